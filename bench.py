@@ -1,0 +1,160 @@
+"""Benchmark of the PatchMatch depth path on one MI355X.
+
+Workload (one "step"): plain single-view PatchMatch on a 1600x1200 float32
+reference with 8 neighbours (seeded 64-facet piecewise-planar heightfield,
+3x3 camera grid, depth range [4, 40] m, patch 7x7, 3 iterations):
+random init + 3 x (red pass + black pass) + decode, all on the device with
+the views already resident in HBM.  Metric: Mpix*views*iters/s =
+(W*H/1e6) * N_src * iterations / seconds.
+
+Also reported: the roofline of the dominant kernel (the colour-pass sweep,
+timed with HIP events on its own stream inside the timed steps) against
+the FP32 vector peak, accuracy against the synthetic ground truth, and the
+CPU oracle timed on the host cores on a bounded sample of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpix·views·iters/s on 1 GPU vs host CPU; % pixels |Δd|/d<1% vs reference"
+UNIT = "Mpix·views·iters/s"
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
+PATCH = 7
+FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap x P taps + 75)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_scene(seed: int):
+    from opensfm_amd import synthetic
+    return synthetic.facet_heightfield(seed=seed, width=1600, height=1200, num_src=8)
+
+
+def configure(est, scene, iters: int, storage: str = "f32"):
+    for im, K, R, t in zip(scene.images, scene.Ks, scene.Rs, scene.ts):
+        est.add_view(K, R, t, im)
+    est.set_depth_range(scene.depth_min, scene.depth_max, 0)
+    est.set_patchmatch_iterations(iters)
+    est.set_patch_size(PATCH)
+    est.set_image_storage(storage)
+
+
+def cpu_baseline(scene, iters: int, rows: int):
+    """CPU oracle on a band of reference rows (all sources), init + iterations."""
+    from oracle import pm_oracle as po
+    y0 = scene.gt_depth.shape[0] // 2 - rows // 2
+    band = scene.crop_rows(y0, y0 + rows)
+    p = po.Params(patch_size=PATCH, depth_min=scene.depth_min, depth_max=scene.depth_max, max_iterations=iters)
+    sc = po.OracleScene(band.images, band.Ks, band.Rs, band.ts)
+    st = po.OracleState(sc.width, sc.height)
+    t0 = time.perf_counter()
+    po.run(sc, p, st)
+    po.decode(sc, p, st)
+    dt = time.perf_counter() - t0
+    units = sc.width * sc.height / 1e6 * band.num_src * iters
+    return {"value": units / dt, "unit": UNIT, "cores": po.max_threads(), "kind": "port",
+            "sample": f"reference rows {y0}..{y0 + rows} of the same 1600x1200 scene ({sc.width}x{rows} px, "
+                      f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iters", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--storage", choices=["f32", "f16"], default="f32")
+    ap.add_argument("--cpu-rows", type=int, default=96, help="reference rows in the CPU-oracle sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by reference "
+              f"view with no cross-GPU traffic at one hierarchy level, so N GPUs would run N independent "
+              f"copies of this workload.", file=sys.stderr)
+        return 2
+
+    from opensfm_amd import native
+    pydense = native()
+    if not pydense.DepthmapEstimator.is_available():
+        raise RuntimeError("no HIP device visible")
+
+    log("rendering scene ...")
+    scene = make_scene(args.seed)
+    h, w = scene.gt_depth.shape
+    n_src = scene.num_src
+    units_per_step = w * h / 1e6 * n_src * args.iters
+
+    est = pydense.DepthmapEstimator()
+    configure(est, scene, args.iters, args.storage)
+    est.upload()
+    for i in range(args.warmup):
+        est.run()
+        log(f"warmup {i + 1}/{args.warmup}: {est.last_timing()['run_ms']:.1f} ms")
+
+    sweep_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        est.run()
+        tm = est.last_timing()
+        sweep_ms += tm["sweep_ms"]
+        launches += tm["sweep_launches"]
+        log(f"step {i + 1}/{args.steps}: {tm['run_ms']:.1f} ms (sweeps {tm['sweep_ms']:.1f} ms)")
+    elapsed = time.perf_counter() - t0
+    value = units_per_step * args.steps / elapsed
+
+    out = est.download()
+    d = out["depth"]
+    gt = scene.gt_depth
+    acc1 = float(np.mean(np.abs(d - gt) < 0.01 * gt))
+    acc5 = float(np.mean(np.abs(d - gt) < 0.05 * gt))
+
+    # Roofline of the sweep kernel: algorithmic FLOP of one colour pass =
+    # (pixels of that colour) x N_src x 15 hypotheses x (21 P + 75).
+    avg_launch_s = sweep_ms / max(launches, 1) / 1e3
+    flop_per_launch = (w * h / 2.0) * n_src * FLOP_PER_UNIT
+    achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel": "sweep_kernel<3,float>", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+                "flop_per_launch": flop_per_launch}
+
+    cpu = None
+    if args.cpu_rows > 0:
+        log(f"CPU oracle on {args.cpu_rows} rows ...")
+        cpu = cpu_baseline(scene, args.iters, args.cpu_rows)
+        cpu["value"] = round(cpu["value"], 5)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else "f16-storage/f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain PatchMatch, one {w}x{h} {args.storage} reference + {n_src} neighbours, "
+                               f"64-facet heightfield (seed {args.seed}), patch {PATCH}x{PATCH}, "
+                               f"{args.iters} iterations, init+sweeps+decode per step",
+                   "width": w, "height": h, "sources": n_src, "iterations": args.iters, "patch": PATCH,
+                   "units_per_step": units_per_step},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
+    }
+    if cpu and cpu["value"] > 0:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,158 @@
+/*
+ * opensfm_pm.h -- C ABI of the MI355X PatchMatch multi-view depth estimator.
+ *
+ * Plain C types only (no torch, no HIP types), so any FFI can bind it.  All
+ * functions return PM_OK (0) or a negative PM_E* code; the message of the
+ * last failure on the calling thread is available from pm_last_error().
+ * One handle = one reference view + its sources + its own HIP stream; a
+ * handle is not thread-safe, distinct handles may be used from distinct
+ * threads.
+ *
+ * Reference interfaces replaced (paths relative to the OpenSfM tree):
+ *   pm_create / pm_destroy      dense::DepthmapEstimator ctor/dtor
+ *                               (opensfm/src/lib/dense/src/estimator.cc:30-31)
+ *                               and DepthmapEstimator() of pydense.pyi:42-43
+ *   pm_add_view                 DepthmapEstimator::AddView (estimator.cc:33-42),
+ *                               add_view(K, R, t, image, mask) (pydense.pyi:45-52)
+ *   pm_set_depth_range          SetDepthRange (estimator.cc:44-47),
+ *                               set_depth_range(min, max, num_planes) (pydense.pyi:56)
+ *   pm_set_params/pm_get_params SetParams(DepthmapParams) (estimator.cc:49-51,
+ *                               estimator.h:16-36); set_patchmatch_iterations,
+ *                               set_patch_size, set_min_patch_sd (pydense.pyi:57-59)
+ *   pm_set_geom_consistency     SetGeomConsistencyWeight / SetPreviousDepths
+ *                               (estimator.cc:53-116)
+ *   pm_set_prior                the prior_planes / plane_masks buffers that
+ *                               ComputeSfMPlanarPrior uploads (estimator.cc:838-1107)
+ *   pm_compute                  Run(DepthmapResult*) (estimator.cc:134-255 +
+ *                               ReadBackResults :1112-1224), compute_patch_match()
+ *                               (pydense.pyi:54)
+ *   pm_upload / pm_run /        the same, split so the device-resident part can be
+ *   pm_download                 timed and re-run without host transfers
+ *   pm_init_state / pm_iterate  RandomInit + PriorReinit (estimator.cc:515-614) and
+ *                               RunIteration (estimator.cc:619-680), for staged parity
+ *   pm_read_state/pm_write_state read/write the per-pixel state buffers
+ *                               (estimator.cc:452-477)
+ *   pm_score_planes             compute_cost_vector + compute_initial_cost_and_views
+ *                               (opencl_kernels.h:381-621), the "ComputePlaneScore"
+ *                               of the estimator, on caller-given planes
+ *   pm_device_count/pm_device_name
+ *                               DepthmapClusterEstimator.num_devices / device_name
+ *                               (opensfm/src/lib/dense/python/pybind.cc:105-122)
+ */
+#ifndef OPENSFM_PM_H_
+#define OPENSFM_PM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_OK 0
+#define PM_E_INVALID (-1)   /* bad argument or call order (std::runtime_error in the reference) */
+#define PM_E_DEVICE (-2)    /* HIP runtime failure / no device */
+#define PM_E_STATE (-3)     /* operation needs a state that does not exist yet */
+#define PM_E_NOMEM (-4)     /* allocation failed */
+
+#define PM_MAX_SOURCES 16   /* reference kMaxSources is 8 (opencl_kernels.h:20) */
+
+/* Pixel formats accepted by pm_add_view. */
+#define PM_U8 0     /* 0..255, stored as value/255 (estimator.cc:313-318) */
+#define PM_F32 1    /* already in [0, 1] */
+#define PM_F16 2    /* IEEE binary16 in [0, 1] */
+
+/* Storage of the images in device memory. */
+#define PM_STORE_F32 0
+#define PM_STORE_F16 1
+
+typedef struct pm_handle pm_handle;
+
+/* DepthmapParams (estimator.h:16-36) plus the textureless-variance gate
+ * (1.5e-6 at opencl_kernels.h:254,516,519) that set_min_patch_sd drives. */
+typedef struct pm_params {
+  int max_iterations;       /* 3 */
+  int patch_size;           /* odd, 3..7 */
+  float sigma_spatial;
+  float sigma_color;        /* in [0,1] intensity units */
+  int top_k;
+  int use_census;           /* 0/1 */
+  float depth_min, depth_max;
+  int hierarchy_levels;     /* only 1 is supported */
+  float smooth_weight;
+  int anchor_views;
+  int checkerboard_filter;  /* 0/1 */
+  float var_threshold;      /* weighted patch variance gate, [0,1]^2 units */
+  int image_storage;        /* PM_STORE_F32 / PM_STORE_F16 */
+  int quantize_half;        /* 1: round inputs to binary16 like the reference upload */
+} pm_params;
+
+void pm_default_params(pm_params* out);
+
+/* Last error message of the calling thread ("" if none). */
+const char* pm_last_error(void);
+
+int pm_device_count(int* count);
+int pm_device_name(int device, char* buf, int buflen);
+
+int pm_create(int device, pm_handle** out);
+int pm_destroy(pm_handle* h);
+
+/* K, R: row-major 3x3 float64; t: 3 float64.  The image is copied.
+ * row_stride in bytes (0 = packed).  mask may be NULL; it is accepted for
+ * the pydense surface and not used (the reference ACMMP has no mask). */
+int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t,
+                const void* image, int format, int width, int height, int64_t row_stride,
+                const uint8_t* mask);
+int pm_clear_views(pm_handle* h);
+int pm_num_views(pm_handle* h, int* n);
+
+int pm_set_depth_range(pm_handle* h, float depth_min, float depth_max);
+int pm_set_params(pm_handle* h, const pm_params* p);
+int pm_get_params(pm_handle* h, pm_params* p);
+
+/* Geometric consistency: weight and, per source slot, a previous depth map
+ * at the reference resolution (NULL = none). */
+int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev_depths, int n);
+
+/* Planar prior: (nx, ny, nz, d) per reference pixel + mask (NULL clears). */
+int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks);
+
+/* Stage views/cameras in device memory and allocate the per-pixel state. */
+int pm_upload(pm_handle* h);
+/* Random init, prior re-init, iterations, optional filter and decode, all on
+ * the device; returns once the stream is idle. */
+int pm_run(pm_handle* h);
+/* Copy the decoded maps out.  Any pointer may be NULL.
+ * depth, cost, confidence: H*W float; plane: H*W*4 float (n, d);
+ * normal: H*W*3 float; views: H*W uint32 selected-view bitmask. */
+int pm_download(pm_handle* h, float* depth, float* plane, float* normal, float* cost,
+                uint32_t* views, float* confidence);
+/* pm_upload + pm_run + pm_download. */
+int pm_compute(pm_handle* h, float* depth, float* plane, float* normal, float* cost,
+               uint32_t* views, float* confidence);
+
+/* Staged execution (after pm_upload). */
+int pm_init_state(pm_handle* h);
+int pm_iterate(pm_handle* h, int iteration, int color /* 0, 1 or -1 for both */);
+int pm_filter(pm_handle* h);
+int pm_decode(pm_handle* h);
+int pm_read_state(pm_handle* h, float* planes, float* costs, uint32_t* rng, uint32_t* sel);
+int pm_write_state(pm_handle* h, const float* planes, const float* costs, const uint32_t* rng,
+                   const uint32_t* sel);
+
+/* Score n (pixel, plane) pairs on the uploaded views: per-view costs (n x
+ * num_sources, row-major), top-k initial cost (n) and view bitmask (n).
+ * xy: 2n int32; planes: 4n float.  init_cost / sel may be NULL. */
+int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs,
+                    float* init_cost, uint32_t* sel);
+
+/* Device time of the last pm_run in milliseconds (HIP events on the
+ * handle's stream), and of its dominant kernel (all sweep launches). */
+int pm_last_timing(pm_handle* h, float* run_ms, float* sweep_ms, int* sweep_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENSFM_PM_H_ */

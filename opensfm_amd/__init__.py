@@ -1,0 +1,27 @@
+"""MI355X-native PatchMatch multi-view depth estimation.
+
+The compute path is the HIP library ``libopensfm_pm.so`` (C ABI in
+``include/opensfm_pm.h``) and the pybind11 module ``pydense`` over it, both
+built in-tree by ``python -m opensfm_amd.build``.  Importing this package
+does not load them; ``opensfm_amd.native()`` does, and raises if they are
+missing.
+"""
+import os
+
+__all__ = ["native", "library_path"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libopensfm_pm.so")
+
+
+def native():
+    """Import the compiled ``pydense`` module (RuntimeError if it was not built)."""
+    try:
+        from . import pydense  # noqa: F401
+    except ImportError as e:  # pragma: no cover - depends on the build
+        raise RuntimeError(
+            "opensfm_amd native extension is not built; run `python -m opensfm_amd.build`") from e
+    return pydense

@@ -1,0 +1,83 @@
+"""Build the native parts in-tree.
+
+  opensfm_amd/libopensfm_pm.so   HIP kernels + C ABI (hipcc, gfx950)
+  opensfm_amd/pydense*.so        pybind11 module over the C ABI (g++)
+
+Run ``python -m opensfm_amd.build``; the package never builds on import.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libopensfm_pm.so")
+ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # numerics contract: no implicit contraction, IEEE divide / sqrt
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    # no packed-FP32 VALU ops anywhere
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def pydense_path() -> str:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "pydense" + suffix)
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_hip(jobs: int = 2) -> str:
+    objs = []
+    procs = []
+    for src in ("pm_kernels.hip", "pm_api.hip"):
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        cmd = ["hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), src))
+        objs.append(obj)
+    for p, src in procs:
+        if p.wait() != 0:
+            raise RuntimeError(f"hipcc failed on {src}")
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def build_pydense() -> str:
+    import pybind11
+
+    out = pydense_path()
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+           f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(CSRC, "pydense.cpp"), "-o", tmp,
+           f"-L{HERE}", "-lopensfm_pm", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_hip()
+    build_pydense()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"all": build_all, "hip": build_hip, "pydense": build_pydense}[what]()

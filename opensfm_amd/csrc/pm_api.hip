@@ -1,0 +1,717 @@
+// C ABI of the PatchMatch estimator (declared in include/opensfm_pm.h).
+//
+// Host orchestration restating dense::DepthmapEstimator (opensfm/src/lib/
+// dense/src/estimator.cc): view staging (:299-510), random init (:515-546),
+// prior re-init (:574-614), iterations (:619-680), checkerboard filter
+// (:685-719) and decode (:1112-1224).  The reference round-trips every colour
+// pass through buffer->image copies (:568-572); here passes update the
+// state buffers in place on one HIP stream and nothing leaves HBM until
+// pm_download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/opensfm_pm.h"
+#include "pm_kernels.h"
+
+using namespace pmhip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define PM_HIP(call, what)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(PM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// IEEE binary16 round-to-nearest-even from binary32 (bit-exact, handles
+// subnormals, overflow to inf and NaN).
+uint16_t f32_to_f16_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t absx = x & 0x7fffffffu;
+  if (absx >= 0x7f800000u) return static_cast<uint16_t>(sign | (absx > 0x7f800000u ? 0x7e00u : 0x7c00u));
+  if (absx >= 0x477ff000u) return static_cast<uint16_t>(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (absx < 0x38800000u) {                                                 // result subnormal or zero
+    if (absx < 0x33000000u) return static_cast<uint16_t>(sign);            // < 2^-25 -> 0 (ties to even 0)
+    const uint32_t mant = (absx & 0x7fffffu) | 0x800000u;
+    const int shift = 126 - static_cast<int>(absx >> 23);                   // 14..24
+    const uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1);
+    uint32_t r = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+    return static_cast<uint16_t>(sign | r);
+  }
+  const uint32_t e = (absx >> 23) - 112u;  // rebias 127 -> 15
+  const uint32_t m = absx & 0x7fffffu;
+  uint32_t r = (e << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+  return static_cast<uint16_t>(sign | r);
+}
+
+float f16_bits_to_f32(uint16_t h) {
+  const uint32_t sign = (static_cast<uint32_t>(h) & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu;
+  uint32_t m = h & 0x3ffu;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) {
+      x = sign;
+    } else {  // subnormal: normalise
+      int k = -1;
+      do { ++k; m <<= 1; } while ((m & 0x400u) == 0);
+      x = sign | ((112u - k) << 23) | ((m & 0x3ffu) << 13);
+    }
+  } else if (e == 31) {
+    x = sign | 0x7f800000u | (m << 13);
+  } else {
+    x = sign | ((e + 112u) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+struct HostView {
+  double K[9], R[9], t[3];
+  int w, h;
+  std::vector<float> pix;  // [0,1] values as given (u8 already /255)
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count == n && p) return PM_OK;
+    release();
+    if (count == 0) return PM_OK;
+    if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return fail(PM_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+    return PM_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct pm_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  pm_params params{};
+  std::vector<HostView> views;
+  float geom_weight = 0.0f;
+  std::vector<std::vector<float>> prev_depths;  // per source slot, empty = none
+  std::vector<float> prior_planes;
+  std::vector<uint32_t> prior_masks;
+
+  // device state
+  bool uploaded = false;
+  bool state_ready = false;
+  int W = 0, H = 0, n_src = 0;
+  bool f16 = false;
+  DevBuf<unsigned char> images;
+  DevBuf<PmView> dviews;
+  DevBuf<PmCam> dcams;
+  DevBuf<float4> planes, prior, snapshot;
+  DevBuf<float> costs, prevd;
+  DevBuf<uint2> rng;
+  DevBuf<uint32_t> sel, masks;
+  uint32_t prev_mask = 0u;
+  DevBuf<float> out_depth, out_normal, out_cost, out_conf;
+  float last_run_ms = 0.0f, last_sweep_ms = 0.0f;
+  int last_sweeps = 0;
+
+  ~pm_handle() {
+    images.release(); dviews.release(); dcams.release(); planes.release(); prior.release();
+    snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
+    out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int check_handle(pm_handle* h) {
+  if (!h) return fail(PM_E_INVALID, "null pm_handle");
+  return PM_OK;
+}
+
+PmArgs make_args(pm_handle* h) {
+  PmArgs a{};
+  a.planes = h->planes.p;
+  a.costs = h->costs.p;
+  a.rng = h->rng.p;
+  a.sel = h->sel.p;
+  a.prior_planes = h->masks.p ? h->prior.p : nullptr;
+  a.plane_masks = h->masks.p;
+  a.prev_depths = h->prevd.p;
+  a.prev_depth_mask = h->prev_mask;
+  a.snapshot = h->snapshot.p;
+  a.images = h->images.p;
+  a.views = h->dviews.p;
+  a.cams = h->dcams.p;
+  a.width = h->W;
+  a.height = h->H;
+  a.n_src = h->n_src;
+  const pm_params& p = h->params;
+  a.depth_min = p.depth_min;
+  a.depth_max = p.depth_max;
+  a.spatial_factor = -1.0f / (2.0f * p.sigma_spatial * p.sigma_spatial);
+  a.color_factor = -1.0f / (2.0f * p.sigma_color * p.sigma_color);
+  a.top_k = p.top_k;
+  a.census_weight = p.use_census ? 1.0f : 0.0f;
+  a.var_threshold = p.var_threshold;
+  a.smooth_w = p.smooth_weight;
+  a.geom_weight = h->geom_weight;
+  a.anchor_views = p.anchor_views;
+  a.color = 0;
+  a.iteration = 0;
+  return a;
+}
+
+// Smooth weight of one iteration: estimator.cc:646-649.
+float smooth_weight_at(const pm_params& p, int iter) {
+  const float decay = 1.0f - 0.9f * static_cast<float>(iter) /
+                                 static_cast<float>(std::max(p.max_iterations - 1, 1));
+  return p.smooth_weight * decay;
+}
+
+PmCam make_cam(const HostView& v) {
+  PmCam c{};
+  for (int i = 0; i < 9; ++i) { c.K[i] = static_cast<float>(v.K[i]); c.R[i] = static_cast<float>(v.R[i]); }
+  for (int i = 0; i < 3; ++i) c.t[i] = static_cast<float>(v.t[i]);
+  c.width = v.w;
+  c.height = v.h;
+  return c;
+}
+
+int validate(pm_handle* h) {
+  const pm_params& p = h->params;
+  if (h->views.empty()) return fail(PM_E_INVALID, "DepthmapEstimator: no views added");
+  if (h->views.size() < 2) return fail(PM_E_INVALID, "DepthmapEstimator: need at least one source view");
+  if (static_cast<int>(h->views.size()) - 1 > PM_MAX_SOURCES)
+    return fail(PM_E_INVALID, "DepthmapEstimator: at most " + std::to_string(PM_MAX_SOURCES) + " source views");
+  if (p.patch_size != 3 && p.patch_size != 5 && p.patch_size != 7)
+    return fail(PM_E_INVALID, "patch_size must be 3, 5 or 7");
+  if (!(p.depth_min > 0.0f) || !(p.depth_max > p.depth_min))
+    return fail(PM_E_INVALID, "depth range must satisfy 0 < min < max");
+  if (p.max_iterations < 0) return fail(PM_E_INVALID, "max_iterations must be >= 0");
+  if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "only hierarchy_levels = 1 is supported");
+  if (!(p.sigma_spatial > 0.0f) || !(p.sigma_color > 0.0f)) return fail(PM_E_INVALID, "sigmas must be > 0");
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_default_params(pm_params* o) {
+  if (!o) return;
+  // Reference pipeline profile (opensfm/config.py:407-466) at one level.
+  o->max_iterations = 3;
+  o->patch_size = 7;
+  o->sigma_spatial = 1.5f;
+  o->sigma_color = 25.0f / 255.0f;
+  o->top_k = 8;
+  o->use_census = 1;
+  o->depth_min = 0.0f;
+  o->depth_max = 1.0f;
+  o->hierarchy_levels = 1;
+  o->smooth_weight = 0.2f;
+  o->anchor_views = 2;
+  o->checkerboard_filter = 0;
+  o->var_threshold = 1.5e-6f;
+  o->image_storage = PM_STORE_F32;
+  o->quantize_half = 0;
+}
+
+const char* pm_last_error(void) { return g_last_error.c_str(); }
+
+int pm_device_count(int* count) {
+  if (!count) return fail(PM_E_INVALID, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return PM_OK;
+}
+
+int pm_device_name(int device, char* buf, int buflen) {
+  if (!buf || buflen <= 0) return fail(PM_E_INVALID, "null buffer");
+  hipDeviceProp_t prop;
+  PM_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  std::snprintf(buf, static_cast<size_t>(buflen), "%s (%s)", prop.name, prop.gcnArchName);
+  return PM_OK;
+}
+
+int pm_create(int device, pm_handle** out) {
+  if (!out) return fail(PM_E_INVALID, "null output handle");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(PM_E_DEVICE, "DepthmapEstimator: no HIP device is available on this system");
+  if (device < 0 || device >= n) return fail(PM_E_INVALID, "device index out of range");
+  PM_HIP(hipSetDevice(device), "hipSetDevice");
+  pm_handle* h = new pm_handle();
+  h->device = device;
+  pm_default_params(&h->params);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(PM_E_DEVICE, "hipStreamCreate failed");
+  }
+  for (auto& e : h->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete h;
+      return fail(PM_E_DEVICE, "hipEventCreate failed");
+    }
+  }
+  *out = h;
+  return PM_OK;
+}
+
+int pm_destroy(pm_handle* h) {
+  if (!h) return PM_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+  return PM_OK;
+}
+
+int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t, const void* image,
+                int format, int width, int height, int64_t row_stride, const uint8_t* /*mask*/) {
+  if (int r = check_handle(h)) return r;
+  if (!K || !R || !t || !image) return fail(PM_E_INVALID, "add_view: null argument");
+  if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "add_view: empty image");
+  int esize = format == PM_U8 ? 1 : format == PM_F32 ? 4 : format == PM_F16 ? 2 : 0;
+  if (!esize) return fail(PM_E_INVALID, "add_view: unknown pixel format");
+  if (row_stride == 0) row_stride = static_cast<int64_t>(width) * esize;
+  if (row_stride < static_cast<int64_t>(width) * esize) return fail(PM_E_INVALID, "add_view: row stride too small");
+  HostView v;
+  std::memcpy(v.K, K, sizeof(v.K));
+  std::memcpy(v.R, R, sizeof(v.R));
+  std::memcpy(v.t, t, sizeof(v.t));
+  v.w = width;
+  v.h = height;
+  v.pix.resize(static_cast<size_t>(width) * height);
+  const unsigned char* base = static_cast<const unsigned char*>(image);
+  const float inv255 = static_cast<float>(1.0 / 255.0);
+  for (int y = 0; y < height; ++y) {
+    const unsigned char* row = base + row_stride * y;
+    float* dst = v.pix.data() + static_cast<size_t>(y) * width;
+    for (int x = 0; x < width; ++x) {
+      if (format == PM_U8) {
+        dst[x] = static_cast<float>(row[x]) * inv255;  // estimator.cc:315
+      } else if (format == PM_F32) {
+        std::memcpy(&dst[x], row + 4 * x, 4);
+      } else {
+        uint16_t b;
+        std::memcpy(&b, row + 2 * x, 2);
+        dst[x] = f16_bits_to_f32(b);
+      }
+    }
+  }
+  h->views.push_back(std::move(v));
+  h->uploaded = false;
+  h->state_ready = false;
+  return PM_OK;
+}
+
+int pm_clear_views(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  h->views.clear();
+  h->uploaded = false;
+  h->state_ready = false;
+  return PM_OK;
+}
+
+int pm_num_views(pm_handle* h, int* n) {
+  if (int r = check_handle(h)) return r;
+  if (!n) return fail(PM_E_INVALID, "null output");
+  *n = static_cast<int>(h->views.size());
+  return PM_OK;
+}
+
+int pm_set_depth_range(pm_handle* h, float dmin, float dmax) {
+  if (int r = check_handle(h)) return r;
+  h->params.depth_min = dmin;
+  h->params.depth_max = dmax;
+  h->state_ready = false;
+  return PM_OK;
+}
+
+int pm_set_params(pm_handle* h, const pm_params* p) {
+  if (int r = check_handle(h)) return r;
+  if (!p) return fail(PM_E_INVALID, "null params");
+  const bool restage = p->image_storage != h->params.image_storage || p->quantize_half != h->params.quantize_half;
+  h->params = *p;
+  if (restage) h->uploaded = false;
+  h->state_ready = false;
+  return PM_OK;
+}
+
+int pm_get_params(pm_handle* h, pm_params* p) {
+  if (int r = check_handle(h)) return r;
+  if (!p) return fail(PM_E_INVALID, "null params");
+  *p = h->params;
+  return PM_OK;
+}
+
+int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev_depths, int n) {
+  if (int r = check_handle(h)) return r;
+  if (n < 0 || n > PM_MAX_SOURCES) return fail(PM_E_INVALID, "too many previous depth maps");
+  if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before previous depths");
+  const size_t npix = static_cast<size_t>(h->views[0].w) * h->views[0].h;
+  h->geom_weight = weight;
+  h->prev_depths.assign(static_cast<size_t>(n), std::vector<float>());
+  for (int i = 0; i < n; ++i)
+    if (prev_depths && prev_depths[i]) h->prev_depths[i].assign(prev_depths[i], prev_depths[i] + npix);
+  h->uploaded = false;
+  return PM_OK;
+}
+
+int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks) {
+  if (int r = check_handle(h)) return r;
+  if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before the prior");
+  const size_t npix = static_cast<size_t>(h->views[0].w) * h->views[0].h;
+  if (!prior_planes || !masks) {
+    h->prior_planes.clear();
+    h->prior_masks.clear();
+  } else {
+    h->prior_planes.assign(prior_planes, prior_planes + 4 * npix);
+    h->prior_masks.assign(masks, masks + npix);
+  }
+  h->uploaded = false;
+  return PM_OK;
+}
+
+int pm_upload(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (int r = validate(h)) return r;
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  const pm_params& p = h->params;
+  const int nv = static_cast<int>(h->views.size());
+  h->W = h->views[0].w;
+  h->H = h->views[0].h;
+  h->n_src = nv - 1;
+  h->f16 = p.image_storage == PM_STORE_F16;
+  const size_t esz = h->f16 ? 2 : 4;
+
+  // Padded images (1-texel replicated border = clamp-to-edge).
+  std::vector<PmView> vt(static_cast<size_t>(nv));
+  int64_t total = 0;
+  for (int i = 0; i < nv; ++i) {
+    const HostView& v = h->views[static_cast<size_t>(i)];
+    vt[i].base = total;
+    vt[i].w = v.w;
+    vt[i].h = v.h;
+    vt[i].pitch = v.w + 2;
+    total += static_cast<int64_t>(v.w + 2) * (v.h + 2);
+    total = (total + 63) & ~int64_t(63);
+  }
+  std::vector<unsigned char> host(static_cast<size_t>(total) * esz, 0);
+  for (int i = 0; i < nv; ++i) {
+    const HostView& v = h->views[static_cast<size_t>(i)];
+    for (int y = -1; y <= v.h; ++y) {
+      const int cy = std::min(std::max(y, 0), v.h - 1);
+      for (int x = -1; x <= v.w; ++x) {
+        const int cx = std::min(std::max(x, 0), v.w - 1);
+        float val = v.pix[static_cast<size_t>(cy) * v.w + cx];
+        const size_t o = static_cast<size_t>(vt[i].base + static_cast<int64_t>(y + 1) * vt[i].pitch + (x + 1));
+        if (h->f16 || p.quantize_half) {
+          const uint16_t b = f32_to_f16_bits(val);
+          if (h->f16) { std::memcpy(&host[o * 2], &b, 2); continue; }
+          val = f16_bits_to_f32(b);
+        }
+        std::memcpy(&host[o * 4], &val, 4);
+      }
+    }
+  }
+  // Cameras, source camera centres (float, opencl_kernels.h:1523-1525) and
+  // angle weights (float64 then cast, estimator.cc:380-407).
+  std::vector<PmCam> cams(static_cast<size_t>(nv));
+  for (int i = 0; i < nv; ++i) cams[i] = make_cam(h->views[static_cast<size_t>(i)]);
+  const HostView& v0 = h->views[0];
+  const double c0[3] = {-(v0.R[0] * v0.t[0] + v0.R[3] * v0.t[1] + v0.R[6] * v0.t[2]),
+                        -(v0.R[1] * v0.t[0] + v0.R[4] * v0.t[1] + v0.R[7] * v0.t[2]),
+                        -(v0.R[2] * v0.t[0] + v0.R[5] * v0.t[1] + v0.R[8] * v0.t[2])};
+  for (int i = 1; i < nv; ++i) {
+    const PmCam& c = cams[i];
+    vt[i].cx = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    vt[i].cy = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    vt[i].cz = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    const HostView& v = h->views[static_cast<size_t>(i)];
+    const double ci[3] = {-(v.R[0] * v.t[0] + v.R[3] * v.t[1] + v.R[6] * v.t[2]),
+                          -(v.R[1] * v.t[0] + v.R[4] * v.t[1] + v.R[7] * v.t[2]),
+                          -(v.R[2] * v.t[0] + v.R[5] * v.t[1] + v.R[8] * v.t[2])};
+    const double b[3] = {ci[0] - c0[0], ci[1] - c0[1], ci[2] - c0[2]};
+    const double bl = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (bl < 1e-12) {
+      vt[i].angle_w = 0.01f;
+    } else {
+      const double ca = std::fabs(v0.R[6] * (b[0] / bl) + v0.R[7] * (b[1] / bl) + v0.R[8] * (b[2] / bl));
+      const double sa = std::sqrt(1.0 - ca * ca);
+      vt[i].angle_w = static_cast<float>(std::max(sa, 0.01));
+    }
+  }
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  if (int r = h->images.alloc(host.size())) return r;
+  if (int r = h->dviews.alloc(vt.size())) return r;
+  if (int r = h->dcams.alloc(cams.size())) return r;
+  if (int r = h->planes.alloc(npix)) return r;
+  if (int r = h->costs.alloc(npix)) return r;
+  if (int r = h->rng.alloc(npix)) return r;
+  if (int r = h->sel.alloc(npix)) return r;
+  if (int r = h->out_depth.alloc(npix)) return r;
+  if (int r = h->out_normal.alloc(3 * npix)) return r;
+  if (int r = h->out_cost.alloc(npix)) return r;
+  if (int r = h->out_conf.alloc(npix)) return r;
+  PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
+  PM_HIP(hipMemcpyAsync(h->dviews.p, vt.data(), vt.size() * sizeof(PmView), hipMemcpyHostToDevice, h->stream),
+         "upload view table");
+  PM_HIP(hipMemcpyAsync(h->dcams.p, cams.data(), cams.size() * sizeof(PmCam), hipMemcpyHostToDevice, h->stream),
+         "upload cameras");
+  // Prior (estimator.cc:480-486) and previous depths (:499-508, 66-116).
+  if (!h->prior_masks.empty()) {
+    if (int r = h->prior.alloc(npix)) return r;
+    if (int r = h->masks.alloc(npix)) return r;
+    if (int r = h->snapshot.alloc(npix)) return r;
+    PM_HIP(hipMemcpyAsync(h->prior.p, h->prior_planes.data(), npix * sizeof(float4), hipMemcpyHostToDevice,
+                          h->stream), "upload prior");
+    PM_HIP(hipMemcpyAsync(h->masks.p, h->prior_masks.data(), npix * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          h->stream), "upload masks");
+  } else {
+    h->prior.release();
+    h->masks.release();
+    h->snapshot.release();
+  }
+  h->prev_mask = 0u;
+  bool any_prev = false;
+  for (auto& d : h->prev_depths) any_prev |= !d.empty();
+  if (any_prev && h->geom_weight > 0.0f) {
+    if (int r = h->prevd.alloc(static_cast<size_t>(h->n_src) * npix)) return r;
+    std::vector<float> flat(static_cast<size_t>(h->n_src) * npix, 0.0f);
+    for (int i = 0; i < static_cast<int>(h->prev_depths.size()) && i < h->n_src; ++i) {
+      if (h->prev_depths[i].empty()) continue;
+      std::memcpy(&flat[static_cast<size_t>(i) * npix], h->prev_depths[i].data(), npix * sizeof(float));
+      h->prev_mask |= 1u << i;
+    }
+    PM_HIP(hipMemcpyAsync(h->prevd.p, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, h->stream),
+           "upload previous depths");
+    PM_HIP(hipStreamSynchronize(h->stream), "upload sync");
+  } else {
+    h->prevd.release();
+  }
+  PM_HIP(hipStreamSynchronize(h->stream), "upload sync");
+  h->uploaded = true;
+  h->state_ready = false;
+  return PM_OK;
+}
+
+int pm_init_state(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_init_state: views are not uploaded (call pm_upload)");
+  if (int r = validate(h)) return r;
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  PmArgs a = make_args(h);
+  const int hp = h->params.patch_size / 2;
+  PM_HIP(launch_init(a, hp, h->f16, h->stream), "init kernel launch");
+  if (h->masks.p) {  // estimator.cc:574-614
+    PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
+                          h->stream), "prior snapshot");
+    PM_HIP(launch_prior(a, hp, h->f16, h->stream), "prior kernel launch");
+  }
+  h->state_ready = true;
+  return PM_OK;
+}
+
+int pm_iterate(pm_handle* h, int iteration, int color) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_iterate: state is not initialised (call pm_init_state)");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  PmArgs a = make_args(h);
+  a.iteration = iteration;
+  a.smooth_w = smooth_weight_at(h->params, iteration);
+  const int hp = h->params.patch_size / 2;
+  for (int c = 0; c < 2; ++c) {
+    if (color >= 0 && c != color) continue;
+    a.color = c;
+    PM_HIP(launch_sweep(a, hp, h->f16, h->stream), "sweep kernel launch");
+  }
+  return PM_OK;
+}
+
+int pm_filter(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_filter: state is not initialised");
+  PmArgs a = make_args(h);
+  for (int c = 0; c < 2; ++c) {
+    a.color = c;
+    PM_HIP(launch_filter(a, h->stream), "filter kernel launch");
+  }
+  return PM_OK;
+}
+
+int pm_decode(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_decode: state is not initialised");
+  PmArgs a = make_args(h);
+  PmDecodeOut o{h->out_depth.p, h->out_normal.p, h->out_cost.p, h->out_conf.p};
+  PM_HIP(launch_decode(a, o, h->stream), "decode kernel launch");
+  return PM_OK;
+}
+
+int pm_run(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) {
+    if (int r = pm_upload(h)) return r;
+  }
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
+  if (int r = pm_init_state(h)) return r;
+  PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
+  for (int it = 0; it < h->params.max_iterations; ++it)
+    if (int r = pm_iterate(h, it, -1)) return r;
+  PM_HIP(hipEventRecord(h->ev[2], h->stream), "event");
+  if (h->params.checkerboard_filter)
+    if (int r = pm_filter(h)) return r;
+  if (int r = pm_decode(h)) return r;
+  PM_HIP(hipEventRecord(h->ev[3], h->stream), "event");
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_run");
+  PM_HIP(hipEventElapsedTime(&h->last_run_ms, h->ev[0], h->ev[3]), "event time");
+  PM_HIP(hipEventElapsedTime(&h->last_sweep_ms, h->ev[1], h->ev[2]), "event time");
+  h->last_sweeps = 2 * h->params.max_iterations;
+  return PM_OK;
+}
+
+int pm_last_timing(pm_handle* h, float* run_ms, float* sweep_ms, int* sweeps) {
+  if (int r = check_handle(h)) return r;
+  if (run_ms) *run_ms = h->last_run_ms;
+  if (sweep_ms) *sweep_ms = h->last_sweep_ms;
+  if (sweeps) *sweeps = h->last_sweeps;
+  return PM_OK;
+}
+
+int pm_download(pm_handle* h, float* depth, float* plane, float* normal, float* cost, uint32_t* views,
+                float* confidence) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_download: nothing computed yet");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (!dst) return PM_OK;
+    PM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream), "download");
+    return PM_OK;
+  };
+  if (int r = cp(depth, h->out_depth.p, npix * 4)) return r;
+  if (int r = cp(plane, h->planes.p, npix * 16)) return r;
+  if (int r = cp(normal, h->out_normal.p, npix * 12)) return r;
+  if (int r = cp(cost, h->out_cost.p, npix * 4)) return r;
+  if (int r = cp(views, h->sel.p, npix * 4)) return r;
+  if (int r = cp(confidence, h->out_conf.p, npix * 4)) return r;
+  PM_HIP(hipStreamSynchronize(h->stream), "download sync");
+  return PM_OK;
+}
+
+int pm_compute(pm_handle* h, float* depth, float* plane, float* normal, float* cost, uint32_t* views,
+               float* confidence) {
+  if (int r = check_handle(h)) return r;
+  if (int r = pm_upload(h)) return r;
+  if (int r = pm_run(h)) return r;
+  return pm_download(h, depth, plane, normal, cost, views, confidence);
+}
+
+int pm_read_state(pm_handle* h, float* planes, float* costs, uint32_t* rng, uint32_t* sel) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_read_state: state is not initialised");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  if (planes) PM_HIP(hipMemcpyAsync(planes, h->planes.p, npix * 16, hipMemcpyDeviceToHost, h->stream), "read");
+  if (costs) PM_HIP(hipMemcpyAsync(costs, h->costs.p, npix * 4, hipMemcpyDeviceToHost, h->stream), "read");
+  if (rng) PM_HIP(hipMemcpyAsync(rng, h->rng.p, npix * 8, hipMemcpyDeviceToHost, h->stream), "read");
+  if (sel) PM_HIP(hipMemcpyAsync(sel, h->sel.p, npix * 4, hipMemcpyDeviceToHost, h->stream), "read");
+  PM_HIP(hipStreamSynchronize(h->stream), "read sync");
+  return PM_OK;
+}
+
+int pm_write_state(pm_handle* h, const float* planes, const float* costs, const uint32_t* rng,
+                   const uint32_t* sel) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_write_state: views are not uploaded (call pm_upload)");
+  if (!planes || !costs || !rng || !sel) return fail(PM_E_INVALID, "pm_write_state needs all four buffers");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  PM_HIP(hipMemcpyAsync(h->planes.p, planes, npix * 16, hipMemcpyHostToDevice, h->stream), "write");
+  PM_HIP(hipMemcpyAsync(h->costs.p, costs, npix * 4, hipMemcpyHostToDevice, h->stream), "write");
+  PM_HIP(hipMemcpyAsync(h->rng.p, rng, npix * 8, hipMemcpyHostToDevice, h->stream), "write");
+  PM_HIP(hipMemcpyAsync(h->sel.p, sel, npix * 4, hipMemcpyHostToDevice, h->stream), "write");
+  PM_HIP(hipStreamSynchronize(h->stream), "write sync");
+  h->state_ready = true;
+  return PM_OK;
+}
+
+int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs, float* init_cost,
+                    uint32_t* sel) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_score_planes: views are not uploaded (call pm_upload)");
+  if (n < 0 || (n > 0 && (!xy || !planes || !costs))) return fail(PM_E_INVALID, "pm_score_planes: bad arguments");
+  if (n == 0) return PM_OK;
+  for (int i = 0; i < n; ++i)
+    if (xy[2 * i] < 0 || xy[2 * i] >= h->W || xy[2 * i + 1] < 0 || xy[2 * i + 1] >= h->H)
+      return fail(PM_E_INVALID, "pm_score_planes: pixel outside the reference view");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  DevBuf<int32_t> dxy;
+  DevBuf<float4> dpl;
+  DevBuf<float> dcost, dinit;
+  DevBuf<uint32_t> dsel;
+  if (int r = dxy.alloc(2 * static_cast<size_t>(n))) return r;
+  if (int r = dpl.alloc(static_cast<size_t>(n))) return r;
+  if (int r = dcost.alloc(static_cast<size_t>(n) * h->n_src)) return r;
+  if (int r = dinit.alloc(static_cast<size_t>(n))) return r;
+  if (int r = dsel.alloc(static_cast<size_t>(n))) return r;
+  int rc = PM_OK;
+  do {
+    PmArgs a = make_args(h);
+    if (hipMemcpyAsync(dxy.p, xy, 8 * static_cast<size_t>(n), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(dpl.p, planes, 16 * static_cast<size_t>(n), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+      rc = fail(PM_E_DEVICE, "pm_score_planes: upload failed");
+      break;
+    }
+    hipError_t e = launch_score(a, h->params.patch_size / 2, h->f16, n, dxy.p, dpl.p, dcost.p, dinit.p, dsel.p,
+                                h->stream);
+    if (e != hipSuccess) { rc = fail(PM_E_DEVICE, std::string("score kernel: ") + hipGetErrorString(e)); break; }
+    if (hipMemcpyAsync(costs, dcost.p, 4 * static_cast<size_t>(n) * h->n_src, hipMemcpyDeviceToHost, h->stream) !=
+        hipSuccess) { rc = fail(PM_E_DEVICE, "pm_score_planes: download failed"); break; }
+    if (init_cost) (void)hipMemcpyAsync(init_cost, dinit.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost, h->stream);
+    if (sel) (void)hipMemcpyAsync(sel, dsel.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost, h->stream);
+    e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(PM_E_DEVICE, std::string("pm_score_planes: ") + hipGetErrorString(e));
+  } while (0);
+  dxy.release(); dpl.release(); dcost.release(); dinit.release(); dsel.release();
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// Device-side building blocks of the PatchMatch path for gfx950.
+//
+// Numerics contract (shared with the CPU oracle so the two agree bit for
+// bit): the library is compiled with -ffp-contract=off, fused multiply-adds
+// appear only as explicit fmaf() (bilinear lerps, warp coordinates, NCC
+// accumulations), exp/log are pm_expf/pm_logf below, sqrt and '/' are IEEE
+// correctly rounded, and bilinear sampling is exact software bilinear with
+// texel centres at integer coordinates and clamp-to-edge addressing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pmhip {
+
+constexpr int kMaxSrc = 16;
+constexpr int kMcSamples = 32;                       // opencl_kernels.h:910
+constexpr float kPiF = 3.14159274101257324219f;     // M_PI_F
+
+// Camera as packed by the reference host (opencl_utils.h:258-280).
+struct PmCam {
+  float K[9];
+  float R[9];
+  float t[3];
+  int width;
+  int height;
+};
+static_assert(sizeof(PmCam) == 92, "camera layout");
+
+// Per-view image descriptor.  Images are stored with a one-texel replicated
+// border (clamp-to-edge baked in), so texel (x, y) of a w x h view lives at
+// base + (y + 1) * pitch + (x + 1), pitch = w + 2.
+struct PmView {
+  int64_t base;      // element offset of texel (-1, -1)
+  int w, h, pitch;
+  float cx, cy, cz;  // camera centre -R^T t (sources; opencl_kernels.h:1523-1525)
+  float angle_w;     // estimator.cc:380-407
+};
+
+struct Plane { float x, y, z, w; };
+struct V3 { float x, y, z; };
+
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(V3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ V3 normalize3(V3 a) {
+  const float l = len3(a);
+  if (l == 0.0f) return a;
+  return V3{a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// e^x: Cody-Waite reduction by ln2, degree-7 Taylor polynomial, 2^k via the
+// exponent field; x < -86 -> 0, x > 88 -> +inf.
+__device__ __forceinline__ float pm_expf(float x) {
+  if (x != x) return x;
+  if (x < -86.0f) return 0.0f;
+  if (x > 88.0f) return __builtin_inff();
+  const float kf = __builtin_rintf(x * 1.44269502162933349609f);
+  float r = fmaf(-kf, 0.693145751953125f, x);
+  r = fmaf(-kf, 1.42860682030941723212e-6f, r);
+  float p = 1.98412698412698412698e-4f;
+  p = fmaf(p, r, 1.38888888888888888889e-3f);
+  p = fmaf(p, r, 8.33333333333333333333e-3f);
+  p = fmaf(p, r, 4.16666666666666666667e-2f);
+  p = fmaf(p, r, 1.66666666666666666667e-1f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const int k = static_cast<int>(kf);
+  return p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
+}
+
+// ln x, x > 0: ln m = 2 atanh((m-1)/(m+1)) with m in [sqrt(1/2), sqrt(2)).
+__device__ __forceinline__ float pm_logf(float x) {
+  if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+  if (x == __builtin_inff()) return x;
+  uint32_t u = __float_as_uint(x);
+  int e = static_cast<int>((u >> 23) & 0xffu) - 127;
+  if (e == -127) {
+    x = x * 8388608.0f;
+    u = __float_as_uint(x);
+    e = static_cast<int>((u >> 23) & 0xffu) - 127 - 23;
+  }
+  float m = __uint_as_float((u & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421353816986083984f) { m = m * 0.5f; e += 1; }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float s2 = s * s;
+  float p = 9.09090909090909090909e-2f;
+  p = fmaf(p, s2, 1.11111111111111111111e-1f);
+  p = fmaf(p, s2, 1.42857142857142857143e-1f);
+  p = fmaf(p, s2, 2.0e-1f);
+  p = fmaf(p, s2, 3.33333333333333333333e-1f);
+  p = fmaf(p, s2, 1.0f);
+  const float lm = (2.0f * s) * p;
+  const float fe = static_cast<float>(e);
+  return fmaf(fe, 0.693145751953125f, fmaf(fe, 1.42860682030941723212e-6f, lm));
+}
+
+// Counter-based per-pixel RNG (opencl_kernels.h:78-97): 4 Philox-style
+// rounds, multiplier 0xD256D193, round keys key + i * 0x9E3779B9.
+__device__ __forceinline__ float rand_float(uint32_t& s0, uint32_t& s1, uint32_t key) {
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t lo = s0 * 0xD256D193u;
+    const uint32_t hi = __umulhi(s0, 0xD256D193u);
+    s0 = s1 ^ (key + i * 0x9E3779B9u) ^ hi;
+    s1 = lo;
+  }
+  return static_cast<float>(s0) / 4294967296.0f;
+}
+
+// Geometry (opencl_kernels.h:102-173).
+__device__ __forceinline__ V3 project(V3 X, const PmCam& c) {
+  V3 xc;
+  xc.x = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
+  xc.y = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
+  xc.z = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
+  V3 p;
+  p.x = c.K[0] * xc.x + c.K[1] * xc.y + c.K[2] * xc.z;
+  p.y = c.K[3] * xc.x + c.K[4] * xc.y + c.K[5] * xc.z;
+  p.z = c.K[6] * xc.x + c.K[7] * xc.y + c.K[8] * xc.z;
+  return p;
+}
+
+__device__ __forceinline__ V3 backproject(float x, float y, float depth, const PmCam& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  V3 pc{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
+  V3 d{pc.x - c.t[0], pc.y - c.t[1], pc.z - c.t[2]};
+  V3 w;
+  w.x = c.R[0] * d.x + c.R[3] * d.y + c.R[6] * d.z;
+  w.y = c.R[1] * d.x + c.R[4] * d.y + c.R[7] * d.z;
+  w.z = c.R[2] * d.x + c.R[5] * d.y + c.R[8] * d.z;
+  return w;
+}
+
+__device__ __forceinline__ float depth_from_plane(float x, float y, Plane p, const PmCam& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  const float denom = p.x * (x - cx) / fx + p.y * (y - cy) / fy + p.z;
+  if (fabsf(denom) < 1e-6f) return 0.0f;
+  return -p.w / denom;
+}
+
+__device__ __forceinline__ Plane plane_from_depth_normal(float x, float y, float depth, V3 n, const PmCam& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  V3 pt{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
+  const float d = -dot3(n, pt);
+  return Plane{n.x, n.y, n.z, d};
+}
+
+// Image access on the padded layout.
+template <typename T>
+__device__ __forceinline__ float ld(const T* p) { return static_cast<float>(*p); }
+
+template <typename T>
+__device__ __forceinline__ float texel(const T* __restrict__ img, const PmView& v, int ix, int iy) {
+  ix = min(max(ix, 0), v.w - 1);
+  iy = min(max(iy, 0), v.h - 1);
+  return ld(img + v.base + static_cast<int64_t>(iy + 1) * v.pitch + (ix + 1));
+}
+
+template <typename T>
+__device__ __forceinline__ float bilinear(const T* __restrict__ img, const PmView& v, float sx, float sy) {
+  sx = clampf(sx, -1.0f, static_cast<float>(v.w - 1));
+  sy = clampf(sy, -1.0f, static_cast<float>(v.h - 1));
+  const float fx0 = floorf(sx), fy0 = floorf(sy);
+  const float ax = sx - fx0, ay = sy - fy0;
+  const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+  const T* r0 = img + v.base + static_cast<int64_t>(y0 + 1) * v.pitch + (x0 + 1);
+  const T* r1 = r0 + v.pitch;
+  const float t00 = ld(r0), t01 = ld(r0 + 1);
+  const float t10 = ld(r1), t11 = ld(r1 + 1);
+  const float v0 = fmaf(ax, t01 - t00, t00);
+  const float v1 = fmaf(ax, t11 - t10, t10);
+  return fmaf(ay, v1 - v0, v0);
+}
+
+}  // namespace pmhip

@@ -1,0 +1,56 @@
+// Host-side launchers of the PatchMatch kernels (implemented in pm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_device.h"
+
+namespace pmhip {
+
+// Everything a kernel of the path needs, passed by value as the kernel
+// argument.  Per-pixel state is row-major over the reference view.
+struct PmArgs {
+  float4* planes;              // (nx, ny, nz, d)
+  float* costs;
+  uint2* rng;
+  uint32_t* sel;
+  const float4* prior_planes;  // may be null
+  const uint32_t* plane_masks; // may be null
+  const float* prev_depths;    // n_src * W * H, may be null
+  const float4* snapshot;      // planes before the prior re-init launch
+  const void* images;          // padded views, float or _Float16
+  const PmView* views;         // 1 + n_src
+  const PmCam* cams;           // 1 + n_src
+  uint32_t prev_depth_mask;
+  int width, height, n_src;
+  float depth_min, depth_max;
+  float spatial_factor, color_factor;
+  int top_k;
+  float census_weight;
+  float var_threshold;
+  float smooth_w;
+  float geom_weight;
+  int anchor_views;
+  int color;
+  int iteration;
+};
+
+struct PmDecodeOut {
+  float* depth;
+  float* normal;  // 3 per pixel
+  float* cost;
+  float* confidence;
+};
+
+// All launchers enqueue on `stream` and return the launch status.
+hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_sweep(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_filter(const PmArgs& a, hipStream_t stream);
+hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stream);
+hipError_t launch_score(const PmArgs& a, int half_patch, bool f16, int n, const int32_t* xy,
+                        const float4* planes, float* costs, float* init_cost, uint32_t* sel,
+                        hipStream_t stream);
+
+}  // namespace pmhip

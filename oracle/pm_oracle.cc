@@ -1,0 +1,1207 @@
+// CPU oracle for the PatchMatch multi-view depth path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity checker and the timed
+// CPU baseline; it is never linked into, loaded by, or called from the
+// product library.  Only tests/, the smoke check and the CPU-baseline leg of
+// the benchmark may load it.
+//
+// It restates, in plain C++17 + OpenMP, the ACMMP-style OpenCL program of the
+// reference (opensfm/src/lib/dense/opencl_kernels.h:22-1856) and the host
+// schedule of opensfm/src/lib/dense/src/estimator.cc:134-255, 515-720,
+// 1112-1224.  Every function cites the lines it follows.
+//
+// Numerics contract (the reference's own GPU numerics are not reproducible:
+// hardware bilinear filtering, -cl-fast-relaxed-math, -cl-mad-enable at
+// opencl_utils.h:84-85).  This oracle fixes them explicitly and the HIP path
+// follows the same contract, so the two can be compared bit for bit:
+//   * compiled with -ffp-contract=off: a*b+c is two roundings unless written
+//     as fmaf(); fmaf() is used exactly where the device code uses it (the
+//     per-tap bilinear lerps, warp coordinates and NCC accumulations -- the
+//     fusions -cl-mad-enable licenses in the reference);
+//   * exp/log are pm_expf/pm_logf below (Cody-Waite reduction + Taylor /
+//     atanh series, written from the textbook method), not libm;
+//   * sqrt and '/' are IEEE correctly rounded;
+//   * bilinear sampling is exact software bilinear with texel centres at
+//     integer coordinates and clamp-to-edge addressing; the reference's
+//     "+0.5f" (opencl_kernels.h:224,229,459,491) only converts pixel-centre
+//     coordinates to OpenCL's texel-corner convention, so the oracle samples
+//     at the pixel-centre coordinate directly.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+constexpr int kMaxSrc = 16;   // reference: MAX_SOURCES 8 (opencl_kernels.h:26), lifted
+constexpr int kMcSamples = 32;  // MC_NUM_SAMPLES (opencl_kernels.h:910)
+constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
+
+}  // namespace
+
+extern "C" {
+
+// Camera layout of the reference's CLCamera (opencl_utils.h:258-280; kernel
+// struct opencl_kernels.h:62-68): K and R row-major, t, size.  92 bytes.
+typedef struct {
+  float K[9];
+  float R[9];
+  float t[3];
+  int width;
+  int height;
+} pmo_camera;
+
+typedef struct {
+  int width, height;          // reference view size
+  int num_src;                // number of source views (<= 16)
+  const float* images;        // all views, row-major, concatenated (ref first)
+  const int64_t* offsets;     // num_src+1 element offsets into images
+  const int* widths;          // num_src+1
+  const int* heights;         // num_src+1
+  const pmo_camera* cams;     // num_src+1 (index 0 = reference)
+  const float* angle_weights; // num_src (estimator.cc:380-407)
+} pmo_scene;
+
+typedef struct {
+  int half_patch;
+  float sigma_spatial;
+  float sigma_color;
+  int top_k;
+  float census_weight;   // use_census ? 1 : 0 (estimator.cc:536)
+  float depth_min, depth_max;
+  int max_iterations;
+  float smooth_weight;
+  int anchor_views;
+  float geom_weight;
+  float var_threshold;   // 1.5e-6 in the reference (opencl_kernels.h:254,516,519)
+  int checkerboard_filter;
+} pmo_params;
+
+typedef struct {
+  float* planes;               // 4 * npix (nx, ny, nz, d)
+  float* costs;                // npix
+  uint32_t* rng;               // 2 * npix
+  uint32_t* sel;               // npix
+  const float* prior_planes;   // 4 * npix or null
+  const uint32_t* plane_masks; // npix or null
+  const float* prev_depths;    // num_src * npix (ref resolution) or null
+  uint32_t prev_depth_mask;
+} pmo_state;
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Elementary functions of the numerics contract.
+// ---------------------------------------------------------------------------
+inline float bits_to_float(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline uint32_t float_to_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+// e^x: k = rint(x/ln2), r = x - k*ln2 in two Cody-Waite steps, e^r by its
+// degree-7 Taylor polynomial (|r| <= 0.35, truncation < 6e-9 relative),
+// scaled by 2^k through the exponent field.  x < -86 returns 0 (no
+// subnormal results), x > 88 returns +inf.
+inline float pm_expf(float x) {
+  if (x != x) return x;
+  if (x < -86.0f) return 0.0f;
+  if (x > 88.0f) return INFINITY;
+  const float kf = std::rint(x * 1.44269502162933349609f);
+  float r = std::fmaf(-kf, 0.693145751953125f, x);
+  r = std::fmaf(-kf, 1.42860682030941723212e-6f, r);
+  float p = 1.98412698412698412698e-4f;           // 1/7!
+  p = std::fmaf(p, r, 1.38888888888888888889e-3f);  // 1/6!
+  p = std::fmaf(p, r, 8.33333333333333333333e-3f);  // 1/5!
+  p = std::fmaf(p, r, 4.16666666666666666667e-2f);  // 1/4!
+  p = std::fmaf(p, r, 1.66666666666666666667e-1f);  // 1/3!
+  p = std::fmaf(p, r, 0.5f);
+  p = std::fmaf(p, r, 1.0f);
+  p = std::fmaf(p, r, 1.0f);
+  const int k = static_cast<int>(kf);
+  return p * bits_to_float(static_cast<uint32_t>(k + 127) << 23);
+}
+
+// ln x for finite x > 0: x = m * 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m-1)/(m+1), series to s^11 (|s| < 0.172).
+inline float pm_logf(float x) {
+  if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  uint32_t u = float_to_bits(x);
+  int e = static_cast<int>((u >> 23) & 0xffu) - 127;
+  if (e == -127) {  // subnormal: renormalise
+    x = x * 8388608.0f;
+    u = float_to_bits(x);
+    e = static_cast<int>((u >> 23) & 0xffu) - 127 - 23;
+  }
+  float m = bits_to_float((u & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421353816986083984f) { m = m * 0.5f; e += 1; }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float s2 = s * s;
+  float p = 9.09090909090909090909e-2f;            // 1/11
+  p = std::fmaf(p, s2, 1.11111111111111111111e-1f); // 1/9
+  p = std::fmaf(p, s2, 1.42857142857142857143e-1f); // 1/7
+  p = std::fmaf(p, s2, 2.0e-1f);                    // 1/5
+  p = std::fmaf(p, s2, 3.33333333333333333333e-1f); // 1/3
+  p = std::fmaf(p, s2, 1.0f);
+  const float lm = (2.0f * s) * p;
+  const float fe = static_cast<float>(e);
+  return std::fmaf(fe, 0.693145751953125f, std::fmaf(fe, 1.42860682030941723212e-6f, lm));
+}
+
+struct V3 { float x, y, z; };
+inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float len3(V3 a) { return std::sqrt(dot3(a, a)); }
+// OpenCL normalize(): all-zero input is returned unchanged.
+inline V3 normalize3(V3 a) {
+  const float l = len3(a);
+  if (l == 0.0f) return a;
+  return V3{a.x / l, a.y / l, a.z / l};
+}
+inline float clampf(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), hi); }
+
+struct Plane { float x, y, z, w; };
+
+// ---------------------------------------------------------------------------
+// RNG: opencl_kernels.h:78-97.
+// ---------------------------------------------------------------------------
+inline void philox_round(uint32_t st[2], uint32_t key) {
+  const uint64_t prod = static_cast<uint64_t>(st[0]) * 0xD256D193ull;
+  const uint32_t lo = static_cast<uint32_t>(prod);
+  const uint32_t hi = static_cast<uint32_t>(prod >> 32);
+  const uint32_t new_x = st[1] ^ key ^ hi;
+  st[0] = new_x;
+  st[1] = lo;
+}
+
+inline float rand_float(uint32_t st[2], uint32_t key) {
+  for (uint32_t i = 0; i < 4; ++i) philox_round(st, key + i * 0x9E3779B9u);
+  return static_cast<float>(st[0]) / 4294967296.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry: opencl_kernels.h:102-173.
+// ---------------------------------------------------------------------------
+inline V3 project(V3 X, const pmo_camera& c) {
+  V3 xc;
+  xc.x = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
+  xc.y = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
+  xc.z = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
+  V3 p;
+  p.x = c.K[0] * xc.x + c.K[1] * xc.y + c.K[2] * xc.z;
+  p.y = c.K[3] * xc.x + c.K[4] * xc.y + c.K[5] * xc.z;
+  p.z = c.K[6] * xc.x + c.K[7] * xc.y + c.K[8] * xc.z;
+  return p;
+}
+
+inline V3 backproject(float x, float y, float depth, const pmo_camera& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  V3 pc{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
+  V3 d{pc.x - c.t[0], pc.y - c.t[1], pc.z - c.t[2]};
+  V3 w;
+  w.x = c.R[0] * d.x + c.R[3] * d.y + c.R[6] * d.z;
+  w.y = c.R[1] * d.x + c.R[4] * d.y + c.R[7] * d.z;
+  w.z = c.R[2] * d.x + c.R[5] * d.y + c.R[8] * d.z;
+  return w;
+}
+
+inline float depth_from_plane(float x, float y, Plane p, const pmo_camera& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  const float denom = p.x * (x - cx) / fx + p.y * (y - cy) / fy + p.z;
+  if (std::fabs(denom) < 1e-6f) return 0.0f;
+  return -p.w / denom;
+}
+
+inline Plane plane_from_depth_normal(float x, float y, float depth, V3 n, const pmo_camera& c) {
+  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  V3 pt{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
+  const float d = -dot3(n, pt);
+  return Plane{n.x, n.y, n.z, d};
+}
+
+// ---------------------------------------------------------------------------
+// Image access.
+// ---------------------------------------------------------------------------
+struct Img {
+  const float* data;
+  int w, h;
+  inline float texel(int ix, int iy) const {
+    ix = std::min(std::max(ix, 0), w - 1);
+    iy = std::min(std::max(iy, 0), h - 1);
+    return data[static_cast<int64_t>(iy) * w + ix];
+  }
+  // Bilinear at a pixel-centre coordinate, clamp-to-edge.  Coordinates are
+  // first clamped to [-1, w-1] x [-1, h-1]: outside that range both taps of
+  // an axis hit the same edge texel, so the clamp does not change the value
+  // (lerp(a, a, t) == a exactly) and keeps the float->int conversion defined.
+  inline float bilinear(float sx, float sy) const {
+    sx = clampf(sx, -1.0f, static_cast<float>(w - 1));
+    sy = clampf(sy, -1.0f, static_cast<float>(h - 1));
+    const float fx0 = std::floor(sx), fy0 = std::floor(sy);
+    const float ax = sx - fx0, ay = sy - fy0;
+    const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+    const float t00 = texel(x0, y0), t01 = texel(x0 + 1, y0);
+    const float t10 = texel(x0, y0 + 1), t11 = texel(x0 + 1, y0 + 1);
+    const float v0 = std::fmaf(ax, t01 - t00, t00);
+    const float v1 = std::fmaf(ax, t11 - t10, t10);
+    return std::fmaf(ay, v1 - v0, v0);
+  }
+};
+
+struct Ctx {
+  const pmo_scene* sc;
+  const pmo_params* pr;
+  Img ref;
+  Img src[kMaxSrc];
+  const pmo_camera* cams;
+  int n_src;
+  float spatial_factor, color_factor;
+  Ctx(const pmo_scene* s, const pmo_params* p) : sc(s), pr(p) {
+    ref = Img{s->images + s->offsets[0], s->widths[0], s->heights[0]};
+    n_src = std::min(s->num_src, kMaxSrc);
+    for (int i = 0; i < n_src; ++i)
+      src[i] = Img{s->images + s->offsets[i + 1], s->widths[i + 1], s->heights[i + 1]};
+    cams = s->cams;
+    spatial_factor = -1.0f / (2.0f * p->sigma_spatial * p->sigma_spatial);
+    color_factor = -1.0f / (2.0f * p->sigma_color * p->sigma_color);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Reference patch: opencl_kernels.h:42-49, 328-366.
+// ---------------------------------------------------------------------------
+struct RefPatch {
+  float ref_center, sum_w, sum_wr, sum_wrr;
+  float w[(2 * 7 + 1) * (2 * 7 + 1)];
+  float wr[(2 * 7 + 1) * (2 * 7 + 1)];
+};
+
+void compute_ref_patch(const Ctx& c, int x, int y, RefPatch* rp) {
+  const int hp = c.pr->half_patch;
+  const float center = c.ref.texel(x, y);
+  float sum_w = 0.0f, sum_wr = 0.0f, sum_wrr = 0.0f;
+  int p = 0;
+  for (int iy = -hp; iy <= hp; ++iy) {
+    for (int ix = -hp; ix <= hp; ++ix) {
+      const float rv = c.ref.texel(x + ix, y + iy);
+      const float r = rv - center;
+      const float w = pm_expf(c.spatial_factor * static_cast<float>(ix * ix + iy * iy) +
+                              c.color_factor * r * r);
+      sum_w += w;
+      const float wr = w * r;
+      sum_wr += wr;
+      sum_wrr = std::fmaf(wr, r, sum_wrr);
+      rp->w[p] = w;
+      rp->wr[p] = wr;
+      ++p;
+    }
+  }
+  rp->ref_center = center;
+  rp->sum_w = sum_w;
+  rp->sum_wr = sum_wr;
+  rp->sum_wrr = sum_wrr;
+}
+
+// Strided ZNCC fallback: opencl_kernels.h:188-258.
+float ncc_at_stride(const Ctx& c, int s, float ref_center, float src_center, float scx, float scy,
+                    int cx, int cy, float stride, float jxx, float jxy, float jyx, float jyy) {
+  const int hp = c.pr->half_patch;
+  float sum_w = 0.0f, sum_r = 0.0f, sum_s = 0.0f, sum_rr = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+  for (int iy = -hp; iy <= hp; ++iy) {
+    for (int ix = -hp; ix <= hp; ++ix) {
+      const float fdx = stride * static_cast<float>(ix);
+      const float fdy = stride * static_cast<float>(iy);
+      const float rx = static_cast<float>(cx) + fdx;
+      const float ry = static_cast<float>(cy) + fdy;
+      const float ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));
+      const float sx = scx + jxx * fdx + jyx * fdy;
+      const float sy = scy + jxy * fdx + jyy * fdy;
+      const float src_val = c.src[s].bilinear(sx, sy);
+      const float r = ref_val - ref_center;
+      const float w = pm_expf(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +
+                              r * r * c.color_factor);
+      const float sv = src_val - src_center;
+      sum_r += w * r;
+      sum_s += w * sv;
+      sum_rr += w * r * r;
+      sum_ss += w * sv * sv;
+      sum_rs += w * r * sv;
+      sum_w += w;
+    }
+  }
+  if (sum_w < 1e-6f) return -1.0f;
+  const float inv_w = 1.0f / sum_w;
+  const float mean_r = sum_r * inv_w;
+  const float mean_s = sum_s * inv_w;
+  const float var_ref = sum_rr * inv_w - mean_r * mean_r;
+  const float var_src = sum_ss * inv_w - mean_s * mean_s;
+  if (var_ref < c.pr->var_threshold || var_src < c.pr->var_threshold) return -1.0f;
+  const float covar = sum_rs * inv_w - mean_r * mean_s;
+  return covar / std::sqrt(var_ref * var_src);
+}
+
+// Census fallback: opencl_kernels.h:270-315.
+float census_inner(const Ctx& c, int s, float ref_center, float src_center, int cx, int cy,
+                   float scx, float scy, float jxx, float jxy, float jyx, float jyy) {
+  uint32_t rc = 0u, sc = 0u;
+  int bit = 0;
+  for (int dy = -2; dy <= 2; ++dy) {
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const float ref_val = c.ref.texel(cx + dx, cy + dy);
+      const float sx = scx + jxx * static_cast<float>(dx) + jyx * static_cast<float>(dy);
+      const float sy = scy + jxy * static_cast<float>(dx) + jyy * static_cast<float>(dy);
+      const float src_val = c.src[s].bilinear(sx, sy);
+      if (ref_val > ref_center) rc |= (1u << bit);
+      if (src_val > src_center) sc |= (1u << bit);
+      ++bit;
+    }
+  }
+  const int hamming = __builtin_popcount(rc ^ sc);
+  if (__builtin_popcount(rc) < 3 && __builtin_popcount(sc) < 3) return 1.0f;
+  return static_cast<float>(hamming) / 24.0f;
+}
+
+// Per-view cost vector ("ComputePlaneScore"): opencl_kernels.h:381-557.
+// The reference loops taps outer / views inner; each view's sums are
+// accumulated over the taps in the same order either way, so the oracle
+// loops views outer.
+int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, float* out) {
+  const int n_src = c.n_src;
+  const int hp = c.pr->half_patch;
+  const pmo_camera& c0 = c.cams[0];
+  const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane, c0);
+  if (depth <= 0.0f) {
+    for (int i = 0; i < n_src; ++i) out[i] = 2.0f;
+    return n_src;
+  }
+  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), plane, c0);
+  const float depth_dy = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), plane, c0);
+  if (depth_dx <= 0.0f || depth_dy <= 0.0f) {
+    for (int i = 0; i < n_src; ++i) out[i] = 2.0f;
+    return n_src;
+  }
+  const V3 cw = backproject(static_cast<float>(x), static_cast<float>(y), depth, c0);
+  const V3 wdx = backproject(static_cast<float>(x + 1), static_cast<float>(y), depth_dx, c0);
+  const V3 wdy = backproject(static_cast<float>(x), static_cast<float>(y + 1), depth_dy, c0);
+
+  const bool ref_ok = rp.sum_w >= 1e-6f;
+  float inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
+  if (ref_ok) {
+    inv_w = 1.0f / rp.sum_w;
+    mean_r = rp.sum_wr * inv_w;
+    var_ref = rp.sum_wrr * inv_w - mean_r * mean_r;
+  }
+
+  for (int s = 0; s < n_src; ++s) {
+    const pmo_camera& cs = c.cams[s + 1];
+    const V3 pc = project(cw, cs);
+    if (pc.z < 1e-6f) { out[s] = 2.0f; continue; }
+    const float scx = pc.x / pc.z;
+    const float scy = pc.y / pc.z;
+    const V3 pdx = project(wdx, cs);
+    const V3 pdy = project(wdy, cs);
+    if (pdx.z < 1e-6f || pdy.z < 1e-6f) { out[s] = 2.0f; continue; }
+    const float jxx = pdx.x / pdx.z - scx;
+    const float jxy = pdx.y / pdx.z - scy;
+    const float jyx = pdy.x / pdy.z - scx;
+    const float jyy = pdy.y / pdy.z - scy;
+    const float det = jxx * jyy - jxy * jyx;
+    if (det < 0.02f || det > 50.0f) { out[s] = 2.0f; continue; }
+    const float scen = c.src[s].bilinear(scx, scy);
+
+    // Fused stride-1 ZNCC over the cached reference weights (:471-499).
+    float sum_ws = 0.0f, sum_wss = 0.0f, sum_wrs = 0.0f;
+    int p = 0;
+    for (int iy = -hp; iy <= hp; ++iy) {
+      const float fiy = static_cast<float>(iy);
+      const float bx = std::fmaf(jyx, fiy, scx);
+      const float by = std::fmaf(jyy, fiy, scy);
+      for (int ix = -hp; ix <= hp; ++ix) {
+        const float fix = static_cast<float>(ix);
+        const float w = rp.w[p];
+        const float wr = rp.wr[p];
+        ++p;
+        const float sx = std::fmaf(jxx, fix, bx);
+        const float sy = std::fmaf(jxy, fix, by);
+        const float sv = c.src[s].bilinear(sx, sy) - scen;
+        const float wsv = w * sv;
+        sum_ws += wsv;
+        sum_wss = std::fmaf(wsv, sv, sum_wss);
+        sum_wrs = std::fmaf(wr, sv, sum_wrs);
+      }
+    }
+    if (!ref_ok) { out[s] = 2.0f; continue; }  // :502-505
+
+    float ncc1 = -1.0f;
+    if (var_ref >= c.pr->var_threshold) {
+      const float mean_s = sum_ws * inv_w;
+      const float var_src = sum_wss * inv_w - mean_s * mean_s;
+      if (var_src >= c.pr->var_threshold) {
+        const float covar = sum_wrs * inv_w - mean_r * mean_s;
+        ncc1 = covar / std::sqrt(var_ref * var_src);
+      }
+    }
+    if (ncc1 > 0.1f) { out[s] = 1.0f - ncc1; continue; }
+    float best = ncc1;
+    const float strides[2] = {2.0f, 3.0f};
+    for (int st = 0; st < 2; ++st) {
+      const float ncc = ncc_at_stride(c, s, rp.ref_center, scen, scx, scy, x, y, strides[st],
+                                      jxx, jxy, jyx, jyy);
+      if (ncc > best) best = ncc;
+      if (best > 0.1f) break;
+    }
+    if (best <= -0.5f) {
+      out[s] = (c.pr->census_weight > 1e-6f)
+                   ? census_inner(c, s, rp.ref_center, scen, x, y, scx, scy, jxx, jxy, jyx, jyy)
+                   : 2.0f;
+    } else {
+      out[s] = 1.0f - best;
+    }
+  }
+  return n_src;
+}
+
+// Top-k initial cost + view bitmask: opencl_kernels.h:565-621.
+float initial_cost_and_views(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp,
+                             uint32_t* out_sel) {
+  float cv[kMaxSrc], cc[kMaxSrc];
+  for (int i = 0; i < kMaxSrc; ++i) { cv[i] = 2.0f; cc[i] = 2.0f; }
+  const int n_src = c.n_src;
+  cost_vector(c, x, y, plane, rp, cv);
+  int num_valid = 0;
+  for (int i = 0; i < n_src; ++i) {
+    cc[i] = cv[i];
+    if (cv[i] < 2.0f) ++num_valid;
+  }
+  for (int i = 1; i < n_src; ++i) {
+    const float tmp = cv[i];
+    int j = i;
+    for (; j >= 1 && tmp < cv[j - 1]; --j) cv[j] = cv[j - 1];
+    cv[j] = tmp;
+  }
+  *out_sel = 0u;
+  const int k = std::min(std::min(c.pr->top_k, num_valid), n_src);
+  if (k <= 0) return 2.0f;
+  float total = 0.0f;
+  for (int i = 0; i < k; ++i) total += cv[i];
+  const float thr = cv[k - 1];
+  for (int i = 0; i < n_src; ++i)
+    if (cc[i] <= thr) *out_sel |= (1u << i);
+  return total / static_cast<float>(k);
+}
+
+inline Plane load_plane(const float* planes, int64_t idx) {
+  return Plane{planes[4 * idx], planes[4 * idx + 1], planes[4 * idx + 2], planes[4 * idx + 3]};
+}
+inline void store_plane(float* planes, int64_t idx, Plane p) {
+  planes[4 * idx] = p.x; planes[4 * idx + 1] = p.y; planes[4 * idx + 2] = p.z; planes[4 * idx + 3] = p.w;
+}
+
+// Smoothness: opencl_kernels.h:649-707.
+float smoothness_cost(const Ctx& c, const float* planes, Plane cur, int x, int y) {
+  const pmo_camera& c0 = c.cams[0];
+  const int W = c.sc->width, H = c.sc->height;
+  const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
+  if (cur_depth <= 0.0f) return 0.0f;
+  const V3 cur_n = normalize3(V3{cur.x, cur.y, cur.z});
+  const float sigma_rel = 0.02f;
+  const int dxs[4] = {-1, 1, 0, 0};
+  const int dys[4] = {0, 0, -1, 1};
+  float tw = 0.0f, tc = 0.0f;
+  for (int k = 0; k < 4; ++k) {
+    const int nx = x + dxs[k], ny = y + dys[k];
+    if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+    const Plane nb = load_plane(planes, static_cast<int64_t>(ny) * W + nx);
+    const float nb_depth = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), nb, c0);
+    if (nb_depth <= 0.0f) continue;
+    const float rel = std::fabs(cur_depth - nb_depth) / cur_depth;
+    const float w = pm_expf(-rel * rel / (2.0f * sigma_rel * sigma_rel));
+    const V3 nb_n = normalize3(V3{nb.x, nb.y, nb.z});
+    const float normal_cost = 1.0f - dot3(cur_n, nb_n);
+    const float predicted = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), cur, c0);
+    const float dev = (predicted > 0.0f) ? std::fabs(nb_depth - predicted) / cur_depth : 0.0f;
+    const float cost = 0.5f * clampf(normal_cost, 0.0f, 1.0f) + 0.5f * clampf(dev * 10.0f, 0.0f, 1.0f);
+    tc += w * cost;
+    tw += w;
+  }
+  if (tw < 1e-6f) return 0.0f;
+  return tc / tw;
+}
+
+// Surface normal from neighbour depths: opencl_kernels.h:725-772.
+V3 surface_normal(const Ctx& c, const float* planes, int x, int y, float center_depth) {
+  const int W = c.sc->width, H = c.sc->height;
+  if (center_depth <= 0.0f) return V3{0, 0, 0};
+  if (x < 1 || x >= W - 1 || y < 1 || y >= H - 1) return V3{0, 0, 0};
+  const pmo_camera& c0 = c.cams[0];
+  const float fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
+  const Plane pl = load_plane(planes, static_cast<int64_t>(y) * W + (x - 1));
+  const Plane pr = load_plane(planes, static_cast<int64_t>(y) * W + (x + 1));
+  const Plane pu = load_plane(planes, static_cast<int64_t>(y - 1) * W + x);
+  const Plane pd = load_plane(planes, static_cast<int64_t>(y + 1) * W + x);
+  const float dl = depth_from_plane(static_cast<float>(x - 1), static_cast<float>(y), pl, c0);
+  const float dr = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), pr, c0);
+  const float du = depth_from_plane(static_cast<float>(x), static_cast<float>(y - 1), pu, c0);
+  const float dd = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), pd, c0);
+  if (dl <= 0.0f || dr <= 0.0f || du <= 0.0f || dd <= 0.0f) return V3{0, 0, 0};
+  const float dx = dr - dl;
+  const float dy = dd - du;
+  V3 n{fx * dx, fy * dy, (cx - static_cast<float>(x)) * dx + (cy - static_cast<float>(y)) * dy - center_depth};
+  const float l = len3(n);
+  if (l < 1e-8f) return V3{0, 0, 0};
+  n = V3{n.x / l, n.y / l, n.z / l};
+  if (n.z > 0.0f) n = V3{-n.x, -n.y, -n.z};
+  return n;
+}
+
+// Geometric consistency: opencl_kernels.h:779-861.
+float geom_cost(const Ctx& c, const pmo_state* st, int sidx, Plane plane, int px, int py) {
+  const int pdw = c.sc->width, pdh = c.sc->height;
+  const pmo_camera& rc = c.cams[0];
+  const pmo_camera& scm = c.cams[sidx + 1];
+  const float bad = 1.0f;
+  const float max_dim = static_cast<float>(std::max(pdw, pdh));
+  if (max_dim < 1.0f) return bad;
+  const float good_thresh = 3.0f / max_dim;
+  const float full_thresh = 6.0f / max_dim;
+  const float depth = depth_from_plane(static_cast<float>(px), static_cast<float>(py), plane, rc);
+  if (depth <= 0.0f) return bad;
+  const V3 wp = backproject(static_cast<float>(px), static_cast<float>(py), depth, rc);
+  const V3 sp = project(wp, scm);
+  if (sp.z < 1e-6f) return bad;
+  const float sx = sp.x / sp.z, sy = sp.y / sp.z;
+  if (sx < 0.0f || sx >= static_cast<float>(pdw) || sy < 0.0f || sy >= static_cast<float>(pdh)) return bad;
+  const int ix = std::min(std::max(static_cast<int>(sx + 0.5f), 0), pdw - 1);
+  const int iy = std::min(std::max(static_cast<int>(sy + 0.5f), 0), pdh - 1);
+  const float sd = st->prev_depths[static_cast<int64_t>(sidx) * pdw * pdh + static_cast<int64_t>(iy) * pdw + ix];
+  if (sd <= 0.0f) return bad;
+  const V3 sw = backproject(sx, sy, sd, scm);
+  const V3 rp = project(sw, rc);
+  if (rp.z < 1e-6f) return bad;
+  const float bx = rp.x / rp.z, by = rp.y / rp.z;
+  const float dx = static_cast<float>(px) - bx;
+  const float dy = static_cast<float>(py) - by;
+  const float err = std::sqrt(dx * dx + dy * dy) / max_dim;
+  if (err <= good_thresh) return 0.0f;
+  if (err >= full_thresh) return 1.0f;
+  return (err - good_thresh) / (full_thresh - good_thresh);
+}
+
+// View-weighted cost with geometry: opencl_kernels.h:871-900.
+float weighted_cost_geom(const Ctx& c, const pmo_state* st, const float* cv, const float* vw,
+                         float wnorm, Plane plane, int x, int y) {
+  if (wnorm < 1e-6f) return 2.0f;
+  float total = 0.0f;
+  const float gw = c.pr->geom_weight;
+  for (int i = 0; i < c.n_src; ++i) {
+    if (vw[i] > 0.0f) {
+      const float photo = cv[i];
+      float geom = 0.0f;
+      if (gw > 0.0f && (st->prev_depth_mask & (1u << i))) geom = geom_cost(c, st, i, plane, x, y);
+      total += vw[i] * (photo + gw * geom);
+    }
+  }
+  return total / wnorm;
+}
+
+// Monte-Carlo view-sampled cost: opencl_kernels.h:910-969.
+float mc_cost(const Ctx& c, const pmo_state* st, const float* cv, const float* vw, float wnorm,
+              Plane plane, int x, int y, uint32_t rs[2], uint32_t key) {
+  if (wnorm < 1e-6f) return 2.0f;
+  const int n_src = c.n_src;
+  float cdf[kMaxSrc];
+  float cum = 0.0f;
+  for (int i = 0; i < n_src; ++i) { cum += vw[i]; cdf[i] = cum; }
+  const float gw = c.pr->geom_weight;
+  float total = 0.0f;
+  int valid = 0;
+  for (int s = 0; s < kMcSamples; ++s) {
+    const float u = rand_float(rs, key + static_cast<uint32_t>(s) * 7919u);
+    const float thr = u * cum;
+    int v = n_src - 1;
+    for (int i = 0; i < n_src; ++i)
+      if (cdf[i] > thr) { v = i; break; }
+    const float photo = cv[v];
+    float geom = 0.0f;
+    if (gw > 0.0f && (st->prev_depth_mask & (1u << v))) geom = geom_cost(c, st, v, plane, x, y);
+    total += photo + gw * geom;
+    ++valid;
+  }
+  return valid > 0 ? total / static_cast<float>(valid) : 2.0f;
+}
+
+// acmmp_random_init: opencl_kernels.h:979-1039.
+void random_init_px(const Ctx& c, pmo_state* st, int x, int y) {
+  const int W = c.sc->width;
+  const int idx = y * W + x;
+  uint32_t rs[2] = {static_cast<uint32_t>(idx) * 1099087573u + 2654435769u,
+                    static_cast<uint32_t>(idx) * 2246822519u + 13u};
+  const uint32_t key = static_cast<uint32_t>(idx + 7919);
+  const float ldmin = pm_logf(std::fmax(c.pr->depth_min, 1e-6f));
+  const float ldmax = pm_logf(std::fmax(c.pr->depth_max, 1e-5f));
+  const float depth = pm_expf(ldmin + rand_float(rs, key) * (ldmax - ldmin));
+  V3 n;
+  n.x = rand_float(rs, key + 3u) * 2.0f - 1.0f;
+  n.y = rand_float(rs, key + 5u) * 2.0f - 1.0f;
+  n.z = -1.0f;
+  n = normalize3(n);
+  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, c.cams[0]);
+  store_plane(st->planes, idx, plane);
+  st->rng[2 * idx] = rs[0];
+  st->rng[2 * idx + 1] = rs[1];
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+  uint32_t sel = 0u;
+  st->costs[idx] = initial_cost_and_views(c, x, y, plane, rp, &sel);
+  st->sel[idx] = sel;
+}
+
+// acmmp_prior_reinit: opencl_kernels.h:1056-1182.
+void prior_reinit_px(const Ctx& c, pmo_state* st, const float* snap, int x, int y) {
+  const int W = c.sc->width;
+  const int idx = y * W + x;
+  if (!st->plane_masks || st->plane_masks[idx] == 0u) return;
+  const pmo_camera& c0 = c.cams[0];
+  const Plane pp = load_plane(st->prior_planes, idx);
+  const float dprior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+  const float dmin = c.pr->depth_min, dmax = c.pr->depth_max;
+  if (dprior <= 0.0f || dprior < dmin || dprior > dmax) return;
+  uint32_t rs[2] = {st->rng[2 * idx], st->rng[2 * idx + 1]};
+  const uint32_t key = static_cast<uint32_t>(idx + 31337u);
+  float dp = dprior * 0.99f + rand_float(rs, key) * dprior * 0.02f;
+  dp = clampf(dp, dmin, dmax);
+  const V3 pn = normalize3(V3{pp.x, pp.y, pp.z});
+  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), dp, pn, c0);
+  const int n_src = c.n_src;
+  const uint32_t cur_sel = st->sel[idx];
+  float vw[kMaxSrc];
+  float wnorm = 0.0f;
+  for (int i = 0; i < kMaxSrc; ++i) {
+    if (i < n_src && (cur_sel & (1u << i))) { vw[i] = 1.0f; wnorm += 1.0f; } else vw[i] = 0.0f;
+  }
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+  const float sw = c.pr->smooth_weight;
+  if (wnorm < 1e-6f) {
+    uint32_t sel = 0u;
+    float cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
+    if (sw > 1e-6f) cost += sw * smoothness_cost(c, snap, plane, x, y);
+    if (cost < st->costs[idx]) {
+      store_plane(st->planes, idx, plane);
+      st->costs[idx] = cost;
+      st->sel[idx] = sel;
+    }
+    st->rng[2 * idx] = rs[0];
+    st->rng[2 * idx + 1] = rs[1];
+    return;
+  }
+  float cv[kMaxSrc];
+  for (int i = 0; i < kMaxSrc; ++i) cv[i] = 2.0f;
+  cost_vector(c, x, y, plane, rp, cv);
+  float cost = weighted_cost_geom(c, st, cv, vw, wnorm, plane, x, y);
+  if (sw > 1e-6f) cost += sw * smoothness_cost(c, snap, plane, x, y);
+  if (cost < st->costs[idx]) {
+    store_plane(st->planes, idx, plane);
+    st->costs[idx] = cost;
+  }
+  st->rng[2 * idx] = rs[0];
+  st->rng[2 * idx + 1] = rs[1];
+}
+
+// acmmp_patchmatch, one pixel of one colour: opencl_kernels.h:1197-1778.
+// Updates in place: every neighbour the pixel reads has the other colour
+// (odd Manhattan offsets), so this equals the reference's snapshot reads.
+void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, float smooth_w) {
+  const int W = c.sc->width, H = c.sc->height;
+  const int idx = y * W + x;
+  const pmo_camera& c0 = c.cams[0];
+  const pmo_params& P = *c.pr;
+  uint32_t rs[2] = {st->rng[2 * idx], st->rng[2 * idx + 1]};
+  const uint32_t key = static_cast<uint32_t>(idx + iteration * 7919 + 42);
+  const int n_src = c.n_src;
+  const float* planes = st->planes;
+  const float* costs = st->costs;
+
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+
+  // Step 1: candidate positions (:1254-1430).
+  int cpx[8], cpy[8];
+  bool flags[8];
+  float ca[8 * kMaxSrc];
+  for (int i = 0; i < 8; ++i) { cpx[i] = x; cpy[i] = y; flags[i] = false; }
+  for (int i = 0; i < 8 * kMaxSrc; ++i) ca[i] = 2.0f;
+  auto cost_at = [&](int px, int py) { return costs[static_cast<int64_t>(py) * W + px]; };
+  auto eval = [&](int slot, int px, int py) {
+    cpx[slot] = px; cpy[slot] = py; flags[slot] = true;
+    cost_vector(c, x, y, load_plane(planes, static_cast<int64_t>(py) * W + px), rp, &ca[slot * kMaxSrc]);
+  };
+  if (y > 2) {  // far up (:1262-1278)
+    int by = y - 3; float bc = cost_at(x, by);
+    for (int i = 1; i < 11; ++i) { const int yy = y - 3 - 2 * i; if (yy < 0) break;
+      const float cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
+    eval(1, x, by);
+  }
+  if (y < H - 3) {  // far down (:1281-1296)
+    int by = y + 3; float bc = cost_at(x, by);
+    for (int i = 1; i < 11; ++i) { const int yy = y + 3 + 2 * i; if (yy >= H) break;
+      const float cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
+    eval(3, x, by);
+  }
+  if (x > 2) {  // far left (:1299-1314)
+    int bx = x - 3; float bc = cost_at(bx, y);
+    for (int i = 1; i < 11; ++i) { const int xx = x - 3 - 2 * i; if (xx < 0) break;
+      const float cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
+    eval(5, bx, y);
+  }
+  if (x < W - 3) {  // far right (:1317-1332)
+    int bx = x + 3; float bc = cost_at(bx, y);
+    for (int i = 1; i < 11; ++i) { const int xx = x + 3 + 2 * i; if (xx >= W) break;
+      const float cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
+    eval(7, bx, y);
+  }
+  if (y > 0) {  // near up (:1335-1356)
+    int bx = x, by = y - 1; float bc = cost_at(bx, by);
+    for (int i = 0; i < 3; ++i) {
+      if (y > 1 + i && x > i) { const int px = x - i - 1, py = y - 2 - i; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+      if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+    }
+    eval(0, bx, by);
+  }
+  if (y < H - 1) {  // near down (:1361-1382)
+    int bx = x, by = y + 1; float bc = cost_at(bx, by);
+    for (int i = 0; i < 3; ++i) {
+      if (y < H - 2 - i && x > i) { const int px = x - i - 1, py = y + 2 + i; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+      if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+    }
+    eval(2, bx, by);
+  }
+  if (x > 0) {  // near left (:1385-1406)
+    int bx = x - 1, by = y; float bc = cost_at(bx, by);
+    for (int i = 0; i < 3; ++i) {
+      if (x > 1 + i && y > i) { const int px = x - 2 - i, py = y - i - 1; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+      if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+    }
+    eval(4, bx, by);
+  }
+  if (x < W - 1) {  // near right (:1409-1430)
+    int bx = x + 1, by = y; float bc = cost_at(bx, by);
+    for (int i = 0; i < 3; ++i) {
+      if (x < W - 2 - i && y > i) { const int px = x + 2 + i, py = y - i - 1; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+      if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const float cc = cost_at(px, py);
+        if (cc < bc) { bc = cc; bx = px; by = py; } }
+    }
+    eval(6, bx, by);
+  }
+
+  // Step 2: view weights (:1452-1563).
+  const float vthr = 0.8f;
+  float vw[kMaxSrc];
+  for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
+  for (int i = 0; i < n_src; ++i) {
+    float vc = 0.0f;
+    for (int j = 0; j < 8; ++j) {
+      const float cc = ca[j * kMaxSrc + i];
+      if (flags[j] && cc < vthr) vc += (1.0f - cc);
+    }
+    if (vc > 0.0f) vw[i] = vc;
+  }
+  const uint32_t prev_sel = st->sel[idx];
+  if (P.anchor_views > 0) {
+    int anchored = 0;
+    for (int i = 0; i < n_src && anchored < P.anchor_views; ++i) {
+      if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
+    }
+  }
+  {
+    const Plane cur = load_plane(planes, idx);
+    V3 cur_n{cur.x, cur.y, cur.z};
+    const float nl = len3(cur_n);
+    const bool have_normal = nl > 0.1f;
+    if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
+    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
+    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
+    const float inc_sigma = 0.6f;
+    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
+    for (int i = 0; i < n_src; ++i) {
+      if (vw[i] < 1e-6f) continue;
+      vw[i] *= c.sc->angle_weights[i];
+      if (have_normal && cur_depth > 0.0f) {
+        const pmo_camera& ci = c.cams[i + 1];
+        V3 C;
+        C.x = -(ci.R[0] * ci.t[0] + ci.R[3] * ci.t[1] + ci.R[6] * ci.t[2]);
+        C.y = -(ci.R[1] * ci.t[0] + ci.R[4] * ci.t[1] + ci.R[7] * ci.t[2]);
+        C.z = -(ci.R[2] * ci.t[0] + ci.R[5] * ci.t[1] + ci.R[8] * ci.t[2]);
+        const V3 ray = normalize3(V3{C.x - p3.x, C.y - p3.y, C.z - p3.z});
+        const float cos_inc = std::fabs(dot3(cur_n, ray));
+        const float xi = 1.0f - cos_inc;
+        vw[i] *= pm_expf(xi * xi * inv2s2);
+      }
+    }
+  }
+  uint32_t new_sel = 0u;
+  float wnorm = 0.0f;
+  for (int i = 0; i < n_src; ++i)
+    if (vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
+  if (wnorm < 1e-6f) {
+    for (int i = 0; i < n_src; ++i) {
+      for (int j = 0; j < 8; ++j) {
+        if (flags[j] && ca[j * kMaxSrc + i] < 2.0f) {
+          vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
+        }
+      }
+    }
+  }
+
+  // Step 3: candidate costs (:1570-1619).
+  float fc[8];
+  for (int i = 0; i < 8; ++i) {
+    if (flags[i]) {
+      const Plane cp = load_plane(planes, static_cast<int64_t>(cpy[i]) * W + cpx[i]);
+      float ph = mc_cost(c, st, &ca[i * kMaxSrc], vw, wnorm, cp, x, y, rs,
+                         key + 600u + static_cast<uint32_t>(i) * 31u);
+      if (smooth_w > 1e-6f) ph += smooth_w * smoothness_cost(c, planes, cp, x, y);
+      fc[i] = ph;
+    } else {
+      fc[i] = 2.0f;
+    }
+  }
+  int best = 0;
+  float best_cost = fc[0];
+  for (int i = 1; i < 8; ++i) if (fc[i] < best_cost) { best_cost = fc[i]; best = i; }
+
+  Plane plane_now = load_plane(planes, idx);
+  float cvn[kMaxSrc];
+  for (int i = 0; i < kMaxSrc; ++i) cvn[i] = 2.0f;
+  cost_vector(c, x, y, plane_now, rp, cvn);
+  float cost_now = mc_cost(c, st, cvn, vw, wnorm, plane_now, x, y, rs, key + 700u);
+  if (smooth_w > 1e-6f) cost_now += smooth_w * smoothness_cost(c, planes, plane_now, x, y);
+  st->costs[idx] = cost_now;
+  float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+
+  const bool have_prior = st->plane_masks && st->plane_masks[idx] > 0u;
+  float depth_prior = 0.0f;
+  V3 prior_n{0.0f, 0.0f, -1.0f};
+  const float depth_sigma = (P.depth_max - P.depth_min) / 32.0f;
+  const float angle_sigma = kPiF * (5.0f / 180.0f);
+  if (have_prior) {
+    const Plane pp = load_plane(st->prior_planes, idx);
+    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+    prior_n = normalize3(V3{pp.x, pp.y, pp.z});
+  }
+
+  // Step 3b (:1643-1656).
+  if (flags[best] && best_cost < cost_now) {
+    const Plane bp = load_plane(planes, static_cast<int64_t>(cpy[best]) * W + cpx[best]);
+    const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
+    if (db >= P.depth_min && db <= P.depth_max) {
+      depth_now = db;
+      plane_now = bp;
+      cost_now = best_cost;
+      st->sel[idx] = new_sel;
+    }
+  }
+
+  // Step 4: refinement (:1669-1772).
+  {
+    const float depth_perturbation = 0.005f;
+    const float normal_perturbation = 0.01f * kPiF;
+    const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
+    float depth_rand, depth_pert;
+    V3 nrand, npert;
+    if (have_prior && depth_prior > 0.0f) {
+      depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(rs, key + 503u) * 6.0f * depth_sigma;
+      depth_rand = clampf(depth_rand, P.depth_min, P.depth_max);
+      const float pa1 = (rand_float(rs, key + 504u) - 0.5f) * angle_sigma;
+      const float pa2 = (rand_float(rs, key + 505u) - 0.5f) * angle_sigma;
+      nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
+      if (nrand.z > 0.0f) nrand.z = -nrand.z;
+    } else {
+      const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
+      const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
+      depth_rand = pm_expf(ldmin + rand_float(rs, key + 503u) * (ldmax - ldmin));
+      nrand.x = rand_float(rs, key + 504u) * 2.0f - 1.0f;
+      nrand.y = rand_float(rs, key + 505u) * 2.0f - 1.0f;
+      nrand.z = -1.0f;
+      nrand = normalize3(nrand);
+    }
+    {
+      const float dlo = (1.0f - depth_perturbation) * depth_now;
+      const float dhi = (1.0f + depth_perturbation) * depth_now;
+      depth_pert = dlo + rand_float(rs, key + 510u) * (dhi - dlo);
+      depth_pert = clampf(depth_pert, P.depth_min, P.depth_max);
+    }
+    {
+      const float pa1 = (rand_float(rs, key + 520u) - 0.5f) * normal_perturbation;
+      const float pa2 = (rand_float(rs, key + 521u) - 0.5f) * normal_perturbation;
+      npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
+      if (npert.z > 0.0f) npert.z = -npert.z;
+    }
+    const V3 sn = surface_normal(c, planes, x, y, depth_now);
+    const bool have_sn = len3(sn) > 0.5f;
+    const float rd[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
+    const V3 rn[6] = {cur_normal, nrand, nrand, npert, cur_normal, sn};
+    const int nc = have_sn ? 6 : 5;
+    for (int h = 0; h < nc; ++h) {
+      const Plane cand = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[h], rn[h], c0);
+      float cv[kMaxSrc];
+      for (int i = 0; i < kMaxSrc; ++i) cv[i] = 2.0f;
+      cost_vector(c, x, y, cand, rp, cv);
+      float tc = mc_cost(c, st, cv, vw, wnorm, cand, x, y, rs, key + 800u + static_cast<uint32_t>(h) * 37u);
+      if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(c, planes, cand, x, y);
+      const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cand, c0);
+      if (db < P.depth_min || db > P.depth_max) continue;
+      if (tc < cost_now) { depth_now = db; plane_now = cand; cost_now = tc; }
+    }
+  }
+  store_plane(st->planes, idx, plane_now);
+  st->costs[idx] = cost_now;
+  st->rng[2 * idx] = rs[0];
+  st->rng[2 * idx + 1] = rs[1];
+}
+
+// acmmp_checkerboard_filter, one pixel: opencl_kernels.h:1883-2039.
+void checkerboard_px(const Ctx& c, pmo_state* st, int x, int y) {
+  const int W = c.sc->width, H = c.sc->height;
+  const int center = y * W + x;
+  const pmo_camera& c0 = c.cams[0];
+  if (st->costs[center] < 0.001f) return;
+  float buf[21];
+  int n = 0;
+  auto dep = [&](int px, int py) {
+    return depth_from_plane(static_cast<float>(px), static_cast<float>(py),
+                            load_plane(st->planes, static_cast<int64_t>(py) * W + px), c0);
+  };
+  buf[n++] = dep(x, y);
+  if (y > 0) buf[n++] = dep(x, y - 1);
+  if (y > 2) buf[n++] = dep(x, y - 3);
+  if (y > 4) buf[n++] = dep(x, y - 5);
+  if (y < H - 1) buf[n++] = dep(x, y + 1);
+  if (y < H - 3) buf[n++] = dep(x, y + 3);
+  if (y < H - 5) buf[n++] = dep(x, y + 5);
+  if (x > 0) buf[n++] = dep(x - 1, y);
+  if (x > 2) buf[n++] = dep(x - 3, y);
+  if (x > 4) buf[n++] = dep(x - 5, y);
+  if (x < W - 1) buf[n++] = dep(x + 1, y);
+  if (x < W - 3) buf[n++] = dep(x + 3, y);
+  if (x < W - 5) buf[n++] = dep(x + 5, y);
+  if (y > 0 && x < W - 2) buf[n++] = dep(x + 2, y - 1);
+  if (y < H - 1 && x < W - 2) buf[n++] = dep(x + 2, y + 1);
+  if (y > 0 && x > 1) buf[n++] = dep(x - 2, y - 1);
+  if (y < H - 1 && x > 1) buf[n++] = dep(x - 2, y + 1);
+  if (x > 0 && y > 2) buf[n++] = dep(x - 1, y - 2);
+  if (x < W - 1 && y > 2) buf[n++] = dep(x + 1, y - 2);
+  if (x > 0 && y < H - 2) buf[n++] = dep(x - 1, y + 2);
+  if (x < W - 1 && y < H - 2) buf[n++] = dep(x + 1, y + 2);
+  {
+    const float percent = 0.03f;
+    const float cd = buf[0];
+    if (cd > 0.0f) {
+      const float lo = cd * (1 - percent);
+      const float hi = cd * (1 + percent);
+      int m = 1;
+      for (int i = 1; i < n; ++i)
+        if (buf[i] > lo && buf[i] < hi) buf[m++] = buf[i];
+      n = m;
+    }
+  }
+  for (int i = 1; i < n; ++i) {
+    const float tmp = buf[i];
+    int j;
+    for (j = i; j >= 1 && tmp < buf[j - 1]; --j) buf[j] = buf[j - 1];
+    buf[j] = tmp;
+  }
+  const int mid = n / 2;
+  const float med = (n % 2 == 0) ? (buf[mid - 1] + buf[mid]) * 0.5f : buf[mid];
+  if (med > 0.0f) {
+    const Plane cp = load_plane(st->planes, center);
+    const V3 nn = normalize3(V3{cp.x, cp.y, cp.z});
+    store_plane(st->planes, center, plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), med, nn, c0));
+  }
+}
+
+template <typename F>
+void for_colour(int W, int H, int color, F f) {
+  const int hw = (W + 1) / 2;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int gy = 0; gy < H; ++gy) {
+    for (int gx = 0; gx < hw; ++gx) {
+      const int x = gx * 2 + ((gy + color) % 2);
+      if (x >= W) continue;
+      f(x, gy);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void pmo_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int pmo_max_threads(void) { return omp_get_max_threads(); }
+
+float pmo_expf(float x) { return pm_expf(x); }
+float pmo_logf(float x) { return pm_logf(x); }
+
+float pmo_rand_float(uint32_t* state, uint32_t key) { return rand_float(state, key); }
+
+float pmo_bilinear(const float* img, int w, int h, float sx, float sy) {
+  Img im{img, w, h};
+  return im.bilinear(sx, sy);
+}
+
+void pmo_project(const pmo_camera* cam, const float* X, float* out) {
+  const V3 p = project(V3{X[0], X[1], X[2]}, *cam);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+void pmo_backproject(const pmo_camera* cam, float x, float y, float depth, float* out) {
+  const V3 p = backproject(x, y, depth, *cam);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+float pmo_depth_from_plane(const pmo_camera* cam, float x, float y, const float* plane) {
+  return depth_from_plane(x, y, Plane{plane[0], plane[1], plane[2], plane[3]}, *cam);
+}
+void pmo_plane_from_depth_normal(const pmo_camera* cam, float x, float y, float depth,
+                                 const float* n, float* out) {
+  const Plane p = plane_from_depth_normal(x, y, depth, V3{n[0], n[1], n[2]}, *cam);
+  out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = p.w;
+}
+
+// Reference patch moments (centre, sum_w, sum_wr, sum_wrr) then w[P], wr[P].
+void pmo_ref_patch(const pmo_scene* sc, const pmo_params* pr, int x, int y, float* out) {
+  Ctx c(sc, pr);
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+  const int P = (2 * pr->half_patch + 1) * (2 * pr->half_patch + 1);
+  out[0] = rp.ref_center; out[1] = rp.sum_w; out[2] = rp.sum_wr; out[3] = rp.sum_wrr;
+  for (int i = 0; i < P; ++i) { out[4 + i] = rp.w[i]; out[4 + P + i] = rp.wr[i]; }
+}
+
+// Score n (pixel, plane) pairs: per-view costs (n x num_src), top-k initial
+// cost and view bitmask.  xy: 2n ints; planes: 4n floats.
+void pmo_score_planes(const pmo_scene* sc, const pmo_params* pr, int n, const int* xy,
+                      const float* planes, float* costs, float* init_cost, uint32_t* sel) {
+  Ctx c(sc, pr);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    RefPatch rp;
+    compute_ref_patch(c, x, y, &rp);
+    const Plane p{planes[4 * i], planes[4 * i + 1], planes[4 * i + 2], planes[4 * i + 3]};
+    float cv[kMaxSrc];
+    for (int k = 0; k < kMaxSrc; ++k) cv[k] = 2.0f;
+    cost_vector(c, x, y, p, rp, cv);
+    for (int k = 0; k < c.n_src; ++k) costs[static_cast<int64_t>(i) * c.n_src + k] = cv[k];
+    uint32_t s = 0u;
+    const float ic = initial_cost_and_views(c, x, y, p, rp, &s);
+    if (init_cost) init_cost[i] = ic;
+    if (sel) sel[i] = s;
+  }
+}
+
+void pmo_random_init(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
+  Ctx c(sc, pr);
+  const int W = sc->width, H = sc->height;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) random_init_px(c, st, x, y);
+}
+
+void pmo_prior_reinit(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
+  Ctx c(sc, pr);
+  const int W = sc->width, H = sc->height;
+  if (!st->plane_masks) return;
+  // Every pixel runs at once here (no colour split), and the smoothness term
+  // reads 4-neighbour planes from the pre-kernel snapshot (planes_img at
+  // opencl_kernels.h:1144,1170), so those reads go to a copy.
+  float* snap = new float[static_cast<size_t>(W) * H * 4];
+  std::memcpy(snap, st->planes, sizeof(float) * 4 * static_cast<size_t>(W) * H);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) prior_reinit_px(c, st, snap, x, y);
+  delete[] snap;
+}
+
+void pmo_sweep(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, int color, int iteration,
+               float smooth_w) {
+  Ctx c(sc, pr);
+  for_colour(sc->width, sc->height, color, [&](int x, int y) { patchmatch_px(c, st, x, y, iteration, smooth_w); });
+}
+
+void pmo_checkerboard(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, int color) {
+  Ctx c(sc, pr);
+  for_colour(sc->width, sc->height, color, [&](int x, int y) { checkerboard_px(c, st, x, y); });
+}
+
+// Smooth weight of iteration `iter` (estimator.cc:646-649).
+float pmo_smooth_weight(const pmo_params* pr, int iter) {
+  const float decay = 1.0f - 0.9f * static_cast<float>(iter) /
+                                 static_cast<float>(std::max(pr->max_iterations - 1, 1));
+  return pr->smooth_weight * decay;
+}
+
+// Decode planes/costs into depth, normal, cost, confidence
+// (estimator.cc:1112-1224; speckle / gap filling off, median off).
+void pmo_decode(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, float* depth,
+                float* normal, float* cost, float* confidence) {
+  const int W = sc->width, H = sc->height;
+  const pmo_camera& c0 = sc->cams[0];
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      const int64_t idx = static_cast<int64_t>(y) * W + x;
+      const float* p = &st->planes[4 * idx];
+      depth[idx] = 0.0f;
+      normal[3 * idx] = 0.0f; normal[3 * idx + 1] = 0.0f; normal[3 * idx + 2] = 0.0f;
+      cost[idx] = 0.0f;
+      confidence[idx] = 0.0f;
+      if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || !std::isfinite(p[3]))
+        continue;
+      const float rx = (static_cast<float>(x) - c0.K[2]) / c0.K[0];
+      const float ry = (static_cast<float>(y) - c0.K[5]) / c0.K[4];
+      const float denom = p[0] * rx + p[1] * ry + p[2] * 1.0f;
+      float d = (std::fabs(denom) > 1e-6f) ? -p[3] / denom : 0.0f;
+      if (!std::isfinite(d)) d = 0.0f;
+      if (d > 0.0f && d >= pr->depth_min && d <= pr->depth_max) {
+        depth[idx] = d;
+        const float len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        if (len > 1e-6f) {
+          normal[3 * idx] = p[0] / len; normal[3 * idx + 1] = p[1] / len; normal[3 * idx + 2] = p[2] / len;
+        }
+      }
+      const float cc = st->costs[idx];
+      cost[idx] = std::isfinite(cc) ? cc : 0.0f;
+      if (depth[idx] > 0.0f)
+        confidence[idx] = 1.0f / (1.0f + pm_expf((cost[idx] - 0.5f) * 5.0f));
+    }
+  }
+}
+
+// Whole single-level run (estimator.cc:134-255 with hierarchy_levels = 1):
+// random init, prior re-init, iterations x {red, black}, optional
+// checkerboard filter.  Decode separately with pmo_decode.
+void pmo_run(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
+  pmo_random_init(sc, pr, st);
+  pmo_prior_reinit(sc, pr, st);
+  for (int it = 0; it < pr->max_iterations; ++it) {
+    const float sw = pmo_smooth_weight(pr, it);
+    pmo_sweep(sc, pr, st, 0, it, sw);
+    pmo_sweep(sc, pr, st, 1, it, sw);
+  }
+  if (pr->checkerboard_filter) {
+    pmo_checkerboard(sc, pr, st, 0);
+    pmo_checkerboard(sc, pr, st, 1);
+  }
+}
+
+}  // extern "C"
