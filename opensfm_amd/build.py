@@ -38,24 +38,30 @@ def _run(cmd, **kw):
     subprocess.run(cmd, check=True, **kw)
 
 
-def build_hip(jobs: int = 2) -> str:
+def build_hip(out: str = LIB, defines=(), sources=None) -> str:
+    """Compile the HIP sources (in parallel) and link ``out``.
+
+    ``defines``: extra -D flags (tuning knobs); ``sources``: override the
+    source paths (default: csrc/pm_kernels.hip and csrc/pm_api.hip).
+    """
+    srcs = sources or [os.path.join(CSRC, "pm_kernels.hip"), os.path.join(CSRC, "pm_api.hip")]
     objs = []
     procs = []
-    for src in ("pm_kernels.hip", "pm_api.hip"):
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = ["hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    for src in srcs:
+        obj = out + "." + os.path.basename(src).replace(".hip", ".o")
+        cmd = ["hipcc", *HIP_FLAGS, *[f"-D{d}" for d in defines], f"-I{CSRC}", "-c", src, "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), src))
         objs.append(obj)
     for p, src in procs:
         if p.wait() != 0:
             raise RuntimeError(f"hipcc failed on {src}")
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run(["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     for o in objs:
         os.remove(o)
-    return LIB
+    return out
 
 
 def build_pydense() -> str:

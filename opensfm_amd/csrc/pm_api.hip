@@ -134,8 +134,8 @@ struct pm_handle {
   int W = 0, H = 0, n_src = 0;
   bool f16 = false;
   DevBuf<unsigned char> images;
-  DevBuf<PmView> dviews;
-  DevBuf<PmCam> dcams;
+  std::vector<PmView> hviews;  // passed by value in the kernel arguments
+  std::vector<PmCam> hcams;
   DevBuf<float4> planes, prior, snapshot;
   DevBuf<float> costs, prevd;
   DevBuf<uint2> rng;
@@ -146,7 +146,7 @@ struct pm_handle {
   int last_sweeps = 0;
 
   ~pm_handle() {
-    images.release(); dviews.release(); dcams.release(); planes.release(); prior.release();
+    images.release(); planes.release(); prior.release();
     snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
     out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
@@ -173,8 +173,10 @@ PmArgs make_args(pm_handle* h) {
   a.prev_depth_mask = h->prev_mask;
   a.snapshot = h->snapshot.p;
   a.images = h->images.p;
-  a.views = h->dviews.p;
-  a.cams = h->dcams.p;
+  for (size_t i = 0; i < h->hviews.size() && i <= static_cast<size_t>(kMaxSrc); ++i) {
+    a.views[i] = h->hviews[i];
+    a.cams[i] = h->hcams[i];
+  }
   a.width = h->W;
   a.height = h->H;
   a.n_src = h->n_src;
@@ -479,8 +481,8 @@ int pm_upload(pm_handle* h) {
   }
   const size_t npix = static_cast<size_t>(h->W) * h->H;
   if (int r = h->images.alloc(host.size())) return r;
-  if (int r = h->dviews.alloc(vt.size())) return r;
-  if (int r = h->dcams.alloc(cams.size())) return r;
+  h->hviews = vt;
+  h->hcams = cams;
   if (int r = h->planes.alloc(npix)) return r;
   if (int r = h->costs.alloc(npix)) return r;
   if (int r = h->rng.alloc(npix)) return r;
@@ -490,10 +492,6 @@ int pm_upload(pm_handle* h) {
   if (int r = h->out_cost.alloc(npix)) return r;
   if (int r = h->out_conf.alloc(npix)) return r;
   PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
-  PM_HIP(hipMemcpyAsync(h->dviews.p, vt.data(), vt.size() * sizeof(PmView), hipMemcpyHostToDevice, h->stream),
-         "upload view table");
-  PM_HIP(hipMemcpyAsync(h->dcams.p, cams.data(), cams.size() * sizeof(PmCam), hipMemcpyHostToDevice, h->stream),
-         "upload cameras");
   // Prior (estimator.cc:480-486) and previous depths (:499-508, 66-116).
   if (!h->prior_masks.empty()) {
     if (int r = h->prior.alloc(npix)) return r;

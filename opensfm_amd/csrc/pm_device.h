@@ -51,12 +51,13 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fm
 
 // e^x: Cody-Waite reduction by ln2, degree-7 Taylor polynomial, 2^k via the
 // exponent field; x < -86 -> 0, x > 88 -> +inf.
+// Branch-free on the device: the polynomial runs on x clamped to the
+// finite range (identical for every x that is not special-cased) and the
+// special cases are selected afterwards.
 __device__ __forceinline__ float pm_expf(float x) {
-  if (x != x) return x;
-  if (x < -86.0f) return 0.0f;
-  if (x > 88.0f) return __builtin_inff();
-  const float kf = __builtin_rintf(x * 1.44269502162933349609f);
-  float r = fmaf(-kf, 0.693145751953125f, x);
+  const float xc = fminf(fmaxf(x, -87.0f), 89.0f);
+  const float kf = __builtin_rintf(xc * 1.44269502162933349609f);
+  float r = fmaf(-kf, 0.693145751953125f, xc);
   r = fmaf(-kf, 1.42860682030941723212e-6f, r);
   float p = 1.98412698412698412698e-4f;
   p = fmaf(p, r, 1.38888888888888888889e-3f);
@@ -67,7 +68,10 @@ __device__ __forceinline__ float pm_expf(float x) {
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
   const int k = static_cast<int>(kf);
-  return p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
+  float res = p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
+  res = (x < -86.0f) ? 0.0f : res;
+  res = (x > 88.0f) ? __builtin_inff() : res;
+  return (x != x) ? x : res;
 }
 
 // ln x, x > 0: ln m = 2 atanh((m-1)/(m+1)) with m in [sqrt(1/2), sqrt(2)).
@@ -148,15 +152,30 @@ __device__ __forceinline__ Plane plane_from_depth_normal(float x, float y, float
   return Plane{n.x, n.y, n.z, d};
 }
 
+// Global-address-space view of a pointer: loads become global_load_*
+// instead of flat loads (which also tie up the LDS counter).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PM_GLOBAL __attribute__((address_space(1)))
+#else
+#define PM_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ const PM_GLOBAL T* gp(const T* p) { return (const PM_GLOBAL T*)(p); }
+template <typename T>
+__device__ __forceinline__ PM_GLOBAL T* gpw(T* p) { return (PM_GLOBAL T*)(p); }
+
 // Image access on the padded layout.
 template <typename T>
-__device__ __forceinline__ float ld(const T* p) { return static_cast<float>(*p); }
+__device__ __forceinline__ float ld(const PM_GLOBAL T* p) { return static_cast<float>(*p); }
 
+// Offsets inside one view are 32-bit (views are < 2^31 texels), so a load
+// is a uniform 64-bit base in SGPRs plus a 32-bit per-lane offset.
 template <typename T>
 __device__ __forceinline__ float texel(const T* __restrict__ img, const PmView& v, int ix, int iy) {
   ix = min(max(ix, 0), v.w - 1);
   iy = min(max(iy, 0), v.h - 1);
-  return ld(img + v.base + static_cast<int64_t>(iy + 1) * v.pitch + (ix + 1));
+  const PM_GLOBAL T* vb = gp(img) + v.base;
+  return ld(vb + static_cast<uint32_t>((iy + 1) * v.pitch + (ix + 1)));
 }
 
 template <typename T>
@@ -166,10 +185,11 @@ __device__ __forceinline__ float bilinear(const T* __restrict__ img, const PmVie
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
   const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-  const T* r0 = img + v.base + static_cast<int64_t>(y0 + 1) * v.pitch + (x0 + 1);
-  const T* r1 = r0 + v.pitch;
-  const float t00 = ld(r0), t01 = ld(r0 + 1);
-  const float t10 = ld(r1), t11 = ld(r1 + 1);
+  const PM_GLOBAL T* vb = gp(img) + v.base;
+  const uint32_t o0 = static_cast<uint32_t>((y0 + 1) * v.pitch + (x0 + 1));
+  const uint32_t o1 = o0 + static_cast<uint32_t>(v.pitch);
+  const float t00 = ld(vb + o0), t01 = ld(vb + (o0 + 1u));
+  const float t10 = ld(vb + o1), t11 = ld(vb + (o1 + 1u));
   const float v0 = fmaf(ax, t01 - t00, t00);
   const float v1 = fmaf(ax, t11 - t10, t10);
   return fmaf(ay, v1 - v0, v0);

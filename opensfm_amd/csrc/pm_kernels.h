@@ -20,8 +20,6 @@ struct PmArgs {
   const float* prev_depths;    // n_src * W * H, may be null
   const float4* snapshot;      // planes before the prior re-init launch
   const void* images;          // padded views, float or _Float16
-  const PmView* views;         // 1 + n_src
-  const PmCam* cams;           // 1 + n_src
   uint32_t prev_depth_mask;
   int width, height, n_src;
   float depth_min, depth_max;
@@ -34,7 +32,12 @@ struct PmArgs {
   int anchor_views;
   int color;
   int iteration;
+  // Per-view descriptors travel in the kernel-argument segment, so every
+  // camera / view read in the kernels is a uniform scalar load.
+  PmCam cams[kMaxSrc + 1];     // index 0 = reference
+  PmView views[kMaxSrc + 1];
 };
+static_assert(sizeof(PmArgs) <= 4096, "kernel argument segment");
 
 struct PmDecodeOut {
   float* depth;

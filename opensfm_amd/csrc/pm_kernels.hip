@@ -63,7 +63,7 @@ __device__ __forceinline__ void compute_ref_patch(const T* __restrict__ img, con
 
 // opencl_kernels.h:188-258
 template <int HP, typename T>
-__device__ __attribute__((noinline)) float ncc_at_stride(const PmArgs& a, const T* __restrict__ img, int s,
+__device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const T* __restrict__ img, int s,
                                                          float ref_center, float src_center, float scx, float scy,
                                                          int cx, int cy, float stride, float jxx, float jxy,
                                                          float jyx, float jyy) {
@@ -105,7 +105,7 @@ __device__ __attribute__((noinline)) float ncc_at_stride(const PmArgs& a, const 
 
 // opencl_kernels.h:270-315
 template <typename T>
-__device__ __attribute__((noinline)) float census_inner(const PmArgs& a, const T* __restrict__ img, int s,
+__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, int s,
                                                         float ref_center, float src_center, int cx, int cy,
                                                         float scx, float scy, float jxx, float jxy, float jyx,
                                                         float jyy) {
@@ -209,12 +209,14 @@ __device__ __forceinline__ void cost_vector(const PmArgs& a, const T* __restrict
           if (ncc1 > 0.1f) {
             cost = 1.0f - ncc1;
           } else {
+            // Textureless fallback: strides 2 then 3, stop once NCC > 0.1 (:532-542).
             float best = ncc1;
-            float ncc = ncc_at_stride<HP>(a, img, s, rp.center, scen, scx, scy, x, y, 2.0f, jxx, jxy, jyx, jyy);
-            if (ncc > best) best = ncc;
-            if (!(best > 0.1f)) {
-              ncc = ncc_at_stride<HP>(a, img, s, rp.center, scen, scx, scy, x, y, 3.0f, jxx, jxy, jyx, jyy);
+#pragma unroll 1
+            for (int st = 0; st < 2; ++st) {
+              const float ncc = ncc_at_stride<HP>(a, img, s, rp.center, scen, scx, scy, x, y,
+                                                  st == 0 ? 2.0f : 3.0f, jxx, jxy, jyx, jyy);
               if (ncc > best) best = ncc;
+              if (best > 0.1f) break;
             }
             if (best <= -0.5f) {
               cost = (a.census_weight > 1e-6f)
@@ -277,7 +279,7 @@ __device__ __forceinline__ void load_nbrs(const PmArgs& a, const float4* __restr
   for (int k = 0; k < 4; ++k) {
     const int nx = x + dxs[k], ny = y + dys[k];
     nb.ok[k] = !(nx < 0 || nx >= W || ny < 0 || ny >= H);
-    nb.p[k] = nb.ok[k] ? to_plane(planes[static_cast<int64_t>(ny) * W + nx]) : Plane{0, 0, 0, 0};
+    nb.p[k] = nb.ok[k] ? to_plane(gp(planes)[static_cast<int64_t>(ny) * W + nx]) : Plane{0, 0, 0, 0};
   }
 }
 
@@ -304,7 +306,7 @@ __device__ __forceinline__ V3 surface_normal(const PmArgs& a, const Nbrs& nb, in
 }
 
 // opencl_kernels.h:779-861
-__device__ __attribute__((noinline)) float geom_cost(const PmArgs& a, int sidx, Plane plane, int px, int py) {
+__device__ __forceinline__ float geom_cost(const PmArgs& a, int sidx, Plane plane, int px, int py) {
   const int pdw = a.width, pdh = a.height;
   const PmCam& rc = a.cams[0];
   const PmCam& scm = a.cams[sidx + 1];
@@ -513,9 +515,24 @@ __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) {
 
 // ---------------------------------------------------------------------------
 // acmmp_patchmatch, one colour (opencl_kernels.h:1197-1778)
+//
+// The 15 plane hypotheses of a pixel (8 propagated candidates, the current
+// plane, up to 6 refinements) run through ONE loop with one inlined copy of
+// the cost function, which keeps the kernel small enough for the
+// instruction cache.  Evaluating the current plane's cost vector together
+// with the candidates is equivalent to the reference order: cost vectors
+// consume no random numbers, so the RNG stream (candidate MC draws, then
+// the current plane's, then the refinement draws) is unchanged.
 // ---------------------------------------------------------------------------
+// Occupancy target of the colour-pass kernel (waves per SIMD); the register
+// allocator trims hoisting to fit.
+#ifndef PM_SWEEP_WAVES_PER_EU
+#define PM_SWEEP_WAVES_PER_EU 1
+#endif
+
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PM_SWEEP_WAVES_PER_EU)))
+void sweep_kernel(PmArgs a) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x;
   const int W = a.width, H = a.height;
@@ -529,50 +546,51 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
   const PmCam& c0 = a.cams[0];
   const int n_src = a.n_src;
   const int idx = y * W + x;
-  const uint2 st = a.rng[idx];
+  const uint2 st = gp(a.rng)[idx];
   uint32_t s0 = st.x, s1 = st.y;
   const uint32_t key = static_cast<uint32_t>(idx + a.iteration * 7919 + 42);
-  // Not __restrict__: the pass writes its own pixel through a.planes / a.costs.
-  const float4* planes = a.planes;
-  const float* costs = a.costs;
+  // Other-colour state is read only; this pixel's own entries are written
+  // once at the end.
+  const PM_GLOBAL float4* planes = gp(a.planes);
+  const PM_GLOBAL float* costs = gp(a.costs);
   float* ca = smem + lane;  // ca[(slot * n_src + view) * kWave]
   int* cpos = reinterpret_cast<int*>(smem + kSweepSlots * n_src * kWave) + lane;  // cpos[slot * kWave]
 
   RefPatch<HP> rp;
   compute_ref_patch<HP>(img, a.views[0], x, y, a.spatial_factor, a.color_factor, rp);
 
-  // Step 1: 8 candidates (:1254-1430).
+  // Step 1: candidate positions (:1254-1430), cheapest-cost probes on the
+  // other colour.  Slot layout 0=near up, 1=far up, 2=near down, 3=far down,
+  // 4=near left, 5=far left, 6=near right, 7=far right.
   uint32_t flags = 0u;
-  for (int i = 0; i < 8 * n_src; ++i) ca[i * kWave] = 2.0f;
   auto cost_at = [&](int px, int py) { return costs[py * W + px]; };
-  auto eval = [&](int slot, int px, int py) {
+  auto put = [&](int slot, int px, int py) {
     cpos[slot * kWave] = py * W + px;
     flags |= 1u << slot;
-    cost_vector<HP>(a, img, x, y, to_plane(planes[py * W + px]), rp, ca + slot * n_src * kWave, kWave);
   };
   if (y > 2) {
     int by = y - 3; float bc = cost_at(x, by);
     for (int i = 1; i < 11; ++i) { const int yy = y - 3 - 2 * i; if (yy < 0) break;
       const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
-    eval(1, x, by);
+    put(1, x, by);
   }
   if (y < H - 3) {
     int by = y + 3; float bc = cost_at(x, by);
     for (int i = 1; i < 11; ++i) { const int yy = y + 3 + 2 * i; if (yy >= H) break;
       const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
-    eval(3, x, by);
+    put(3, x, by);
   }
   if (x > 2) {
     int bx = x - 3; float bc = cost_at(bx, y);
     for (int i = 1; i < 11; ++i) { const int xx = x - 3 - 2 * i; if (xx < 0) break;
       const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
-    eval(5, bx, y);
+    put(5, bx, y);
   }
   if (x < W - 3) {
     int bx = x + 3; float bc = cost_at(bx, y);
     for (int i = 1; i < 11; ++i) { const int xx = x + 3 + 2 * i; if (xx >= W) break;
       const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
-    eval(7, bx, y);
+    put(7, bx, y);
   }
   if (y > 0) {
     int bx = x, by = y - 1; float bc = cost_at(bx, by);
@@ -582,7 +600,7 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
       if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const float c = cost_at(px, py);
         if (c < bc) { bc = c; bx = px; by = py; } }
     }
-    eval(0, bx, by);
+    put(0, bx, by);
   }
   if (y < H - 1) {
     int bx = x, by = y + 1; float bc = cost_at(bx, by);
@@ -592,7 +610,7 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
       if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const float c = cost_at(px, py);
         if (c < bc) { bc = c; bx = px; by = py; } }
     }
-    eval(2, bx, by);
+    put(2, bx, by);
   }
   if (x > 0) {
     int bx = x - 1, by = y; float bc = cost_at(bx, by);
@@ -602,7 +620,7 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
       if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const float c = cost_at(px, py);
         if (c < bc) { bc = c; bx = px; by = py; } }
     }
-    eval(4, bx, by);
+    put(4, bx, by);
   }
   if (x < W - 1) {
     int bx = x + 1, by = y; float bc = cost_at(bx, by);
@@ -612,165 +630,46 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
       if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const float c = cost_at(px, py);
         if (c < bc) { bc = c; bx = px; by = py; } }
     }
-    eval(6, bx, by);
-  }
-
-  // Step 2: view weights (:1452-1563).
-  const float vthr = 0.8f;
-  float vw[kMaxSrc];
-#pragma unroll
-  for (int i = 0; i < kMaxSrc; ++i) {
-    vw[i] = 0.0f;
-    if (i < n_src) {
-      float vc = 0.0f;
-      for (int j = 0; j < 8; ++j) {
-        const float c = ca[(j * n_src + i) * kWave];
-        if ((flags & (1u << j)) && c < vthr) vc += (1.0f - c);
-      }
-      if (vc > 0.0f) vw[i] = vc;
-    }
-  }
-  const uint32_t prev_sel = a.sel[idx];
-  if (a.anchor_views > 0) {
-    int anchored = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxSrc; ++i) {
-      if (i < n_src && anchored < a.anchor_views) {
-        if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
-      }
-    }
-  }
-  Plane plane_now = to_plane(planes[idx]);
-  {
-    V3 cur_n{plane_now.x, plane_now.y, plane_now.z};
-    const float nl = len3(cur_n);
-    const bool have_normal = nl > 0.1f;
-    if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
-    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
-    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
-    const float inc_sigma = 0.6f;
-    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
-#pragma unroll
-    for (int i = 0; i < kMaxSrc; ++i) {
-      if (i >= n_src || vw[i] < 1e-6f) continue;
-      const PmView v = a.views[i + 1];
-      vw[i] *= v.angle_w;
-      if (have_normal && cur_depth > 0.0f) {
-        const V3 ray = normalize3(V3{v.cx - p3.x, v.cy - p3.y, v.cz - p3.z});
-        const float cos_inc = fabsf(dot3(cur_n, ray));
-        const float xi = 1.0f - cos_inc;
-        vw[i] *= pm_expf(xi * xi * inv2s2);
-      }
-    }
-  }
-  uint32_t new_sel = 0u;
-  float wnorm = 0.0f;
-#pragma unroll
-  for (int i = 0; i < kMaxSrc; ++i)
-    if (i < n_src && vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
-  if (wnorm < 1e-6f) {
-#pragma unroll
-    for (int i = 0; i < kMaxSrc; ++i) {
-      if (i >= n_src) continue;
-      for (int j = 0; j < 8; ++j) {
-        if ((flags & (1u << j)) && ca[(j * n_src + i) * kWave] < 2.0f) {
-          vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
-        }
-      }
-    }
+    put(6, bx, by);
   }
 
   Nbrs nb;
-  load_nbrs(a, planes, x, y, nb);
+  load_nbrs(a, a.planes, x, y, nb);
   const float smooth_w = a.smooth_w;
-
-  // Step 3: candidate costs (:1570-1619).
-  float best_cost = 2.0f;
-  int best = 0;
-  for (int i = 0; i < 8; ++i) {
-    float fc = 2.0f;
-    if (flags & (1u << i)) {
-      const Plane cp = to_plane(planes[cpos[i * kWave]]);
-      fc = mc_cost(a, ca + i * n_src * kWave, vw, wnorm, cp, x, y, s0, s1,
-                   key + 600u + static_cast<uint32_t>(i) * 31u);
-      if (smooth_w > 1e-6f) fc += smooth_w * smoothness_cost(a, nb, cp, x, y);
-    }
-    if (i == 0) { best_cost = fc; } else if (fc < best_cost) { best_cost = fc; best = i; }
-  }
-
-  float* cvn = ca + 8 * n_src * kWave;
-  cost_vector<HP>(a, img, x, y, plane_now, rp, cvn, kWave);
-  float cost_now = mc_cost(a, cvn, vw, wnorm, plane_now, x, y, s0, s1, key + 700u);
-  if (smooth_w > 1e-6f) cost_now += smooth_w * smoothness_cost(a, nb, plane_now, x, y);
-  float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
-
-  const bool have_prior = a.plane_masks != nullptr && a.plane_masks[idx] > 0u;
-  float depth_prior = 0.0f;
-  V3 prior_n{0.0f, 0.0f, -1.0f};
-  const float depth_sigma = (a.depth_max - a.depth_min) / 32.0f;
-  const float angle_sigma = kPiF * (5.0f / 180.0f);
-  if (have_prior) {
-    const Plane pp = to_plane(a.prior_planes[idx]);
-    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
-    prior_n = normalize3(V3{pp.x, pp.y, pp.z});
-  }
-
-  // Step 3b (:1643-1656).
+  const uint32_t prev_sel = gp(a.sel)[idx];
+  Plane plane_now = to_plane(planes[idx]);
+  float cost_now = 2.0f, depth_now = 0.0f;
   uint32_t sel_out = prev_sel;
-  if ((flags & (1u << best)) && best_cost < cost_now) {
-    const Plane bp = to_plane(planes[cpos[best * kWave]]);
-    const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
-    if (db >= a.depth_min && db <= a.depth_max) {
-      depth_now = db;
-      plane_now = bp;
-      cost_now = best_cost;
-      sel_out = new_sel;
-    }
-  }
 
-  // Step 4: refinement (:1669-1772).
-  {
-    const float depth_perturbation = 0.005f;
-    const float normal_perturbation = 0.01f * kPiF;
-    const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
-    float depth_rand, depth_pert;
-    V3 nrand, npert;
-    if (have_prior && depth_prior > 0.0f) {
-      depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(s0, s1, key + 503u) * 6.0f * depth_sigma;
-      depth_rand = clampf(depth_rand, a.depth_min, a.depth_max);
-      const float pa1 = (rand_float(s0, s1, key + 504u) - 0.5f) * angle_sigma;
-      const float pa2 = (rand_float(s0, s1, key + 505u) - 0.5f) * angle_sigma;
-      nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
-      if (nrand.z > 0.0f) nrand.z = -nrand.z;
+  float vw[kMaxSrc];
+#pragma unroll
+  for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
+  float wnorm = 0.0f;
+  // refinement hypotheses (filled after the current plane is scored)
+  float depth_rand = 0.0f, depth_pert = 0.0f, depth0 = 0.0f;
+  V3 cur_normal{0, 0, 0}, nrand{0, 0, 0}, npert{0, 0, 0}, sn{0, 0, 0};
+  int n_ref = 0;
+
+#pragma unroll 1
+  for (int h = 0; h < 9 + 6; ++h) {
+    Plane hyp;
+    bool active;
+    int slot;
+    if (h < 8) {
+      active = (flags >> h) & 1u;
+      slot = h;
+      hyp = active ? to_plane(planes[cpos[h * kWave]]) : Plane{0, 0, 0, 0};
+    } else if (h == 8) {
+      active = true;
+      slot = 8;
+      hyp = plane_now;
     } else {
-      const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
-      const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
-      depth_rand = pm_expf(ldmin + rand_float(s0, s1, key + 503u) * (ldmax - ldmin));
-      nrand.x = rand_float(s0, s1, key + 504u) * 2.0f - 1.0f;
-      nrand.y = rand_float(s0, s1, key + 505u) * 2.0f - 1.0f;
-      nrand.z = -1.0f;
-      nrand = normalize3(nrand);
-    }
-    {
-      const float dlo = (1.0f - depth_perturbation) * depth_now;
-      const float dhi = (1.0f + depth_perturbation) * depth_now;
-      depth_pert = dlo + rand_float(s0, s1, key + 510u) * (dhi - dlo);
-      depth_pert = clampf(depth_pert, a.depth_min, a.depth_max);
-    }
-    {
-      const float pa1 = (rand_float(s0, s1, key + 520u) - 0.5f) * normal_perturbation;
-      const float pa2 = (rand_float(s0, s1, key + 521u) - 0.5f) * normal_perturbation;
-      npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
-      if (npert.z > 0.0f) npert.z = -npert.z;
-    }
-    const V3 sn = surface_normal(a, nb, x, y, depth_now);
-    const bool have_sn = len3(sn) > 0.5f;
-    const int nc = have_sn ? 6 : 5;
-    const float depth0 = depth_now;  // the hypotheses are fixed before the loop (:1733-1741)
-    for (int h = 0; h < nc; ++h) {
+      const int r = h - 9;
+      active = r < n_ref;
+      slot = 8;
       float rd;
       V3 rn;
-      switch (h) {
+      switch (r) {  // :1733-1741
         case 0: rd = depth_rand; rn = cur_normal; break;
         case 1: rd = depth0; rn = nrand; break;
         case 2: rd = depth_rand; rn = nrand; break;
@@ -778,19 +677,171 @@ __global__ __launch_bounds__(kWave) void sweep_kernel(PmArgs a) {
         case 4: rd = depth_pert; rn = cur_normal; break;
         default: rd = depth0; rn = sn; break;
       }
-      const Plane cand = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd, rn, c0);
-      cost_vector<HP>(a, img, x, y, cand, rp, cvn, kWave);
-      float tc = mc_cost(a, cvn, vw, wnorm, cand, x, y, s0, s1, key + 800u + static_cast<uint32_t>(h) * 37u);
-      if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(a, nb, cand, x, y);
-      const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cand, c0);
-      if (db < a.depth_min || db > a.depth_max) continue;
-      if (tc < cost_now) { depth_now = db; plane_now = cand; cost_now = tc; }
+      hyp = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd, rn, c0);
+    }
+    if (active) cost_vector<HP>(a, img, x, y, hyp, rp, ca + slot * n_src * kWave, kWave);
+
+    if (h == 8) {
+      // Step 2: view weights (:1452-1563).
+      const float vthr = 0.8f;
+#pragma unroll
+      for (int i = 0; i < kMaxSrc; ++i) {
+        if (i < n_src) {
+          float vc = 0.0f;
+          for (int j = 0; j < 8; ++j) {
+            const float c = ca[(j * n_src + i) * kWave];
+            if (((flags >> j) & 1u) && c < vthr) vc += (1.0f - c);
+          }
+          if (vc > 0.0f) vw[i] = vc;
+        }
+      }
+      if (a.anchor_views > 0) {
+        int anchored = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxSrc; ++i) {
+          if (i < n_src && anchored < a.anchor_views) {
+            if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
+          }
+        }
+      }
+      {
+        V3 cur_n{plane_now.x, plane_now.y, plane_now.z};
+        const float nl = len3(cur_n);
+        const bool have_normal = nl > 0.1f;
+        if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
+        const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+        const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
+        const float inc_sigma = 0.6f;
+        const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
+#pragma unroll
+        for (int i = 0; i < kMaxSrc; ++i) {
+          if (i >= n_src || vw[i] < 1e-6f) continue;
+          const PmView& v = a.views[i + 1];
+          vw[i] *= v.angle_w;
+          if (have_normal && cur_depth > 0.0f) {
+            const V3 ray = normalize3(V3{v.cx - p3.x, v.cy - p3.y, v.cz - p3.z});
+            const float cos_inc = fabsf(dot3(cur_n, ray));
+            const float xi = 1.0f - cos_inc;
+            vw[i] *= pm_expf(xi * xi * inv2s2);
+          }
+        }
+      }
+      uint32_t new_sel = 0u;
+#pragma unroll
+      for (int i = 0; i < kMaxSrc; ++i)
+        if (i < n_src && vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
+      if (wnorm < 1e-6f) {
+#pragma unroll
+        for (int i = 0; i < kMaxSrc; ++i) {
+          if (i >= n_src) continue;
+          for (int j = 0; j < 8; ++j) {
+            if (((flags >> j) & 1u) && ca[(j * n_src + i) * kWave] < 2.0f) {
+              vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
+            }
+          }
+        }
+      }
+
+      // Step 3: Monte-Carlo costs of the candidates, then of the current
+      // plane (:1570-1619).
+      float best_cost = 2.0f;
+      int best = 0;
+#pragma unroll 1
+      for (int i = 0; i < 9; ++i) {
+        const bool flagged = i == 8 || ((flags >> i) & 1u);
+        float fc = 2.0f;
+        if (flagged) {
+          const Plane cp = i == 8 ? plane_now : to_plane(planes[cpos[i * kWave]]);
+          const uint32_t k2 = i == 8 ? key + 700u : key + 600u + static_cast<uint32_t>(i) * 31u;
+          fc = mc_cost(a, ca + i * n_src * kWave, vw, wnorm, cp, x, y, s0, s1, k2);
+          if (smooth_w > 1e-6f) fc += smooth_w * smoothness_cost(a, nb, cp, x, y);
+        }
+        if (i == 8) {
+          cost_now = fc;
+        } else if (i == 0) {
+          best_cost = fc;
+        } else if (fc < best_cost) {
+          best_cost = fc;
+          best = i;
+        }
+      }
+      depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+
+      const bool have_prior = a.plane_masks != nullptr && gp(a.plane_masks)[idx] > 0u;
+      float depth_prior = 0.0f;
+      V3 prior_n{0.0f, 0.0f, -1.0f};
+      const float depth_sigma = (a.depth_max - a.depth_min) / 32.0f;
+      const float angle_sigma = kPiF * (5.0f / 180.0f);
+      if (have_prior) {
+        const Plane pp = to_plane(gp(a.prior_planes)[idx]);
+        depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+        prior_n = normalize3(V3{pp.x, pp.y, pp.z});
+      }
+
+      // Step 3b (:1643-1656).
+      if (((flags >> best) & 1u) && best_cost < cost_now) {
+        const Plane bp = to_plane(planes[cpos[best * kWave]]);
+        const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
+        if (db >= a.depth_min && db <= a.depth_max) {
+          depth_now = db;
+          plane_now = bp;
+          cost_now = best_cost;
+          sel_out = new_sel;
+        }
+      }
+
+      // Step 4 set-up: the refinement hypotheses (:1669-1743).
+      const float depth_perturbation = 0.005f;
+      const float normal_perturbation = 0.01f * kPiF;
+      cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
+      if (have_prior && depth_prior > 0.0f) {
+        depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(s0, s1, key + 503u) * 6.0f * depth_sigma;
+        depth_rand = clampf(depth_rand, a.depth_min, a.depth_max);
+        const float pa1 = (rand_float(s0, s1, key + 504u) - 0.5f) * angle_sigma;
+        const float pa2 = (rand_float(s0, s1, key + 505u) - 0.5f) * angle_sigma;
+        nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
+        if (nrand.z > 0.0f) nrand.z = -nrand.z;
+      } else {
+        const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
+        const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
+        depth_rand = pm_expf(ldmin + rand_float(s0, s1, key + 503u) * (ldmax - ldmin));
+        nrand.x = rand_float(s0, s1, key + 504u) * 2.0f - 1.0f;
+        nrand.y = rand_float(s0, s1, key + 505u) * 2.0f - 1.0f;
+        nrand.z = -1.0f;
+        nrand = normalize3(nrand);
+      }
+      {
+        const float dlo = (1.0f - depth_perturbation) * depth_now;
+        const float dhi = (1.0f + depth_perturbation) * depth_now;
+        depth_pert = dlo + rand_float(s0, s1, key + 510u) * (dhi - dlo);
+        depth_pert = clampf(depth_pert, a.depth_min, a.depth_max);
+      }
+      {
+        const float pa1 = (rand_float(s0, s1, key + 520u) - 0.5f) * normal_perturbation;
+        const float pa2 = (rand_float(s0, s1, key + 521u) - 0.5f) * normal_perturbation;
+        npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
+        if (npert.z > 0.0f) npert.z = -npert.z;
+      }
+      sn = surface_normal(a, nb, x, y, depth_now);
+      n_ref = len3(sn) > 0.5f ? 6 : 5;
+      depth0 = depth_now;
+    } else if (h > 8 && active) {
+      // Step 4: score one refinement hypothesis (:1744-1771).
+      float tc = mc_cost(a, ca + 8 * n_src * kWave, vw, wnorm, hyp, x, y, s0, s1,
+                         key + 800u + static_cast<uint32_t>(h - 9) * 37u);
+      if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(a, nb, hyp, x, y);
+      const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), hyp, c0);
+      if (!(db < a.depth_min || db > a.depth_max) && tc < cost_now) {
+        depth_now = db;
+        plane_now = hyp;
+        cost_now = tc;
+      }
     }
   }
-  a.planes[idx] = to_f4(plane_now);
-  a.costs[idx] = cost_now;
-  a.rng[idx] = make_uint2(s0, s1);
-  a.sel[idx] = sel_out;
+  gpw(a.planes)[idx] = to_f4(plane_now);
+  gpw(a.costs)[idx] = cost_now;
+  gpw(a.rng)[idx] = make_uint2(s0, s1);
+  gpw(a.sel)[idx] = sel_out;
 }
 
 // ---------------------------------------------------------------------------
