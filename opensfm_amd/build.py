@@ -38,18 +38,19 @@ def _run(cmd, **kw):
     subprocess.run(cmd, check=True, **kw)
 
 
-def build_hip(out: str = LIB, defines=(), sources=None) -> str:
+def build_hip(out: str = LIB, defines=(), sources=None, extra=()) -> str:
     """Compile the HIP sources (in parallel) and link ``out``.
 
     ``defines``: extra -D flags (tuning knobs); ``sources``: override the
-    source paths (default: csrc/pm_kernels.hip and csrc/pm_api.hip).
+    source paths (default: csrc/pm_kernels.hip and csrc/pm_api.hip);
+    ``extra``: further compiler flags.
     """
     srcs = sources or [os.path.join(CSRC, "pm_kernels.hip"), os.path.join(CSRC, "pm_api.hip")]
     objs = []
     procs = []
     for src in srcs:
         obj = out + "." + os.path.basename(src).replace(".hip", ".o")
-        cmd = ["hipcc", *HIP_FLAGS, *[f"-D{d}" for d in defines], f"-I{CSRC}", "-c", src, "-o", obj]
+        cmd = ["hipcc", *HIP_FLAGS, *extra, *[f"-D{d}" for d in defines], f"-I{CSRC}", "-c", src, "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), src))
         objs.append(obj)

@@ -134,8 +134,15 @@ struct pm_handle {
   int W = 0, H = 0, n_src = 0;
   bool f16 = false;
   DevBuf<unsigned char> images;
-  std::vector<PmView> hviews;  // passed by value in the kernel arguments
-  std::vector<PmCam> hcams;
+  DevBuf<PmView> gviews;       // view table (index 0 = reference)
+  DevBuf<PmCam> gcams;         // cameras
+  DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
+  DevBuf<int> cpos, nref;
+  DevBuf<float4> refp;
+  DevBuf<float2> ref_taps;     // reference-side patch data, strides 1..3
+  DevBuf<float4> ref_sums;
+  int64_t ref_plane = 0;
+  bool ref_ready = false;
   DevBuf<float4> planes, prior, snapshot;
   DevBuf<float> costs, prevd;
   DevBuf<uint2> rng;
@@ -146,7 +153,9 @@ struct pm_handle {
   int last_sweeps = 0;
 
   ~pm_handle() {
-    images.release(); planes.release(); prior.release();
+    images.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
+    ca.release(); vwb.release(); wnb.release(); cpos.release(); nref.release(); refp.release();
+    planes.release(); prior.release();
     snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
     out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
@@ -173,10 +182,17 @@ PmArgs make_args(pm_handle* h) {
   a.prev_depth_mask = h->prev_mask;
   a.snapshot = h->snapshot.p;
   a.images = h->images.p;
-  for (size_t i = 0; i < h->hviews.size() && i <= static_cast<size_t>(kMaxSrc); ++i) {
-    a.views[i] = h->hviews[i];
-    a.cams[i] = h->hcams[i];
-  }
+  a.ref_taps = h->ref_taps.p;
+  a.ref_sums = h->ref_sums.p;
+  a.ref_plane = h->ref_plane;
+  a.cams = h->gcams.p;
+  a.views = h->gviews.p;
+  a.ca = h->ca.p;
+  a.cpos = h->cpos.p;
+  a.vwb = h->vwb.p;
+  a.wnb = h->wnb.p;
+  a.refp = h->refp.p;
+  a.nref = h->nref.p;
   a.width = h->W;
   a.height = h->H;
   a.n_src = h->n_src;
@@ -225,6 +241,16 @@ int validate(pm_handle* h) {
   if (p.max_iterations < 0) return fail(PM_E_INVALID, "max_iterations must be >= 0");
   if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "only hierarchy_levels = 1 is supported");
   if (!(p.sigma_spatial > 0.0f) || !(p.sigma_color > 0.0f)) return fail(PM_E_INVALID, "sigmas must be > 0");
+  return PM_OK;
+}
+
+// Reference-side patch data for all strides, once per upload (it depends
+// only on the reference image and the patch parameters).
+int ensure_refpatch(pm_handle* h) {
+  if (h->ref_ready) return PM_OK;
+  PmArgs a = make_args(h);
+  PM_HIP(launch_refpatch(a, h->params.patch_size / 2, h->f16, h->stream), "refpatch kernel launch");
+  h->ref_ready = true;
   return PM_OK;
 }
 
@@ -368,7 +394,9 @@ int pm_set_depth_range(pm_handle* h, float dmin, float dmax) {
 int pm_set_params(pm_handle* h, const pm_params* p) {
   if (int r = check_handle(h)) return r;
   if (!p) return fail(PM_E_INVALID, "null params");
-  const bool restage = p->image_storage != h->params.image_storage || p->quantize_half != h->params.quantize_half;
+  const bool restage = p->image_storage != h->params.image_storage || p->quantize_half != h->params.quantize_half ||
+                       p->patch_size != h->params.patch_size;
+  if (p->sigma_spatial != h->params.sigma_spatial || p->sigma_color != h->params.sigma_color) h->ref_ready = false;
   h->params = *p;
   if (restage) h->uploaded = false;
   h->state_ready = false;
@@ -481,8 +509,20 @@ int pm_upload(pm_handle* h) {
   }
   const size_t npix = static_cast<size_t>(h->W) * h->H;
   if (int r = h->images.alloc(host.size())) return r;
-  h->hviews = vt;
-  h->hcams = cams;
+  if (int r = h->gviews.alloc(vt.size())) return r;
+  if (int r = h->gcams.alloc(cams.size())) return r;
+  h->ref_plane = 2 * static_cast<int64_t>(h->H) * ((h->W + 1) / 2);
+  const int P = (p.patch_size / 2 * 2 + 1) * (p.patch_size / 2 * 2 + 1);
+  if (int r = h->ref_taps.alloc(static_cast<size_t>(3 * P) * h->ref_plane)) return r;
+  if (int r = h->ref_sums.alloc(static_cast<size_t>(3) * h->ref_plane)) return r;
+  const size_t rp = static_cast<size_t>(h->ref_plane);
+  if (int r = h->ca.alloc(9 * static_cast<size_t>(h->n_src) * rp)) return r;
+  if (int r = h->vwb.alloc(static_cast<size_t>(h->n_src) * rp)) return r;
+  if (int r = h->wnb.alloc(rp)) return r;
+  if (int r = h->cpos.alloc(8 * rp)) return r;
+  if (int r = h->nref.alloc(rp)) return r;
+  if (int r = h->refp.alloc(6 * rp)) return r;
+  h->ref_ready = false;
   if (int r = h->planes.alloc(npix)) return r;
   if (int r = h->costs.alloc(npix)) return r;
   if (int r = h->rng.alloc(npix)) return r;
@@ -492,6 +532,10 @@ int pm_upload(pm_handle* h) {
   if (int r = h->out_cost.alloc(npix)) return r;
   if (int r = h->out_conf.alloc(npix)) return r;
   PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
+  PM_HIP(hipMemcpyAsync(h->gviews.p, vt.data(), vt.size() * sizeof(PmView), hipMemcpyHostToDevice, h->stream),
+         "upload view table");
+  PM_HIP(hipMemcpyAsync(h->gcams.p, cams.data(), cams.size() * sizeof(PmCam), hipMemcpyHostToDevice, h->stream),
+         "upload cameras");
   // Prior (estimator.cc:480-486) and previous depths (:499-508, 66-116).
   if (!h->prior_masks.empty()) {
     if (int r = h->prior.alloc(npix)) return r;
@@ -536,6 +580,7 @@ int pm_init_state(pm_handle* h) {
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
+  if (int r = ensure_refpatch(h)) return r;
   PM_HIP(launch_init(a, hp, h->f16, h->stream), "init kernel launch");
   if (h->masks.p) {  // estimator.cc:574-614
     PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
@@ -550,6 +595,7 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
   if (int r = check_handle(h)) return r;
   if (!h->state_ready) return fail(PM_E_STATE, "pm_iterate: state is not initialised (call pm_init_state)");
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  if (int r = ensure_refpatch(h)) return r;
   PmArgs a = make_args(h);
   a.iteration = iteration;
   a.smooth_w = smooth_weight_at(h->params, iteration);
@@ -691,6 +737,7 @@ int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes,
   if (int r = dinit.alloc(static_cast<size_t>(n))) return r;
   if (int r = dsel.alloc(static_cast<size_t>(n))) return r;
   int rc = PM_OK;
+  if (int r = ensure_refpatch(h)) return r;
   do {
     PmArgs a = make_args(h);
     if (hipMemcpyAsync(dxy.p, xy, 8 * static_cast<size_t>(n), hipMemcpyHostToDevice, h->stream) != hipSuccess ||

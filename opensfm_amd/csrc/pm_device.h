@@ -156,8 +156,10 @@ __device__ __forceinline__ Plane plane_from_depth_normal(float x, float y, float
 // instead of flat loads (which also tie up the LDS counter).
 #if defined(__HIP_DEVICE_COMPILE__)
 #define PM_GLOBAL __attribute__((address_space(1)))
+#define PM_CONST __attribute__((address_space(4)))
 #else
 #define PM_GLOBAL
+#define PM_CONST
 #endif
 template <typename T>
 __device__ __forceinline__ const PM_GLOBAL T* gp(const T* p) { return (const PM_GLOBAL T*)(p); }
@@ -165,17 +167,74 @@ template <typename T>
 __device__ __forceinline__ PM_GLOBAL T* gpw(T* p) { return (PM_GLOBAL T*)(p); }
 
 // Image access on the padded layout.
+// Byte offsets inside one view are 32-bit (a view is < 2^29 texels), so a
+// load is a uniform 64-bit base in SGPRs plus a 32-bit per-lane offset, and
+// the two horizontally adjacent texels of a bilinear row come in one load.
 template <typename T>
-__device__ __forceinline__ float ld(const PM_GLOBAL T* p) { return static_cast<float>(*p); }
+__device__ __forceinline__ const PM_GLOBAL char* view_base(const T* img, const PmView& v) {
+  return reinterpret_cast<const PM_GLOBAL char*>(gp(img) + v.base);
+}
 
-// Offsets inside one view are 32-bit (views are < 2^31 texels), so a load
-// is a uniform 64-bit base in SGPRs plus a 32-bit per-lane offset.
+__device__ __forceinline__ void ld_pair(const PM_GLOBAL char* p, float& a, float& b, float) {
+  float2 t;
+  __builtin_memcpy(&t, p, 8);
+  a = t.x;
+  b = t.y;
+}
+__device__ __forceinline__ void ld_pair(const PM_GLOBAL char* p, float& a, float& b, _Float16) {
+  _Float16 t[2];
+  __builtin_memcpy(t, p, 4);
+  a = static_cast<float>(t[0]);
+  b = static_cast<float>(t[1]);
+}
+
 template <typename T>
 __device__ __forceinline__ float texel(const T* __restrict__ img, const PmView& v, int ix, int iy) {
   ix = min(max(ix, 0), v.w - 1);
   iy = min(max(iy, 0), v.h - 1);
-  const PM_GLOBAL T* vb = gp(img) + v.base;
-  return ld(vb + static_cast<uint32_t>((iy + 1) * v.pitch + (ix + 1)));
+  const uint32_t o = (__umul24(static_cast<uint32_t>(iy + 1), static_cast<uint32_t>(v.pitch)) +
+                      static_cast<uint32_t>(ix + 1)) * static_cast<uint32_t>(sizeof(T));
+  T t;
+  __builtin_memcpy(&t, view_base(img, v) + o, sizeof(T));
+  return static_cast<float>(t);
+}
+
+// A view prepared for repeated bilinear sampling (wave-uniform fields).
+struct Sampler {
+  const PM_GLOBAL char* base;
+  uint32_t row_bytes;   // pitch * sizeof(T)
+  float row_bytes_f;    // the same, as float (exact: < 2^24)
+  float origin_f;       // byte offset of texel (0, 0): (pitch + 1) * sizeof(T)
+  float xmax, ymax;     // w - 1, h - 1
+};
+
+template <typename T>
+__device__ __forceinline__ Sampler make_sampler(const T* img, const PmView& v) {
+  const uint32_t rb = static_cast<uint32_t>(v.pitch) * static_cast<uint32_t>(sizeof(T));
+  return Sampler{view_base(img, v), rb, static_cast<float>(rb),
+                 static_cast<float>((v.pitch + 1) * static_cast<int>(sizeof(T))), static_cast<float>(v.w - 1),
+                 static_cast<float>(v.h - 1)};
+}
+
+// Same value as bilinear() below, with fewer instructions: the clamp is one
+// med3 per axis (coordinates reaching here are finite), and the byte offset
+// of the top-left texel is formed in float, exact because every term is an
+// integer below 2^24 (a view is < 16M texels).
+template <typename T>
+__device__ __forceinline__ float sample(const Sampler& v, float sx, float sy) {
+  sx = __builtin_amdgcn_fmed3f(sx, -1.0f, v.xmax);
+  sy = __builtin_amdgcn_fmed3f(sy, -1.0f, v.ymax);
+  const float fx0 = floorf(sx), fy0 = floorf(sy);
+  const float ax = sx - fx0, ay = sy - fy0;
+  const float fo = fmaf(fy0, v.row_bytes_f, fmaf(fx0, static_cast<float>(sizeof(T)), v.origin_f));
+  const uint32_t o0 = static_cast<uint32_t>(fo);
+  const uint32_t o1 = o0 + v.row_bytes;
+  float t00, t01, t10, t11;
+  ld_pair(v.base + o0, t00, t01, T{});
+  ld_pair(v.base + o1, t10, t11, T{});
+  const float v0 = fmaf(ax, t01 - t00, t00);
+  const float v1 = fmaf(ax, t11 - t10, t10);
+  return fmaf(ay, v1 - v0, v0);
 }
 
 template <typename T>
@@ -185,11 +244,13 @@ __device__ __forceinline__ float bilinear(const T* __restrict__ img, const PmVie
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
   const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-  const PM_GLOBAL T* vb = gp(img) + v.base;
-  const uint32_t o0 = static_cast<uint32_t>((y0 + 1) * v.pitch + (x0 + 1));
-  const uint32_t o1 = o0 + static_cast<uint32_t>(v.pitch);
-  const float t00 = ld(vb + o0), t01 = ld(vb + (o0 + 1u));
-  const float t10 = ld(vb + o1), t11 = ld(vb + (o1 + 1u));
+  const PM_GLOBAL char* vb = view_base(img, v);
+  const uint32_t o0 = (__umul24(static_cast<uint32_t>(y0 + 1), static_cast<uint32_t>(v.pitch)) +
+                       static_cast<uint32_t>(x0 + 1)) * static_cast<uint32_t>(sizeof(T));
+  const uint32_t o1 = o0 + static_cast<uint32_t>(v.pitch) * static_cast<uint32_t>(sizeof(T));
+  float t00, t01, t10, t11;
+  ld_pair(vb + o0, t00, t01, T{});
+  ld_pair(vb + o1, t10, t11, T{});
   const float v0 = fmaf(ax, t01 - t00, t00);
   const float v1 = fmaf(ax, t11 - t10, t10);
   return fmaf(ay, v1 - v0, v0);

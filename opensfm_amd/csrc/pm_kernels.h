@@ -20,6 +20,21 @@ struct PmArgs {
   const float* prev_depths;    // n_src * W * H, may be null
   const float4* snapshot;      // planes before the prior re-init launch
   const void* images;          // padded views, float or _Float16
+  float2* ref_taps;            // precomputed reference taps (w, w*r), strides 1..3
+  float4* ref_sums;            // precomputed reference moments, strides 1..3
+  int64_t ref_plane;           // entries per tap plane (= 2 * H * ceil(W/2))
+  // Camera and view tables in device memory (index 0 = reference).  Read
+  // through the constant address space, so wave-uniform indices become
+  // scalar loads; per-lane indices become vector loads.
+  const PmCam* cams;
+  const PmView* views;
+  // Colour-pass intermediate state, indexed [slot][cidx] with stride ref_plane.
+  float* ca;                   // per-view costs, (9 * n_src) slots
+  int* cpos;                   // candidate pixel of each of the 8 slots (-1: none)
+  float* vwb;                  // view weights, n_src slots
+  float* wnb;                  // sum of the view weights
+  float4* refp;                // refinement hypotheses, 6 slots
+  int* nref;                   // number of refinement hypotheses (5 or 6)
   uint32_t prev_depth_mask;
   int width, height, n_src;
   float depth_min, depth_max;
@@ -32,12 +47,7 @@ struct PmArgs {
   int anchor_views;
   int color;
   int iteration;
-  // Per-view descriptors travel in the kernel-argument segment, so every
-  // camera / view read in the kernels is a uniform scalar load.
-  PmCam cams[kMaxSrc + 1];     // index 0 = reference
-  PmView views[kMaxSrc + 1];
 };
-static_assert(sizeof(PmArgs) <= 4096, "kernel argument segment");
 
 struct PmDecodeOut {
   float* depth;
@@ -47,6 +57,7 @@ struct PmDecodeOut {
 };
 
 // All launchers enqueue on `stream` and return the launch status.
+hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_sweep(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);

@@ -21,104 +21,307 @@ constexpr int kWave = 64;
 constexpr int kTileX = 16;
 constexpr int kTileY = 4;
 constexpr int kSweepSlots = 9;   // 8 propagation candidates + current/refinement
+constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
 
-template <int HP>
-struct RefPatch {
-  static constexpr int P = (2 * HP + 1) * (2 * HP + 1);
-  float center, sum_w, sum_wr, sum_wrr;
-  float w[P];
-  float wr[P];
-};
+// Tile of this workgroup.  The dispatcher hands workgroups to the XCDs
+// round-robin (linear id mod 8), so consecutive ids land on different L2s;
+// remapping id -> tile gives every XCD one contiguous band of tiles, whose
+// source-image footprint stays resident in that XCD's L2.  The grid is 1-D
+// with kXcds * ceil(tiles / kXcds) workgroups.
+__device__ __forceinline__ bool tile_of(int tiles_x, int tiles_y, int& tx, int& ty) {
+  const int ntiles = tiles_x * tiles_y;
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int b = static_cast<int>(blockIdx.x);
+  const int t = (b % kXcds) * per + b / kXcds;
+  if (t >= ntiles) return false;
+  tx = t % tiles_x;
+  ty = t / tiles_x;
+  return true;
+}
+
+__device__ __forceinline__ bool full_tile_pixel(int width, int height, int& x, int& y) {
+  int tx, ty;
+  if (!tile_of((width + kTileX - 1) / kTileX, (height + kTileY - 1) / kTileY, tx, ty)) return false;
+  x = tx * kTileX + static_cast<int>(threadIdx.x % kTileX);
+  y = ty * kTileY + static_cast<int>(threadIdx.x / kTileX);
+  return x < width && y < height;
+}
 
 __device__ __forceinline__ Plane to_plane(float4 v) { return Plane{v.x, v.y, v.z, v.w}; }
 __device__ __forceinline__ float4 to_f4(Plane p) { return make_float4(p.x, p.y, p.z, p.w); }
+// Camera / view i for a wave-uniform i (scalar loads from the constant space).
+__device__ __forceinline__ PmCam kcam(const PmArgs& a, int i) {
+  return ((const PM_CONST PmCam*)(a.cams))[i];
+}
+__device__ __forceinline__ PmView kview(const PmArgs& a, int i) {
+  return ((const PM_CONST PmView*)(a.views))[i];
+}
+template <typename T>
+__device__ __forceinline__ const T* images_of(const PmArgs& a) { return static_cast<const T*>(a.images); }
 
-// opencl_kernels.h:328-366
-template <int HP, typename T>
-__device__ __forceinline__ void compute_ref_patch(const T* __restrict__ img, const PmView& rv, int x, int y,
-                                                  float sf, float cf, RefPatch<HP>& rp) {
-  const float center = texel(img, rv, x, y);
-  float sum_w = 0.0f, sum_wr = 0.0f, sum_wrr = 0.0f;
-  int p = 0;
-#pragma unroll
-  for (int iy = -HP; iy <= HP; ++iy) {
-#pragma unroll
-    for (int ix = -HP; ix <= HP; ++ix) {
-      const float r = texel(img, rv, x + ix, y + iy) - center;
-      const float w = pm_expf(sf * static_cast<float>(ix * ix + iy * iy) + cf * r * r);
-      sum_w += w;
-      const float wr = w * r;
-      sum_wr += wr;
-      sum_wrr = fmaf(wr, r, sum_wrr);
-      rp.w[p] = w;
-      rp.wr[p] = wr;
-      ++p;
-    }
-  }
-  rp.center = center;
-  rp.sum_w = sum_w;
-  rp.sum_wr = sum_wr;
-  rp.sum_wrr = sum_wrr;
+// ---------------------------------------------------------------------------
+// Reference-side patch data.  The bilateral weights w and the products w*r
+// of every tap, and the reference moments, depend only on the reference
+// image, the pixel and the sampling stride, so they are computed once per
+// run for strides 1, 2 and 3 and kept in HBM (about 1.2 KB per pixel):
+//   ref_taps[(s * P + p) * ref_plane + cidx] = (w, w*r)     s = stride - 1
+//   ref_sums[s * ref_plane + cidx] = (centre, sum_w, sum_wr, sum_wrr)
+// cidx is the colour-major pixel index, so the 64 lanes of a colour-pass
+// wave read consecutive entries.  Values are exactly those the reference
+// recomputes for every hypothesis (opencl_kernels.h:328-366 for stride 1,
+// the reference half of :216-244 for strides 2 and 3).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int colour_index(const PmArgs& a, int x, int y) {
+  const int hw = (a.width + 1) / 2;
+  return ((x + y) & 1) * (a.height * hw) + y * hw + (x >> 1);
 }
 
-// opencl_kernels.h:188-258
 template <int HP, typename T>
-__device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const T* __restrict__ img, int s,
-                                                         float ref_center, float src_center, float scx, float scy,
-                                                         int cx, int cy, float stride, float jxx, float jxy,
-                                                         float jyx, float jyy) {
-  const PmView rv = a.views[0];
-  const PmView sv = a.views[s + 1];
-  float sum_w = 0.0f, sum_r = 0.0f, sum_s = 0.0f, sum_rr = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+__global__ __launch_bounds__(kWave) void refpatch_kernel(PmArgs a) {
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  const int lane = threadIdx.x;
+  int x, y;
+  if (!full_tile_pixel(a.width, a.height, x, y)) return;
+  const T* img = images_of<T>(a);
+  const PmView rv = kview(a, 0);
+  const int cidx = colour_index(a, x, y);
+  const float sf = a.spatial_factor, cf = a.color_factor;
+  const float center = texel(img, rv, x, y);
+  PM_GLOBAL float2* taps = gpw(a.ref_taps);
+  PM_GLOBAL float4* sums = gpw(a.ref_sums);
+  {  // stride 1 (opencl_kernels.h:340-365)
+    float sum_w = 0.0f, sum_wr = 0.0f, sum_wrr = 0.0f;
+    int p = 0;
+    for (int iy = -HP; iy <= HP; ++iy) {
+      for (int ix = -HP; ix <= HP; ++ix) {
+        const float r = texel(img, rv, x + ix, y + iy) - center;
+        const float w = pm_expf(sf * static_cast<float>(ix * ix + iy * iy) + cf * r * r);
+        sum_w += w;
+        const float wr = w * r;
+        sum_wr += wr;
+        sum_wrr = fmaf(wr, r, sum_wrr);
+        taps[static_cast<int64_t>(p) * a.ref_plane + cidx] = make_float2(w, wr);
+        ++p;
+      }
+    }
+    sums[cidx] = make_float4(center, sum_w, sum_wr, sum_wrr);
+  }
+  for (int si = 1; si < 3; ++si) {  // strides 2 and 3 (opencl_kernels.h:216-244)
+    const float stride = static_cast<float>(si + 1);
+    float sum_w = 0.0f, sum_r = 0.0f, sum_rr = 0.0f;
+    int p = 0;
+    for (int iy = -HP; iy <= HP; ++iy) {
+      for (int ix = -HP; ix <= HP; ++ix) {
+        const float fdx = stride * static_cast<float>(ix);
+        const float fdy = stride * static_cast<float>(iy);
+        const float rx = static_cast<float>(x) + fdx;
+        const float ry = static_cast<float>(y) + fdy;
+        const float r = texel(img, rv, static_cast<int>(rx), static_cast<int>(ry)) - center;
+        const float w = pm_expf(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
+        const float wr = w * r;
+        sum_r += wr;
+        sum_rr += wr * r;
+        sum_w += w;
+        taps[(static_cast<int64_t>(si) * P + p) * a.ref_plane + cidx] = make_float2(w, wr);
+        ++p;
+      }
+    }
+    sums[static_cast<int64_t>(si) * a.ref_plane + cidx] = make_float4(center, sum_w, sum_r, sum_rr);
+  }
+}
+
+// Camera / view of a per-lane (divergent) source index: read from the
+// device copies instead of the kernel arguments.
+__device__ __forceinline__ PmView lane_view(const PmArgs& a, int s) { return gp(a.views)[s + 1]; }
+__device__ __forceinline__ PmCam lane_cam(const PmArgs& a, int s) { return gp(a.cams)[s + 1]; }
+
+// Warp of one source view for a plane (opencl_kernels.h:432-461):
+// false = invalid (cost 2.0).
+struct Warp { float scx, scy, jxx, jxy, jyx, jyy; };
+
+__device__ __forceinline__ bool warp_of(V3 cw, V3 wdx, V3 wdy, const PmCam& cs, Warp& w) {
+  const V3 pc = project(cw, cs);
+  if (pc.z < 1e-6f) return false;
+  w.scx = pc.x / pc.z;
+  w.scy = pc.y / pc.z;
+  const V3 pdx = project(wdx, cs);
+  const V3 pdy = project(wdy, cs);
+  if (pdx.z < 1e-6f || pdy.z < 1e-6f) return false;
+  w.jxx = pdx.x / pdx.z - w.scx;
+  w.jxy = pdx.y / pdx.z - w.scy;
+  w.jyx = pdy.x / pdy.z - w.scx;
+  w.jyy = pdy.y / pdy.z - w.scy;
+  const float det = w.jxx * w.jyy - w.jxy * w.jyx;
+  return !(det < 0.02f || det > 50.0f);
+}
+
+// The three reference points of a plane at (x, y) (opencl_kernels.h:401-416).
+struct PlanePts { V3 cw, wdx, wdy; };
+
+__device__ __forceinline__ bool plane_points(const PmCam& c0, int x, int y, Plane pl, PlanePts& pp) {
+  const float fx = static_cast<float>(x), fy = static_cast<float>(y);
+  const float depth = depth_from_plane(fx, fy, pl, c0);
+  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), fy, pl, c0);
+  const float depth_dy = depth_from_plane(fx, static_cast<float>(y + 1), pl, c0);
+  if (depth <= 0.0f || depth_dx <= 0.0f || depth_dy <= 0.0f) return false;
+  pp.cw = backproject(fx, fy, depth, c0);
+  pp.wdx = backproject(static_cast<float>(x + 1), fy, depth_dx, c0);
+  pp.wdy = backproject(fx, static_cast<float>(y + 1), depth_dy, c0);
+  return true;
+}
+
+// Stride-1 per-view costs of one plane (opencl_kernels.h:381-557 without
+// the textureless fallback).  Views run in chunks of kViewChunk inside the
+// tap loop, so each tap's (w, w*r) is loaded once per chunk; each view
+// still accumulates its taps in row-major order.  out[s * os] gets the
+// cost, or -- for views whose NCC is <= 0.1 -- the stride-1 NCC, with bit s
+// set in the returned pending mask for resolve_fallbacks().
+constexpr int kViewChunk = 4;
+
+template <int HP, typename T>
+__device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
+                                                Plane pl, float* out, int64_t os) {
+  constexpr int D = 2 * HP + 1;
+  const int n_src = a.n_src;
+  PlanePts pp;
+  if (!plane_points(kcam(a, 0), x, y, pl, pp)) {
+    for (int s = 0; s < n_src; ++s) out[s * os] = 2.0f;
+    return 0u;
+  }
+  const float4 rs = gp(a.ref_sums)[cidx];
+  const bool ref_ok = rs.y >= 1e-6f;
+  float inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
+  if (ref_ok) {
+    inv_w = 1.0f / rs.y;
+    mean_r = rs.z * inv_w;
+    var_ref = rs.w * inv_w - mean_r * mean_r;
+  }
+  const int64_t plane_stride = a.ref_plane;
+  const uint32_t ucidx = static_cast<uint32_t>(cidx);
+  uint32_t pending = 0u;
+#pragma unroll 1
+  for (int c0v = 0; c0v < n_src; c0v += kViewChunk) {
+    // Views of the chunk past n_src, or with an invalid warp, are sampled
+    // anyway (coordinates are clamped, so every access stays inside the
+    // view) and their sums discarded: the tap loop stays branch-free and
+    // the loads of all views and taps of a row can be in flight together.
+    Warp wp[kViewChunk];
+    Sampler sm[kViewChunk];
+    float scen[kViewChunk], s_ws[kViewChunk], s_wss[kViewChunk], s_wrs[kViewChunk];
+    bool ok[kViewChunk];
+#pragma unroll
+    for (int k = 0; k < kViewChunk; ++k) {
+      const int s = c0v + k;
+      const int sv = s < n_src ? s : n_src - 1;
+      ok[k] = s < n_src && warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, sv + 1), wp[k]) && ref_ok;
+      if (!ok[k]) wp[k] = Warp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      sm[k] = make_sampler(img, kview(a, sv + 1));
+      scen[k] = sample<T>(sm[k], wp[k].scx, wp[k].scy);
+      s_ws[k] = 0.0f; s_wss[k] = 0.0f; s_wrs[k] = 0.0f;
+    }
+#pragma unroll 1
+    for (int iy = -HP; iy <= HP; ++iy) {
+      const float fiy = static_cast<float>(iy);
+      float bx[kViewChunk], by[kViewChunk];
+#pragma unroll
+      for (int k = 0; k < kViewChunk; ++k) {
+        bx[k] = fmaf(wp[k].jyx, fiy, wp[k].scx);
+        by[k] = fmaf(wp[k].jyy, fiy, wp[k].scy);
+      }
+      const PM_GLOBAL float2* trow = gp(a.ref_taps) + static_cast<int64_t>((iy + HP) * D) * plane_stride;
+#pragma unroll
+      for (int ix = -HP; ix <= HP; ++ix) {
+        const float fix = static_cast<float>(ix);
+        const PM_GLOBAL char* tp = reinterpret_cast<const PM_GLOBAL char*>(trow + static_cast<int64_t>(ix + HP) * plane_stride);
+        const float2 wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + (ucidx << 3));
+#pragma unroll
+        for (int k = 0; k < kViewChunk; ++k) {
+          const float sx = fmaf(wp[k].jxx, fix, bx[k]);
+          const float sy = fmaf(wp[k].jxy, fix, by[k]);
+          const float sd = sample<T>(sm[k], sx, sy) - scen[k];
+          const float wsv = wv.x * sd;
+          s_ws[k] += wsv;
+          s_wss[k] = fmaf(wsv, sd, s_wss[k]);
+          s_wrs[k] = fmaf(wv.y, sd, s_wrs[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kViewChunk; ++k) {
+      const int s = c0v + k;
+      if (s >= n_src) continue;
+      float cost = 2.0f;
+      if (ok[k]) {
+        float ncc1 = -1.0f;
+        if (var_ref >= a.var_threshold) {
+          const float mean_s = s_ws[k] * inv_w;
+          const float var_src = s_wss[k] * inv_w - mean_s * mean_s;
+          if (var_src >= a.var_threshold) {
+            const float covar = s_wrs[k] * inv_w - mean_r * mean_s;
+            ncc1 = covar / sqrtf(var_ref * var_src);
+          }
+        }
+        if (ncc1 > 0.1f) {
+          cost = 1.0f - ncc1;
+        } else {
+          cost = ncc1;
+          pending |= 1u << s;
+        }
+      }
+      out[s * os] = cost;
+    }
+  }
+  return pending;
+}
+
+// Strided ZNCC on precomputed reference data (opencl_kernels.h:188-258).
+template <int HP, typename T>
+__device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const T* __restrict__ img, const PmView& sv, int si,
+                                               int cidx, float src_center, const Warp& w) {
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  const float stride = static_cast<float>(si + 1);
+  const PM_GLOBAL float2* taps = gp(a.ref_taps) + (static_cast<int64_t>(si) * P) * a.ref_plane + cidx;
+  float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+  int p = 0;
   for (int iy = -HP; iy <= HP; ++iy) {
     for (int ix = -HP; ix <= HP; ++ix) {
       const float fdx = stride * static_cast<float>(ix);
       const float fdy = stride * static_cast<float>(iy);
-      const float rx = static_cast<float>(cx) + fdx;
-      const float ry = static_cast<float>(cy) + fdy;
-      const float ref_val = texel(img, rv, static_cast<int>(rx), static_cast<int>(ry));
-      const float sx = scx + jxx * fdx + jyx * fdy;
-      const float sy = scy + jxy * fdx + jyy * fdy;
-      const float src_val = bilinear(img, sv, sx, sy);
-      const float r = ref_val - ref_center;
-      const float w = pm_expf(fdx * fdx * a.spatial_factor + fdy * fdy * a.spatial_factor +
-                              r * r * a.color_factor);
-      const float sd = src_val - src_center;
-      sum_r += w * r;
-      sum_s += w * sd;
-      sum_rr += w * r * r;
-      sum_ss += w * sd * sd;
-      sum_rs += w * r * sd;
-      sum_w += w;
+      const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
+      const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
+      const float sd = bilinear(img, sv, sx, sy) - src_center;
+      const float2 wv = taps[static_cast<int64_t>(p) * a.ref_plane];
+      sum_s += wv.x * sd;
+      sum_ss += wv.x * sd * sd;
+      sum_rs += wv.y * sd;
+      ++p;
     }
   }
-  if (sum_w < 1e-6f) return -1.0f;
-  const float inv_w = 1.0f / sum_w;
-  const float mean_r = sum_r * inv_w;
+  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+  if (rs.y < 1e-6f) return -1.0f;
+  const float inv_w = 1.0f / rs.y;
+  const float mean_r = rs.z * inv_w;
   const float mean_s = sum_s * inv_w;
-  const float var_ref = sum_rr * inv_w - mean_r * mean_r;
+  const float var_ref = rs.w * inv_w - mean_r * mean_r;
   const float var_src = sum_ss * inv_w - mean_s * mean_s;
   if (var_ref < a.var_threshold || var_src < a.var_threshold) return -1.0f;
   const float covar = sum_rs * inv_w - mean_r * mean_s;
   return covar / sqrtf(var_ref * var_src);
 }
 
-// opencl_kernels.h:270-315
+// Census fallback (opencl_kernels.h:270-315).
 template <typename T>
-__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, int s,
-                                                        float ref_center, float src_center, int cx, int cy,
-                                                        float scx, float scy, float jxx, float jxy, float jyx,
-                                                        float jyy) {
-  const PmView rv = a.views[0];
-  const PmView sv = a.views[s + 1];
+__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, const PmView& sv,
+                                              float ref_center, float src_center, int cx, int cy, const Warp& w) {
+  const PmView rv = kview(a, 0);
   uint32_t rc = 0u, sc = 0u;
   int bit = 0;
   for (int dy = -2; dy <= 2; ++dy) {
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
       const float ref_val = texel(img, rv, cx + dx, cy + dy);
-      const float sx = scx + jxx * static_cast<float>(dx) + jyx * static_cast<float>(dy);
-      const float sy = scy + jxy * static_cast<float>(dx) + jyy * static_cast<float>(dy);
+      const float sx = w.scx + w.jxx * static_cast<float>(dx) + w.jyx * static_cast<float>(dy);
+      const float sy = w.scy + w.jxy * static_cast<float>(dx) + w.jyy * static_cast<float>(dy);
       const float src_val = bilinear(img, sv, sx, sy);
       if (ref_val > ref_center) rc |= (1u << bit);
       if (src_val > src_center) sc |= (1u << bit);
@@ -130,107 +333,50 @@ __device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restri
   return static_cast<float>(hamming) / 24.0f;
 }
 
-// Per-view cost vector of one plane at (x, y): opencl_kernels.h:381-557.
-// Writes n_src costs to out[s * ostride].  Views are the outer loop; each
-// view accumulates its taps in row-major order, as in the reference.
+// Textureless fallback of the views left pending by cost_vector()
+// (opencl_kernels.h:531-553): strides 2 then 3 (stop once NCC > 0.1), then
+// census or 2.0.  Each lane walks its own pending views, so the wave loops
+// max(pending views per lane) times instead of once per view that any lane
+// needs; the view index is per lane, so its camera comes from device memory.
 template <int HP, typename T>
-__device__ __forceinline__ void cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, Plane pl,
-                                            const RefPatch<HP>& rp, float* out, int ostride) {
-  const int n_src = a.n_src;
-  const PmCam& c0 = a.cams[0];
-  const float fx = static_cast<float>(x), fy = static_cast<float>(y);
-  const float depth = depth_from_plane(fx, fy, pl, c0);
-  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), fy, pl, c0);
-  const float depth_dy = depth_from_plane(fx, static_cast<float>(y + 1), pl, c0);
-  if (depth <= 0.0f || depth_dx <= 0.0f || depth_dy <= 0.0f) {  // :402-412
-    for (int s = 0; s < n_src; ++s) out[s * ostride] = 2.0f;
-    return;
-  }
-  const V3 cw = backproject(fx, fy, depth, c0);
-  const V3 wdx = backproject(static_cast<float>(x + 1), fy, depth_dx, c0);
-  const V3 wdy = backproject(fx, static_cast<float>(y + 1), depth_dy, c0);
-
-  const bool ref_ok = rp.sum_w >= 1e-6f;
-  float inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
-  if (ref_ok) {
-    inv_w = 1.0f / rp.sum_w;
-    mean_r = rp.sum_wr * inv_w;
-    var_ref = rp.sum_wrr * inv_w - mean_r * mean_r;
-  }
-
-  for (int s = 0; s < n_src; ++s) {
-    const PmCam& cs = a.cams[s + 1];
-    const PmView sv = a.views[s + 1];
-    float cost = 2.0f;
-    const V3 pc = project(cw, cs);
-    if (!(pc.z < 1e-6f)) {
-      const float scx = pc.x / pc.z;
-      const float scy = pc.y / pc.z;
-      const V3 pdx = project(wdx, cs);
-      const V3 pdy = project(wdy, cs);
-      if (!(pdx.z < 1e-6f || pdy.z < 1e-6f)) {
-        const float jxx = pdx.x / pdx.z - scx;
-        const float jxy = pdx.y / pdx.z - scy;
-        const float jyx = pdy.x / pdy.z - scx;
-        const float jyy = pdy.y / pdy.z - scy;
-        const float det = jxx * jyy - jxy * jyx;
-        if (!(det < 0.02f || det > 50.0f) && ref_ok) {
-          const float scen = bilinear(img, sv, scx, scy);
-          // Fused stride-1 ZNCC over the cached reference weights (:471-499).
-          float sum_ws = 0.0f, sum_wss = 0.0f, sum_wrs = 0.0f;
-          int p = 0;
-#pragma unroll
-          for (int iy = -HP; iy <= HP; ++iy) {
-            const float fiy = static_cast<float>(iy);
-            const float bx = fmaf(jyx, fiy, scx);
-            const float by = fmaf(jyy, fiy, scy);
-#pragma unroll
-            for (int ix = -HP; ix <= HP; ++ix) {
-              const float fix = static_cast<float>(ix);
-              const float sx = fmaf(jxx, fix, bx);
-              const float sy = fmaf(jxy, fix, by);
-              const float sd = bilinear(img, sv, sx, sy) - scen;
-              const float wsv = rp.w[p] * sd;
-              sum_ws += wsv;
-              sum_wss = fmaf(wsv, sd, sum_wss);
-              sum_wrs = fmaf(rp.wr[p], sd, sum_wrs);
-              ++p;
-            }
-          }
-          float ncc1 = -1.0f;
-          if (var_ref >= a.var_threshold) {
-            const float mean_s = sum_ws * inv_w;
-            const float var_src = sum_wss * inv_w - mean_s * mean_s;
-            if (var_src >= a.var_threshold) {
-              const float covar = sum_wrs * inv_w - mean_r * mean_s;
-              ncc1 = covar / sqrtf(var_ref * var_src);
-            }
-          }
-          if (ncc1 > 0.1f) {
-            cost = 1.0f - ncc1;
-          } else {
-            // Textureless fallback: strides 2 then 3, stop once NCC > 0.1 (:532-542).
-            float best = ncc1;
+__device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
+                                                  Plane pl, float* out, int64_t os, uint32_t pending) {
+  PlanePts pp{};
+  if (pending) plane_points(kcam(a, 0), x, y, pl, pp);
+  const float ref_center = gp(a.ref_sums)[cidx].x;
+  while (__any(pending != 0u)) {
+    if (pending) {
+      const int s = __ffs(pending) - 1;
+      pending &= pending - 1u;
+      const PmCam cs = lane_cam(a, s);
+      const PmView sv = lane_view(a, s);
+      Warp w;
+      warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
+      const float scen = bilinear(img, sv, w.scx, w.scy);
+      float best = out[s * os];  // stride-1 NCC
 #pragma unroll 1
-            for (int st = 0; st < 2; ++st) {
-              const float ncc = ncc_at_stride<HP>(a, img, s, rp.center, scen, scx, scy, x, y,
-                                                  st == 0 ? 2.0f : 3.0f, jxx, jxy, jyx, jyy);
-              if (ncc > best) best = ncc;
-              if (best > 0.1f) break;
-            }
-            if (best <= -0.5f) {
-              cost = (a.census_weight > 1e-6f)
-                         ? census_inner(a, img, s, rp.center, scen, x, y, scx, scy, jxx, jxy, jyx, jyy)
-                         : 2.0f;
-            } else {
-              cost = 1.0f - best;
-            }
-          }
-        }
+      for (int si = 1; si < 3; ++si) {
+        const float ncc = ncc_at_stride<HP>(a, img, sv, si, cidx, scen, w);
+        if (ncc > best) best = ncc;
+        if (best > 0.1f) break;
       }
+      float cost;
+      if (best <= -0.5f) {
+        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sv, ref_center, scen, x, y, w) : 2.0f;
+      } else {
+        cost = 1.0f - best;
+      }
+      out[s * os] = cost;
     }
-    out[s * ostride] = cost;
   }
+}
+
+// Full cost vector: stride 1 for all views, then the fallbacks.
+template <int HP, typename T>
+__device__ __forceinline__ void full_cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
+                                                 Plane pl, float* out, int64_t os = kWave) {
+  const uint32_t pending = cost_vector<HP>(a, img, x, y, cidx, pl, out, os);
+  resolve_fallbacks<HP>(a, img, x, y, cidx, pl, out, os, pending);
 }
 
 // Smoothness against the 4 neighbours (opencl_kernels.h:649-707); the
@@ -242,7 +388,7 @@ struct Nbrs {
 };
 
 __device__ __forceinline__ float smoothness_cost(const PmArgs& a, const Nbrs& nb, Plane cur, int x, int y) {
-  const PmCam& c0 = a.cams[0];
+  const PmCam c0 = kcam(a, 0);
   const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
   if (cur_depth <= 0.0f) return 0.0f;
   const V3 cur_n = normalize3(V3{cur.x, cur.y, cur.z});
@@ -288,7 +434,7 @@ __device__ __forceinline__ V3 surface_normal(const PmArgs& a, const Nbrs& nb, in
   const int W = a.width, H = a.height;
   if (center_depth <= 0.0f) return V3{0, 0, 0};
   if (x < 1 || x >= W - 1 || y < 1 || y >= H - 1) return V3{0, 0, 0};
-  const PmCam& c0 = a.cams[0];
+  const PmCam c0 = kcam(a, 0);
   const float fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
   const float dl = depth_from_plane(static_cast<float>(x - 1), static_cast<float>(y), nb.p[0], c0);
   const float dr = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), nb.p[1], c0);
@@ -308,8 +454,8 @@ __device__ __forceinline__ V3 surface_normal(const PmArgs& a, const Nbrs& nb, in
 // opencl_kernels.h:779-861
 __device__ __forceinline__ float geom_cost(const PmArgs& a, int sidx, Plane plane, int px, int py) {
   const int pdw = a.width, pdh = a.height;
-  const PmCam& rc = a.cams[0];
-  const PmCam& scm = a.cams[sidx + 1];
+  const PmCam rc = kcam(a, 0);
+  const PmCam scm = lane_cam(a, sidx);
   const float max_dim = static_cast<float>(max(pdw, pdh));
   if (max_dim < 1.0f) return 1.0f;
   const float good_thresh = 3.0f / max_dim;
@@ -338,14 +484,16 @@ __device__ __forceinline__ float geom_cost(const PmArgs& a, int sidx, Plane plan
 }
 
 // opencl_kernels.h:910-969; cv is this lane's LDS column (stride kWave).
-__device__ __forceinline__ float mc_cost(const PmArgs& a, const float* cv, const float (&vw)[kMaxSrc], float wnorm,
-                                         Plane plane, int x, int y, uint32_t& s0, uint32_t& s1, uint32_t key) {
+template <int NV>
+__device__ __forceinline__ float mc_cost(const PmArgs& a, const float* cv, int64_t cs, const float (&vw)[NV],
+                                         float wnorm, Plane plane, int x, int y, uint32_t& s0, uint32_t& s1,
+                                         uint32_t key) {
   if (wnorm < 1e-6f) return 2.0f;
   const int n_src = a.n_src;
-  float cdf[kMaxSrc];
+  float cdf[NV];
   float cum = 0.0f;
 #pragma unroll
-  for (int i = 0; i < kMaxSrc; ++i) {
+  for (int i = 0; i < NV; ++i) {
     if (i < n_src) cum += vw[i];
     cdf[i] = cum;
   }
@@ -357,9 +505,9 @@ __device__ __forceinline__ float mc_cost(const PmArgs& a, const float* cv, const
     const float thr = u * cum;
     int v = n_src - 1;
 #pragma unroll
-    for (int i = kMaxSrc - 1; i >= 0; --i)
+    for (int i = NV - 1; i >= 0; --i)
       if (i < n_src && cdf[i] > thr) v = i;
-    const float photo = cv[v * kWave];
+    const float photo = cv[v * cs];
     float geom = 0.0f;
     if (use_geom && (a.prev_depth_mask & (1u << v))) geom = geom_cost(a, v, plane, x, y);
     total += photo + gw * geom;
@@ -413,9 +561,6 @@ __device__ __forceinline__ float initial_cost_and_views(const PmArgs& a, const f
   return total / static_cast<float>(k);
 }
 
-template <typename T>
-__device__ __forceinline__ const T* images_of(const PmArgs& a) { return static_cast<const T*>(a.images); }
-
 // ---------------------------------------------------------------------------
 // acmmp_random_init (opencl_kernels.h:979-1039)
 // ---------------------------------------------------------------------------
@@ -423,9 +568,8 @@ template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x;
-  const int x = blockIdx.x * kTileX + (lane % kTileX);
-  const int y = blockIdx.y * kTileY + (lane / kTileX);
-  if (x >= a.width || y >= a.height) return;
+  int x, y;
+  if (!full_tile_pixel(a.width, a.height, x, y)) return;
   const T* img = images_of<T>(a);
   const int idx = y * a.width + x;
   uint32_t s0 = static_cast<uint32_t>(idx) * 1099087573u + 2654435769u;
@@ -439,14 +583,12 @@ __global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) {
   n.y = rand_float(s0, s1, key + 5u) * 2.0f - 1.0f;
   n.z = -1.0f;
   n = normalize3(n);
-  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, a.cams[0]);
+  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, kcam(a, 0));
   a.planes[idx] = to_f4(plane);
   a.rng[idx] = make_uint2(s0, s1);
-  RefPatch<HP> rp;
-  compute_ref_patch<HP>(img, a.views[0], x, y, a.spatial_factor, a.color_factor, rp);
   float* cv = smem + lane;
   float* tmp = smem + a.n_src * kWave + lane;
-  cost_vector<HP>(a, img, x, y, plane, rp, cv, kWave);
+  full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), plane, cv);
   uint32_t sel = 0u;
   a.costs[idx] = initial_cost_and_views(a, cv, tmp, &sel);
   a.sel[idx] = sel;
@@ -460,13 +602,12 @@ template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x;
-  const int x = blockIdx.x * kTileX + (lane % kTileX);
-  const int y = blockIdx.y * kTileY + (lane / kTileX);
-  if (x >= a.width || y >= a.height) return;
+  int x, y;
+  if (!full_tile_pixel(a.width, a.height, x, y)) return;
   const int idx = y * a.width + x;
   if (a.plane_masks[idx] == 0u) return;
   const T* img = images_of<T>(a);
-  const PmCam& c0 = a.cams[0];
+  const PmCam c0 = kcam(a, 0);
   const Plane pp = to_plane(a.prior_planes[idx]);
   const float dprior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
   if (dprior <= 0.0f || dprior < a.depth_min || dprior > a.depth_max) return;
@@ -485,14 +626,12 @@ __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) {
   for (int i = 0; i < kMaxSrc; ++i) {
     if (i < n_src && (cur_sel & (1u << i))) { vw[i] = 1.0f; wnorm += 1.0f; } else { vw[i] = 0.0f; }
   }
-  RefPatch<HP> rp;
-  compute_ref_patch<HP>(img, a.views[0], x, y, a.spatial_factor, a.color_factor, rp);
   Nbrs nb;
   load_nbrs(a, a.snapshot, x, y, nb);
   const float sw = a.smooth_w;
   float* cv = smem + lane;
   float* tmp = smem + n_src * kWave + lane;
-  cost_vector<HP>(a, img, x, y, plane, rp, cv, kWave);
+  full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), plane, cv);
   if (wnorm < 1e-6f) {
     uint32_t sel = 0u;
     float cost = initial_cost_and_views(a, cv, tmp, &sel);
@@ -514,351 +653,397 @@ __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// acmmp_patchmatch, one colour (opencl_kernels.h:1197-1778)
-//
-// The 15 plane hypotheses of a pixel (8 propagated candidates, the current
-// plane, up to 6 refinements) run through ONE loop with one inlined copy of
-// the cost function, which keeps the kernel small enough for the
-// instruction cache.  Evaluating the current plane's cost vector together
-// with the candidates is equivalent to the reference order: cost vectors
-// consume no random numbers, so the RNG stream (candidate MC draws, then
-// the current plane's, then the refinement draws) is unchanged.
+// acmmp_patchmatch, one colour (opencl_kernels.h:1197-1778), as four kernels:
+//   pass_candidates  cost vectors of the 8 propagated candidates and of the
+//                    current plane, one lane per (pixel, hypothesis);
+//   pass_select      view weights, Monte-Carlo costs, acceptance of the best
+//                    candidate, refinement hypotheses (one lane per pixel);
+//   pass_refine_cost cost vectors of the refinement hypotheses, one lane per
+//                    (pixel, hypothesis);
+//   pass_refine      their Monte-Carlo costs and acceptance (lane per pixel).
+// Cost vectors draw no random numbers, so computing them ahead of the MC
+// steps keeps the reference's RNG stream: candidate MC draws, current-plane
+// MC draws, refinement set-up draws, refinement MC draws.  Per-pixel
+// intermediate state lives in HBM, indexed by the colour-major index.
 // ---------------------------------------------------------------------------
-// Occupancy target of the colour-pass kernel (waves per SIMD); the register
-// allocator trims hoisting to fit.
-#ifndef PM_SWEEP_WAVES_PER_EU
-#define PM_SWEEP_WAVES_PER_EU 1
-#endif
+struct ColourPixel {
+  int x, y, idx, cidx;
+};
+
+__device__ __forceinline__ bool colour_pixel(const PmArgs& a, ColourPixel& cp) {
+  const int hw = (a.width + 1) / 2;
+  int tx, ty;
+  if (!tile_of((hw + kTileX - 1) / kTileX, (a.height + kTileY - 1) / kTileY, tx, ty)) return false;
+  const int gx = tx * kTileX + static_cast<int>(threadIdx.x % kTileX);
+  const int y = ty * kTileY + static_cast<int>(threadIdx.x / kTileX);
+  if (gx >= hw || y >= a.height) return false;
+  const int x = gx * 2 + ((y + a.color) & 1);
+  if (x >= a.width) return false;
+  cp.x = x;
+  cp.y = y;
+  cp.idx = y * a.width + x;
+  cp.cidx = colour_index(a, x, y);
+  return true;
+}
+
+// Candidate slot h of pixel (x, y) (opencl_kernels.h:1254-1430): the
+// cheapest-cost probe of that search pattern, or -1 when the pattern does
+// not fit in the image.  0 near up, 1 far up, 2 near down, 3 far down,
+// 4 near left, 5 far left, 6 near right, 7 far right.
+__device__ __forceinline__ int candidate_position(const PmArgs& a, int h, int x, int y) {
+  const int W = a.width, H = a.height;
+  const PM_GLOBAL float* costs = gp(a.costs);
+  auto cost_at = [&](int px, int py) { return costs[py * W + px]; };
+  switch (h) {
+    case 1:
+      if (y > 2) {
+        int by = y - 3; float bc = cost_at(x, by);
+        for (int i = 1; i < 11; ++i) { const int yy = y - 3 - 2 * i; if (yy < 0) break;
+          const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
+        return by * W + x;
+      }
+      return -1;
+    case 3:
+      if (y < H - 3) {
+        int by = y + 3; float bc = cost_at(x, by);
+        for (int i = 1; i < 11; ++i) { const int yy = y + 3 + 2 * i; if (yy >= H) break;
+          const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
+        return by * W + x;
+      }
+      return -1;
+    case 5:
+      if (x > 2) {
+        int bx = x - 3; float bc = cost_at(bx, y);
+        for (int i = 1; i < 11; ++i) { const int xx = x - 3 - 2 * i; if (xx < 0) break;
+          const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
+        return y * W + bx;
+      }
+      return -1;
+    case 7:
+      if (x < W - 3) {
+        int bx = x + 3; float bc = cost_at(bx, y);
+        for (int i = 1; i < 11; ++i) { const int xx = x + 3 + 2 * i; if (xx >= W) break;
+          const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
+        return y * W + bx;
+      }
+      return -1;
+    case 0:
+      if (y > 0) {
+        int bx = x, by = y - 1; float bc = cost_at(bx, by);
+        for (int i = 0; i < 3; ++i) {
+          if (y > 1 + i && x > i) { const int px = x - i - 1, py = y - 2 - i; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+          if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+        }
+        return by * W + bx;
+      }
+      return -1;
+    case 2:
+      if (y < H - 1) {
+        int bx = x, by = y + 1; float bc = cost_at(bx, by);
+        for (int i = 0; i < 3; ++i) {
+          if (y < H - 2 - i && x > i) { const int px = x - i - 1, py = y + 2 + i; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+          if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+        }
+        return by * W + bx;
+      }
+      return -1;
+    case 4:
+      if (x > 0) {
+        int bx = x - 1, by = y; float bc = cost_at(bx, by);
+        for (int i = 0; i < 3; ++i) {
+          if (x > 1 + i && y > i) { const int px = x - 2 - i, py = y - i - 1; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+          if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+        }
+        return by * W + bx;
+      }
+      return -1;
+    default:  // 6
+      if (x < W - 1) {
+        int bx = x + 1, by = y; float bc = cost_at(bx, by);
+        for (int i = 0; i < 3; ++i) {
+          if (x < W - 2 - i && y > i) { const int px = x + 2 + i, py = y - i - 1; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+          if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const float c = cost_at(px, py);
+            if (c < bc) { bc = c; bx = px; by = py; } }
+        }
+        return by * W + bx;
+      }
+      return -1;
+  }
+}
 
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PM_SWEEP_WAVES_PER_EU)))
-void sweep_kernel(PmArgs a) {
-  extern __shared__ float smem[];
-  const int lane = threadIdx.x;
-  const int W = a.width, H = a.height;
-  const int gx = blockIdx.x * kTileX + (lane % kTileX);
-  const int y = blockIdx.y * kTileY + (lane / kTileX);
-  const int half_width = (W + 1) / 2;
-  if (gx >= half_width || y >= H) return;
-  const int x = gx * 2 + ((y + a.color) & 1);
-  if (x >= W) return;
-  const T* img = images_of<T>(a);
-  const PmCam& c0 = a.cams[0];
+__global__ __launch_bounds__(kWave * 9) void pass_candidates_kernel(PmArgs a) {
+  ColourPixel cp;
+  if (!colour_pixel(a, cp)) return;
+  const int h = threadIdx.y;  // 0..7 candidates, 8 = current plane (one wave each)
+  const int pos = h == 8 ? cp.idx : candidate_position(a, h, cp.x, cp.y);
+  if (h < 8) gpw(a.cpos)[static_cast<int64_t>(h) * a.ref_plane + cp.cidx] = pos;
+  if (pos < 0) return;
+  const Plane hyp = to_plane(gp(a.planes)[pos]);
+  float* out = a.ca + static_cast<int64_t>(h) * a.n_src * a.ref_plane + cp.cidx;
+  full_cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+}
+
+template <int NV>
+__global__ __launch_bounds__(kWave) void pass_select_kernel(PmArgs a) {
+  ColourPixel cp;
+  if (!colour_pixel(a, cp)) return;
+  const int x = cp.x, y = cp.y, idx = cp.idx;
+  const int64_t pl = a.ref_plane;
+  const PmCam c0 = kcam(a, 0);
   const int n_src = a.n_src;
-  const int idx = y * W + x;
   const uint2 st = gp(a.rng)[idx];
   uint32_t s0 = st.x, s1 = st.y;
   const uint32_t key = static_cast<uint32_t>(idx + a.iteration * 7919 + 42);
-  // Other-colour state is read only; this pixel's own entries are written
-  // once at the end.
   const PM_GLOBAL float4* planes = gp(a.planes);
-  const PM_GLOBAL float* costs = gp(a.costs);
-  float* ca = smem + lane;  // ca[(slot * n_src + view) * kWave]
-  int* cpos = reinterpret_cast<int*>(smem + kSweepSlots * n_src * kWave) + lane;  // cpos[slot * kWave]
-
-  RefPatch<HP> rp;
-  compute_ref_patch<HP>(img, a.views[0], x, y, a.spatial_factor, a.color_factor, rp);
-
-  // Step 1: candidate positions (:1254-1430), cheapest-cost probes on the
-  // other colour.  Slot layout 0=near up, 1=far up, 2=near down, 3=far down,
-  // 4=near left, 5=far left, 6=near right, 7=far right.
+  const float* ca = a.ca + cp.cidx;  // ca[(slot * n_src + view) * pl]
+  int cpos[8];
   uint32_t flags = 0u;
-  auto cost_at = [&](int px, int py) { return costs[py * W + px]; };
-  auto put = [&](int slot, int px, int py) {
-    cpos[slot * kWave] = py * W + px;
-    flags |= 1u << slot;
-  };
-  if (y > 2) {
-    int by = y - 3; float bc = cost_at(x, by);
-    for (int i = 1; i < 11; ++i) { const int yy = y - 3 - 2 * i; if (yy < 0) break;
-      const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
-    put(1, x, by);
+#pragma unroll
+  for (int h = 0; h < 8; ++h) {
+    cpos[h] = gp(a.cpos)[h * pl + cp.cidx];
+    if (cpos[h] >= 0) flags |= 1u << h;
   }
-  if (y < H - 3) {
-    int by = y + 3; float bc = cost_at(x, by);
-    for (int i = 1; i < 11; ++i) { const int yy = y + 3 + 2 * i; if (yy >= H) break;
-      const float c = cost_at(x, yy); if (c < bc) { bc = c; by = yy; } }
-    put(3, x, by);
-  }
-  if (x > 2) {
-    int bx = x - 3; float bc = cost_at(bx, y);
-    for (int i = 1; i < 11; ++i) { const int xx = x - 3 - 2 * i; if (xx < 0) break;
-      const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
-    put(5, bx, y);
-  }
-  if (x < W - 3) {
-    int bx = x + 3; float bc = cost_at(bx, y);
-    for (int i = 1; i < 11; ++i) { const int xx = x + 3 + 2 * i; if (xx >= W) break;
-      const float c = cost_at(xx, y); if (c < bc) { bc = c; bx = xx; } }
-    put(7, bx, y);
-  }
-  if (y > 0) {
-    int bx = x, by = y - 1; float bc = cost_at(bx, by);
-    for (int i = 0; i < 3; ++i) {
-      if (y > 1 + i && x > i) { const int px = x - i - 1, py = y - 2 - i; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
-      if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
+  const uint32_t prev_sel = gp(a.sel)[idx];
+  Plane plane_now = to_plane(planes[idx]);
+
+  // Step 2: view weights (:1452-1563).
+  float vw[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) vw[i] = 0.0f;
+  const float vthr = 0.8f;
+  for (int j = 0; j < 8; ++j) {  // per view the sum runs over j = 0..7, as in the reference
+    if (!((flags >> j) & 1u)) continue;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      if (i < n_src) {
+        const float c = ca[(j * n_src + i) * pl];
+        if (c < vthr) vw[i] += (1.0f - c);
+      }
     }
-    put(0, bx, by);
   }
-  if (y < H - 1) {
-    int bx = x, by = y + 1; float bc = cost_at(bx, by);
-    for (int i = 0; i < 3; ++i) {
-      if (y < H - 2 - i && x > i) { const int px = x - i - 1, py = y + 2 + i; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
-      if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
+  if (a.anchor_views > 0) {
+    int anchored = 0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      if (i < n_src && anchored < a.anchor_views) {
+        if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
+      }
     }
-    put(2, bx, by);
   }
-  if (x > 0) {
-    int bx = x - 1, by = y; float bc = cost_at(bx, by);
-    for (int i = 0; i < 3; ++i) {
-      if (x > 1 + i && y > i) { const int px = x - 2 - i, py = y - i - 1; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
-      if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
+  {
+    V3 cur_n{plane_now.x, plane_now.y, plane_now.z};
+    const float nl = len3(cur_n);
+    const bool have_normal = nl > 0.1f;
+    if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
+    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
+    const float inc_sigma = 0.6f;
+    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      if (i >= n_src || vw[i] < 1e-6f) continue;
+      const PmView v = kview(a, i + 1);
+      vw[i] *= v.angle_w;
+      if (have_normal && cur_depth > 0.0f) {
+        const V3 ray = normalize3(V3{v.cx - p3.x, v.cy - p3.y, v.cz - p3.z});
+        const float cos_inc = fabsf(dot3(cur_n, ray));
+        const float xi = 1.0f - cos_inc;
+        vw[i] *= pm_expf(xi * xi * inv2s2);
+      }
     }
-    put(4, bx, by);
   }
-  if (x < W - 1) {
-    int bx = x + 1, by = y; float bc = cost_at(bx, by);
-    for (int i = 0; i < 3; ++i) {
-      if (x < W - 2 - i && y > i) { const int px = x + 2 + i, py = y - i - 1; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
-      if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const float c = cost_at(px, py);
-        if (c < bc) { bc = c; bx = px; by = py; } }
+  uint32_t new_sel = 0u;
+  float wnorm = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    if (i < n_src && vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
+  if (wnorm < 1e-6f) {  // :1552-1563
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      if (i >= n_src) continue;
+      bool any = false;
+      for (int j = 0; j < 8; ++j)
+        any |= ((flags >> j) & 1u) && ca[(j * n_src + i) * pl] < 2.0f;
+      if (any) { vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; }
     }
-    put(6, bx, by);
   }
 
   Nbrs nb;
   load_nbrs(a, a.planes, x, y, nb);
   const float smooth_w = a.smooth_w;
-  const uint32_t prev_sel = gp(a.sel)[idx];
-  Plane plane_now = to_plane(planes[idx]);
-  float cost_now = 2.0f, depth_now = 0.0f;
-  uint32_t sel_out = prev_sel;
 
-  float vw[kMaxSrc];
-#pragma unroll
-  for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
-  float wnorm = 0.0f;
-  // refinement hypotheses (filled after the current plane is scored)
-  float depth_rand = 0.0f, depth_pert = 0.0f, depth0 = 0.0f;
-  V3 cur_normal{0, 0, 0}, nrand{0, 0, 0}, npert{0, 0, 0}, sn{0, 0, 0};
-  int n_ref = 0;
-
-#pragma unroll 1
-  for (int h = 0; h < 9 + 6; ++h) {
-    Plane hyp;
-    bool active;
-    int slot;
-    if (h < 8) {
-      active = (flags >> h) & 1u;
-      slot = h;
-      hyp = active ? to_plane(planes[cpos[h * kWave]]) : Plane{0, 0, 0, 0};
-    } else if (h == 8) {
-      active = true;
-      slot = 8;
-      hyp = plane_now;
-    } else {
-      const int r = h - 9;
-      active = r < n_ref;
-      slot = 8;
-      float rd;
-      V3 rn;
-      switch (r) {  // :1733-1741
-        case 0: rd = depth_rand; rn = cur_normal; break;
-        case 1: rd = depth0; rn = nrand; break;
-        case 2: rd = depth_rand; rn = nrand; break;
-        case 3: rd = depth0; rn = npert; break;
-        case 4: rd = depth_pert; rn = cur_normal; break;
-        default: rd = depth0; rn = sn; break;
-      }
-      hyp = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd, rn, c0);
+  // Step 3: Monte-Carlo costs of the candidates, then of the current plane
+  // (:1570-1619).
+  float best_cost = 2.0f, cost_now = 2.0f;
+  int best = 0;
+  for (int i = 0; i < 9; ++i) {
+    const bool flagged = i == 8 || ((flags >> i) & 1u);
+    float fc = 2.0f;
+    if (flagged) {
+      const Plane cand = i == 8 ? plane_now : to_plane(planes[cpos[i]]);
+      const uint32_t k2 = i == 8 ? key + 700u : key + 600u + static_cast<uint32_t>(i) * 31u;
+      fc = mc_cost<NV>(a, ca + static_cast<int64_t>(i) * n_src * pl, pl, vw, wnorm, cand, x, y, s0, s1, k2);
+      if (smooth_w > 1e-6f) fc += smooth_w * smoothness_cost(a, nb, cand, x, y);
     }
-    if (active) cost_vector<HP>(a, img, x, y, hyp, rp, ca + slot * n_src * kWave, kWave);
+    if (i == 8) {
+      cost_now = fc;
+    } else if (i == 0) {
+      best_cost = fc;
+    } else if (fc < best_cost) {
+      best_cost = fc;
+      best = i;
+    }
+  }
+  float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
 
-    if (h == 8) {
-      // Step 2: view weights (:1452-1563).
-      const float vthr = 0.8f;
-#pragma unroll
-      for (int i = 0; i < kMaxSrc; ++i) {
-        if (i < n_src) {
-          float vc = 0.0f;
-          for (int j = 0; j < 8; ++j) {
-            const float c = ca[(j * n_src + i) * kWave];
-            if (((flags >> j) & 1u) && c < vthr) vc += (1.0f - c);
-          }
-          if (vc > 0.0f) vw[i] = vc;
-        }
-      }
-      if (a.anchor_views > 0) {
-        int anchored = 0;
-#pragma unroll
-        for (int i = 0; i < kMaxSrc; ++i) {
-          if (i < n_src && anchored < a.anchor_views) {
-            if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
-          }
-        }
-      }
-      {
-        V3 cur_n{plane_now.x, plane_now.y, plane_now.z};
-        const float nl = len3(cur_n);
-        const bool have_normal = nl > 0.1f;
-        if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
-        const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
-        const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
-        const float inc_sigma = 0.6f;
-        const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
-#pragma unroll
-        for (int i = 0; i < kMaxSrc; ++i) {
-          if (i >= n_src || vw[i] < 1e-6f) continue;
-          const PmView& v = a.views[i + 1];
-          vw[i] *= v.angle_w;
-          if (have_normal && cur_depth > 0.0f) {
-            const V3 ray = normalize3(V3{v.cx - p3.x, v.cy - p3.y, v.cz - p3.z});
-            const float cos_inc = fabsf(dot3(cur_n, ray));
-            const float xi = 1.0f - cos_inc;
-            vw[i] *= pm_expf(xi * xi * inv2s2);
-          }
-        }
-      }
-      uint32_t new_sel = 0u;
-#pragma unroll
-      for (int i = 0; i < kMaxSrc; ++i)
-        if (i < n_src && vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
-      if (wnorm < 1e-6f) {
-#pragma unroll
-        for (int i = 0; i < kMaxSrc; ++i) {
-          if (i >= n_src) continue;
-          for (int j = 0; j < 8; ++j) {
-            if (((flags >> j) & 1u) && ca[(j * n_src + i) * kWave] < 2.0f) {
-              vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
-            }
-          }
-        }
-      }
+  const bool have_prior = a.plane_masks != nullptr && gp(a.plane_masks)[idx] > 0u;
+  float depth_prior = 0.0f;
+  V3 prior_n{0.0f, 0.0f, -1.0f};
+  const float depth_sigma = (a.depth_max - a.depth_min) / 32.0f;
+  const float angle_sigma = kPiF * (5.0f / 180.0f);
+  if (have_prior) {
+    const Plane pp = to_plane(gp(a.prior_planes)[idx]);
+    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+    prior_n = normalize3(V3{pp.x, pp.y, pp.z});
+  }
 
-      // Step 3: Monte-Carlo costs of the candidates, then of the current
-      // plane (:1570-1619).
-      float best_cost = 2.0f;
-      int best = 0;
-#pragma unroll 1
-      for (int i = 0; i < 9; ++i) {
-        const bool flagged = i == 8 || ((flags >> i) & 1u);
-        float fc = 2.0f;
-        if (flagged) {
-          const Plane cp = i == 8 ? plane_now : to_plane(planes[cpos[i * kWave]]);
-          const uint32_t k2 = i == 8 ? key + 700u : key + 600u + static_cast<uint32_t>(i) * 31u;
-          fc = mc_cost(a, ca + i * n_src * kWave, vw, wnorm, cp, x, y, s0, s1, k2);
-          if (smooth_w > 1e-6f) fc += smooth_w * smoothness_cost(a, nb, cp, x, y);
-        }
-        if (i == 8) {
-          cost_now = fc;
-        } else if (i == 0) {
-          best_cost = fc;
-        } else if (fc < best_cost) {
-          best_cost = fc;
-          best = i;
-        }
-      }
-      depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+  // Step 3b (:1643-1656).
+  uint32_t sel_out = prev_sel;
+  if (((flags >> best) & 1u) && best_cost < cost_now) {
+    const Plane bp = to_plane(planes[cpos[best]]);
+    const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
+    if (db >= a.depth_min && db <= a.depth_max) {
+      depth_now = db;
+      plane_now = bp;
+      cost_now = best_cost;
+      sel_out = new_sel;
+    }
+  }
 
-      const bool have_prior = a.plane_masks != nullptr && gp(a.plane_masks)[idx] > 0u;
-      float depth_prior = 0.0f;
-      V3 prior_n{0.0f, 0.0f, -1.0f};
-      const float depth_sigma = (a.depth_max - a.depth_min) / 32.0f;
-      const float angle_sigma = kPiF * (5.0f / 180.0f);
-      if (have_prior) {
-        const Plane pp = to_plane(gp(a.prior_planes)[idx]);
-        depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
-        prior_n = normalize3(V3{pp.x, pp.y, pp.z});
-      }
+  // Step 4 set-up: the refinement hypotheses (:1669-1743).
+  const float depth_perturbation = 0.005f;
+  const float normal_perturbation = 0.01f * kPiF;
+  const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
+  float depth_rand, depth_pert;
+  V3 nrand, npert;
+  if (have_prior && depth_prior > 0.0f) {
+    depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(s0, s1, key + 503u) * 6.0f * depth_sigma;
+    depth_rand = clampf(depth_rand, a.depth_min, a.depth_max);
+    const float pa1 = (rand_float(s0, s1, key + 504u) - 0.5f) * angle_sigma;
+    const float pa2 = (rand_float(s0, s1, key + 505u) - 0.5f) * angle_sigma;
+    nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
+    if (nrand.z > 0.0f) nrand.z = -nrand.z;
+  } else {
+    const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
+    const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
+    depth_rand = pm_expf(ldmin + rand_float(s0, s1, key + 503u) * (ldmax - ldmin));
+    nrand.x = rand_float(s0, s1, key + 504u) * 2.0f - 1.0f;
+    nrand.y = rand_float(s0, s1, key + 505u) * 2.0f - 1.0f;
+    nrand.z = -1.0f;
+    nrand = normalize3(nrand);
+  }
+  {
+    const float dlo = (1.0f - depth_perturbation) * depth_now;
+    const float dhi = (1.0f + depth_perturbation) * depth_now;
+    depth_pert = dlo + rand_float(s0, s1, key + 510u) * (dhi - dlo);
+    depth_pert = clampf(depth_pert, a.depth_min, a.depth_max);
+  }
+  {
+    const float pa1 = (rand_float(s0, s1, key + 520u) - 0.5f) * normal_perturbation;
+    const float pa2 = (rand_float(s0, s1, key + 521u) - 0.5f) * normal_perturbation;
+    npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
+    if (npert.z > 0.0f) npert.z = -npert.z;
+  }
+  const V3 sn = surface_normal(a, nb, x, y, depth_now);
+  const int n_ref = len3(sn) > 0.5f ? 6 : 5;
+  // The hypotheses are fixed here, before any of them is scored (:1733-1741).
+  const float rd[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
+  const V3 rn[6] = {cur_normal, nrand, nrand, npert, cur_normal, sn};
+  PM_GLOBAL float4* refp = gpw(a.refp) + cp.cidx;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+    if (r < n_ref)
+      refp[r * pl] = to_f4(plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[r], rn[r], c0));
+  gpw(a.nref)[cp.cidx] = n_ref;
+  PM_GLOBAL float* vwb = gpw(a.vwb) + cp.cidx;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    if (i < n_src) vwb[i * pl] = vw[i];
+  gpw(a.wnb)[cp.cidx] = wnorm;
+  gpw(a.planes)[idx] = to_f4(plane_now);
+  gpw(a.costs)[idx] = cost_now;
+  gpw(a.sel)[idx] = sel_out;
+  gpw(a.rng)[idx] = make_uint2(s0, s1);
+}
 
-      // Step 3b (:1643-1656).
-      if (((flags >> best) & 1u) && best_cost < cost_now) {
-        const Plane bp = to_plane(planes[cpos[best * kWave]]);
-        const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
-        if (db >= a.depth_min && db <= a.depth_max) {
-          depth_now = db;
-          plane_now = bp;
-          cost_now = best_cost;
-          sel_out = new_sel;
-        }
-      }
+template <int HP, typename T>
+__global__ __launch_bounds__(kWave * 6) void pass_refine_cost_kernel(PmArgs a) {
+  ColourPixel cp;
+  if (!colour_pixel(a, cp)) return;
+  const int r = threadIdx.y;  // one wave per refinement hypothesis
+  if (r >= gp(a.nref)[cp.cidx]) return;
+  const Plane hyp = to_plane(gp(a.refp)[r * a.ref_plane + cp.cidx]);
+  float* out = a.ca + static_cast<int64_t>(r) * a.n_src * a.ref_plane + cp.cidx;
+  full_cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+}
 
-      // Step 4 set-up: the refinement hypotheses (:1669-1743).
-      const float depth_perturbation = 0.005f;
-      const float normal_perturbation = 0.01f * kPiF;
-      cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
-      if (have_prior && depth_prior > 0.0f) {
-        depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(s0, s1, key + 503u) * 6.0f * depth_sigma;
-        depth_rand = clampf(depth_rand, a.depth_min, a.depth_max);
-        const float pa1 = (rand_float(s0, s1, key + 504u) - 0.5f) * angle_sigma;
-        const float pa2 = (rand_float(s0, s1, key + 505u) - 0.5f) * angle_sigma;
-        nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
-        if (nrand.z > 0.0f) nrand.z = -nrand.z;
-      } else {
-        const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
-        const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
-        depth_rand = pm_expf(ldmin + rand_float(s0, s1, key + 503u) * (ldmax - ldmin));
-        nrand.x = rand_float(s0, s1, key + 504u) * 2.0f - 1.0f;
-        nrand.y = rand_float(s0, s1, key + 505u) * 2.0f - 1.0f;
-        nrand.z = -1.0f;
-        nrand = normalize3(nrand);
-      }
-      {
-        const float dlo = (1.0f - depth_perturbation) * depth_now;
-        const float dhi = (1.0f + depth_perturbation) * depth_now;
-        depth_pert = dlo + rand_float(s0, s1, key + 510u) * (dhi - dlo);
-        depth_pert = clampf(depth_pert, a.depth_min, a.depth_max);
-      }
-      {
-        const float pa1 = (rand_float(s0, s1, key + 520u) - 0.5f) * normal_perturbation;
-        const float pa2 = (rand_float(s0, s1, key + 521u) - 0.5f) * normal_perturbation;
-        npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
-        if (npert.z > 0.0f) npert.z = -npert.z;
-      }
-      sn = surface_normal(a, nb, x, y, depth_now);
-      n_ref = len3(sn) > 0.5f ? 6 : 5;
-      depth0 = depth_now;
-    } else if (h > 8 && active) {
-      // Step 4: score one refinement hypothesis (:1744-1771).
-      float tc = mc_cost(a, ca + 8 * n_src * kWave, vw, wnorm, hyp, x, y, s0, s1,
-                         key + 800u + static_cast<uint32_t>(h - 9) * 37u);
-      if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(a, nb, hyp, x, y);
-      const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), hyp, c0);
-      if (!(db < a.depth_min || db > a.depth_max) && tc < cost_now) {
-        depth_now = db;
-        plane_now = hyp;
-        cost_now = tc;
-      }
+template <int NV>
+__global__ __launch_bounds__(kWave) void pass_refine_kernel(PmArgs a) {
+  ColourPixel cp;
+  if (!colour_pixel(a, cp)) return;
+  const int x = cp.x, y = cp.y, idx = cp.idx;
+  const int64_t pl = a.ref_plane;
+  const PmCam c0 = kcam(a, 0);
+  const uint2 st = gp(a.rng)[idx];
+  uint32_t s0 = st.x, s1 = st.y;
+  const uint32_t key = static_cast<uint32_t>(idx + a.iteration * 7919 + 42);
+  Plane plane_now = to_plane(gp(a.planes)[idx]);
+  float cost_now = gp(a.costs)[idx];
+  float vw[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) vw[i] = i < a.n_src ? gp(a.vwb)[i * pl + cp.cidx] : 0.0f;
+  const float wnorm = gp(a.wnb)[cp.cidx];
+  const int n_ref = gp(a.nref)[cp.cidx];
+  Nbrs nb;
+  load_nbrs(a, a.planes, x, y, nb);
+  const float smooth_w = a.smooth_w;
+  const float* ca = a.ca + cp.cidx;
+  // Step 4: score the refinement hypotheses in order (:1744-1771).
+  for (int r = 0; r < n_ref; ++r) {
+    const Plane hyp = to_plane(gp(a.refp)[r * pl + cp.cidx]);
+    float tc = mc_cost<NV>(a, ca + static_cast<int64_t>(r) * a.n_src * pl, pl, vw, wnorm, hyp, x, y, s0, s1,
+                           key + 800u + static_cast<uint32_t>(r) * 37u);
+    if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(a, nb, hyp, x, y);
+    const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), hyp, c0);
+    if (!(db < a.depth_min || db > a.depth_max) && tc < cost_now) {
+      plane_now = hyp;
+      cost_now = tc;
     }
   }
   gpw(a.planes)[idx] = to_f4(plane_now);
   gpw(a.costs)[idx] = cost_now;
   gpw(a.rng)[idx] = make_uint2(s0, s1);
-  gpw(a.sel)[idx] = sel_out;
 }
 
 // ---------------------------------------------------------------------------
 // acmmp_checkerboard_filter, one colour (opencl_kernels.h:1883-2039)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void filter_kernel(PmArgs a) {
-  const int lane = threadIdx.x;
   const int W = a.width, H = a.height;
-  const int gx = blockIdx.x * kTileX + (lane % kTileX);
-  const int y = blockIdx.y * kTileY + (lane / kTileX);
-  const int half_width = (W + 1) / 2;
-  if (gx >= half_width || y >= H) return;
-  const int x = gx * 2 + ((y + a.color) & 1);
-  if (x >= W) return;
-  const int center = y * W + x;
+  ColourPixel cp;
+  if (!colour_pixel(a, cp)) return;
+  const int x = cp.x, y = cp.y;
+  const int center = cp.idx;
   if (a.costs[center] < 0.001f) return;
-  const PmCam& c0 = a.cams[0];
+  const PmCam c0 = kcam(a, 0);
   float buf[21];
   int n = 0;
   auto dep = [&](int px, int py) {
@@ -920,7 +1105,7 @@ __global__ __launch_bounds__(256) void decode_kernel(PmArgs a, PmDecodeOut o) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= static_cast<int64_t>(W) * H) return;
   const int x = static_cast<int>(idx % W), y = static_cast<int>(idx / W);
-  const PmCam& c0 = a.cams[0];
+  const PmCam c0 = kcam(a, 0);
   const float4 p = a.planes[idx];
   float depth = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, cost = 0.0f, conf = 0.0f;
   if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w)) {
@@ -957,11 +1142,9 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
   if (i >= n) return;
   const T* img = images_of<T>(a);
   const int x = xy[2 * i], y = xy[2 * i + 1];
-  RefPatch<HP> rp;
-  compute_ref_patch<HP>(img, a.views[0], x, y, a.spatial_factor, a.color_factor, rp);
   float* cv = smem + lane;
   float* tmp = smem + a.n_src * kWave + lane;
-  cost_vector<HP>(a, img, x, y, to_plane(pl[i]), rp, cv, kWave);
+  full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), to_plane(pl[i]), cv);
   for (int s = 0; s < a.n_src; ++s) costs[static_cast<int64_t>(i) * a.n_src + s] = cv[s * kWave];
   uint32_t sv = 0u;
   const float ic = initial_cost_and_views(a, cv, tmp, &sv);
@@ -969,15 +1152,33 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
   if (sel) sel[i] = sv;
 }
 
+dim3 xcd_grid(int tiles) { return dim3(static_cast<unsigned>(kXcds * ((tiles + kXcds - 1) / kXcds))); }
 dim3 full_grid(const PmArgs& a) {
-  return dim3((a.width + kTileX - 1) / kTileX, (a.height + kTileY - 1) / kTileY);
+  return xcd_grid(((a.width + kTileX - 1) / kTileX) * ((a.height + kTileY - 1) / kTileY));
 }
 dim3 colour_grid(const PmArgs& a) {
   const int hw = (a.width + 1) / 2;
-  return dim3((hw + kTileX - 1) / kTileX, (a.height + kTileY - 1) / kTileY);
+  return xcd_grid(((hw + kTileX - 1) / kTileX) * ((a.height + kTileY - 1) / kTileY));
 }
 
 }  // namespace
+
+#define PM_DISPATCH_NORET(KERNEL, HP, F16, GRID, BLOCK, LDS, STREAM, ...)                                   \
+  do {                                                                                            \
+    if (F16) {                                                                                    \
+      switch (HP) {                                                                               \
+        case 1: hipLaunchKernelGGL((KERNEL<1, _Float16>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+        case 2: hipLaunchKernelGGL((KERNEL<2, _Float16>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL((KERNEL<3, _Float16>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+      }                                                                                           \
+    } else {                                                                                      \
+      switch (HP) {                                                                               \
+        case 1: hipLaunchKernelGGL((KERNEL<1, float>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+        case 2: hipLaunchKernelGGL((KERNEL<2, float>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL((KERNEL<3, float>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); break; \
+      }                                                                                           \
+    }                                                                                             \
+  } while (0)
 
 #define PM_DISPATCH(KERNEL, HP, F16, GRID, LDS, STREAM, ...)                                         \
   do {                                                                                            \
@@ -999,6 +1200,10 @@ dim3 colour_grid(const PmArgs& a) {
     return hipGetLastError();                                                                     \
   } while (0)
 
+hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
+  PM_DISPATCH(refpatch_kernel, hp, f16, full_grid(a), 0, stream, a);
+}
+
 hipError_t launch_init(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   const size_t lds = sizeof(float) * 2 * a.n_src * kWave;
   PM_DISPATCH(init_kernel, hp, f16, full_grid(a), lds, stream, a);
@@ -1009,9 +1214,20 @@ hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(prior_kernel, hp, f16, full_grid(a), lds, stream, a);
 }
 
+template <int NV>
+hipError_t launch_pass_nv(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
+  const dim3 g = colour_grid(a);
+  PM_DISPATCH_NORET(pass_candidates_kernel, hp, f16, g, dim3(kWave, 9), 0, stream, a);
+  hipLaunchKernelGGL((pass_select_kernel<NV>), g, dim3(kWave), 0, stream, a);
+  PM_DISPATCH_NORET(pass_refine_cost_kernel, hp, f16, g, dim3(kWave, 6), 0, stream, a);
+  hipLaunchKernelGGL((pass_refine_kernel<NV>), g, dim3(kWave), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_sweep(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
-  const size_t lds = sizeof(float) * (kSweepSlots * a.n_src + 8) * kWave;
-  PM_DISPATCH(sweep_kernel, hp, f16, colour_grid(a), lds, stream, a);
+  if (hp < 1 || hp > 3) return hipErrorInvalidValue;
+  if (a.n_src <= 8) return launch_pass_nv<8>(a, hp, f16, stream);
+  return launch_pass_nv<kMaxSrc>(a, hp, f16, stream);
 }
 
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream) {
