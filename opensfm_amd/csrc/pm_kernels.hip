@@ -335,28 +335,69 @@ __device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restri
 
 // Textureless fallback of the views left pending by cost_vector()
 // (opencl_kernels.h:531-553): strides 2 then 3 (stop once NCC > 0.1), then
-// census or 2.0.  Each lane walks its own pending views, so the wave loops
-// max(pending views per lane) times instead of once per view that any lane
-// needs; the view index is per lane, so its camera comes from device memory.
+// census or 2.0.  The wave walks the views in order and the lanes with that
+// view pending work on it together: the view is wave-uniform (scalar
+// descriptors, one source image per gather instruction, neighbouring
+// pixels), which keeps the gathers coherent.
+template <int HP, typename T>
+__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const Sampler& sm, int si, int cidx,
+                                                 float src_center, const Warp& w) {
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  const float stride = static_cast<float>(si + 1);
+  const uint32_t toff = static_cast<uint32_t>(cidx) << 3;
+  float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+  int p = 0;
+#pragma unroll 1
+  for (int iy = -HP; iy <= HP; ++iy) {
+#pragma unroll
+    for (int ix = -HP; ix <= HP; ++ix) {
+      const float fdx = stride * static_cast<float>(ix);
+      const float fdy = stride * static_cast<float>(iy);
+      const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
+      const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
+      const float sd = sample<T>(sm, sx, sy) - src_center;
+      const PM_GLOBAL char* tp = reinterpret_cast<const PM_GLOBAL char*>(
+          gp(a.ref_taps) + (static_cast<int64_t>(si) * P + (p + ix + HP)) * a.ref_plane);
+      const float2 wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + toff);
+      sum_s += wv.x * sd;
+      sum_ss += wv.x * sd * sd;
+      sum_rs += wv.y * sd;
+    }
+    p += 2 * HP + 1;
+  }
+  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+  if (rs.y < 1e-6f) return -1.0f;
+  const float inv_w = 1.0f / rs.y;
+  const float mean_r = rs.z * inv_w;
+  const float mean_s = sum_s * inv_w;
+  const float var_ref = rs.w * inv_w - mean_r * mean_r;
+  const float var_src = sum_ss * inv_w - mean_s * mean_s;
+  if (var_ref < a.var_threshold || var_src < a.var_threshold) return -1.0f;
+  const float covar = sum_rs * inv_w - mean_r * mean_s;
+  return covar / sqrtf(var_ref * var_src);
+}
+
 template <int HP, typename T>
 __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
                                                   Plane pl, float* out, int64_t os, uint32_t pending) {
   PlanePts pp{};
   if (pending) plane_points(kcam(a, 0), x, y, pl, pp);
   const float ref_center = gp(a.ref_sums)[cidx].x;
-  while (__any(pending != 0u)) {
-    if (pending) {
-      const int s = __ffs(pending) - 1;
-      pending &= pending - 1u;
-      const PmCam cs = lane_cam(a, s);
-      const PmView sv = lane_view(a, s);
+  const int n_src = a.n_src;
+#pragma unroll 1
+  for (int s = 0; s < n_src; ++s) {
+    const bool mine = (pending >> s) & 1u;
+    if (!__any(mine)) continue;
+    if (mine) {
+      const PmView sv = kview(a, s + 1);
+      const Sampler sm = make_sampler(img, sv);
       Warp w;
-      warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
-      const float scen = bilinear(img, sv, w.scx, w.scy);
+      warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, s + 1), w);  // valid: it was valid at stride 1
+      const float scen = sample<T>(sm, w.scx, w.scy);
       float best = out[s * os];  // stride-1 NCC
 #pragma unroll 1
       for (int si = 1; si < 3; ++si) {
-        const float ncc = ncc_at_stride<HP>(a, img, sv, si, cidx, scen, w);
+        const float ncc = ncc_at_stride_u<HP, T>(a, sm, si, cidx, scen, w);
         if (ncc > best) best = ncc;
         if (best > 0.1f) break;
       }
