@@ -134,11 +134,15 @@ struct pm_handle {
   int W = 0, H = 0, n_src = 0;
   bool f16 = false;
   DevBuf<unsigned char> images;
+  DevBuf<unsigned char> quads;  // 2 x 2 quad copy of the padded views
   DevBuf<PmView> gviews;       // view table (index 0 = reference)
   DevBuf<PmCam> gcams;         // cameras
   DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
+  DevBuf<uint2> jobs;
+  DevBuf<uint32_t> jobcount;
+  int64_t job_cap = 0;
   DevBuf<float2> ref_taps;     // reference-side patch data, strides 1..3
   DevBuf<float4> ref_sums;
   int64_t ref_plane = 0;
@@ -153,8 +157,9 @@ struct pm_handle {
   int last_sweeps = 0;
 
   ~pm_handle() {
-    images.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
+    images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
     ca.release(); vwb.release(); wnb.release(); cpos.release(); nref.release(); refp.release();
+    jobs.release(); jobcount.release();
     planes.release(); prior.release();
     snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
     out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
@@ -182,6 +187,7 @@ PmArgs make_args(pm_handle* h) {
   a.prev_depth_mask = h->prev_mask;
   a.snapshot = h->snapshot.p;
   a.images = h->images.p;
+  a.quads = h->quads.p;
   a.ref_taps = h->ref_taps.p;
   a.ref_sums = h->ref_sums.p;
   a.ref_plane = h->ref_plane;
@@ -193,6 +199,10 @@ PmArgs make_args(pm_handle* h) {
   a.wnb = h->wnb.p;
   a.refp = h->refp.p;
   a.nref = h->nref.p;
+  a.jobs = h->jobs.p;
+  a.jobcount = h->jobcount.p;
+  a.job_cap = h->job_cap;
+  a.phase = 0;
   a.width = h->W;
   a.height = h->H;
   a.n_src = h->n_src;
@@ -452,9 +462,11 @@ int pm_upload(pm_handle* h) {
 
   // Padded images (1-texel replicated border = clamp-to-edge).
   std::vector<PmView> vt(static_cast<size_t>(nv));
-  int64_t total = 0;
+  int64_t total = 0, qtotal = 0;
   for (int i = 0; i < nv; ++i) {
     const HostView& v = h->views[static_cast<size_t>(i)];
+    vt[i].qbase = qtotal;
+    qtotal += static_cast<int64_t>(v.w + 2) * (v.h + 2);
     vt[i].base = total;
     vt[i].w = v.w;
     vt[i].h = v.h;
@@ -509,6 +521,7 @@ int pm_upload(pm_handle* h) {
   }
   const size_t npix = static_cast<size_t>(h->W) * h->H;
   if (int r = h->images.alloc(host.size())) return r;
+  if (int r = h->quads.alloc(static_cast<size_t>(qtotal) * 4 * esz)) return r;
   if (int r = h->gviews.alloc(vt.size())) return r;
   if (int r = h->gcams.alloc(cams.size())) return r;
   h->ref_plane = 2 * static_cast<int64_t>(h->H) * ((h->W + 1) / 2);
@@ -522,6 +535,9 @@ int pm_upload(pm_handle* h) {
   if (int r = h->cpos.alloc(8 * rp)) return r;
   if (int r = h->nref.alloc(rp)) return r;
   if (int r = h->refp.alloc(6 * rp)) return r;
+  h->job_cap = fallback_queue_capacity(h->W, h->H, h->n_src);
+  if (int r = h->jobs.alloc(8 * static_cast<size_t>(h->job_cap))) return r;
+  if (int r = h->jobcount.alloc(8)) return r;
   h->ref_ready = false;
   if (int r = h->planes.alloc(npix)) return r;
   if (int r = h->costs.alloc(npix)) return r;
@@ -536,6 +552,13 @@ int pm_upload(pm_handle* h) {
          "upload view table");
   PM_HIP(hipMemcpyAsync(h->gcams.p, cams.data(), cams.size() * sizeof(PmCam), hipMemcpyHostToDevice, h->stream),
          "upload cameras");
+  {
+    PmArgs qa{};
+    qa.images = h->images.p;
+    qa.quads = h->quads.p;
+    qa.views = h->gviews.p;
+    PM_HIP(launch_quads(qa, nv, h->f16, h->stream), "quad kernel launch");
+  }
   // Prior (estimator.cc:480-486) and previous depths (:499-508, 66-116).
   if (!h->prior_masks.empty()) {
     if (int r = h->prior.alloc(npix)) return r;

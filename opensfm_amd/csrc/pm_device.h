@@ -32,6 +32,7 @@ static_assert(sizeof(PmCam) == 92, "camera layout");
 // base + (y + 1) * pitch + (x + 1), pitch = w + 2.
 struct PmView {
   int64_t base;      // element offset of texel (-1, -1)
+  int64_t qbase;     // the same in the quad copy (see Sampler)
   int w, h, pitch;
   float cx, cy, cz;  // camera centre -R^T t (sources; opencl_kernels.h:1523-1525)
   float angle_w;     // estimator.cc:380-407
@@ -166,26 +167,11 @@ __device__ __forceinline__ const PM_GLOBAL T* gp(const T* p) { return (const PM_
 template <typename T>
 __device__ __forceinline__ PM_GLOBAL T* gpw(T* p) { return (PM_GLOBAL T*)(p); }
 
-// Image access on the padded layout.
-// Byte offsets inside one view are 32-bit (a view is < 2^29 texels), so a
-// load is a uniform 64-bit base in SGPRs plus a 32-bit per-lane offset, and
-// the two horizontally adjacent texels of a bilinear row come in one load.
+// Texel reads on the padded layout.  Byte offsets inside one view are
+// 32-bit, so a load is a 64-bit view base plus a 32-bit per-lane offset.
 template <typename T>
 __device__ __forceinline__ const PM_GLOBAL char* view_base(const T* img, const PmView& v) {
   return reinterpret_cast<const PM_GLOBAL char*>(gp(img) + v.base);
-}
-
-__device__ __forceinline__ void ld_pair(const PM_GLOBAL char* p, float& a, float& b, float) {
-  float2 t;
-  __builtin_memcpy(&t, p, 8);
-  a = t.x;
-  b = t.y;
-}
-__device__ __forceinline__ void ld_pair(const PM_GLOBAL char* p, float& a, float& b, _Float16) {
-  _Float16 t[2];
-  __builtin_memcpy(t, p, 4);
-  a = static_cast<float>(t[0]);
-  b = static_cast<float>(t[1]);
 }
 
 template <typename T>
@@ -199,58 +185,54 @@ __device__ __forceinline__ float texel(const T* __restrict__ img, const PmView& 
   return static_cast<float>(t);
 }
 
-// A view prepared for repeated bilinear sampling (wave-uniform fields).
+// Bilinear sampling goes through a second copy of every view in which
+// texel (x, y) of the padded layout holds its 2 x 2 neighbourhood
+// (T[y][x], T[y][x+1], T[y+1][x], T[y+1][x+1]): one aligned 16-byte (f32) or
+// 8-byte (f16) load per bilinear sample instead of two unaligned pair loads,
+// which halves the work of the texture address unit -- the bottleneck of
+// the gather-bound cost kernels.  4x the image memory, a few hundred MB at
+// most.
+template <typename T> struct QuadOf;
+template <> struct QuadOf<float> { static constexpr uint32_t bytes = 16; };
+template <> struct QuadOf<_Float16> { static constexpr uint32_t bytes = 8; };
+
+// A view prepared for repeated bilinear sampling.
 struct Sampler {
-  const PM_GLOBAL char* base;
-  uint32_t row_bytes;   // pitch * sizeof(T)
-  float row_bytes_f;    // the same, as float (exact: < 2^24)
-  float origin_f;       // byte offset of texel (0, 0): (pitch + 1) * sizeof(T)
-  float xmax, ymax;     // w - 1, h - 1
+  const PM_GLOBAL char* base;  // quad copy of the view (texel (-1, -1) first)
+  float pitch_f;               // pitch in texels
+  float origin_f;              // index of texel (0, 0): pitch + 1
+  float xmax, ymax;            // w - 1, h - 1
 };
 
-template <typename T>
-__device__ __forceinline__ Sampler make_sampler(const T* img, const PmView& v) {
-  const uint32_t rb = static_cast<uint32_t>(v.pitch) * static_cast<uint32_t>(sizeof(T));
-  return Sampler{view_base(img, v), rb, static_cast<float>(rb),
-                 static_cast<float>((v.pitch + 1) * static_cast<int>(sizeof(T))), static_cast<float>(v.w - 1),
-                 static_cast<float>(v.h - 1)};
+__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& t01, float& t10, float& t11,
+                                        float) {
+  const float4 q = *reinterpret_cast<const PM_GLOBAL float4*>(p);
+  t00 = q.x; t01 = q.y; t10 = q.z; t11 = q.w;
+}
+__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& t01, float& t10, float& t11,
+                                        _Float16) {
+  const uint2 q = *reinterpret_cast<const PM_GLOBAL uint2*>(p);
+  t00 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.x & 0xffffu)));
+  t01 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.x >> 16)));
+  t10 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.y & 0xffffu)));
+  t11 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.y >> 16)));
 }
 
-// Same value as bilinear() below, with fewer instructions: the clamp is one
-// med3 per axis (coordinates reaching here are finite), and the byte offset
-// of the top-left texel is formed in float, exact because every term is an
-// integer below 2^24 (a view is < 16M texels).
+// Exact software bilinear at a pixel-centre coordinate with clamp-to-edge:
+// clamp to [-1, w-1] x [-1, h-1] (outside that range both taps of an axis
+// are the same edge texel, so lerp(a, a, t) == a keeps the clamp-to-edge
+// value; one med3 per axis, coordinates reaching here are finite), floor,
+// lerp along x on both rows, then along y.  The quad index is formed in float, exact
+// because it is an integer below 2^24 (a view is < 16M texels).
 template <typename T>
 __device__ __forceinline__ float sample(const Sampler& v, float sx, float sy) {
   sx = __builtin_amdgcn_fmed3f(sx, -1.0f, v.xmax);
   sy = __builtin_amdgcn_fmed3f(sy, -1.0f, v.ymax);
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
-  const float fo = fmaf(fy0, v.row_bytes_f, fmaf(fx0, static_cast<float>(sizeof(T)), v.origin_f));
-  const uint32_t o0 = static_cast<uint32_t>(fo);
-  const uint32_t o1 = o0 + v.row_bytes;
+  const uint32_t qi = static_cast<uint32_t>(fmaf(fy0, v.pitch_f, fx0 + v.origin_f));
   float t00, t01, t10, t11;
-  ld_pair(v.base + o0, t00, t01, T{});
-  ld_pair(v.base + o1, t10, t11, T{});
-  const float v0 = fmaf(ax, t01 - t00, t00);
-  const float v1 = fmaf(ax, t11 - t10, t10);
-  return fmaf(ay, v1 - v0, v0);
-}
-
-template <typename T>
-__device__ __forceinline__ float bilinear(const T* __restrict__ img, const PmView& v, float sx, float sy) {
-  sx = clampf(sx, -1.0f, static_cast<float>(v.w - 1));
-  sy = clampf(sy, -1.0f, static_cast<float>(v.h - 1));
-  const float fx0 = floorf(sx), fy0 = floorf(sy);
-  const float ax = sx - fx0, ay = sy - fy0;
-  const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-  const PM_GLOBAL char* vb = view_base(img, v);
-  const uint32_t o0 = (__umul24(static_cast<uint32_t>(y0 + 1), static_cast<uint32_t>(v.pitch)) +
-                       static_cast<uint32_t>(x0 + 1)) * static_cast<uint32_t>(sizeof(T));
-  const uint32_t o1 = o0 + static_cast<uint32_t>(v.pitch) * static_cast<uint32_t>(sizeof(T));
-  float t00, t01, t10, t11;
-  ld_pair(vb + o0, t00, t01, T{});
-  ld_pair(vb + o1, t10, t11, T{});
+  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, t01, t10, t11, T{});
   const float v0 = fmaf(ax, t01 - t00, t00);
   const float v1 = fmaf(ax, t11 - t10, t10);
   return fmaf(ay, v1 - v0, v0);

@@ -20,6 +20,7 @@ struct PmArgs {
   const float* prev_depths;    // n_src * W * H, may be null
   const float4* snapshot;      // planes before the prior re-init launch
   const void* images;          // padded views, float or _Float16
+  void* quads;                 // the same as 2 x 2 quads (bilinear sampling)
   float2* ref_taps;            // precomputed reference taps (w, w*r), strides 1..3
   float4* ref_sums;            // precomputed reference moments, strides 1..3
   int64_t ref_plane;           // entries per tap plane (= 2 * H * ceil(W/2))
@@ -35,6 +36,10 @@ struct PmArgs {
   float* wnb;                  // sum of the view weights
   float4* refp;                // refinement hypotheses, 6 slots
   int* nref;                   // number of refinement hypotheses (5 or 6)
+  uint2* jobs;                 // textureless-fallback jobs, one queue per XCD
+  uint32_t* jobcount;          // queue lengths (8)
+  int64_t job_cap;             // capacity of one queue
+  int phase;                   // fallback jobs of the candidate (0) or refinement (1) slots
   uint32_t prev_depth_mask;
   int width, height, n_src;
   float depth_min, depth_max;
@@ -58,6 +63,9 @@ struct PmDecodeOut {
 
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream);
+// Capacity one XCD's fallback-job queue needs for a w x h reference.
+int64_t fallback_queue_capacity(int width, int height, int n_src);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_sweep(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);

@@ -22,6 +22,7 @@ constexpr int kTileX = 16;
 constexpr int kTileY = 4;
 constexpr int kSweepSlots = 9;   // 8 propagation candidates + current/refinement
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
+constexpr int kFallbackBlocksPerXcd = 256;
 
 // Tile of this workgroup.  The dispatcher hands workgroups to the XCDs
 // round-robin (linear id mod 8), so consecutive ids land on different L2s;
@@ -49,6 +50,34 @@ __device__ __forceinline__ bool full_tile_pixel(int width, int height, int& x, i
 
 __device__ __forceinline__ Plane to_plane(float4 v) { return Plane{v.x, v.y, v.z, v.w}; }
 __device__ __forceinline__ float4 to_f4(Plane p) { return make_float4(p.x, p.y, p.z, p.w); }
+template <typename T>
+__device__ __forceinline__ Sampler make_sampler(const PmArgs& a, const PmView& v) {
+  return Sampler{reinterpret_cast<const PM_GLOBAL char*>(gp(static_cast<const char*>(a.quads))) +
+                     v.qbase * QuadOf<T>::bytes,
+                 static_cast<float>(v.pitch), static_cast<float>(v.pitch + 1), static_cast<float>(v.w - 1),
+                 static_cast<float>(v.h - 1)};
+}
+
+// Quad copy of the padded views (see Sampler): one lane per padded texel.
+template <typename T>
+__global__ __launch_bounds__(256) void quad_kernel(PmArgs a, int nviews) {
+  for (int vi = 0; vi < nviews; ++vi) {
+    const PmView v = gp(a.views)[vi];
+    const int64_t n = static_cast<int64_t>(v.pitch) * (v.h + 2);
+    const PM_GLOBAL T* src = gp(static_cast<const T*>(a.images)) + v.base;
+    PM_GLOBAL T* dst = gpw(static_cast<T*>(a.quads)) + v.qbase * 4;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+      const int x = static_cast<int>(i % v.pitch), y = static_cast<int>(i / v.pitch);
+      const int x1 = min(x + 1, v.pitch - 1), y1 = min(y + 1, v.h + 1);
+      dst[4 * i + 0] = src[static_cast<int64_t>(y) * v.pitch + x];
+      dst[4 * i + 1] = src[static_cast<int64_t>(y) * v.pitch + x1];
+      dst[4 * i + 2] = src[static_cast<int64_t>(y1) * v.pitch + x];
+      dst[4 * i + 3] = src[static_cast<int64_t>(y1) * v.pitch + x1];
+    }
+  }
+}
+
 // Camera / view i for a wave-uniform i (scalar loads from the constant space).
 __device__ __forceinline__ PmCam kcam(const PmArgs& a, int i) {
   return ((const PM_CONST PmCam*)(a.cams))[i];
@@ -215,7 +244,7 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
       const int sv = s < n_src ? s : n_src - 1;
       ok[k] = s < n_src && warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, sv + 1), wp[k]) && ref_ok;
       if (!ok[k]) wp[k] = Warp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-      sm[k] = make_sampler(img, kview(a, sv + 1));
+      sm[k] = make_sampler<T>(a, kview(a, sv + 1));
       scen[k] = sample<T>(sm[k], wp[k].scx, wp[k].scy);
       s_ws[k] = 0.0f; s_wss[k] = 0.0f; s_wrs[k] = 0.0f;
     }
@@ -274,44 +303,9 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
   return pending;
 }
 
-// Strided ZNCC on precomputed reference data (opencl_kernels.h:188-258).
-template <int HP, typename T>
-__device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const T* __restrict__ img, const PmView& sv, int si,
-                                               int cidx, float src_center, const Warp& w) {
-  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
-  const float stride = static_cast<float>(si + 1);
-  const PM_GLOBAL float2* taps = gp(a.ref_taps) + (static_cast<int64_t>(si) * P) * a.ref_plane + cidx;
-  float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
-  int p = 0;
-  for (int iy = -HP; iy <= HP; ++iy) {
-    for (int ix = -HP; ix <= HP; ++ix) {
-      const float fdx = stride * static_cast<float>(ix);
-      const float fdy = stride * static_cast<float>(iy);
-      const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
-      const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
-      const float sd = bilinear(img, sv, sx, sy) - src_center;
-      const float2 wv = taps[static_cast<int64_t>(p) * a.ref_plane];
-      sum_s += wv.x * sd;
-      sum_ss += wv.x * sd * sd;
-      sum_rs += wv.y * sd;
-      ++p;
-    }
-  }
-  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
-  if (rs.y < 1e-6f) return -1.0f;
-  const float inv_w = 1.0f / rs.y;
-  const float mean_r = rs.z * inv_w;
-  const float mean_s = sum_s * inv_w;
-  const float var_ref = rs.w * inv_w - mean_r * mean_r;
-  const float var_src = sum_ss * inv_w - mean_s * mean_s;
-  if (var_ref < a.var_threshold || var_src < a.var_threshold) return -1.0f;
-  const float covar = sum_rs * inv_w - mean_r * mean_s;
-  return covar / sqrtf(var_ref * var_src);
-}
-
 // Census fallback (opencl_kernels.h:270-315).
 template <typename T>
-__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, const PmView& sv,
+__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, const Sampler& sm,
                                               float ref_center, float src_center, int cx, int cy, const Warp& w) {
   const PmView rv = kview(a, 0);
   uint32_t rc = 0u, sc = 0u;
@@ -322,7 +316,7 @@ __device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restri
       const float ref_val = texel(img, rv, cx + dx, cy + dy);
       const float sx = w.scx + w.jxx * static_cast<float>(dx) + w.jyx * static_cast<float>(dy);
       const float sy = w.scy + w.jxy * static_cast<float>(dx) + w.jyy * static_cast<float>(dy);
-      const float src_val = bilinear(img, sv, sx, sy);
+      const float src_val = sample<T>(sm, sx, sy);
       if (ref_val > ref_center) rc |= (1u << bit);
       if (src_val > src_center) sc |= (1u << bit);
       ++bit;
@@ -390,7 +384,7 @@ __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __re
     if (!__any(mine)) continue;
     if (mine) {
       const PmView sv = kview(a, s + 1);
-      const Sampler sm = make_sampler(img, sv);
+      const Sampler sm = make_sampler<T>(a, sv);
       Warp w;
       warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, s + 1), w);  // valid: it was valid at stride 1
       const float scen = sample<T>(sm, w.scx, w.scy);
@@ -403,13 +397,98 @@ __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __re
       }
       float cost;
       if (best <= -0.5f) {
-        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sv, ref_center, scen, x, y, w) : 2.0f;
+        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
       } else {
         cost = 1.0f - best;
       }
       out[s * os] = cost;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Textureless fallbacks of the colour-pass cost kernels go through a job
+// queue instead of being resolved in place: a wave appends one job per
+// (pixel, hypothesis slot, view) left pending, grouped by view, to the queue
+// of its XCD; fallback_kernel then runs the jobs with every lane busy.  The
+// stride-1 NCC is already in the cost slot, so a job is just the pixel, the
+// slot and the view.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void emit_jobs(const PmArgs& a, int x, int y, int slot, uint32_t pending) {
+  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+  const int lane = static_cast<int>(threadIdx.x);
+  PM_GLOBAL uint2* q = gpw(a.jobs) + static_cast<int64_t>(xcd) * a.job_cap;
+#pragma unroll 1
+  for (int s = 0; s < a.n_src; ++s) {
+    const bool mine = (pending >> s) & 1u;
+    const uint64_t ballot = __ballot(mine);
+    if (ballot == 0ull) continue;
+    const int leader = __ffsll(static_cast<unsigned long long>(ballot)) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(a.jobcount + xcd, static_cast<uint32_t>(__popcll(ballot)));
+    base = __shfl(base, leader);
+    if (mine) {
+      const uint32_t rank = static_cast<uint32_t>(__popcll(ballot & ((1ull << lane) - 1ull)));
+      q[base + rank] = make_uint2(static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16),
+                                  static_cast<uint32_t>(slot) | (static_cast<uint32_t>(s) << 8));
+    }
+  }
+}
+
+// Runs the queued fallbacks (opencl_kernels.h:531-553 for one view):
+// strides 2 then 3 (stop once NCC > 0.1), then census or 2.0.  Workgroups
+// on XCD k drain queue k; the loop bound is the queue length read once, so
+// every wave exits.  a.phase: 0 = candidate slots, 1 = refinement slots.
+template <int HP, typename T>
+__global__ __launch_bounds__(kWave) void fallback_kernel(PmArgs a) {
+  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+  const uint32_t j = blockIdx.x / kXcds;
+  const uint32_t nb = gridDim.x / kXcds;
+  const uint32_t count = gp(a.jobcount)[xcd];
+  const T* img = images_of<T>(a);
+  const PM_GLOBAL uint2* q = gp(a.jobs) + static_cast<int64_t>(xcd) * a.job_cap;
+  const int64_t pl = a.ref_plane;
+  for (uint32_t c = j * kWave; c < count; c += nb * kWave) {
+    const uint32_t qi = c + threadIdx.x;
+    if (qi >= count) continue;
+    const uint2 job = q[qi];
+    const int x = static_cast<int>(job.x & 0xffffu), y = static_cast<int>(job.x >> 16);
+    const int slot = static_cast<int>(job.y & 0xffu), s = static_cast<int>(job.y >> 8);
+    const int cidx = colour_index(a, x, y);
+    Plane pl4;
+    if (a.phase == 0) {
+      const int pos = slot < 8 ? gp(a.cpos)[slot * pl + cidx] : y * a.width + x;
+      pl4 = to_plane(gp(a.planes)[pos]);
+    } else {
+      pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
+    }
+    float* out = a.ca + (static_cast<int64_t>(slot) * a.n_src + s) * pl + cidx;
+    PlanePts pp;
+    plane_points(kcam(a, 0), x, y, pl4, pp);
+    const PmView sv = lane_view(a, s);
+    const Sampler sm = make_sampler<T>(a, sv);
+    Warp w;
+    warp_of(pp.cw, pp.wdx, pp.wdy, lane_cam(a, s), w);  // valid: it was valid at stride 1
+    const float scen = sample<T>(sm, w.scx, w.scy);
+    float best = *out;  // stride-1 NCC
+    for (int si = 1; si < 3; ++si) {
+      const float ncc = ncc_at_stride_u<HP, T>(a, sm, si, cidx, scen, w);
+      if (ncc > best) best = ncc;
+      if (best > 0.1f) break;
+    }
+    float cost;
+    if (best <= -0.5f) {
+      const float ref_center = gp(a.ref_sums)[cidx].x;
+      cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
+    } else {
+      cost = 1.0f - best;
+    }
+    *out = cost;
+  }
+}
+
+__global__ void zero_counters_kernel(uint32_t* c, int n) {
+  if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0u;
 }
 
 // Full cost vector: stride 1 for all views, then the fallbacks.
@@ -829,7 +908,8 @@ __global__ __launch_bounds__(kWave * 9) void pass_candidates_kernel(PmArgs a) {
   if (pos < 0) return;
   const Plane hyp = to_plane(gp(a.planes)[pos]);
   float* out = a.ca + static_cast<int64_t>(h) * a.n_src * a.ref_plane + cp.cidx;
-  full_cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+  const uint32_t pending = cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+  emit_jobs(a, cp.x, cp.y, h, pending);
 }
 
 template <int NV>
@@ -1033,7 +1113,8 @@ __global__ __launch_bounds__(kWave * 6) void pass_refine_cost_kernel(PmArgs a) {
   if (r >= gp(a.nref)[cp.cidx]) return;
   const Plane hyp = to_plane(gp(a.refp)[r * a.ref_plane + cp.cidx]);
   float* out = a.ca + static_cast<int64_t>(r) * a.n_src * a.ref_plane + cp.cidx;
-  full_cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+  const uint32_t pending = cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
+  emit_jobs(a, cp.x, cp.y, r, pending);
 }
 
 template <int NV>
@@ -1241,6 +1322,23 @@ dim3 colour_grid(const PmArgs& a) {
     return hipGetLastError();                                                                     \
   } while (0)
 
+hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream) {
+  if (f16) {
+    hipLaunchKernelGGL(quad_kernel<_Float16>, dim3(2048), dim3(256), 0, stream, a, nviews);
+  } else {
+    hipLaunchKernelGGL(quad_kernel<float>, dim3(2048), dim3(256), 0, stream, a, nviews);
+  }
+  return hipGetLastError();
+}
+
+int64_t fallback_queue_capacity(int width, int height, int n_src) {
+  // XCD k owns at most ceil(tiles / 8) colour tiles of 64 pixels, each with
+  // at most 9 slots x n_src pending views.
+  const int hw = (width + 1) / 2;
+  const int64_t tiles = static_cast<int64_t>((hw + kTileX - 1) / kTileX) * ((height + kTileY - 1) / kTileY);
+  return ((tiles + kXcds - 1) / kXcds) * kWave * 9 * n_src;
+}
+
 hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(refpatch_kernel, hp, f16, full_grid(a), 0, stream, a);
 }
@@ -1258,9 +1356,17 @@ hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
 template <int NV>
 hipError_t launch_pass_nv(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   const dim3 g = colour_grid(a);
+  const dim3 gf(kXcds * kFallbackBlocksPerXcd);
+  PmArgs b = a;
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kWave), 0, stream, a.jobcount, kXcds);
   PM_DISPATCH_NORET(pass_candidates_kernel, hp, f16, g, dim3(kWave, 9), 0, stream, a);
+  b.phase = 0;
+  PM_DISPATCH_NORET(fallback_kernel, hp, f16, gf, dim3(kWave), 0, stream, b);
   hipLaunchKernelGGL((pass_select_kernel<NV>), g, dim3(kWave), 0, stream, a);
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kWave), 0, stream, a.jobcount, kXcds);
   PM_DISPATCH_NORET(pass_refine_cost_kernel, hp, f16, g, dim3(kWave, 6), 0, stream, a);
+  b.phase = 1;
+  PM_DISPATCH_NORET(fallback_kernel, hp, f16, gf, dim3(kWave), 0, stream, b);
   hipLaunchKernelGGL((pass_refine_kernel<NV>), g, dim3(kWave), 0, stream, a);
   return hipGetLastError();
 }
