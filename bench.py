@@ -7,10 +7,12 @@ random init + 3 x (red pass + black pass) + decode, all on the device with
 the views already resident in HBM.  Metric: Mpix*views*iters/s =
 (W*H/1e6) * N_src * iterations / seconds.
 
-Also reported: the roofline of the dominant kernel (the colour-pass sweep,
-timed with HIP events on its own stream inside the timed steps) against
-the FP32 vector peak, accuracy against the synthetic ground truth, and the
-CPU oracle timed on the host cores on a bounded sample of the same scene.
+Also reported: the roofline of the dominant kernel (pass_candidates, timed
+with HIP events on the estimator's stream inside the timed steps) against
+the FP32 vector peak, per-stage device time, accuracy against the synthetic
+ground truth, and the CPU oracle timed on the host cores on a bounded sample
+of the same scene.  Each est.run() returns only after its HIP stream is idle
+(hipStreamSynchronize), so the wall clock brackets complete GPU work.
 """
 from __future__ import annotations
 
@@ -30,6 +32,10 @@ UNIT = "Mpix·views·iters/s"
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PATCH = 7
 FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap x P taps + 75)
+# HBM bytes per pass_candidates launch: rocprofv3 --pmc FETCH_SIZE (x 1.9999)
+# + WRITE_SIZE (x 1.0) on this workload, each factor read off the same counter
+# on a 1 GiB device-to-device copy (profiles/*_hbm_traffic.txt).
+TRAFFIC_PER_LAUNCH = 8.661e9
 
 
 def log(msg: str) -> None:
@@ -68,6 +74,25 @@ def cpu_baseline(scene, iters: int, rows: int):
                       f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
 
 
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of ``mib`` MiB through the HIP runtime (a
+    plain copy kernel of known size, to scale the HBM traffic counters)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("copy probe: hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, n, 3):  # hipMemcpyDeviceToDevice
+        raise RuntimeError("copy probe: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {mib} MiB read + {mib} MiB written")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +102,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--storage", choices=["f32", "f16"], default="f32")
     ap.add_argument("--cpu-rows", type=int, default=96, help="reference rows in the CPU-oracle sample (0 = skip)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the timed steps, run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by reference "
@@ -102,33 +129,44 @@ def main() -> int:
         est.run()
         log(f"warmup {i + 1}/{args.warmup}: {est.last_timing()['run_ms']:.1f} ms")
 
-    sweep_ms = 0.0
-    launches = 0
+    stages = {}
     t0 = time.perf_counter()
     for i in range(args.steps):
         est.run()
         tm = est.last_timing()
-        sweep_ms += tm["sweep_ms"]
-        launches += tm["sweep_launches"]
-        log(f"step {i + 1}/{args.steps}: {tm['run_ms']:.1f} ms (sweeps {tm['sweep_ms']:.1f} ms)")
+        for k, (ms, n) in est.stage_timing().items():
+            acc = stages.setdefault(k, [0.0, 0])
+            acc[0] += ms
+            acc[1] += n
+        log(f"step {i + 1}/{args.steps}: {tm['run_ms']:.1f} ms (iterations {tm['sweep_ms']:.1f} ms)")
     elapsed = time.perf_counter() - t0
     value = units_per_step * args.steps / elapsed
 
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
+
     out = est.download()
+    # Host-buffer rate: padding/conversion + upload + run + download.
+    t1 = time.perf_counter()
+    est.compute_patch_match()
+    pcie_value = units_per_step / (time.perf_counter() - t1)
     d = out["depth"]
     gt = scene.gt_depth
     acc1 = float(np.mean(np.abs(d - gt) < 0.01 * gt))
     acc5 = float(np.mean(np.abs(d - gt) < 0.05 * gt))
 
-    # Roofline of the sweep kernel: algorithmic FLOP of one colour pass =
-    # (pixels of that colour) x N_src x 15 hypotheses x (21 P + 75).
-    avg_launch_s = sweep_ms / max(launches, 1) / 1e3
-    flop_per_launch = (w * h / 2.0) * n_src * FLOP_PER_UNIT
+    # Roofline of the dominant kernel, pass_candidates: one launch scores the
+    # 8 propagated candidates + the current plane of every pixel of one
+    # colour, i.e. (W*H/2) x 9 hypotheses x N_src x (21 P + 75) FLOP.
+    cand_ms, cand_n = stages["candidates"]
+    avg_launch_s = cand_ms / max(cand_n, 1) / 1e3
+    flop_per_launch = (w * h / 2.0) * 9 * n_src * (21 * PATCH * PATCH + 75)
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "kernel": "sweep_kernel<3,float>", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-                "flop_per_launch": flop_per_launch}
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": TRAFFIC_PER_LAUNCH,
+                "kernel": f"pass_candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'}>",
+                "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
+    stage_ms = {k: round(v[0] / args.steps, 3) for k, v in stages.items()}
 
     cpu = None
     if args.cpu_rows > 0:
@@ -149,6 +187,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
+        "stage_ms_per_step": stage_ms,
+        "pcie_inclusive_value": round(pcie_value, 3),
     }
     if cpu and cpu["value"] > 0:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)

@@ -148,8 +148,19 @@ int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes,
                     float* init_cost, uint32_t* sel);
 
 /* Device time of the last pm_run in milliseconds (HIP events on the
- * handle's stream), and of its dominant kernel (all sweep launches). */
+ * handle's stream): the whole run, and the iterations (all colour passes). */
 int pm_last_timing(pm_handle* h, float* run_ms, float* sweep_ms, int* sweep_launches);
+
+/* Kernel stages of a run; per-stage device time of the last pm_run (sum
+ * over its launches, HIP events bracketing each launch) and launch count. */
+#define PM_STAGE_INIT 0         /* random init + initial costs (acmmp_random_init) */
+#define PM_STAGE_CANDIDATES 1   /* cost vectors of 8 propagated candidates + current plane */
+#define PM_STAGE_FALLBACK 2     /* textureless strided-NCC / census fallbacks */
+#define PM_STAGE_SELECT 3       /* view weights, MC costs, candidate acceptance */
+#define PM_STAGE_REFINE_COST 4  /* cost vectors of the refinement hypotheses */
+#define PM_STAGE_REFINE 5       /* MC costs and acceptance of the refinements */
+#define PM_STAGES 6
+int pm_last_stage_timing(pm_handle* h, int stage, float* ms, int* launches);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,14 @@ struct pm_handle {
   DevBuf<float> out_depth, out_normal, out_cost, out_conf;
   float last_run_ms = 0.0f, last_sweep_ms = 0.0f;
   int last_sweeps = 0;
+  // Per-kernel device time of the last pm_run: HIP events around every
+  // launch of each stage, on the handle's stream.
+  std::vector<hipEvent_t> kev;
+  std::vector<std::pair<int, int>> kev_marks;  // (stage, index of the start event)
+  size_t kev_used = 0;
+  float stage_ms[PM_STAGES] = {};
+  int stage_launches[PM_STAGES] = {};
+  bool timing = false;
 
   ~pm_handle() {
     images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
@@ -164,6 +172,7 @@ struct pm_handle {
     snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
     out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : kev) if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -251,6 +260,27 @@ int validate(pm_handle* h) {
   if (p.max_iterations < 0) return fail(PM_E_INVALID, "max_iterations must be >= 0");
   if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "only hierarchy_levels = 1 is supported");
   if (!(p.sigma_spatial > 0.0f) || !(p.sigma_color > 0.0f)) return fail(PM_E_INVALID, "sigmas must be > 0");
+  return PM_OK;
+}
+
+// Launch `fn` on the handle's stream; while a pm_run is being timed, bracket
+// it with a pair of events attributed to `stage`.
+template <typename F>
+int timed(pm_handle* h, int stage, const char* what, F fn) {
+  if (h->timing) {
+    while (h->kev.size() < h->kev_used + 2) {
+      hipEvent_t e;
+      PM_HIP(hipEventCreate(&e), "hipEventCreate");
+      h->kev.push_back(e);
+    }
+    PM_HIP(hipEventRecord(h->kev[h->kev_used], h->stream), "event");
+  }
+  PM_HIP(fn(), what);
+  if (h->timing) {
+    PM_HIP(hipEventRecord(h->kev[h->kev_used + 1], h->stream), "event");
+    h->kev_marks.emplace_back(stage, static_cast<int>(h->kev_used));
+    h->kev_used += 2;
+  }
   return PM_OK;
 }
 
@@ -604,7 +634,8 @@ int pm_init_state(pm_handle* h) {
   PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
   if (int r = ensure_refpatch(h)) return r;
-  PM_HIP(launch_init(a, hp, h->f16, h->stream), "init kernel launch");
+  if (int r = timed(h, PM_STAGE_INIT, "init kernel launch", [&] { return launch_init(a, hp, h->f16, h->stream); }))
+    return r;
   if (h->masks.p) {  // estimator.cc:574-614
     PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
                           h->stream), "prior snapshot");
@@ -623,10 +654,23 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
   a.iteration = iteration;
   a.smooth_w = smooth_weight_at(h->params, iteration);
   const int hp = h->params.patch_size / 2;
+  const bool f16 = h->f16;
+  hipStream_t st = h->stream;
   for (int c = 0; c < 2; ++c) {
     if (color >= 0 && c != color) continue;
     a.color = c;
-    PM_HIP(launch_sweep(a, hp, h->f16, h->stream), "sweep kernel launch");
+    PM_HIP(launch_zero_counters(a, st), "zero kernel launch");
+    if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
+                      [&] { return launch_candidates(a, hp, f16, st); })) return r;
+    if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
+                      [&] { return launch_fallback(a, hp, f16, 0, st); })) return r;
+    if (int r = timed(h, PM_STAGE_SELECT, "select kernel launch", [&] { return launch_select(a, st); })) return r;
+    PM_HIP(launch_zero_counters(a, st), "zero kernel launch");
+    if (int r = timed(h, PM_STAGE_REFINE_COST, "refine-cost kernel launch",
+                      [&] { return launch_refine_cost(a, hp, f16, st); })) return r;
+    if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
+                      [&] { return launch_fallback(a, hp, f16, 1, st); })) return r;
+    if (int r = timed(h, PM_STAGE_REFINE, "refine kernel launch", [&] { return launch_refine(a, st); })) return r;
   }
   return PM_OK;
 }
@@ -657,11 +701,15 @@ int pm_run(pm_handle* h) {
     if (int r = pm_upload(h)) return r;
   }
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  h->timing = true;
+  h->kev_used = 0;
+  h->kev_marks.clear();
   PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
-  if (int r = pm_init_state(h)) return r;
+  if (int r = pm_init_state(h)) { h->timing = false; return r; }
   PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
   for (int it = 0; it < h->params.max_iterations; ++it)
-    if (int r = pm_iterate(h, it, -1)) return r;
+    if (int r = pm_iterate(h, it, -1)) { h->timing = false; return r; }
+  h->timing = false;
   PM_HIP(hipEventRecord(h->ev[2], h->stream), "event");
   if (h->params.checkerboard_filter)
     if (int r = pm_filter(h)) return r;
@@ -671,6 +719,21 @@ int pm_run(pm_handle* h) {
   PM_HIP(hipEventElapsedTime(&h->last_run_ms, h->ev[0], h->ev[3]), "event time");
   PM_HIP(hipEventElapsedTime(&h->last_sweep_ms, h->ev[1], h->ev[2]), "event time");
   h->last_sweeps = 2 * h->params.max_iterations;
+  for (int k = 0; k < PM_STAGES; ++k) { h->stage_ms[k] = 0.0f; h->stage_launches[k] = 0; }
+  for (const auto& m : h->kev_marks) {
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, h->kev[m.second], h->kev[m.second + 1]), "event time");
+    h->stage_ms[m.first] += ms;
+    h->stage_launches[m.first] += 1;
+  }
+  return PM_OK;
+}
+
+int pm_last_stage_timing(pm_handle* h, int stage, float* ms, int* launches) {
+  if (int r = check_handle(h)) return r;
+  if (stage < 0 || stage >= PM_STAGES) return fail(PM_E_INVALID, "unknown stage");
+  if (ms) *ms = h->stage_ms[stage];
+  if (launches) *launches = h->stage_launches[stage];
   return PM_OK;
 }
 

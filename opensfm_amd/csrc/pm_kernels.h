@@ -68,7 +68,15 @@ hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t strea
 int64_t fallback_queue_capacity(int width, int height, int n_src);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
-hipError_t launch_sweep(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+// One colour pass (a.color) of iteration a.iteration, in this order:
+// zero_counters, candidates, fallback(0), select, zero_counters,
+// refine_cost, fallback(1), refine.
+hipError_t launch_zero_counters(const PmArgs& a, hipStream_t stream);
+hipError_t launch_candidates(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, hipStream_t stream);
+hipError_t launch_select(const PmArgs& a, hipStream_t stream);
+hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+hipError_t launch_refine(const PmArgs& a, hipStream_t stream);
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream);
 hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stream);
 hipError_t launch_score(const PmArgs& a, int half_patch, bool f16, int n, const int32_t* xy,

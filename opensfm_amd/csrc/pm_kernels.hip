@@ -1353,28 +1353,47 @@ hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(prior_kernel, hp, f16, full_grid(a), lds, stream, a);
 }
 
-template <int NV>
-hipError_t launch_pass_nv(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
-  const dim3 g = colour_grid(a);
-  const dim3 gf(kXcds * kFallbackBlocksPerXcd);
-  PmArgs b = a;
+hipError_t launch_zero_counters(const PmArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kWave), 0, stream, a.jobcount, kXcds);
-  PM_DISPATCH_NORET(pass_candidates_kernel, hp, f16, g, dim3(kWave, 9), 0, stream, a);
-  b.phase = 0;
-  PM_DISPATCH_NORET(fallback_kernel, hp, f16, gf, dim3(kWave), 0, stream, b);
-  hipLaunchKernelGGL((pass_select_kernel<NV>), g, dim3(kWave), 0, stream, a);
-  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kWave), 0, stream, a.jobcount, kXcds);
-  PM_DISPATCH_NORET(pass_refine_cost_kernel, hp, f16, g, dim3(kWave, 6), 0, stream, a);
-  b.phase = 1;
-  PM_DISPATCH_NORET(fallback_kernel, hp, f16, gf, dim3(kWave), 0, stream, b);
-  hipLaunchKernelGGL((pass_refine_kernel<NV>), g, dim3(kWave), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_sweep(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
+hipError_t launch_candidates(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   if (hp < 1 || hp > 3) return hipErrorInvalidValue;
-  if (a.n_src <= 8) return launch_pass_nv<8>(a, hp, f16, stream);
-  return launch_pass_nv<kMaxSrc>(a, hp, f16, stream);
+  PM_DISPATCH_NORET(pass_candidates_kernel, hp, f16, colour_grid(a), dim3(kWave, 9), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine_cost(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
+  if (hp < 1 || hp > 3) return hipErrorInvalidValue;
+  PM_DISPATCH_NORET(pass_refine_cost_kernel, hp, f16, colour_grid(a), dim3(kWave, 6), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fallback(const PmArgs& a, int hp, bool f16, int phase, hipStream_t stream) {
+  if (hp < 1 || hp > 3) return hipErrorInvalidValue;
+  PmArgs b = a;
+  b.phase = phase;
+  PM_DISPATCH_NORET(fallback_kernel, hp, f16, dim3(kXcds * kFallbackBlocksPerXcd), dim3(kWave), 0, stream, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_select(const PmArgs& a, hipStream_t stream) {
+  if (a.n_src <= 8) {
+    hipLaunchKernelGGL((pass_select_kernel<8>), colour_grid(a), dim3(kWave), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((pass_select_kernel<kMaxSrc>), colour_grid(a), dim3(kWave), 0, stream, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_refine(const PmArgs& a, hipStream_t stream) {
+  if (a.n_src <= 8) {
+    hipLaunchKernelGGL((pass_refine_kernel<8>), colour_grid(a), dim3(kWave), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((pass_refine_kernel<kMaxSrc>), colour_grid(a), dim3(kWave), 0, stream, a);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream) {

@@ -194,6 +194,18 @@ class Estimator {
     return py::make_tuple(costs, init, sel);
   }
 
+  py::dict stage_timing() {
+    static const char* names[PM_STAGES] = {"init", "candidates", "fallback", "select", "refine_cost", "refine"};
+    py::dict d;
+    for (int k = 0; k < PM_STAGES; ++k) {
+      float ms = 0;
+      int n = 0;
+      check(pm_last_stage_timing(h_, k, &ms, &n));
+      d[names[k]] = py::make_tuple(ms, n);
+    }
+    return d;
+  }
+
   py::dict last_timing() {
     float run_ms = 0, sweep_ms = 0;
     int sweeps = 0;
@@ -307,6 +319,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("write_state", &Estimator::write_state)
       .def("score_planes", &Estimator::score_planes, py::arg("xy"), py::arg("planes"))
       .def("last_timing", &Estimator::last_timing)
+      .def("stage_timing", &Estimator::stage_timing,
+           "Per-stage device time of the last run: {stage: (milliseconds, launches)}")
       .def_static("is_available", []() {
         int n = 0;
         return pm_device_count(&n) == PM_OK && n > 0;

@@ -26,6 +26,15 @@
 //     "+0.5f" (opencl_kernels.h:224,229,459,491) only converts pixel-centre
 //     coordinates to OpenCL's texel-corner convention, so the oracle samples
 //     at the pixel-centre coordinate directly.
+//
+// Parity unpinned: the parts below have no closed form, no golden vectors in
+// the reference's tests, and the reference cannot run in this environment,
+// so they rest on this restatement alone (tests/test_oracle.py pins the
+// rest): the view-weight heuristics (opencl_kernels.h:1452-1563), the
+// Monte-Carlo view sampling of mc_cost (:910-969), the candidate search
+// patterns (:1262-1430), the refinement recipe (:1669-1772), the smoothness
+// cost and surface normal (:649-772), geometric consistency (:779-900),
+// prior re-init (:1056-1182) and the checkerboard median (:1883-2039).
 
 #include <cmath>
 #include <cstdint>
