@@ -565,9 +565,9 @@ int pm_upload(pm_handle* h) {
   if (int r = h->cpos.alloc(8 * rp)) return r;
   if (int r = h->nref.alloc(rp)) return r;
   if (int r = h->refp.alloc(6 * rp)) return r;
-  h->job_cap = fallback_queue_capacity(h->W, h->H, h->n_src);
-  if (int r = h->jobs.alloc(8 * static_cast<size_t>(h->job_cap))) return r;
-  if (int r = h->jobcount.alloc(8)) return r;
+  h->job_cap = fallback_queue_capacity(h->W, h->H);
+  if (int r = h->jobs.alloc(static_cast<size_t>(fallback_queue_count(h->n_src)) * h->job_cap)) return r;
+  if (int r = h->jobcount.alloc(static_cast<size_t>(job_counter_count()))) return r;
   h->ref_ready = false;
   if (int r = h->planes.alloc(npix)) return r;
   if (int r = h->costs.alloc(npix)) return r;

@@ -36,9 +36,9 @@ struct PmArgs {
   float* wnb;                  // sum of the view weights
   float4* refp;                // refinement hypotheses, 6 slots
   int* nref;                   // number of refinement hypotheses (5 or 6)
-  uint2* jobs;                 // textureless-fallback jobs, one queue per XCD
-  uint32_t* jobcount;          // queue lengths (8)
-  int64_t job_cap;             // capacity of one queue
+  uint2* jobs;                 // textureless-fallback jobs, one queue per (XCD, view)
+  uint32_t* jobcount;          // queue lengths, [xcd * 16 + view]
+  int64_t job_cap;             // capacity of one (xcd, view) queue
   int phase;                   // fallback jobs of the candidate (0) or refinement (1) slots
   uint32_t prev_depth_mask;
   int width, height, n_src;
@@ -64,8 +64,10 @@ struct PmDecodeOut {
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream);
-// Capacity one XCD's fallback-job queue needs for a w x h reference.
-int64_t fallback_queue_capacity(int width, int height, int n_src);
+// Fallback-job queues for a w x h reference: one per (XCD, source view).
+int64_t fallback_queue_capacity(int width, int height);  // jobs per (xcd, view) queue
+int fallback_queue_count(int n_src);                      // number of queues
+int job_counter_count();                                  // length of the jobcount array
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 // One colour pass (a.color) of iteration a.iteration, in this order:

@@ -22,7 +22,6 @@ constexpr int kTileX = 16;
 constexpr int kTileY = 4;
 constexpr int kSweepSlots = 9;   // 8 propagation candidates + current/refinement
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
-constexpr int kFallbackBlocksPerXcd = 256;
 
 // Tile of this workgroup.  The dispatcher hands workgroups to the XCDs
 // round-robin (linear id mod 8), so consecutive ids land on different L2s;
@@ -205,7 +204,10 @@ __device__ __forceinline__ bool plane_points(const PmCam& c0, int x, int y, Plan
 // still accumulates its taps in row-major order.  out[s * os] gets the
 // cost, or -- for views whose NCC is <= 0.1 -- the stride-1 NCC, with bit s
 // set in the returned pending mask for resolve_fallbacks().
-constexpr int kViewChunk = 4;
+#ifndef PM_VIEW_CHUNK
+#define PM_VIEW_CHUNK 2
+#endif
+constexpr int kViewChunk = PM_VIEW_CHUNK;
 
 template <int HP, typename T>
 __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
@@ -333,12 +335,27 @@ __device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restri
 // view pending work on it together: the view is wave-uniform (scalar
 // descriptors, one source image per gather instruction, neighbouring
 // pixels), which keeps the gathers coherent.
+// The reference-side (w, w*r) of a strided tap are either read from
+// ref_taps or recomputed from the reference image with the arithmetic of
+// refpatch_kernel (bit-identical).  Recomputing costs one exp per tap and
+// one texel read from the small, cache-resident reference footprint instead
+// of an 8-byte read that misses L2 for scattered fallback jobs.
+#ifndef PM_FB_RECOMPUTE
+#define PM_FB_RECOMPUTE 1
+#endif
+
 template <int HP, typename T>
-__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const Sampler& sm, int si, int cidx,
-                                                 float src_center, const Warp& w) {
+__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const T* __restrict__ img, const Sampler& sm, int si,
+                                                 int x, int y, int cidx, float src_center, const Warp& w) {
   constexpr int P = (2 * HP + 1) * (2 * HP + 1);
   const float stride = static_cast<float>(si + 1);
+  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+#if PM_FB_RECOMPUTE
+  const PmView rv = kview(a, 0);
+  const float sf = a.spatial_factor, cf = a.color_factor;
+#else
   const uint32_t toff = static_cast<uint32_t>(cidx) << 3;
+#endif
   float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
   int p = 0;
 #pragma unroll 1
@@ -350,16 +367,24 @@ __device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const Sampler&
       const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
       const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
       const float sd = sample<T>(sm, sx, sy) - src_center;
+#if PM_FB_RECOMPUTE
+      const float rx = static_cast<float>(x) + fdx;
+      const float ry = static_cast<float>(y) + fdy;
+      const float r = texel(img, rv, static_cast<int>(rx), static_cast<int>(ry)) - rs.x;
+      const float wt = pm_expf(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
+      const float2 wv = make_float2(wt, wt * r);
+#else
       const PM_GLOBAL char* tp = reinterpret_cast<const PM_GLOBAL char*>(
           gp(a.ref_taps) + (static_cast<int64_t>(si) * P + (p + ix + HP)) * a.ref_plane);
       const float2 wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + toff);
+#endif
       sum_s += wv.x * sd;
       sum_ss += wv.x * sd * sd;
       sum_rs += wv.y * sd;
     }
     p += 2 * HP + 1;
   }
-  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+  (void)p;
   if (rs.y < 1e-6f) return -1.0f;
   const float inv_w = 1.0f / rs.y;
   const float mean_r = rs.z * inv_w;
@@ -391,7 +416,7 @@ __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __re
       float best = out[s * os];  // stride-1 NCC
 #pragma unroll 1
       for (int si = 1; si < 3; ++si) {
-        const float ncc = ncc_at_stride_u<HP, T>(a, sm, si, cidx, scen, w);
+        const float ncc = ncc_at_stride_u<HP, T>(a, img, sm, si, x, y, cidx, scen, w);
         if (ncc > best) best = ncc;
         if (best > 0.1f) break;
       }
@@ -407,17 +432,25 @@ __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __re
 }
 
 // ---------------------------------------------------------------------------
-// Textureless fallbacks of the colour-pass cost kernels go through a job
-// queue instead of being resolved in place: a wave appends one job per
-// (pixel, hypothesis slot, view) left pending, grouped by view, to the queue
-// of its XCD; fallback_kernel then runs the jobs with every lane busy.  The
-// stride-1 NCC is already in the cost slot, so a job is just the pixel, the
-// slot and the view.
+// Textureless fallbacks of the colour-pass cost kernels go through job
+// queues instead of being resolved in place: a wave appends one job per
+// (pixel, hypothesis slot) left pending to the queue of (its XCD, the view);
+// fallback_kernel then runs the jobs with every lane busy.  One queue per
+// view keeps the jobs running together on one XCD inside one source image
+// and one band of the reference, a footprint its L2 holds.  The stride-1
+// NCC is already in the cost slot, so a job is just the pixel and the slot.
 // ---------------------------------------------------------------------------
+#ifndef PM_FB_BLOCKS
+#define PM_FB_BLOCKS 512
+#endif
+constexpr int kFallbackBlocksPerXcd = PM_FB_BLOCKS;
+constexpr int kJobCounters = kXcds * kMaxSrc;
+
 __device__ __forceinline__ void emit_jobs(const PmArgs& a, int x, int y, int slot, uint32_t pending) {
   const int xcd = static_cast<int>(blockIdx.x) % kXcds;
   const int lane = static_cast<int>(threadIdx.x);
-  PM_GLOBAL uint2* q = gpw(a.jobs) + static_cast<int64_t>(xcd) * a.job_cap;
+  const uint2 job = make_uint2(static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16),
+                               static_cast<uint32_t>(slot));
 #pragma unroll 1
   for (int s = 0; s < a.n_src; ++s) {
     const bool mine = (pending >> s) & 1u;
@@ -425,65 +458,81 @@ __device__ __forceinline__ void emit_jobs(const PmArgs& a, int x, int y, int slo
     if (ballot == 0ull) continue;
     const int leader = __ffsll(static_cast<unsigned long long>(ballot)) - 1;
     uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(a.jobcount + xcd, static_cast<uint32_t>(__popcll(ballot)));
+    if (lane == leader) base = atomicAdd(a.jobcount + xcd * kMaxSrc + s, static_cast<uint32_t>(__popcll(ballot)));
     base = __shfl(base, leader);
     if (mine) {
       const uint32_t rank = static_cast<uint32_t>(__popcll(ballot & ((1ull << lane) - 1ull)));
-      q[base + rank] = make_uint2(static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16),
-                                  static_cast<uint32_t>(slot) | (static_cast<uint32_t>(s) << 8));
+      gpw(a.jobs)[static_cast<int64_t>(xcd * a.n_src + s) * a.job_cap + base + rank] = job;
     }
   }
 }
 
 // Runs the queued fallbacks (opencl_kernels.h:531-553 for one view):
 // strides 2 then 3 (stop once NCC > 0.1), then census or 2.0.  Workgroups
-// on XCD k drain queue k; the loop bound is the queue length read once, so
-// every wave exits.  a.phase: 0 = candidate slots, 1 = refinement slots.
+// on XCD k drain the queues of XCD k, view after view; each loop bound is a
+// queue length read once, so every wave exits.  a.phase: 0 = candidate
+// slots, 1 = refinement slots.
+#ifndef PM_FB_WAVES
+#define PM_FB_WAVES 0
+#endif
+#if PM_FB_WAVES
+#define PM_FB_ATTR __attribute__((amdgpu_waves_per_eu(PM_FB_WAVES)))
+#else
+#define PM_FB_ATTR
+#endif
+
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) void fallback_kernel(PmArgs a) {
+__global__ __launch_bounds__(kWave) PM_FB_ATTR void fallback_kernel(PmArgs a) {
   const int xcd = static_cast<int>(blockIdx.x) % kXcds;
   const uint32_t j = blockIdx.x / kXcds;
   const uint32_t nb = gridDim.x / kXcds;
-  const uint32_t count = gp(a.jobcount)[xcd];
   const T* img = images_of<T>(a);
-  const PM_GLOBAL uint2* q = gp(a.jobs) + static_cast<int64_t>(xcd) * a.job_cap;
   const int64_t pl = a.ref_plane;
-  for (uint32_t c = j * kWave; c < count; c += nb * kWave) {
-    const uint32_t qi = c + threadIdx.x;
-    if (qi >= count) continue;
-    const uint2 job = q[qi];
-    const int x = static_cast<int>(job.x & 0xffffu), y = static_cast<int>(job.x >> 16);
-    const int slot = static_cast<int>(job.y & 0xffu), s = static_cast<int>(job.y >> 8);
-    const int cidx = colour_index(a, x, y);
-    Plane pl4;
-    if (a.phase == 0) {
-      const int pos = slot < 8 ? gp(a.cpos)[slot * pl + cidx] : y * a.width + x;
-      pl4 = to_plane(gp(a.planes)[pos]);
-    } else {
-      pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
+  const PmCam c0 = kcam(a, 0);
+#pragma unroll 1
+  for (int s = 0; s < a.n_src; ++s) {
+    const uint32_t count = gp(a.jobcount)[xcd * kMaxSrc + s];
+    if (j * kWave >= count) continue;
+    const PM_GLOBAL uint2* q = gp(a.jobs) + static_cast<int64_t>(xcd * a.n_src + s) * a.job_cap;
+    const PmCam cs = kcam(a, s + 1);
+    const Sampler sm = make_sampler<T>(a, kview(a, s + 1));
+#pragma unroll 1
+    for (uint32_t c = j * kWave; c < count; c += nb * kWave) {
+      const uint32_t qi = c + threadIdx.x;
+      if (qi >= count) continue;
+      const uint2 job = q[qi];
+      const int x = static_cast<int>(job.x & 0xffffu), y = static_cast<int>(job.x >> 16);
+      const int slot = static_cast<int>(job.y);
+      const int cidx = colour_index(a, x, y);
+      Plane pl4;
+      if (a.phase == 0) {
+        const int pos = slot < 8 ? gp(a.cpos)[slot * pl + cidx] : y * a.width + x;
+        pl4 = to_plane(gp(a.planes)[pos]);
+      } else {
+        pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
+      }
+      float* out = a.ca + (static_cast<int64_t>(slot) * a.n_src + s) * pl + cidx;
+      PlanePts pp;
+      plane_points(c0, x, y, pl4, pp);
+      Warp w;
+      warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
+      const float scen = sample<T>(sm, w.scx, w.scy);
+      float best = *out;  // stride-1 NCC
+#pragma unroll 1
+      for (int si = 1; si < 3; ++si) {
+        const float ncc = ncc_at_stride_u<HP, T>(a, img, sm, si, x, y, cidx, scen, w);
+        if (ncc > best) best = ncc;
+        if (best > 0.1f) break;
+      }
+      float cost;
+      if (best <= -0.5f) {
+        const float ref_center = gp(a.ref_sums)[cidx].x;
+        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
+      } else {
+        cost = 1.0f - best;
+      }
+      *out = cost;
     }
-    float* out = a.ca + (static_cast<int64_t>(slot) * a.n_src + s) * pl + cidx;
-    PlanePts pp;
-    plane_points(kcam(a, 0), x, y, pl4, pp);
-    const PmView sv = lane_view(a, s);
-    const Sampler sm = make_sampler<T>(a, sv);
-    Warp w;
-    warp_of(pp.cw, pp.wdx, pp.wdy, lane_cam(a, s), w);  // valid: it was valid at stride 1
-    const float scen = sample<T>(sm, w.scx, w.scy);
-    float best = *out;  // stride-1 NCC
-    for (int si = 1; si < 3; ++si) {
-      const float ncc = ncc_at_stride_u<HP, T>(a, sm, si, cidx, scen, w);
-      if (ncc > best) best = ncc;
-      if (best > 0.1f) break;
-    }
-    float cost;
-    if (best <= -0.5f) {
-      const float ref_center = gp(a.ref_sums)[cidx].x;
-      cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
-    } else {
-      cost = 1.0f - best;
-    }
-    *out = cost;
   }
 }
 
@@ -899,7 +948,15 @@ __device__ __forceinline__ int candidate_position(const PmArgs& a, int h, int x,
 }
 
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave * 9) void pass_candidates_kernel(PmArgs a) {
+#ifndef PM_CAND_WAVES
+#define PM_CAND_WAVES 0
+#endif
+#if PM_CAND_WAVES
+#define PM_CAND_ATTR __attribute__((amdgpu_waves_per_eu(PM_CAND_WAVES)))
+#else
+#define PM_CAND_ATTR
+#endif
+__global__ __launch_bounds__(kWave * 9) PM_CAND_ATTR void pass_candidates_kernel(PmArgs a) {
   ColourPixel cp;
   if (!colour_pixel(a, cp)) return;
   const int h = threadIdx.y;  // 0..7 candidates, 8 = current plane (one wave each)
@@ -1106,7 +1163,7 @@ __global__ __launch_bounds__(kWave) void pass_select_kernel(PmArgs a) {
 }
 
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave * 6) void pass_refine_cost_kernel(PmArgs a) {
+__global__ __launch_bounds__(kWave * 6) PM_CAND_ATTR void pass_refine_cost_kernel(PmArgs a) {
   ColourPixel cp;
   if (!colour_pixel(a, cp)) return;
   const int r = threadIdx.y;  // one wave per refinement hypothesis
@@ -1331,13 +1388,16 @@ hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t strea
   return hipGetLastError();
 }
 
-int64_t fallback_queue_capacity(int width, int height, int n_src) {
+int64_t fallback_queue_capacity(int width, int height) {
   // XCD k owns at most ceil(tiles / 8) colour tiles of 64 pixels, each with
-  // at most 9 slots x n_src pending views.
+  // at most 9 pending slots per view.
   const int hw = (width + 1) / 2;
   const int64_t tiles = static_cast<int64_t>((hw + kTileX - 1) / kTileX) * ((height + kTileY - 1) / kTileY);
-  return ((tiles + kXcds - 1) / kXcds) * kWave * 9 * n_src;
+  return ((tiles + kXcds - 1) / kXcds) * kWave * 9;
 }
+
+int fallback_queue_count(int n_src) { return kXcds * n_src; }
+int job_counter_count() { return kJobCounters; }
 
 hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(refpatch_kernel, hp, f16, full_grid(a), 0, stream, a);
@@ -1354,7 +1414,7 @@ hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
 }
 
 hipError_t launch_zero_counters(const PmArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kWave), 0, stream, a.jobcount, kXcds);
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kJobCounters), 0, stream, a.jobcount, kJobCounters);
   return hipGetLastError();
 }
 
