@@ -23,7 +23,8 @@
  *                               (estimator.cc:53-116)
  *   pm_set_prior                the prior_planes / plane_masks buffers that
  *                               ComputeSfMPlanarPrior uploads (estimator.cc:838-1107)
- *   pm_compute                  Run(DepthmapResult*) (estimator.cc:134-255 +
+ *   pm_compute                  Run(DepthmapResult*) (estimator.cc:134-255 with
+ *                               RunLevel, BuildImagePyramid, UpsampleDepthNormal +
  *                               ReadBackResults :1112-1224), compute_patch_match()
  *                               (pydense.pyi:54)
  *   pm_upload / pm_run /        the same, split so the device-resident part can be
@@ -78,13 +79,18 @@ typedef struct pm_params {
   int top_k;
   int use_census;           /* 0/1 */
   float depth_min, depth_max;
-  int hierarchy_levels;     /* only 1 is supported */
+  int hierarchy_levels;     /* 1..8 pyramid levels, coarse to fine (estimator.cc:134-255);
+                               level l of N runs at 1/2^(N-1-l) of the input size */
   float smooth_weight;
   int anchor_views;
   int checkerboard_filter;  /* 0/1 */
   float var_threshold;      /* weighted patch variance gate, [0,1]^2 units */
   int image_storage;        /* PM_STORE_F32 / PM_STORE_F16 */
   int quantize_half;        /* 1: round inputs to binary16 like the reference upload */
+  int speckle_min_size;     /* > 0: drop connected depth segments smaller than this
+                               (RemoveSmallSegments, estimator.cc:1231-1312); 0 = off */
+  int gap_max_size;         /* > 0: fill row/column depth gaps up to this length
+                               (GapInterpolation, estimator.cc:1319-1418); 0 = off */
 } pm_params;
 
 void pm_default_params(pm_params* out);
@@ -94,6 +100,10 @@ const char* pm_last_error(void);
 
 int pm_device_count(int* count);
 int pm_device_name(int device, char* buf, int buflen);
+/* Total and currently free device memory in bytes (GlobalMemSize of
+ * DepthmapClusterEstimator.device_memory_bytes, pybind.cc:105-122).  Either
+ * pointer may be NULL. */
+int pm_device_memory(int device, uint64_t* total_bytes, uint64_t* free_bytes);
 
 int pm_create(int device, pm_handle** out);
 int pm_destroy(pm_handle* h);
@@ -112,17 +122,42 @@ int pm_set_params(pm_handle* h, const pm_params* p);
 int pm_get_params(pm_handle* h, pm_params* p);
 
 /* Geometric consistency: weight and, per source slot, a previous depth map
- * at the reference resolution (NULL = none). */
+ * at the reference resolution (NULL entry = none); prev_depths = NULL sets
+ * the weight only.  Coarser pyramid levels use the maps bilinearly resized
+ * (UploadPreviousDepths, estimator.cc:66-116). */
 int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev_depths, int n);
 
-/* Planar prior: (nx, ny, nz, d) per reference pixel + mask (NULL clears). */
+/* One previous depth map for source slot `slot` (0-based), any size; it is
+ * resized to each level bilinearly.  depth = NULL clears the slot.
+ * SetPreviousDepths / ClearPreviousDepths (estimator.cc:59-64); the weight
+ * comes from pm_set_geom_consistency (prev_depths may be NULL there). */
+int pm_set_previous_depth(pm_handle* h, int slot, const float* depth, int width, int height);
+int pm_clear_previous_depths(pm_handle* h);
+
+/* Planar prior: (nx, ny, nz, d) per reference pixel + mask (NULL clears).
+ * Planes live in the camera frame, so a coarser pyramid level takes the
+ * prior of the full-resolution pixel at the centre of its footprint. */
 int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks);
 
-/* Stage views/cameras in device memory and allocate the per-pixel state. */
+/* Stage views/cameras in device memory and allocate the per-pixel state
+ * (full resolution; with several hierarchy levels pm_run stages each level
+ * itself and ends with the full-resolution level staged). */
 int pm_upload(pm_handle* h);
-/* Random init, prior re-init, iterations, optional filter and decode, all on
- * the device; returns once the stream is idle. */
+/* Per hierarchy level, coarse to fine: random init (coarsest) or transfer
+ * from the decoded coarser level, prior re-init, iterations, optional
+ * filter and decode, all on the device; returns once the stream is idle. */
 int pm_run(pm_handle* h);
+/* One pyramid level of params.hierarchy_levels (RunLevel, estimator.cc:
+ * 172-255), for callers that feed other views' intermediate depths between
+ * levels (DepthmapClusterEstimator::Run, cluster_estimator.cc:82-156).
+ * Levels run in order from 0; the decoded maps of the level can be read with
+ * pm_download (their size from pm_level_size); the last level is full size. */
+int pm_run_level(pm_handle* h, int level);
+int pm_level_size(pm_handle* h, int* width, int* height);
+/* Free the device memory of the handle except the decoded coarser level that
+ * the next pm_run_level starts from (ReleaseGpuBuffers, estimator.cc:1420). */
+int pm_release_device(pm_handle* h);
+
 /* Copy the decoded maps out.  Any pointer may be NULL.
  * depth, cost, confidence: H*W float; plane: H*W*4 float (n, d);
  * normal: H*W*3 float; views: H*W uint32 selected-view bitmask. */

@@ -89,7 +89,8 @@ float f16_bits_to_f32(uint16_t h) {
 struct HostView {
   double K[9], R[9], t[3];
   int w, h;
-  std::vector<float> pix;  // [0,1] values as given (u8 already /255)
+  std::vector<float> pix;    // [0,1] values as given (u8 already /255)
+  std::vector<uint8_t> raw;  // the u8 input, kept for the image pyramid
 };
 
 template <typename T>
@@ -124,7 +125,13 @@ struct pm_handle {
   pm_params params{};
   std::vector<HostView> views;
   float geom_weight = 0.0f;
-  std::vector<std::vector<float>> prev_depths;  // per source slot, empty = none
+  struct PrevDepth {
+    int w = 0, h = 0;
+    std::vector<float> d;  // empty = no previous depth for this source slot
+  };
+  std::vector<PrevDepth> prev_depths = std::vector<PrevDepth>(PM_MAX_SOURCES);
+  int run_levels = 0;        // levels of the run in progress (pm_run_level)
+  int next_level = 0;        // next level pm_run_level accepts
   std::vector<float> prior_planes;
   std::vector<uint32_t> prior_masks;
 
@@ -140,9 +147,7 @@ struct pm_handle {
   DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
-  DevBuf<uint2> jobs;
-  DevBuf<uint32_t> jobcount;
-  int64_t job_cap = 0;
+  DevBuf<uint64_t> fb_masks;   // pending textureless fallbacks per (tile, slot, view)
   DevBuf<float2> ref_taps;     // reference-side patch data, strides 1..3
   DevBuf<float4> ref_sums;
   int64_t ref_plane = 0;
@@ -153,6 +158,8 @@ struct pm_handle {
   DevBuf<uint32_t> sel, masks;
   uint32_t prev_mask = 0u;
   DevBuf<float> out_depth, out_normal, out_cost, out_conf;
+  DevBuf<float> coarse_depth, coarse_normal;  // decoded previous pyramid level
+  int coarse_w = 0, coarse_h = 0;
   float last_run_ms = 0.0f, last_sweep_ms = 0.0f;
   int last_sweeps = 0;
   // Per-kernel device time of the last pm_run: HIP events around every
@@ -167,10 +174,11 @@ struct pm_handle {
   ~pm_handle() {
     images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
     ca.release(); vwb.release(); wnb.release(); cpos.release(); nref.release(); refp.release();
-    jobs.release(); jobcount.release();
+    fb_masks.release();
     planes.release(); prior.release();
     snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
     out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
+    coarse_depth.release(); coarse_normal.release();
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : kev) if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -208,9 +216,7 @@ PmArgs make_args(pm_handle* h) {
   a.wnb = h->wnb.p;
   a.refp = h->refp.p;
   a.nref = h->nref.p;
-  a.jobs = h->jobs.p;
-  a.jobcount = h->jobcount.p;
-  a.job_cap = h->job_cap;
+  a.fb_masks = h->fb_masks.p;
   a.phase = 0;
   a.width = h->W;
   a.height = h->H;
@@ -258,8 +264,13 @@ int validate(pm_handle* h) {
   if (!(p.depth_min > 0.0f) || !(p.depth_max > p.depth_min))
     return fail(PM_E_INVALID, "depth range must satisfy 0 < min < max");
   if (p.max_iterations < 0) return fail(PM_E_INVALID, "max_iterations must be >= 0");
-  if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "only hierarchy_levels = 1 is supported");
+  if (p.hierarchy_levels < 1 || p.hierarchy_levels > 8)
+    return fail(PM_E_INVALID, "hierarchy_levels must be in 1..8");
+  const int div = 1 << (p.hierarchy_levels - 1);
+  if (h->views[0].w / div < 8 || h->views[0].h / div < 8)
+    return fail(PM_E_INVALID, "hierarchy_levels too large for the reference image (coarsest level below 8 px)");
   if (!(p.sigma_spatial > 0.0f) || !(p.sigma_color > 0.0f)) return fail(PM_E_INVALID, "sigmas must be > 0");
+  if (p.speckle_min_size < 0 || p.gap_max_size < 0) return fail(PM_E_INVALID, "speckle/gap sizes must be >= 0");
   return PM_OK;
 }
 
@@ -316,6 +327,8 @@ void pm_default_params(pm_params* o) {
   o->var_threshold = 1.5e-6f;
   o->image_storage = PM_STORE_F32;
   o->quantize_half = 0;
+  o->speckle_min_size = 0;  // config.py: depthmap_speckle_min_size
+  o->gap_max_size = 0;      // config.py: depthmap_gap_max_size
 }
 
 const char* pm_last_error(void) { return g_last_error.c_str(); }
@@ -333,6 +346,23 @@ int pm_device_name(int device, char* buf, int buflen) {
   hipDeviceProp_t prop;
   PM_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   std::snprintf(buf, static_cast<size_t>(buflen), "%s (%s)", prop.name, prop.gcnArchName);
+  return PM_OK;
+}
+
+int pm_device_memory(int device, uint64_t* total_bytes, uint64_t* free_bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(PM_E_DEVICE, "no HIP device is available on this system");
+  if (device < 0 || device >= n) return fail(PM_E_INVALID, "device index out of range");
+  int prev = 0;
+  PM_HIP(hipGetDevice(&prev), "hipGetDevice");
+  PM_HIP(hipSetDevice(device), "hipSetDevice");
+  size_t fr = 0, tot = 0;
+  const hipError_t e = hipMemGetInfo(&fr, &tot);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) return fail(PM_E_DEVICE, std::string("hipMemGetInfo: ") + hipGetErrorString(e));
+  if (total_bytes) *total_bytes = tot;
+  if (free_bytes) *free_bytes = fr;
   return PM_OK;
 }
 
@@ -393,6 +423,7 @@ int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t,
     for (int x = 0; x < width; ++x) {
       if (format == PM_U8) {
         dst[x] = static_cast<float>(row[x]) * inv255;  // estimator.cc:315
+        v.raw.push_back(row[x]);
       } else if (format == PM_F32) {
         std::memcpy(&dst[x], row + 4 * x, 4);
       } else {
@@ -454,11 +485,35 @@ int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev
   if (int r = check_handle(h)) return r;
   if (n < 0 || n > PM_MAX_SOURCES) return fail(PM_E_INVALID, "too many previous depth maps");
   if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before previous depths");
-  const size_t npix = static_cast<size_t>(h->views[0].w) * h->views[0].h;
   h->geom_weight = weight;
-  h->prev_depths.assign(static_cast<size_t>(n), std::vector<float>());
-  for (int i = 0; i < n; ++i)
-    if (prev_depths && prev_depths[i]) h->prev_depths[i].assign(prev_depths[i], prev_depths[i] + npix);
+  if (prev_depths) {
+    for (auto& e : h->prev_depths) e = pm_handle::PrevDepth{};
+    const int w = h->views[0].w, hh = h->views[0].h;
+    for (int i = 0; i < n; ++i)
+      if (prev_depths[i]) h->prev_depths[i] = pm_handle::PrevDepth{w, hh, std::vector<float>(prev_depths[i],
+                                                                   prev_depths[i] + static_cast<size_t>(w) * hh)};
+  }
+  h->uploaded = false;
+  return PM_OK;
+}
+
+int pm_set_previous_depth(pm_handle* h, int slot, const float* depth, int width, int height) {
+  if (int r = check_handle(h)) return r;
+  if (slot < 0 || slot >= PM_MAX_SOURCES) return fail(PM_E_INVALID, "previous depth: source slot out of range");
+  if (!depth) {
+    h->prev_depths[slot] = pm_handle::PrevDepth{};
+  } else {
+    if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "previous depth: empty map");
+    h->prev_depths[slot] =
+        pm_handle::PrevDepth{width, height, std::vector<float>(depth, depth + static_cast<size_t>(width) * height)};
+  }
+  h->uploaded = false;
+  return PM_OK;
+}
+
+int pm_clear_previous_depths(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  for (auto& e : h->prev_depths) e = pm_handle::PrevDepth{};
   h->uploaded = false;
   return PM_OK;
 }
@@ -478,14 +533,229 @@ int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks)
   return PM_OK;
 }
 
-int pm_upload(pm_handle* h) {
-  if (int r = check_handle(h)) return r;
-  if (int r = validate(h)) return r;
+}  // extern "C"
+
+namespace {
+
+// Pixel-area downscale of a w x h image to nw x nh, the resampling
+// cv::resize(INTER_AREA) performs for BuildImagePyramid (estimator.cc:
+// 727-756): each output pixel is the mean of the input over its footprint,
+// pixels on the footprint's edge weighted by the covered fraction.  Sums in
+// double; with round_int the result is rounded to an integer (u8 images).
+void resize_area(const float* src, int w, int h, int nw, int nh, bool round_int, float* dst) {
+  const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
+  for (int dy = 0; dy < nh; ++dy) {
+    const double y0 = dy * sy, y1 = std::min((dy + 1) * sy, static_cast<double>(h));
+    const int iy0 = static_cast<int>(std::floor(y0)), iy1 = std::min(h, static_cast<int>(std::ceil(y1)));
+    for (int dx = 0; dx < nw; ++dx) {
+      const double x0 = dx * sx, x1 = std::min((dx + 1) * sx, static_cast<double>(w));
+      const int ix0 = static_cast<int>(std::floor(x0)), ix1 = std::min(w, static_cast<int>(std::ceil(x1)));
+      double acc = 0.0;
+      for (int yy = iy0; yy < iy1; ++yy) {
+        const double wy = std::min(y1, yy + 1.0) - std::max(y0, static_cast<double>(yy));
+        double row = 0.0;
+        for (int xx = ix0; xx < ix1; ++xx) {
+          const double wx = std::min(x1, xx + 1.0) - std::max(x0, static_cast<double>(xx));
+          row += wx * static_cast<double>(src[static_cast<size_t>(yy) * w + xx]);
+        }
+        acc += wy * row;
+      }
+      double v = acc / ((x1 - x0) * (y1 - y0));
+      if (round_int) v = std::nearbyint(v);
+      dst[static_cast<size_t>(dy) * nw + dx] = static_cast<float>(v);
+    }
+  }
+}
+
+// Bilinear resize with pixel-centre alignment and edge clamping, the
+// cv::resize(INTER_LINEAR) of UploadPreviousDepths (estimator.cc:92-101).
+void resize_linear(const float* src, int w, int h, int nw, int nh, float* dst) {
+  const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
+  for (int dy = 0; dy < nh; ++dy) {
+    double fy = (dy + 0.5) * sy - 0.5;
+    int y0 = static_cast<int>(std::floor(fy));
+    double ay = fy - y0;
+    if (y0 < 0) { y0 = 0; ay = 0.0; }
+    if (y0 >= h - 1) { y0 = h - 1; ay = 0.0; }
+    const int y1 = std::min(y0 + 1, h - 1);
+    for (int dx = 0; dx < nw; ++dx) {
+      double fx = (dx + 0.5) * sx - 0.5;
+      int x0 = static_cast<int>(std::floor(fx));
+      double ax = fx - x0;
+      if (x0 < 0) { x0 = 0; ax = 0.0; }
+      if (x0 >= w - 1) { x0 = w - 1; ax = 0.0; }
+      const int x1 = std::min(x0 + 1, w - 1);
+      const double v00 = src[static_cast<size_t>(y0) * w + x0], v01 = src[static_cast<size_t>(y0) * w + x1];
+      const double v10 = src[static_cast<size_t>(y1) * w + x0], v11 = src[static_cast<size_t>(y1) * w + x1];
+      const double v = (1.0 - ay) * ((1.0 - ax) * v00 + ax * v01) + ay * ((1.0 - ax) * v10 + ax * v11);
+      dst[static_cast<size_t>(dy) * nw + dx] = static_cast<float>(v);
+    }
+  }
+}
+
+// One pyramid level of a view (BuildImagePyramid, estimator.cc:727-756):
+// size int(w * scale), area-resampled pixels, fx, cx scaled by new_w / w
+// and fy, cy by new_h / h.  u8 inputs are resampled as u8.
+HostView scaled_view(const HostView& v, float scale) {
+  HostView o;
+  o.w = std::max(static_cast<int>(static_cast<float>(v.w) * scale), 1);
+  o.h = std::max(static_cast<int>(static_cast<float>(v.h) * scale), 1);
+  std::memcpy(o.K, v.K, sizeof(o.K));
+  std::memcpy(o.R, v.R, sizeof(o.R));
+  std::memcpy(o.t, v.t, sizeof(o.t));
+  const float fsx = static_cast<float>(o.w) / static_cast<float>(v.w);
+  const float fsy = static_cast<float>(o.h) / static_cast<float>(v.h);
+  o.K[0] *= fsx;
+  o.K[2] *= fsx;
+  o.K[4] *= fsy;
+  o.K[5] *= fsy;
+  o.pix.resize(static_cast<size_t>(o.w) * o.h);
+  if (!v.raw.empty()) {
+    std::vector<float> src(v.raw.begin(), v.raw.end());
+    resize_area(src.data(), v.w, v.h, o.w, o.h, true, o.pix.data());
+    const float inv255 = static_cast<float>(1.0 / 255.0);
+    for (float& f : o.pix) f *= inv255;
+  } else {
+    resize_area(v.pix.data(), v.w, v.h, o.w, o.h, false, o.pix.data());
+  }
+  return o;
+}
+
+// Speckle removal (RemoveSmallSegments, estimator.cc:1231-1312): 4-connected
+// segments of valid depths whose neighbours differ by less than `thr`
+// relative depth; segments smaller than min_size lose depth and normal.
+void remove_small_segments(std::vector<float>& depth, std::vector<float>& normal, int w, int h, int min_size,
+                           float thr) {
+  const int npix = w * h;
+  std::vector<char> visited(static_cast<size_t>(npix), 0);
+  std::vector<int> seg;
+  const int dx[4] = {1, -1, 0, 0};
+  const int dy[4] = {0, 0, 1, -1};
+  for (int idx0 = 0; idx0 < npix; ++idx0) {
+    if (visited[idx0] || depth[idx0] <= 0.0f) {
+      visited[idx0] = 1;
+      continue;
+    }
+    seg.clear();
+    seg.push_back(idx0);
+    visited[idx0] = 1;
+    for (size_t head = 0; head < seg.size(); ++head) {
+      const int cur = seg[head];
+      const int cy = cur / w, cx = cur % w;
+      const float dc = depth[cur];
+      for (int k = 0; k < 4; ++k) {
+        const int nx = cx + dx[k], ny = cy + dy[k];
+        if (nx < 0 || nx >= w || ny < 0 || ny >= h) continue;
+        const int ni = ny * w + nx;
+        if (visited[ni]) continue;
+        const float dn = depth[ni];
+        if (dn <= 0.0f) {
+          visited[ni] = 1;
+          continue;
+        }
+        if (std::fabs(dc - dn) / std::max(dc, dn) < thr) {
+          visited[ni] = 1;
+          seg.push_back(ni);
+        }
+      }
+    }
+    if (static_cast<int>(seg.size()) < min_size) {
+      for (int i : seg) {
+        depth[i] = 0.0f;
+        normal[3 * i] = normal[3 * i + 1] = normal[3 * i + 2] = 0.0f;
+      }
+    }
+  }
+}
+
+// One gap fill between valid samples a (depth da, normal na) and b, at
+// fraction t (GapInterpolation, estimator.cc:1348-1362).
+inline void gap_fill(float da, const float* na, float db, const float* nb, float t, float* d, float* n) {
+  *d = da + t * (db - da);
+  float v[3];
+  for (int k = 0; k < 3; ++k) v[k] = (1.0f - t) * na[k] + t * nb[k];
+  const float len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (len > 1e-6f)
+    for (int k = 0; k < 3; ++k) v[k] /= len;
+  for (int k = 0; k < 3; ++k) n[k] = v[k];
+}
+
+// Gap interpolation (GapInterpolation, estimator.cc:1319-1418): runs of at
+// most max_gap invalid pixels between two valid ones within `thr` relative
+// depth are filled linearly, rows first, then columns (only still-empty
+// pixels).
+void gap_interpolation(std::vector<float>& depth, std::vector<float>& normal, int w, int h, int max_gap, float thr) {
+  for (int y = 0; y < h; ++y) {
+    int gs = -1;
+    for (int x = 0; x < w; ++x) {
+      const int i = y * w + x;
+      const float d = depth[i];
+      if (d <= 0.0f) {
+        if (gs < 0) gs = x;
+        continue;
+      }
+      if (gs >= 0) {
+        const int lx = gs - 1;
+        if (lx >= 0 && x - gs <= max_gap) {
+          const float dl = depth[y * w + lx];
+          if (std::fabs(dl - d) / std::max(dl, d) < thr) {
+            const float nl[3] = {normal[3 * (y * w + lx)], normal[3 * (y * w + lx) + 1], normal[3 * (y * w + lx) + 2]};
+            const float nr[3] = {normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]};
+            for (int gx = gs; gx < x; ++gx) {
+              const float t = static_cast<float>(gx - lx) / static_cast<float>(x - lx);
+              gap_fill(dl, nl, d, nr, t, &depth[y * w + gx], &normal[3 * (y * w + gx)]);
+            }
+          }
+        }
+      }
+      gs = -1;
+    }
+  }
+  for (int x = 0; x < w; ++x) {
+    int gs = -1;
+    for (int y = 0; y < h; ++y) {
+      const int i = y * w + x;
+      const float d = depth[i];
+      if (d <= 0.0f) {
+        if (gs < 0) gs = y;
+        continue;
+      }
+      if (gs >= 0) {
+        const int ty = gs - 1;
+        if (ty >= 0 && y - gs <= max_gap) {
+          const float dt = depth[ty * w + x];
+          if (std::fabs(dt - d) / std::max(dt, d) < thr) {
+            const float nt[3] = {normal[3 * (ty * w + x)], normal[3 * (ty * w + x) + 1], normal[3 * (ty * w + x) + 2]};
+            const float nb[3] = {normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]};
+            for (int gy = gs; gy < y; ++gy) {
+              const float t = static_cast<float>(gy - ty) / static_cast<float>(y - ty);
+              if (depth[gy * w + x] <= 0.0f) gap_fill(dt, nt, d, nb, t, &depth[gy * w + x], &normal[3 * (gy * w + x)]);
+            }
+          }
+        }
+      }
+      gs = -1;
+    }
+  }
+}
+
+int level_count(const pm_params& p) { return std::max(1, p.hierarchy_levels); }
+
+// Stages the views of pyramid level `level` of `nlevels` in device memory and
+// sizes the per-pixel state for it (UploadData, estimator.cc:299-510).
+int upload_level(pm_handle* h, int level, int nlevels) {
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  // Level views: level l of N is scaled by 1 / 2^(N-1-l) (estimator.cc:182-201).
+  std::vector<HostView> scaled;
+  const double scale = 1.0 / std::pow(2.0, nlevels - 1 - level);
+  if (scale < 0.999) {
+    for (const HostView& v : h->views) scaled.push_back(scaled_view(v, static_cast<float>(scale)));
+  }
+  const std::vector<HostView>& views = scale < 0.999 ? scaled : h->views;
   const pm_params& p = h->params;
-  const int nv = static_cast<int>(h->views.size());
-  h->W = h->views[0].w;
-  h->H = h->views[0].h;
+  const int nv = static_cast<int>(views.size());
+  h->W = views[0].w;
+  h->H = views[0].h;
   h->n_src = nv - 1;
   h->f16 = p.image_storage == PM_STORE_F16;
   const size_t esz = h->f16 ? 2 : 4;
@@ -494,7 +764,7 @@ int pm_upload(pm_handle* h) {
   std::vector<PmView> vt(static_cast<size_t>(nv));
   int64_t total = 0, qtotal = 0;
   for (int i = 0; i < nv; ++i) {
-    const HostView& v = h->views[static_cast<size_t>(i)];
+    const HostView& v = views[static_cast<size_t>(i)];
     vt[i].qbase = qtotal;
     qtotal += static_cast<int64_t>(v.w + 2) * (v.h + 2);
     vt[i].base = total;
@@ -506,7 +776,7 @@ int pm_upload(pm_handle* h) {
   }
   std::vector<unsigned char> host(static_cast<size_t>(total) * esz, 0);
   for (int i = 0; i < nv; ++i) {
-    const HostView& v = h->views[static_cast<size_t>(i)];
+    const HostView& v = views[static_cast<size_t>(i)];
     for (int y = -1; y <= v.h; ++y) {
       const int cy = std::min(std::max(y, 0), v.h - 1);
       for (int x = -1; x <= v.w; ++x) {
@@ -525,8 +795,8 @@ int pm_upload(pm_handle* h) {
   // Cameras, source camera centres (float, opencl_kernels.h:1523-1525) and
   // angle weights (float64 then cast, estimator.cc:380-407).
   std::vector<PmCam> cams(static_cast<size_t>(nv));
-  for (int i = 0; i < nv; ++i) cams[i] = make_cam(h->views[static_cast<size_t>(i)]);
-  const HostView& v0 = h->views[0];
+  for (int i = 0; i < nv; ++i) cams[i] = make_cam(views[static_cast<size_t>(i)]);
+  const HostView& v0 = views[0];
   const double c0[3] = {-(v0.R[0] * v0.t[0] + v0.R[3] * v0.t[1] + v0.R[6] * v0.t[2]),
                         -(v0.R[1] * v0.t[0] + v0.R[4] * v0.t[1] + v0.R[7] * v0.t[2]),
                         -(v0.R[2] * v0.t[0] + v0.R[5] * v0.t[1] + v0.R[8] * v0.t[2])};
@@ -535,7 +805,7 @@ int pm_upload(pm_handle* h) {
     vt[i].cx = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
     vt[i].cy = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
     vt[i].cz = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
-    const HostView& v = h->views[static_cast<size_t>(i)];
+    const HostView& v = views[static_cast<size_t>(i)];
     const double ci[3] = {-(v.R[0] * v.t[0] + v.R[3] * v.t[1] + v.R[6] * v.t[2]),
                           -(v.R[1] * v.t[0] + v.R[4] * v.t[1] + v.R[7] * v.t[2]),
                           -(v.R[2] * v.t[0] + v.R[5] * v.t[1] + v.R[8] * v.t[2])};
@@ -565,9 +835,7 @@ int pm_upload(pm_handle* h) {
   if (int r = h->cpos.alloc(8 * rp)) return r;
   if (int r = h->nref.alloc(rp)) return r;
   if (int r = h->refp.alloc(6 * rp)) return r;
-  h->job_cap = fallback_queue_capacity(h->W, h->H);
-  if (int r = h->jobs.alloc(static_cast<size_t>(fallback_queue_count(h->n_src)) * h->job_cap)) return r;
-  if (int r = h->jobcount.alloc(static_cast<size_t>(job_counter_count()))) return r;
+  if (int r = h->fb_masks.alloc(static_cast<size_t>(fallback_mask_count(h->W, h->H, h->n_src)))) return r;
   h->ref_ready = false;
   if (int r = h->planes.alloc(npix)) return r;
   if (int r = h->costs.alloc(npix)) return r;
@@ -594,10 +862,25 @@ int pm_upload(pm_handle* h) {
     if (int r = h->prior.alloc(npix)) return r;
     if (int r = h->masks.alloc(npix)) return r;
     if (int r = h->snapshot.alloc(npix)) return r;
-    PM_HIP(hipMemcpyAsync(h->prior.p, h->prior_planes.data(), npix * sizeof(float4), hipMemcpyHostToDevice,
-                          h->stream), "upload prior");
-    PM_HIP(hipMemcpyAsync(h->masks.p, h->prior_masks.data(), npix * sizeof(uint32_t), hipMemcpyHostToDevice,
-                          h->stream), "upload masks");
+    // Planes are in the reference camera frame, so a coarse level takes the
+    // prior of the full-resolution pixel at the centre of its footprint.
+    std::vector<float> pp(4 * npix);
+    std::vector<uint32_t> pm(npix);
+    const int fw = h->views[0].w, fh = h->views[0].h;
+    for (int y = 0; y < h->H; ++y) {
+      const int fy = std::min(static_cast<int>((y + 0.5) * fh / h->H), fh - 1);
+      for (int x = 0; x < h->W; ++x) {
+        const int fx = std::min(static_cast<int>((x + 0.5) * fw / h->W), fw - 1);
+        const size_t si = static_cast<size_t>(fy) * fw + fx, di = static_cast<size_t>(y) * h->W + x;
+        std::memcpy(&pp[4 * di], &h->prior_planes[4 * si], 4 * sizeof(float));
+        pm[di] = h->prior_masks[si];
+      }
+    }
+    PM_HIP(hipMemcpyAsync(h->prior.p, pp.data(), npix * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+           "upload prior");
+    PM_HIP(hipMemcpyAsync(h->masks.p, pm.data(), npix * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream),
+           "upload masks");
+    PM_HIP(hipStreamSynchronize(h->stream), "upload sync");
   } else {
     h->prior.release();
     h->masks.release();
@@ -605,13 +888,19 @@ int pm_upload(pm_handle* h) {
   }
   h->prev_mask = 0u;
   bool any_prev = false;
-  for (auto& d : h->prev_depths) any_prev |= !d.empty();
+  for (int i = 0; i < h->n_src; ++i) any_prev |= !h->prev_depths[i].d.empty();
   if (any_prev && h->geom_weight > 0.0f) {
     if (int r = h->prevd.alloc(static_cast<size_t>(h->n_src) * npix)) return r;
     std::vector<float> flat(static_cast<size_t>(h->n_src) * npix, 0.0f);
-    for (int i = 0; i < static_cast<int>(h->prev_depths.size()) && i < h->n_src; ++i) {
-      if (h->prev_depths[i].empty()) continue;
-      std::memcpy(&flat[static_cast<size_t>(i) * npix], h->prev_depths[i].data(), npix * sizeof(float));
+    for (int i = 0; i < h->n_src; ++i) {
+      const pm_handle::PrevDepth& e = h->prev_depths[i];
+      if (e.d.empty()) continue;
+      float* dst = &flat[static_cast<size_t>(i) * npix];
+      if (e.w == h->W && e.h == h->H) {
+        std::memcpy(dst, e.d.data(), npix * sizeof(float));
+      } else {  // UploadPreviousDepths resizes to the level (estimator.cc:92-101)
+        resize_linear(e.d.data(), e.w, e.h, h->W, h->H, dst);
+      }
       h->prev_mask |= 1u << i;
     }
     PM_HIP(hipMemcpyAsync(h->prevd.p, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, h->stream),
@@ -626,6 +915,56 @@ int pm_upload(pm_handle* h) {
   return PM_OK;
 }
 
+
+}  // namespace
+
+extern "C" {
+
+int pm_upload(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  if (int r = validate(h)) return r;
+  const int n = level_count(h->params);
+  return upload_level(h, n - 1, n);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Prior re-seeding (PriorReinit, estimator.cc:574-614), when a prior is set.
+int run_prior(pm_handle* h, const PmArgs& a) {
+  if (!h->masks.p) return PM_OK;
+  PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
+                        h->stream), "prior snapshot");
+  PM_HIP(launch_prior(a, h->params.patch_size / 2, h->f16, h->stream), "prior kernel launch");
+  return PM_OK;
+}
+
+// Host post-processing of the decoded level (ReadBackResults,
+// estimator.cc:1194-1222): speckle removal, gap filling, then confidence
+// again for the changed depth map.
+int postprocess(pm_handle* h) {
+  const pm_params& p = h->params;
+  if (p.speckle_min_size <= 0 && p.gap_max_size <= 0) return PM_OK;
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  std::vector<float> depth(npix), normal(3 * npix);
+  PM_HIP(hipMemcpyAsync(depth.data(), h->out_depth.p, npix * 4, hipMemcpyDeviceToHost, h->stream), "download");
+  PM_HIP(hipMemcpyAsync(normal.data(), h->out_normal.p, npix * 12, hipMemcpyDeviceToHost, h->stream), "download");
+  PM_HIP(hipStreamSynchronize(h->stream), "postprocess sync");
+  if (p.speckle_min_size > 0) remove_small_segments(depth, normal, h->W, h->H, p.speckle_min_size, 0.01f);
+  if (p.gap_max_size > 0) gap_interpolation(depth, normal, h->W, h->H, p.gap_max_size, 0.025f);
+  PM_HIP(hipMemcpyAsync(h->out_depth.p, depth.data(), npix * 4, hipMemcpyHostToDevice, h->stream), "upload");
+  PM_HIP(hipMemcpyAsync(h->out_normal.p, normal.data(), npix * 12, hipMemcpyHostToDevice, h->stream), "upload");
+  PM_HIP(launch_confidence(static_cast<int64_t>(npix), h->out_depth.p, h->out_cost.p, h->out_conf.p, h->stream),
+         "confidence kernel launch");
+  PM_HIP(hipStreamSynchronize(h->stream), "postprocess sync");
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pm_init_state(pm_handle* h) {
   if (int r = check_handle(h)) return r;
   if (!h->uploaded) return fail(PM_E_STATE, "pm_init_state: views are not uploaded (call pm_upload)");
@@ -636,11 +975,7 @@ int pm_init_state(pm_handle* h) {
   if (int r = ensure_refpatch(h)) return r;
   if (int r = timed(h, PM_STAGE_INIT, "init kernel launch", [&] { return launch_init(a, hp, h->f16, h->stream); }))
     return r;
-  if (h->masks.p) {  // estimator.cc:574-614
-    PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
-                          h->stream), "prior snapshot");
-    PM_HIP(launch_prior(a, hp, h->f16, h->stream), "prior kernel launch");
-  }
+  if (int r = run_prior(h, a)) return r;  // estimator.cc:574-614
   h->state_ready = true;
   return PM_OK;
 }
@@ -659,13 +994,11 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
   for (int c = 0; c < 2; ++c) {
     if (color >= 0 && c != color) continue;
     a.color = c;
-    PM_HIP(launch_zero_counters(a, st), "zero kernel launch");
     if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
                       [&] { return launch_candidates(a, hp, f16, st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
                       [&] { return launch_fallback(a, hp, f16, 0, st); })) return r;
     if (int r = timed(h, PM_STAGE_SELECT, "select kernel launch", [&] { return launch_select(a, st); })) return r;
-    PM_HIP(launch_zero_counters(a, st), "zero kernel launch");
     if (int r = timed(h, PM_STAGE_REFINE_COST, "refine-cost kernel launch",
                       [&] { return launch_refine_cost(a, hp, f16, st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
@@ -695,25 +1028,75 @@ int pm_decode(pm_handle* h) {
   return PM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One pyramid level of `nlevels` (RunLevel, estimator.cc:172-255): stage
+// the level, start from random planes (level 0) or from the decoded
+// coarser level, prior re-init, iterations, filter, decode and host
+// post-processing; a level below the last is kept for the next one.
+int run_level(pm_handle* h, int level, int nlevels) {
+  const bool last = level == nlevels - 1;
+  if (nlevels > 1 || !h->uploaded)
+    if (int r = upload_level(h, level, nlevels)) return r;
+  if (level == 0) {
+    if (int r = pm_init_state(h)) return r;
+  } else {
+    if (h->coarse_w <= 0 || !h->coarse_depth.p) return fail(PM_E_STATE, "no coarser level to start from");
+    if (int r = ensure_refpatch(h)) return r;
+    PmArgs a = make_args(h);
+    if (int r = timed(h, PM_STAGE_INIT, "upsample kernel launch", [&] {
+          return launch_upsample(a, h->coarse_depth.p, h->coarse_normal.p, h->coarse_w, h->coarse_h, h->stream);
+        }))
+      return r;
+    if (int r = run_prior(h, a)) return r;
+    h->state_ready = true;
+  }
+  if (last && h->timing) PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
+  for (int it = 0; it < h->params.max_iterations; ++it)
+    if (int r = pm_iterate(h, it, -1)) return r;
+  if (last && h->timing) PM_HIP(hipEventRecord(h->ev[2], h->stream), "event");
+  if (h->params.checkerboard_filter)
+    if (int r = pm_filter(h)) return r;
+  if (int r = pm_decode(h)) return r;
+  if (int r = postprocess(h)) return r;
+  if (!last) {  // keep the decoded level for the next one (estimator.cc:238-246)
+    const size_t npix = static_cast<size_t>(h->W) * h->H;
+    if (int r = h->coarse_depth.alloc(npix)) return r;
+    if (int r = h->coarse_normal.alloc(3 * npix)) return r;
+    PM_HIP(hipMemcpyAsync(h->coarse_depth.p, h->out_depth.p, npix * sizeof(float), hipMemcpyDeviceToDevice,
+                          h->stream), "keep coarse depth");
+    PM_HIP(hipMemcpyAsync(h->coarse_normal.p, h->out_normal.p, 3 * npix * sizeof(float), hipMemcpyDeviceToDevice,
+                          h->stream), "keep coarse normal");
+    h->coarse_w = h->W;
+    h->coarse_h = h->H;
+  }
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pm_run(pm_handle* h) {
   if (int r = check_handle(h)) return r;
-  if (!h->uploaded) {
-    if (int r = pm_upload(h)) return r;
-  }
+  if (int r = validate(h)) return r;
+  const int nlevels = level_count(h->params);
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   h->timing = true;
   h->kev_used = 0;
   h->kev_marks.clear();
   PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
-  if (int r = pm_init_state(h)) { h->timing = false; return r; }
-  PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
-  for (int it = 0; it < h->params.max_iterations; ++it)
-    if (int r = pm_iterate(h, it, -1)) { h->timing = false; return r; }
+  // Coarse to fine (Run, estimator.cc:134-139).
+  for (int level = 0; level < nlevels; ++level) {
+    if (int r = run_level(h, level, nlevels)) {
+      h->timing = false;
+      return r;
+    }
+  }
   h->timing = false;
-  PM_HIP(hipEventRecord(h->ev[2], h->stream), "event");
-  if (h->params.checkerboard_filter)
-    if (int r = pm_filter(h)) return r;
-  if (int r = pm_decode(h)) return r;
+  h->next_level = nlevels;
   PM_HIP(hipEventRecord(h->ev[3], h->stream), "event");
   PM_HIP(hipStreamSynchronize(h->stream), "pm_run");
   PM_HIP(hipEventElapsedTime(&h->last_run_ms, h->ev[0], h->ev[3]), "event time");
@@ -726,6 +1109,46 @@ int pm_run(pm_handle* h) {
     h->stage_ms[m.first] += ms;
     h->stage_launches[m.first] += 1;
   }
+  return PM_OK;
+}
+
+int pm_run_level(pm_handle* h, int level) {
+  if (int r = check_handle(h)) return r;
+  if (int r = validate(h)) return r;
+  const int nlevels = level_count(h->params);
+  if (level < 0 || level >= nlevels) return fail(PM_E_INVALID, "pm_run_level: level out of range");
+  if (level == 0) h->next_level = 0;
+  if (level != h->next_level) return fail(PM_E_STATE, "pm_run_level: levels run in order, starting from 0");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  if (int r = run_level(h, level, nlevels)) return r;
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_run_level");
+  h->next_level = level + 1;
+  return PM_OK;
+}
+
+int pm_level_size(pm_handle* h, int* width, int* height) {
+  if (int r = check_handle(h)) return r;
+  if (!width || !height) return fail(PM_E_INVALID, "null output");
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_level_size: no level is staged");
+  *width = h->W;
+  *height = h->H;
+  return PM_OK;
+}
+
+int pm_release_device(pm_handle* h) {
+  if (int r = check_handle(h)) return r;
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_release_device");
+  h->images.release(); h->quads.release(); h->gviews.release(); h->gcams.release();
+  h->ref_taps.release(); h->ref_sums.release();
+  h->ca.release(); h->vwb.release(); h->wnb.release(); h->cpos.release(); h->nref.release(); h->refp.release();
+  h->fb_masks.release();
+  h->planes.release(); h->prior.release(); h->snapshot.release(); h->costs.release(); h->prevd.release();
+  h->rng.release(); h->sel.release(); h->masks.release();
+  h->out_depth.release(); h->out_normal.release(); h->out_cost.release(); h->out_conf.release();
+  h->uploaded = false;
+  h->state_ready = false;
+  h->ref_ready = false;
   return PM_OK;
 }
 
@@ -769,7 +1192,10 @@ int pm_download(pm_handle* h, float* depth, float* plane, float* normal, float* 
 int pm_compute(pm_handle* h, float* depth, float* plane, float* normal, float* cost, uint32_t* views,
                float* confidence) {
   if (int r = check_handle(h)) return r;
-  if (int r = pm_upload(h)) return r;
+  if (int r = validate(h)) return r;
+  // with several levels pm_run stages every level itself
+  if (level_count(h->params) == 1)
+    if (int r = pm_upload(h)) return r;
   if (int r = pm_run(h)) return r;
   return pm_download(h, depth, plane, normal, cost, views, confidence);
 }

@@ -49,6 +49,7 @@ __device__ __forceinline__ V3 normalize3(V3 a) {
   return V3{a.x / l, a.y / l, a.z / l};
 }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // e^x: Cody-Waite reduction by ln2, degree-7 Taylor polynomial, 2^k via the
 // exponent field; x < -86 -> 0, x > 88 -> +inf.

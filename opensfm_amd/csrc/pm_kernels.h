@@ -36,9 +36,7 @@ struct PmArgs {
   float* wnb;                  // sum of the view weights
   float4* refp;                // refinement hypotheses, 6 slots
   int* nref;                   // number of refinement hypotheses (5 or 6)
-  uint2* jobs;                 // textureless-fallback jobs, one queue per (XCD, view)
-  uint32_t* jobcount;          // queue lengths, [xcd * 16 + view]
-  int64_t job_cap;             // capacity of one (xcd, view) queue
+  uint64_t* fb_masks;          // pending textureless fallbacks, [(tile * 9 + slot) * n_src + view]
   int phase;                   // fallback jobs of the candidate (0) or refinement (1) slots
   uint32_t prev_depth_mask;
   int width, height, n_src;
@@ -64,16 +62,16 @@ struct PmDecodeOut {
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream);
-// Fallback-job queues for a w x h reference: one per (XCD, source view).
-int64_t fallback_queue_capacity(int width, int height);  // jobs per (xcd, view) queue
-int fallback_queue_count(int n_src);                      // number of queues
-int job_counter_count();                                  // length of the jobcount array
+// Length of the fallback mask array for a w x h reference.
+int64_t fallback_mask_count(int width, int height, int n_src);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+// Coarse-to-fine transfer from a decoded src_w x src_h level (depth, normal
+// 3 per pixel) into the planes/costs/rng/sel of the current level.
+hipError_t launch_upsample(const PmArgs& a, const float* cdepth, const float* cnormal, int src_w, int src_h,
+                           hipStream_t stream);
 // One colour pass (a.color) of iteration a.iteration, in this order:
-// zero_counters, candidates, fallback(0), select, zero_counters,
-// refine_cost, fallback(1), refine.
-hipError_t launch_zero_counters(const PmArgs& a, hipStream_t stream);
+// candidates, fallback(0), select, refine_cost, fallback(1), refine.
 hipError_t launch_candidates(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, hipStream_t stream);
 hipError_t launch_select(const PmArgs& a, hipStream_t stream);
@@ -81,6 +79,8 @@ hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, hipStre
 hipError_t launch_refine(const PmArgs& a, hipStream_t stream);
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream);
 hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stream);
+// conf = logistic(cost) where depth > 0, else 0 (n pixels).
+hipError_t launch_confidence(int64_t n, const float* depth, const float* cost, float* conf, hipStream_t stream);
 hipError_t launch_score(const PmArgs& a, int half_patch, bool f16, int n, const int32_t* xy,
                         const float4* planes, float* costs, float* init_cost, uint32_t* sel,
                         hipStream_t stream);

@@ -47,6 +47,15 @@ __device__ __forceinline__ bool full_tile_pixel(int width, int height, int& x, i
   return x < width && y < height;
 }
 
+// Colour tile of this workgroup (false: past the last tile, for the whole
+// workgroup).
+__device__ __forceinline__ bool colour_tile(const PmArgs& a, int& tx, int& ty) {
+  const int hw = (a.width + 1) / 2;
+  return tile_of((hw + kTileX - 1) / kTileX, (a.height + kTileY - 1) / kTileY, tx, ty);
+}
+
+__device__ __forceinline__ int colour_tiles_x(const PmArgs& a) { return ((a.width + 1) / 2 + kTileX - 1) / kTileX; }
+
 __device__ __forceinline__ Plane to_plane(float4 v) { return Plane{v.x, v.y, v.z, v.w}; }
 __device__ __forceinline__ float4 to_f4(Plane p) { return make_float4(p.x, p.y, p.z, p.w); }
 template <typename T>
@@ -209,9 +218,11 @@ __device__ __forceinline__ bool plane_points(const PmCam& c0, int x, int y, Plan
 #endif
 constexpr int kViewChunk = PM_VIEW_CHUNK;
 
-template <int HP, typename T>
+// kLdsTaps: the pixel's stride-1 (w, w*r) are read from the workgroup's LDS
+// copy (lt[tap * kWave], see stage_tile_taps) instead of ref_taps.
+template <int HP, typename T, bool kLdsTaps = false>
 __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
-                                                Plane pl, float* out, int64_t os) {
+                                                Plane pl, float* out, int64_t os, const float2* lt = nullptr) {
   constexpr int D = 2 * HP + 1;
   const int n_src = a.n_src;
   PlanePts pp;
@@ -263,8 +274,14 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
 #pragma unroll
       for (int ix = -HP; ix <= HP; ++ix) {
         const float fix = static_cast<float>(ix);
-        const PM_GLOBAL char* tp = reinterpret_cast<const PM_GLOBAL char*>(trow + static_cast<int64_t>(ix + HP) * plane_stride);
-        const float2 wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + (ucidx << 3));
+        float2 wv;
+        if constexpr (kLdsTaps) {
+          wv = lt[((iy + HP) * D + (ix + HP)) * kWave];
+        } else {
+          const PM_GLOBAL char* tp =
+              reinterpret_cast<const PM_GLOBAL char*>(trow + static_cast<int64_t>(ix + HP) * plane_stride);
+          wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + (ucidx << 3));
+        }
 #pragma unroll
         for (int k = 0; k < kViewChunk; ++k) {
           const float sx = fmaf(wp[k].jxx, fix, bx[k]);
@@ -305,17 +322,30 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
   return pending;
 }
 
-// Census fallback (opencl_kernels.h:270-315).
+// Reference texels for the fallbacks: straight from the padded image, or
+// from a workgroup's LDS copy of the clamped texels around its tile.
 template <typename T>
-__device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restrict__ img, const Sampler& sm,
-                                              float ref_center, float src_center, int cx, int cy, const Warp& w) {
-  const PmView rv = kview(a, 0);
+struct GlobalRef {
+  const T* img;
+  PmView rv;
+  __device__ __forceinline__ float operator()(int x, int y) const { return texel(img, rv, x, y); }
+};
+struct LdsRef {
+  const float* t;
+  int x0, y0, pitch;
+  __device__ __forceinline__ float operator()(int x, int y) const { return t[(y - y0) * pitch + (x - x0)]; }
+};
+
+// Census fallback (opencl_kernels.h:270-315).
+template <typename T, typename Ref>
+__device__ __forceinline__ float census_inner(const Ref& ref, const Sampler& sm, float ref_center, float src_center,
+                                              int cx, int cy, const Warp& w) {
   uint32_t rc = 0u, sc = 0u;
   int bit = 0;
   for (int dy = -2; dy <= 2; ++dy) {
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
-      const float ref_val = texel(img, rv, cx + dx, cy + dy);
+      const float ref_val = ref(cx + dx, cy + dy);
       const float sx = w.scx + w.jxx * static_cast<float>(dx) + w.jyx * static_cast<float>(dy);
       const float sy = w.scy + w.jxy * static_cast<float>(dx) + w.jyy * static_cast<float>(dy);
       const float src_val = sample<T>(sm, sx, sy);
@@ -335,29 +365,18 @@ __device__ __forceinline__ float census_inner(const PmArgs& a, const T* __restri
 // view pending work on it together: the view is wave-uniform (scalar
 // descriptors, one source image per gather instruction, neighbouring
 // pixels), which keeps the gathers coherent.
-// The reference-side (w, w*r) of a strided tap are either read from
-// ref_taps or recomputed from the reference image with the arithmetic of
-// refpatch_kernel (bit-identical).  Recomputing costs one exp per tap and
-// one texel read from the small, cache-resident reference footprint instead
-// of an 8-byte read that misses L2 for scattered fallback jobs.
-#ifndef PM_FB_RECOMPUTE
-#define PM_FB_RECOMPUTE 1
-#endif
-
-template <int HP, typename T>
-__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const T* __restrict__ img, const Sampler& sm, int si,
-                                                 int x, int y, int cidx, float src_center, const Warp& w) {
-  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+// Strided NCC of the textureless fallback (opencl_kernels.h:188-258).  The
+// reference-side (w, w*r) of each tap are recomputed from the reference
+// texels with the arithmetic of refpatch_kernel (bit-identical to the
+// stored strides 2 and 3), which is cheaper than reading them back for
+// scattered jobs; the moments come from ref_sums.
+template <int HP, typename T, typename Ref>
+__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const Ref& ref, const Sampler& sm, int si, int x,
+                                                 int y, int cidx, float src_center, const Warp& w) {
   const float stride = static_cast<float>(si + 1);
   const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
-#if PM_FB_RECOMPUTE
-  const PmView rv = kview(a, 0);
   const float sf = a.spatial_factor, cf = a.color_factor;
-#else
-  const uint32_t toff = static_cast<uint32_t>(cidx) << 3;
-#endif
   float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
-  int p = 0;
 #pragma unroll 1
   for (int iy = -HP; iy <= HP; ++iy) {
 #pragma unroll
@@ -367,24 +386,16 @@ __device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const T* __res
       const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
       const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
       const float sd = sample<T>(sm, sx, sy) - src_center;
-#if PM_FB_RECOMPUTE
       const float rx = static_cast<float>(x) + fdx;
       const float ry = static_cast<float>(y) + fdy;
-      const float r = texel(img, rv, static_cast<int>(rx), static_cast<int>(ry)) - rs.x;
+      const float r = ref(static_cast<int>(rx), static_cast<int>(ry)) - rs.x;
       const float wt = pm_expf(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
-      const float2 wv = make_float2(wt, wt * r);
-#else
-      const PM_GLOBAL char* tp = reinterpret_cast<const PM_GLOBAL char*>(
-          gp(a.ref_taps) + (static_cast<int64_t>(si) * P + (p + ix + HP)) * a.ref_plane);
-      const float2 wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + toff);
-#endif
-      sum_s += wv.x * sd;
-      sum_ss += wv.x * sd * sd;
-      sum_rs += wv.y * sd;
+      const float wr = wt * r;
+      sum_s += wt * sd;
+      sum_ss += wt * sd * sd;
+      sum_rs += wr * sd;
     }
-    p += 2 * HP + 1;
   }
-  (void)p;
   if (rs.y < 1e-6f) return -1.0f;
   const float inv_w = 1.0f / rs.y;
   const float mean_r = rs.z * inv_w;
@@ -396,148 +407,160 @@ __device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const T* __res
   return covar / sqrtf(var_ref * var_src);
 }
 
+// The fallback of one (pixel, plane, view) whose stride-1 NCC was <= 0.1
+// (opencl_kernels.h:531-553): strides 2 then 3 (stop once NCC > 0.1), then
+// census or 2.0.  Returns the cost.
+template <int HP, typename T, typename Ref>
+__device__ __forceinline__ float fallback_cost(const PmArgs& a, const Ref& ref, const Sampler& sm, int x, int y,
+                                               int cidx, float ref_center, const Warp& w, float ncc1) {
+  const float scen = sample<T>(sm, w.scx, w.scy);
+  float best = ncc1;
+#pragma unroll 1
+  for (int si = 1; si < 3; ++si) {
+    const float ncc = ncc_at_stride_u<HP, T>(a, ref, sm, si, x, y, cidx, scen, w);
+    if (ncc > best) best = ncc;
+    if (best > 0.1f) break;
+  }
+  if (best <= -0.5f) return (a.census_weight > 1e-6f) ? census_inner<T>(ref, sm, ref_center, scen, x, y, w) : 2.0f;
+  return 1.0f - best;
+}
+
 template <int HP, typename T>
 __device__ __forceinline__ void resolve_fallbacks(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
                                                   Plane pl, float* out, int64_t os, uint32_t pending) {
   PlanePts pp{};
   if (pending) plane_points(kcam(a, 0), x, y, pl, pp);
   const float ref_center = gp(a.ref_sums)[cidx].x;
+  const GlobalRef<T> ref{img, kview(a, 0)};
   const int n_src = a.n_src;
 #pragma unroll 1
   for (int s = 0; s < n_src; ++s) {
     const bool mine = (pending >> s) & 1u;
     if (!__any(mine)) continue;
     if (mine) {
-      const PmView sv = kview(a, s + 1);
-      const Sampler sm = make_sampler<T>(a, sv);
+      const Sampler sm = make_sampler<T>(a, kview(a, s + 1));
       Warp w;
       warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, s + 1), w);  // valid: it was valid at stride 1
-      const float scen = sample<T>(sm, w.scx, w.scy);
-      float best = out[s * os];  // stride-1 NCC
-#pragma unroll 1
-      for (int si = 1; si < 3; ++si) {
-        const float ncc = ncc_at_stride_u<HP, T>(a, img, sm, si, x, y, cidx, scen, w);
-        if (ncc > best) best = ncc;
-        if (best > 0.1f) break;
-      }
-      float cost;
-      if (best <= -0.5f) {
-        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
-      } else {
-        cost = 1.0f - best;
-      }
-      out[s * os] = cost;
+      out[s * os] = fallback_cost<HP, T>(a, ref, sm, x, y, cidx, ref_center, w, out[s * os]);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Textureless fallbacks of the colour-pass cost kernels go through job
-// queues instead of being resolved in place: a wave appends one job per
-// (pixel, hypothesis slot) left pending to the queue of (its XCD, the view);
-// fallback_kernel then runs the jobs with every lane busy.  One queue per
-// view keeps the jobs running together on one XCD inside one source image
-// and one band of the reference, a footprint its L2 holds.  The stride-1
-// NCC is already in the cost slot, so a job is just the pixel and the slot.
+// Textureless fallbacks of the colour-pass cost kernels are resolved by a
+// separate kernel with every lane busy.  Each hypothesis wave records, per
+// source view, the ballot of its lanes left pending:
+//   fb_masks[(tile * 9 + slot) * n_src + view]   (bit = lane of the tile)
+// and fallback_kernel gives each (view, tile) one wave that walks the
+// pending (slot, lane) pairs 64 at a time.  The lanes of a wave are then
+// neighbouring pixels warped into one source view -- coherent gathers --
+// and the reference texels around the tile sit in LDS.
+// Every hypothesis wave writes its masks on every pass (zero when nothing
+// is pending), so no clearing is needed.  The stride-1 NCC is already in
+// the cost slot.
 // ---------------------------------------------------------------------------
-#ifndef PM_FB_BLOCKS
-#define PM_FB_BLOCKS 512
-#endif
-constexpr int kFallbackBlocksPerXcd = PM_FB_BLOCKS;
-constexpr int kJobCounters = kXcds * kMaxSrc;
-
-__device__ __forceinline__ void emit_jobs(const PmArgs& a, int x, int y, int slot, uint32_t pending) {
-  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+__device__ __forceinline__ void write_fb_masks(const PmArgs& a, int tile, int slot, uint32_t pending) {
   const int lane = static_cast<int>(threadIdx.x);
-  const uint2 job = make_uint2(static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16),
-                               static_cast<uint32_t>(slot));
+  uint64_t mine = 0ull;
 #pragma unroll 1
   for (int s = 0; s < a.n_src; ++s) {
-    const bool mine = (pending >> s) & 1u;
-    const uint64_t ballot = __ballot(mine);
-    if (ballot == 0ull) continue;
-    const int leader = __ffsll(static_cast<unsigned long long>(ballot)) - 1;
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(a.jobcount + xcd * kMaxSrc + s, static_cast<uint32_t>(__popcll(ballot)));
-    base = __shfl(base, leader);
-    if (mine) {
-      const uint32_t rank = static_cast<uint32_t>(__popcll(ballot & ((1ull << lane) - 1ull)));
-      gpw(a.jobs)[static_cast<int64_t>(xcd * a.n_src + s) * a.job_cap + base + rank] = job;
-    }
+    const uint64_t b = __ballot((pending >> s) & 1u);
+    if (lane == s) mine = b;
   }
+  if (lane < a.n_src) gpw(a.fb_masks)[(static_cast<int64_t>(tile) * 9 + slot) * a.n_src + lane] = mine;
 }
 
-// Runs the queued fallbacks (opencl_kernels.h:531-553 for one view):
-// strides 2 then 3 (stop once NCC > 0.1), then census or 2.0.  Workgroups
-// on XCD k drain the queues of XCD k, view after view; each loop bound is a
-// queue length read once, so every wave exits.  a.phase: 0 = candidate
-// slots, 1 = refinement slots.
-#ifndef PM_FB_WAVES
-#define PM_FB_WAVES 0
-#endif
-#if PM_FB_WAVES
-#define PM_FB_ATTR __attribute__((amdgpu_waves_per_eu(PM_FB_WAVES)))
-#else
-#define PM_FB_ATTR
-#endif
+// Index of the n-th (0-based) set bit of m; m has more than n set bits.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
+  int pos = 0;
+#pragma unroll
+  for (int width = 32; width >= 1; width >>= 1) {
+    const uint64_t lo = m & ((1ull << width) - 1ull);
+    const int c = __popcll(lo);
+    if (n >= c) {
+      n -= c;
+      m >>= width;
+      pos += width;
+    } else {
+      m = lo;
+    }
+  }
+  return pos;
+}
 
+// One wave per (source view, tile).  Each XCD takes the tiles of its band
+// (the tiles its colour-pass workgroups had, see tile_of) view by view, so
+// the waves in flight on an XCD gather from one source image and one band of
+// the reference, a footprint its L2 holds.
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) PM_FB_ATTR void fallback_kernel(PmArgs a) {
-  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
-  const uint32_t j = blockIdx.x / kXcds;
-  const uint32_t nb = gridDim.x / kXcds;
-  const T* img = images_of<T>(a);
+__global__ __launch_bounds__(kWave) void fallback_kernel(PmArgs a) {
+  constexpr int R = (3 * HP > 2) ? 3 * HP : 2;       // reach of stride 3 and of the 5 x 5 census
+  constexpr int RW = 2 * kTileX + 2 * R, RH = kTileY + 2 * R;
+  __shared__ float ref_tile[RW * RH];
+  const int tiles_x = colour_tiles_x(a);
+  const int ntiles = tiles_x * ((a.height + kTileY - 1) / kTileY);
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int b = static_cast<int>(blockIdx.x);
+  const int j = b / kXcds;
+  const int s = j / per;
+  const int tile = (b % kXcds) * per + j % per;
+  const int n_src = a.n_src;
+  if (s >= n_src || tile >= ntiles) return;
+  const int nslots = a.phase == 0 ? 9 : 6;
+  const PM_GLOBAL uint64_t* mk = gp(a.fb_masks) + static_cast<int64_t>(tile) * 9 * n_src + s;
+  uint64_t m[9];
+  int cum[9];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    m[k] = k < nslots ? mk[k * n_src] : 0ull;
+    total += __popcll(m[k]);
+    cum[k] = total;
+  }
+  if (total == 0) return;  // uniform over the wave
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int x0 = tx * 2 * kTileX - R, y0 = ty * kTileY - R;
+  {
+    const T* img = images_of<T>(a);
+    const PmView rv = kview(a, 0);
+    for (int i = static_cast<int>(threadIdx.x); i < RW * RH; i += kWave)
+      ref_tile[i] = texel(img, rv, x0 + i % RW, y0 + i / RW);
+  }
+  __syncthreads();
+  const LdsRef ref{ref_tile, x0, y0, RW};
   const int64_t pl = a.ref_plane;
   const PmCam c0 = kcam(a, 0);
+  const PmCam cs = kcam(a, s + 1);
+  const Sampler sm = make_sampler<T>(a, kview(a, s + 1));
 #pragma unroll 1
-  for (int s = 0; s < a.n_src; ++s) {
-    const uint32_t count = gp(a.jobcount)[xcd * kMaxSrc + s];
-    if (j * kWave >= count) continue;
-    const PM_GLOBAL uint2* q = gp(a.jobs) + static_cast<int64_t>(xcd * a.n_src + s) * a.job_cap;
-    const PmCam cs = kcam(a, s + 1);
-    const Sampler sm = make_sampler<T>(a, kview(a, s + 1));
-#pragma unroll 1
-    for (uint32_t c = j * kWave; c < count; c += nb * kWave) {
-      const uint32_t qi = c + threadIdx.x;
-      if (qi >= count) continue;
-      const uint2 job = q[qi];
-      const int x = static_cast<int>(job.x & 0xffffu), y = static_cast<int>(job.x >> 16);
-      const int slot = static_cast<int>(job.y);
-      const int cidx = colour_index(a, x, y);
-      Plane pl4;
-      if (a.phase == 0) {
-        const int pos = slot < 8 ? gp(a.cpos)[slot * pl + cidx] : y * a.width + x;
-        pl4 = to_plane(gp(a.planes)[pos]);
-      } else {
-        pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
-      }
-      float* out = a.ca + (static_cast<int64_t>(slot) * a.n_src + s) * pl + cidx;
-      PlanePts pp;
-      plane_points(c0, x, y, pl4, pp);
-      Warp w;
-      warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
-      const float scen = sample<T>(sm, w.scx, w.scy);
-      float best = *out;  // stride-1 NCC
-#pragma unroll 1
-      for (int si = 1; si < 3; ++si) {
-        const float ncc = ncc_at_stride_u<HP, T>(a, img, sm, si, x, y, cidx, scen, w);
-        if (ncc > best) best = ncc;
-        if (best > 0.1f) break;
-      }
-      float cost;
-      if (best <= -0.5f) {
-        const float ref_center = gp(a.ref_sums)[cidx].x;
-        cost = (a.census_weight > 1e-6f) ? census_inner(a, img, sm, ref_center, scen, x, y, w) : 2.0f;
-      } else {
-        cost = 1.0f - best;
-      }
-      *out = cost;
+  for (int q0 = 0; q0 < total; q0 += kWave) {
+    const int q = q0 + static_cast<int>(threadIdx.x);
+    if (q >= total) continue;
+    int slot = 0, prev = 0;
+    uint64_t mm = m[0];
+#pragma unroll
+    for (int k = 1; k < 9; ++k) {
+      if (q >= cum[k - 1]) { slot = k; prev = cum[k - 1]; mm = m[k]; }
     }
+    const int l = nth_set_bit(mm, q - prev);
+    const int gx = tx * kTileX + (l % kTileX);
+    const int y = ty * kTileY + (l / kTileX);
+    const int x = gx * 2 + ((y + a.color) & 1);
+    const int cidx = colour_index(a, x, y);
+    Plane pl4;
+    if (a.phase == 0) {
+      const int pos = slot < 8 ? gp(a.cpos)[slot * pl + cidx] : y * a.width + x;
+      pl4 = to_plane(gp(a.planes)[pos]);
+    } else {
+      pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
+    }
+    float* out = a.ca + (static_cast<int64_t>(slot) * n_src + s) * pl + cidx;
+    PlanePts pp;
+    plane_points(c0, x, y, pl4, pp);
+    Warp w;
+    warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
+    *out = fallback_cost<HP, T>(a, ref, sm, x, y, cidx, ref(x, y), w, *out);
   }
-}
-
-__global__ void zero_counters_kernel(uint32_t* c, int n) {
-  if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0u;
 }
 
 // Full cost vector: stride 1 for all views, then the fallbacks.
@@ -733,6 +756,66 @@ __device__ __forceinline__ float initial_cost_and_views(const PmArgs& a, const f
 // ---------------------------------------------------------------------------
 // acmmp_random_init (opencl_kernels.h:979-1039)
 // ---------------------------------------------------------------------------
+// Random plane of pixel (x, y) from its seeded stream (opencl_kernels.h:
+// 1003-1021): log-uniform depth in [depth_min, depth_max], normal
+// (u, v, -1) normalised.
+__device__ __forceinline__ Plane random_plane(const PmArgs& a, int x, int y, uint32_t& s0, uint32_t& s1,
+                                              uint32_t key) {
+  const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
+  const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
+  const float depth = pm_expf(ldmin + rand_float(s0, s1, key) * (ldmax - ldmin));
+  V3 n;
+  n.x = rand_float(s0, s1, key + 3u) * 2.0f - 1.0f;
+  n.y = rand_float(s0, s1, key + 5u) * 2.0f - 1.0f;
+  n.z = -1.0f;
+  n = normalize3(n);
+  return plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, kcam(a, 0));
+}
+
+// Coarse-to-fine transfer between hierarchy levels (acmmp_upsample,
+// opencl_kernels.h:1792-1856, with the coarse planes that
+// UpsampleDepthNormal builds from the decoded coarse depth and normal,
+// estimator.cc:775-791).  The nearest coarse pixel's plane is re-anchored
+// at the fine pixel; where it gives no depth the pixel gets a random
+// plane.  Costs become 2.0 so that the fine level re-evaluates every plane;
+// the random streams are re-seeded.
+__global__ __launch_bounds__(256) void upsample_kernel(PmArgs a, const float* __restrict__ cdepth,
+                                                       const float* __restrict__ cnormal, int src_w, int src_h) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(a.width) * a.height) return;
+  const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
+  const int idx = static_cast<int>(i);
+  uint32_t s0 = static_cast<uint32_t>(idx) * 1099087573u + 2654435769u;
+  uint32_t s1 = static_cast<uint32_t>(idx) * 2246822519u + 13u;
+  const PmCam c0 = kcam(a, 0);
+  const float scale_x = static_cast<float>(src_w) / static_cast<float>(a.width);
+  const float scale_y = static_cast<float>(src_h) / static_cast<float>(a.height);
+  const int sx = clampi(static_cast<int>(static_cast<float>(x) * scale_x), 0, src_w - 1);
+  const int sy = clampi(static_cast<int>(static_cast<float>(y) * scale_y), 0, src_h - 1);
+  const int sidx = sy * src_w + sx;
+  // coarse plane through the decoded coarse point (estimator.cc:775-791)
+  const float d = cdepth[sidx];
+  const V3 cn{cnormal[3 * sidx], cnormal[3 * sidx + 1], cnormal[3 * sidx + 2]};
+  const float fx = c0.K[0] * static_cast<float>(src_w) / static_cast<float>(a.width);
+  const float fy = c0.K[4] * static_cast<float>(src_h) / static_cast<float>(a.height);
+  const float cxs = c0.K[2] * static_cast<float>(src_w) / static_cast<float>(a.width);
+  const float cys = c0.K[5] * static_cast<float>(src_h) / static_cast<float>(a.height);
+  const V3 pt{d * (static_cast<float>(sx) - cxs) / fx, d * (static_cast<float>(sy) - cys) / fy, d};
+  const Plane cp{cn.x, cn.y, cn.z, -(cn.x * pt.x + cn.y * pt.y + cn.z * pt.z)};
+  const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cp, c0);
+  Plane out;
+  if (depth > 0.0f) {
+    out = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth,
+                                  normalize3(V3{cp.x, cp.y, cp.z}), c0);
+  } else {
+    out = random_plane(a, x, y, s0, s1, static_cast<uint32_t>(idx + 7919));
+  }
+  a.planes[idx] = to_f4(out);
+  a.costs[idx] = 2.0f;
+  a.rng[idx] = make_uint2(s0, s1);
+  a.sel[idx] = 0u;
+}
+
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) {
   extern __shared__ float smem[];
@@ -743,16 +826,7 @@ __global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) {
   const int idx = y * a.width + x;
   uint32_t s0 = static_cast<uint32_t>(idx) * 1099087573u + 2654435769u;
   uint32_t s1 = static_cast<uint32_t>(idx) * 2246822519u + 13u;
-  const uint32_t key = static_cast<uint32_t>(idx + 7919);
-  const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
-  const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
-  const float depth = pm_expf(ldmin + rand_float(s0, s1, key) * (ldmax - ldmin));
-  V3 n;
-  n.x = rand_float(s0, s1, key + 3u) * 2.0f - 1.0f;
-  n.y = rand_float(s0, s1, key + 5u) * 2.0f - 1.0f;
-  n.z = -1.0f;
-  n = normalize3(n);
-  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, kcam(a, 0));
+  const Plane plane = random_plane(a, x, y, s0, s1, static_cast<uint32_t>(idx + 7919));
   a.planes[idx] = to_f4(plane);
   a.rng[idx] = make_uint2(s0, s1);
   float* cv = smem + lane;
@@ -838,6 +912,42 @@ __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) {
 struct ColourPixel {
   int x, y, idx, cidx;
 };
+
+// Pixel of this lane in colour tile (tx, ty).
+__device__ __forceinline__ bool colour_lane_pixel(const PmArgs& a, int tx, int ty, ColourPixel& cp) {
+  const int hw = (a.width + 1) / 2;
+  const int gx = tx * kTileX + static_cast<int>(threadIdx.x % kTileX);
+  const int y = ty * kTileY + static_cast<int>(threadIdx.x / kTileX);
+  if (gx >= hw || y >= a.height) return false;
+  const int x = gx * 2 + ((y + a.color) & 1);
+  if (x >= a.width) return false;
+  cp.x = x;
+  cp.y = y;
+  cp.idx = y * a.width + x;
+  cp.cidx = colour_index(a, x, y);
+  return true;
+}
+
+// Copies the stride-1 reference taps of the 64 pixels of colour tile
+// (tx, ty) to LDS, lt[tap * kWave + lane], with every thread of the
+// workgroup; the hypothesis waves then read them with LDS loads, which
+// takes a third of the vector-memory instructions off the texture path.
+template <int P>
+__device__ __forceinline__ void stage_tile_taps(const PmArgs& a, int tx, int ty, float2* lt) {
+  const int hw = (a.width + 1) / 2;
+  const int nthreads = static_cast<int>(blockDim.x * blockDim.y);
+  const int t = static_cast<int>(threadIdx.y * blockDim.x + threadIdx.x);
+  const int64_t cbase = static_cast<int64_t>(a.color) * a.height * hw;
+  const PM_GLOBAL float2* taps = gp(a.ref_taps);
+  for (int k = t; k < P * kWave; k += nthreads) {
+    const int tap = k / kWave, l = k % kWave;
+    const int gx = tx * kTileX + l % kTileX, y = ty * kTileY + l / kTileX;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (gx < hw && y < a.height) v = taps[static_cast<int64_t>(tap) * a.ref_plane + cbase + y * hw + gx];
+    lt[k] = v;
+  }
+  __syncthreads();
+}
 
 __device__ __forceinline__ bool colour_pixel(const PmArgs& a, ColourPixel& cp) {
   const int hw = (a.width + 1) / 2;
@@ -957,16 +1067,25 @@ template <int HP, typename T>
 #define PM_CAND_ATTR
 #endif
 __global__ __launch_bounds__(kWave * 9) PM_CAND_ATTR void pass_candidates_kernel(PmArgs a) {
-  ColourPixel cp;
-  if (!colour_pixel(a, cp)) return;
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  __shared__ float2 lt[P * kWave];
+  int tx, ty;
+  if (!colour_tile(a, tx, ty)) return;
+  stage_tile_taps<P>(a, tx, ty, lt);
   const int h = threadIdx.y;  // 0..7 candidates, 8 = current plane (one wave each)
-  const int pos = h == 8 ? cp.idx : candidate_position(a, h, cp.x, cp.y);
-  if (h < 8) gpw(a.cpos)[static_cast<int64_t>(h) * a.ref_plane + cp.cidx] = pos;
-  if (pos < 0) return;
-  const Plane hyp = to_plane(gp(a.planes)[pos]);
-  float* out = a.ca + static_cast<int64_t>(h) * a.n_src * a.ref_plane + cp.cidx;
-  const uint32_t pending = cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
-  emit_jobs(a, cp.x, cp.y, h, pending);
+  ColourPixel cp;
+  uint32_t pending = 0u;
+  if (colour_lane_pixel(a, tx, ty, cp)) {
+    const int pos = h == 8 ? cp.idx : candidate_position(a, h, cp.x, cp.y);
+    if (h < 8) gpw(a.cpos)[static_cast<int64_t>(h) * a.ref_plane + cp.cidx] = pos;
+    if (pos >= 0) {
+      const Plane hyp = to_plane(gp(a.planes)[pos]);
+      float* out = a.ca + static_cast<int64_t>(h) * a.n_src * a.ref_plane + cp.cidx;
+      pending = cost_vector<HP, T, true>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane,
+                                         lt + threadIdx.x);
+    }
+  }
+  write_fb_masks(a, ty * colour_tiles_x(a) + tx, h, pending);
 }
 
 template <int NV>
@@ -1164,14 +1283,21 @@ __global__ __launch_bounds__(kWave) void pass_select_kernel(PmArgs a) {
 
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave * 6) PM_CAND_ATTR void pass_refine_cost_kernel(PmArgs a) {
-  ColourPixel cp;
-  if (!colour_pixel(a, cp)) return;
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  __shared__ float2 lt[P * kWave];
+  int tx, ty;
+  if (!colour_tile(a, tx, ty)) return;
+  stage_tile_taps<P>(a, tx, ty, lt);
   const int r = threadIdx.y;  // one wave per refinement hypothesis
-  if (r >= gp(a.nref)[cp.cidx]) return;
-  const Plane hyp = to_plane(gp(a.refp)[r * a.ref_plane + cp.cidx]);
-  float* out = a.ca + static_cast<int64_t>(r) * a.n_src * a.ref_plane + cp.cidx;
-  const uint32_t pending = cost_vector<HP>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane);
-  emit_jobs(a, cp.x, cp.y, r, pending);
+  ColourPixel cp;
+  uint32_t pending = 0u;
+  if (colour_lane_pixel(a, tx, ty, cp) && r < gp(a.nref)[cp.cidx]) {
+    const Plane hyp = to_plane(gp(a.refp)[r * a.ref_plane + cp.cidx]);
+    float* out = a.ca + static_cast<int64_t>(r) * a.n_src * a.ref_plane + cp.cidx;
+    pending = cost_vector<HP, T, true>(a, images_of<T>(a), cp.x, cp.y, cp.cidx, hyp, out, a.ref_plane,
+                                       lt + threadIdx.x);
+  }
+  write_fb_masks(a, ty * colour_tiles_x(a) + tx, r, pending);
 }
 
 template <int NV>
@@ -1279,6 +1405,15 @@ __global__ __launch_bounds__(kWave) void filter_kernel(PmArgs a) {
 // ---------------------------------------------------------------------------
 // Decode (estimator.cc:1112-1224; median, speckle and gap filling off).
 // ---------------------------------------------------------------------------
+// Confidence after the depth map was post-processed on the host
+// (ReadBackResults computes it last, estimator.cc:1211-1222).
+__global__ __launch_bounds__(256) void confidence_kernel(int64_t n, const float* __restrict__ depth,
+                                                         const float* __restrict__ cost, float* __restrict__ conf) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  conf[i] = depth[i] > 0.0f ? 1.0f / (1.0f + pm_expf((cost[i] - 0.5f) * 5.0f)) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void decode_kernel(PmArgs a, PmDecodeOut o) {
   const int W = a.width, H = a.height;
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1388,16 +1523,11 @@ hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t strea
   return hipGetLastError();
 }
 
-int64_t fallback_queue_capacity(int width, int height) {
-  // XCD k owns at most ceil(tiles / 8) colour tiles of 64 pixels, each with
-  // at most 9 pending slots per view.
+int64_t fallback_mask_count(int width, int height, int n_src) {
   const int hw = (width + 1) / 2;
   const int64_t tiles = static_cast<int64_t>((hw + kTileX - 1) / kTileX) * ((height + kTileY - 1) / kTileY);
-  return ((tiles + kXcds - 1) / kXcds) * kWave * 9;
+  return tiles * 9 * n_src;
 }
-
-int fallback_queue_count(int n_src) { return kXcds * n_src; }
-int job_counter_count() { return kJobCounters; }
 
 hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(refpatch_kernel, hp, f16, full_grid(a), 0, stream, a);
@@ -1413,8 +1543,11 @@ hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   PM_DISPATCH(prior_kernel, hp, f16, full_grid(a), lds, stream, a);
 }
 
-hipError_t launch_zero_counters(const PmArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(kJobCounters), 0, stream, a.jobcount, kJobCounters);
+hipError_t launch_upsample(const PmArgs& a, const float* cdepth, const float* cnormal, int src_w, int src_h,
+                           hipStream_t stream) {
+  const int64_t n = static_cast<int64_t>(a.width) * a.height;
+  hipLaunchKernelGGL(upsample_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, a, cdepth,
+                     cnormal, src_w, src_h);
   return hipGetLastError();
 }
 
@@ -1434,7 +1567,11 @@ hipError_t launch_fallback(const PmArgs& a, int hp, bool f16, int phase, hipStre
   if (hp < 1 || hp > 3) return hipErrorInvalidValue;
   PmArgs b = a;
   b.phase = phase;
-  PM_DISPATCH_NORET(fallback_kernel, hp, f16, dim3(kXcds * kFallbackBlocksPerXcd), dim3(kWave), 0, stream, b);
+  const int hw = (a.width + 1) / 2;
+  const int64_t tiles = static_cast<int64_t>((hw + kTileX - 1) / kTileX) * ((a.height + kTileY - 1) / kTileY);
+  const int64_t per = (tiles + kXcds - 1) / kXcds;
+  const dim3 grid(static_cast<unsigned>(kXcds * per * a.n_src));
+  PM_DISPATCH_NORET(fallback_kernel, hp, f16, grid, dim3(kWave), 0, stream, b);
   return hipGetLastError();
 }
 
@@ -1458,6 +1595,12 @@ hipError_t launch_refine(const PmArgs& a, hipStream_t stream) {
 
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(filter_kernel, colour_grid(a), dim3(kWave), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_confidence(int64_t n, const float* depth, const float* cost, float* conf, hipStream_t stream) {
+  hipLaunchKernelGGL(confidence_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, n, depth,
+                     cost, conf);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <initializer_list>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "opensfm_pm.h"
@@ -34,6 +35,33 @@ std::vector<double> as_f64(py::handle obj, size_t n, const char* what) {
   return std::vector<double>(a.data(), a.data() + n);
 }
 
+// Adds one view to a handle: u8, f16 or (anything else, cast to) f32 (H, W).
+void add_view_to(pm_handle* h, py::handle K, py::handle R, py::handle t, py::array image, py::object mask) {
+  auto k = as_f64(K, 9, "K");
+  auto r = as_f64(R, 9, "R");
+  auto tt = as_f64(t, 3, "t");
+  if (image.ndim() != 2) throw std::invalid_argument("image must be a 2-D (H, W) array");
+  int fmt;
+  py::array img;
+  if (image.dtype().is(py::dtype::of<uint8_t>())) {
+    fmt = PM_U8;
+    img = py::array_t<uint8_t, py::array::c_style>::ensure(image);
+  } else if (image.dtype().kind() == 'f' && image.dtype().itemsize() == 2) {
+    fmt = PM_F16;
+    img = py::array::ensure(image, py::array::c_style);
+  } else {
+    fmt = PM_F32;
+    img = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(image);
+  }
+  std::vector<uint8_t> m;
+  if (!mask.is_none()) {
+    auto ma = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>::ensure(mask);
+    m.assign(ma.data(), ma.data() + ma.size());
+  }
+  check(pm_add_view(h, k.data(), r.data(), tt.data(), img.data(), fmt, static_cast<int>(img.shape(1)),
+                    static_cast<int>(img.shape(0)), 0, m.empty() ? nullptr : m.data()));
+}
+
 class Estimator {
  public:
   explicit Estimator(int device) { check(pm_create(device, &h_)); }
@@ -42,29 +70,7 @@ class Estimator {
   Estimator& operator=(const Estimator&) = delete;
 
   void add_view(py::handle K, py::handle R, py::handle t, py::array image, py::object mask) {
-    auto k = as_f64(K, 9, "K");
-    auto r = as_f64(R, 9, "R");
-    auto tt = as_f64(t, 3, "t");
-    if (image.ndim() != 2) throw std::invalid_argument("image must be a 2-D (H, W) array");
-    int fmt;
-    py::array img;
-    if (image.dtype().is(py::dtype::of<uint8_t>())) {
-      fmt = PM_U8;
-      img = py::array_t<uint8_t, py::array::c_style>::ensure(image);
-    } else if (image.dtype().kind() == 'f' && image.dtype().itemsize() == 2) {
-      fmt = PM_F16;
-      img = py::array::ensure(image, py::array::c_style);
-    } else {
-      fmt = PM_F32;
-      img = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(image);
-    }
-    std::vector<uint8_t> m;
-    if (!mask.is_none()) {
-      auto ma = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>::ensure(mask);
-      m.assign(ma.data(), ma.data() + ma.size());
-    }
-    check(pm_add_view(h_, k.data(), r.data(), tt.data(), img.data(), fmt, static_cast<int>(img.shape(1)),
-                      static_cast<int>(img.shape(0)), 0, m.empty() ? nullptr : m.data()));
+    add_view_to(h_, K, R, t, image, mask);
   }
 
   void set_depth_range(float dmin, float dmax, int /*num_planes*/) { check(pm_set_depth_range(h_, dmin, dmax)); }
@@ -96,6 +102,8 @@ class Estimator {
     d["var_threshold"] = p.var_threshold;
     d["image_storage"] = p.image_storage == PM_STORE_F16 ? "f16" : "f32";
     d["quantize_half"] = static_cast<bool>(p.quantize_half);
+    d["speckle_min_size"] = p.speckle_min_size;
+    d["gap_max_size"] = p.gap_max_size;
     return d;
   }
 
@@ -110,9 +118,11 @@ class Estimator {
   }
 
   py::dict compute() {
+    pm_params p;
+    check(pm_get_params(h_, &p));
     {
       py::gil_scoped_release nogil;
-      check(pm_upload(h_));
+      if (p.hierarchy_levels <= 1) check(pm_upload(h_));  // several levels: pm_run stages each one
       check(pm_run(h_));
     }
     return download();
@@ -264,6 +274,147 @@ class Estimator {
   py::ssize_t ref_w_ = -1, ref_h_ = -1;
 };
 
+// Per-device memory reservations of reserve_device_memory.
+uint64_t& reserved_bytes(int device) {
+  static std::vector<uint64_t> r(64, 0);
+  if (device < 0 || device >= static_cast<int>(r.size())) throw std::invalid_argument("device index out of range");
+  return r[static_cast<size_t>(device)];
+}
+
+// DepthmapClusterEstimator (cluster_estimator.cc:42-156, bound at
+// pybind.cc:64-123): several reference views, each with its own sources,
+// run level by level; between levels a reference receives the previous
+// level's depth maps of the references its geometric links name.  Every
+// reference has its own handle; its device memory is released after each
+// level, as the reference releases its GPU buffers.
+class ClusterEstimator {
+ public:
+  ClusterEstimator() { pm_default_params(&params_); }
+  ~ClusterEstimator() { clear(); }
+  ClusterEstimator(const ClusterEstimator&) = delete;
+  ClusterEstimator& operator=(const ClusterEstimator&) = delete;
+
+  pm_params& params() { return params_; }
+  void set_geom_consistency_weight(float w) { geom_weight_ = w; }
+  void set_device(int d) { device_ = d; }
+
+  int begin_ref_view(py::handle K, py::handle R, py::handle t, py::array image, float dmin, float dmax) {
+    pm_handle* h = nullptr;
+    check(pm_create(device_, &h));
+    refs_.push_back(Ref{h, dmin, dmax, {}});
+    add_view_to(h, K, R, t, image, py::none());
+    return static_cast<int>(refs_.size()) - 1;
+  }
+
+  void add_source_view(py::handle K, py::handle R, py::handle t, py::array image) {
+    if (refs_.empty()) throw std::runtime_error("DepthmapClusterEstimator: no ref view started");
+    add_view_to(refs_.back().h, K, R, t, image, py::none());
+  }
+
+  void set_sfm_points(py::array points) {
+    if (refs_.empty()) throw std::runtime_error("DepthmapClusterEstimator: no ref view started");
+    if (points.size() > 0)
+      throw std::runtime_error(
+          "DepthmapClusterEstimator: the SfM planar prior is not available; pass prior planes to "
+          "DepthmapEstimator.set_prior instead");
+  }
+
+  void add_geom_link(int ref_idx, int source_view_idx, int from_ref_idx) {
+    if (ref_idx < 0 || ref_idx >= static_cast<int>(refs_.size()))
+      throw std::runtime_error("DepthmapClusterEstimator::AddGeomLink: invalid ref_idx");
+    refs_[ref_idx].links.emplace_back(source_view_idx, from_ref_idx);
+  }
+
+  // Run (cluster_estimator.cc:82-156): [depth, normal, cost, confidence] per reference.
+  py::list run() {
+    const int K = static_cast<int>(refs_.size());
+    if (K == 0) throw std::runtime_error("DepthmapClusterEstimator: no reference views");
+    for (auto& r : refs_) {
+      pm_params p = params_;
+      p.depth_min = r.dmin;
+      p.depth_max = r.dmax;
+      check(pm_set_params(r.h, &p));
+      check(pm_set_geom_consistency(r.h, geom_weight_, nullptr, 0));
+    }
+    const int levels = std::max(1, params_.hierarchy_levels);
+    struct Map {
+      int w = 0, h = 0;
+      std::vector<float> d;
+    };
+    std::vector<Map> prev(static_cast<size_t>(K));
+    std::vector<std::vector<float>> out(static_cast<size_t>(K));
+    std::vector<std::pair<int, int>> size(static_cast<size_t>(K));
+    {
+      py::gil_scoped_release nogil;
+      for (int level = 0; level < levels; ++level) {
+        const bool last = level == levels - 1;
+        std::vector<Map> cur(static_cast<size_t>(K));
+        for (int i = 0; i < K; ++i) {
+          Ref& r = refs_[i];
+          check(pm_clear_previous_depths(r.h));
+          if (level > 0 && geom_weight_ > 0.0f) {
+            for (const auto& link : r.links) {
+              const int from = link.second;
+              if (from >= 0 && from < K && !prev[from].d.empty())
+                check(pm_set_previous_depth(r.h, link.first, prev[from].d.data(), prev[from].w, prev[from].h));
+            }
+          }
+          check(pm_run_level(r.h, level));
+          int w = 0, h = 0;
+          check(pm_level_size(r.h, &w, &h));
+          const size_t n = static_cast<size_t>(w) * h;
+          if (last) {
+            std::vector<float>& o = out[i];
+            o.resize(6 * n);  // depth, normal (3), cost, confidence
+            check(pm_download(r.h, o.data(), nullptr, o.data() + n, o.data() + 4 * n, nullptr, o.data() + 5 * n));
+            size[i] = {w, h};
+          } else {
+            cur[i].w = w;
+            cur[i].h = h;
+            cur[i].d.resize(n);
+            check(pm_download(r.h, cur[i].d.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+          }
+          check(pm_release_device(r.h));
+        }
+        prev = std::move(cur);
+      }
+    }
+    py::list results;
+    for (int i = 0; i < K; ++i) {
+      const py::ssize_t w = size[i].first, h = size[i].second, n = w * h;
+      const float* o = out[i].data();
+      py::array_t<float> depth(shape({h, w})), normal(shape({h, w, 3})), cost(shape({h, w})), conf(shape({h, w}));
+      std::memcpy(depth.mutable_data(), o, n * sizeof(float));
+      std::memcpy(normal.mutable_data(), o + n, 3 * n * sizeof(float));
+      std::memcpy(cost.mutable_data(), o + 4 * n, n * sizeof(float));
+      std::memcpy(conf.mutable_data(), o + 5 * n, n * sizeof(float));
+      py::list entry;
+      entry.append(depth);
+      entry.append(normal);
+      entry.append(cost);
+      entry.append(conf);
+      results.append(entry);
+    }
+    return results;
+  }
+
+  void clear() {
+    for (auto& r : refs_) pm_destroy(r.h);
+    refs_.clear();
+  }
+
+ private:
+  struct Ref {
+    pm_handle* h;
+    float dmin, dmax;
+    std::vector<std::pair<int, int>> links;  // (source view slot, reference it takes depths from)
+  };
+  pm_params params_{};
+  float geom_weight_ = 0.0f;
+  int device_ = 0;
+  std::vector<Ref> refs_;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(pydense, m) {
@@ -302,6 +453,8 @@ PYBIND11_MODULE(pydense, m) {
              e.update([s](pm_params& p) { p.image_storage = s == "f16" ? PM_STORE_F16 : PM_STORE_F32; });
            })
       .def("set_quantize_half", [](Estimator& e, bool v) { e.update([v](pm_params& p) { p.quantize_half = v; }); })
+      .def("set_speckle_min_size", [](Estimator& e, int v) { e.update([v](pm_params& p) { p.speckle_min_size = v; }); })
+      .def("set_gap_max_size", [](Estimator& e, int v) { e.update([v](pm_params& p) { p.gap_max_size = v; }); })
       .def("set_geom_consistency", &Estimator::set_geom_consistency, py::arg("weight"), py::arg("prev_depths"))
       .def("set_prior", &Estimator::set_prior, py::arg("prior_planes"), py::arg("masks"))
       .def("params", &Estimator::params)
@@ -336,4 +489,81 @@ PYBIND11_MODULE(pydense, m) {
         return std::string(buf);
       });
   m.attr("MAX_SOURCES") = PM_MAX_SOURCES;
+
+  using C = ClusterEstimator;
+  auto set = [](auto field) {
+    return [field](C& c, decltype(std::declval<pm_params&>().*field) v) { c.params().*field = v; };
+  };
+  py::class_<C>(m, "DepthmapClusterEstimator")
+      .def(py::init<>())
+      .def("set_max_iterations", set(&pm_params::max_iterations))
+      .def("set_patch_size", set(&pm_params::patch_size))
+      // images are used at the size they are given
+      .def("set_max_image_size", [](C&, int) {})
+      .def("set_hierarchy_levels", set(&pm_params::hierarchy_levels))
+      .def("set_sigma_spatial", set(&pm_params::sigma_spatial))
+      .def("set_sigma_color", set(&pm_params::sigma_color))
+      .def("set_top_k", set(&pm_params::top_k))
+      .def("set_use_census", [](C& c, bool v) { c.params().use_census = v; })
+      .def("set_smooth_weight", set(&pm_params::smooth_weight))
+      .def("set_anchor_views", set(&pm_params::anchor_views))
+      .def("set_checkerboard_filter", [](C& c, bool v) { c.params().checkerboard_filter = v; })
+      .def("set_speckle_min_size", set(&pm_params::speckle_min_size))
+      .def("set_gap_max_size", set(&pm_params::gap_max_size))
+      .def("set_geom_consistency_weight", &C::set_geom_consistency_weight)
+      .def("set_device", &C::set_device)
+      .def("begin_ref_view", &C::begin_ref_view, py::arg("K"), py::arg("R"), py::arg("t"), py::arg("image"),
+           py::arg("depth_min"), py::arg("depth_max"))
+      .def("add_source_view", &C::add_source_view, py::arg("K"), py::arg("R"), py::arg("t"), py::arg("image"))
+      .def("set_sfm_points", &C::set_sfm_points, py::arg("points"))
+      .def("add_geom_link", &C::add_geom_link, py::arg("ref_idx"), py::arg("source_view_idx"),
+           py::arg("from_ref_idx"))
+      .def("run", &C::run, "Run every reference view; returns [depth, normal, cost, confidence] per reference")
+      .def("clear", &C::clear)
+      .def_static("is_available", []() {
+        int n = 0;
+        return pm_device_count(&n) == PM_OK && n > 0;
+      })
+      .def_static("num_devices", []() {
+        int n = 0;
+        pm_device_count(&n);
+        return n;
+      })
+      .def_static("device_name", [](int i) {
+        char buf[256];
+        check(pm_device_name(i, buf, sizeof(buf)));
+        return std::string(buf);
+      })
+      .def_static("device_is_gpu", [](int i) {
+        int n = 0;
+        return pm_device_count(&n) == PM_OK && i >= 0 && i < n;
+      })
+      .def_static("device_memory_bytes", [](int i) {
+        uint64_t total = 0;
+        check(pm_device_memory(i, &total, nullptr));
+        return total;
+      })
+      .def_static("gpu_memory_bytes", []() {
+        int n = 0;
+        uint64_t best = 0;
+        if (pm_device_count(&n) != PM_OK) return best;
+        for (int i = 0; i < n; ++i) {
+          uint64_t t = 0;
+          if (pm_device_memory(i, &t, nullptr) == PM_OK) best = std::max(best, t);
+        }
+        return best;
+      })
+      // Reservations are bookkeeping for callers that share a device
+      // between clusters (available = total - reserved).
+      .def_static("device_available_memory", [](int i) {
+        uint64_t total = 0;
+        check(pm_device_memory(i, &total, nullptr));
+        const uint64_t r = reserved_bytes(i);
+        return total > r ? total - r : 0;
+      })
+      .def_static("reserve_device_memory", [](int i, uint64_t bytes) { reserved_bytes(i) += bytes; })
+      .def_static("release_device_memory", [](int i, uint64_t bytes) {
+        uint64_t& r = reserved_bytes(i);
+        r = r > bytes ? r - bytes : 0;
+      });
 }

@@ -40,6 +40,7 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <vector>
 #include <omp.h>
 
 namespace {
@@ -720,6 +721,57 @@ void prior_reinit_px(const Ctx& c, pmo_state* st, const float* snap, int x, int 
   st->rng[2 * idx + 1] = rs[1];
 }
 
+// Refinement hypotheses of acmmp_patchmatch (opencl_kernels.h:1669-1742):
+//   0 (random depth, current normal)   1 (current depth, random normal)
+//   2 (random depth, random normal)    3 (current depth, perturbed normal)
+//   4 (perturbed depth, current normal) 5 (current depth, surface normal)
+// With a prior the random draws sit around the prior.  Returns 6, or 5 when
+// no surface normal is available.
+int refinement_hypotheses(const Ctx& c, const float* planes, int x, int y, Plane plane_now, float depth_now,
+                          bool have_prior, float depth_prior, V3 prior_n, float depth_sigma, float angle_sigma,
+                          uint32_t rs[2], uint32_t key, float rd[6], V3 rn[6]) {
+  const pmo_params& P = *c.pr;
+  const float depth_perturbation = 0.005f;
+  const float normal_perturbation = 0.01f * kPiF;
+  const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
+  float depth_rand, depth_pert;
+  V3 nrand, npert;
+  if (have_prior && depth_prior > 0.0f) {
+    depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(rs, key + 503u) * 6.0f * depth_sigma;
+    depth_rand = clampf(depth_rand, P.depth_min, P.depth_max);
+    const float pa1 = (rand_float(rs, key + 504u) - 0.5f) * angle_sigma;
+    const float pa2 = (rand_float(rs, key + 505u) - 0.5f) * angle_sigma;
+    nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
+    if (nrand.z > 0.0f) nrand.z = -nrand.z;
+  } else {
+    const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
+    const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
+    depth_rand = pm_expf(ldmin + rand_float(rs, key + 503u) * (ldmax - ldmin));
+    nrand.x = rand_float(rs, key + 504u) * 2.0f - 1.0f;
+    nrand.y = rand_float(rs, key + 505u) * 2.0f - 1.0f;
+    nrand.z = -1.0f;
+    nrand = normalize3(nrand);
+  }
+  {
+    const float dlo = (1.0f - depth_perturbation) * depth_now;
+    const float dhi = (1.0f + depth_perturbation) * depth_now;
+    depth_pert = dlo + rand_float(rs, key + 510u) * (dhi - dlo);
+    depth_pert = clampf(depth_pert, P.depth_min, P.depth_max);
+  }
+  {
+    const float pa1 = (rand_float(rs, key + 520u) - 0.5f) * normal_perturbation;
+    const float pa2 = (rand_float(rs, key + 521u) - 0.5f) * normal_perturbation;
+    npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
+    if (npert.z > 0.0f) npert.z = -npert.z;
+  }
+  const V3 sn = surface_normal(c, planes, x, y, depth_now);
+  const bool have_sn = len3(sn) > 0.5f;
+  const float d6[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
+  const V3 n6[6] = {cur_normal, nrand, nrand, npert, cur_normal, sn};
+  for (int h = 0; h < 6; ++h) { rd[h] = d6[h]; rn[h] = n6[h]; }
+  return have_sn ? 6 : 5;
+}
+
 // acmmp_patchmatch, one pixel of one colour: opencl_kernels.h:1197-1778.
 // Updates in place: every neighbour the pixel reads has the other colour
 // (odd Manhattan offsets), so this equals the reference's snapshot reads.
@@ -923,44 +975,10 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
 
   // Step 4: refinement (:1669-1772).
   {
-    const float depth_perturbation = 0.005f;
-    const float normal_perturbation = 0.01f * kPiF;
-    const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
-    float depth_rand, depth_pert;
-    V3 nrand, npert;
-    if (have_prior && depth_prior > 0.0f) {
-      depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(rs, key + 503u) * 6.0f * depth_sigma;
-      depth_rand = clampf(depth_rand, P.depth_min, P.depth_max);
-      const float pa1 = (rand_float(rs, key + 504u) - 0.5f) * angle_sigma;
-      const float pa2 = (rand_float(rs, key + 505u) - 0.5f) * angle_sigma;
-      nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
-      if (nrand.z > 0.0f) nrand.z = -nrand.z;
-    } else {
-      const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
-      const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
-      depth_rand = pm_expf(ldmin + rand_float(rs, key + 503u) * (ldmax - ldmin));
-      nrand.x = rand_float(rs, key + 504u) * 2.0f - 1.0f;
-      nrand.y = rand_float(rs, key + 505u) * 2.0f - 1.0f;
-      nrand.z = -1.0f;
-      nrand = normalize3(nrand);
-    }
-    {
-      const float dlo = (1.0f - depth_perturbation) * depth_now;
-      const float dhi = (1.0f + depth_perturbation) * depth_now;
-      depth_pert = dlo + rand_float(rs, key + 510u) * (dhi - dlo);
-      depth_pert = clampf(depth_pert, P.depth_min, P.depth_max);
-    }
-    {
-      const float pa1 = (rand_float(rs, key + 520u) - 0.5f) * normal_perturbation;
-      const float pa2 = (rand_float(rs, key + 521u) - 0.5f) * normal_perturbation;
-      npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
-      if (npert.z > 0.0f) npert.z = -npert.z;
-    }
-    const V3 sn = surface_normal(c, planes, x, y, depth_now);
-    const bool have_sn = len3(sn) > 0.5f;
-    const float rd[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
-    const V3 rn[6] = {cur_normal, nrand, nrand, npert, cur_normal, sn};
-    const int nc = have_sn ? 6 : 5;
+    float rd[6];
+    V3 rn[6];
+    const int nc = refinement_hypotheses(c, planes, x, y, plane_now, depth_now, have_prior, depth_prior, prior_n,
+                                         depth_sigma, angle_sigma, rs, key, rd, rn);
     for (int h = 0; h < nc; ++h) {
       const Plane cand = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[h], rn[h], c0);
       float cv[kMaxSrc];
@@ -1094,6 +1112,251 @@ void pmo_ref_patch(const pmo_scene* sc, const pmo_params* pr, int x, int y, floa
   const int P = (2 * pr->half_patch + 1) * (2 * pr->half_patch + 1);
   out[0] = rp.ref_center; out[1] = rp.sum_w; out[2] = rp.sum_wr; out[3] = rp.sum_wrr;
   for (int i = 0; i < P; ++i) { out[4 + i] = rp.w[i]; out[4 + P + i] = rp.wr[i]; }
+}
+
+// Pixel-area downscale (the cv::resize INTER_AREA of BuildImagePyramid,
+// estimator.cc:727-756): each output pixel is the mean of the input over its
+// footprint, edge pixels weighted by the covered fraction; sums in double,
+// optionally rounded to an integer (u8 images).
+void pmo_resize_area(const float* src, int w, int h, int nw, int nh, int round_int, float* dst) {
+  const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
+  for (int dy = 0; dy < nh; ++dy) {
+    const double y0 = dy * sy, y1 = std::min((dy + 1) * sy, static_cast<double>(h));
+    const int iy0 = static_cast<int>(std::floor(y0)), iy1 = std::min(h, static_cast<int>(std::ceil(y1)));
+    for (int dx = 0; dx < nw; ++dx) {
+      const double x0 = dx * sx, x1 = std::min((dx + 1) * sx, static_cast<double>(w));
+      const int ix0 = static_cast<int>(std::floor(x0)), ix1 = std::min(w, static_cast<int>(std::ceil(x1)));
+      double acc = 0.0;
+      for (int yy = iy0; yy < iy1; ++yy) {
+        const double wy = std::min(y1, yy + 1.0) - std::max(y0, static_cast<double>(yy));
+        double row = 0.0;
+        for (int xx = ix0; xx < ix1; ++xx) {
+          const double wx = std::min(x1, xx + 1.0) - std::max(x0, static_cast<double>(xx));
+          row += wx * static_cast<double>(src[static_cast<int64_t>(yy) * w + xx]);
+        }
+        acc += wy * row;
+      }
+      double v = acc / ((x1 - x0) * (y1 - y0));
+      if (round_int) v = std::nearbyint(v);
+      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<float>(v);
+    }
+  }
+}
+
+// Bilinear resize with pixel-centre alignment and edge clamping (the
+// cv::resize INTER_LINEAR of UploadPreviousDepths, estimator.cc:92-101).
+void pmo_resize_linear(const float* src, int w, int h, int nw, int nh, float* dst) {
+  const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
+  for (int dy = 0; dy < nh; ++dy) {
+    const double fy = (dy + 0.5) * sy - 0.5;
+    int y0 = static_cast<int>(std::floor(fy));
+    double ay = fy - y0;
+    if (y0 < 0) { y0 = 0; ay = 0.0; }
+    if (y0 >= h - 1) { y0 = h - 1; ay = 0.0; }
+    const int y1 = std::min(y0 + 1, h - 1);
+    for (int dx = 0; dx < nw; ++dx) {
+      const double fx = (dx + 0.5) * sx - 0.5;
+      int x0 = static_cast<int>(std::floor(fx));
+      double ax = fx - x0;
+      if (x0 < 0) { x0 = 0; ax = 0.0; }
+      if (x0 >= w - 1) { x0 = w - 1; ax = 0.0; }
+      const int x1 = std::min(x0 + 1, w - 1);
+      const double v00 = src[static_cast<int64_t>(y0) * w + x0], v01 = src[static_cast<int64_t>(y0) * w + x1];
+      const double v10 = src[static_cast<int64_t>(y1) * w + x0], v11 = src[static_cast<int64_t>(y1) * w + x1];
+      const double v = (1.0 - ay) * ((1.0 - ax) * v00 + ax * v01) + ay * ((1.0 - ax) * v10 + ax * v11);
+      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<float>(v);
+    }
+  }
+}
+
+// Coarse-to-fine transfer (acmmp_upsample, opencl_kernels.h:1792-1856) from a
+// decoded src_w x src_h level; the coarse plane of each coarse pixel is
+// built from its decoded depth and normal as UpsampleDepthNormal does
+// (estimator.cc:775-791), with the fine camera's intrinsics scaled by the
+// size ratio.  Sets planes, costs (2.0), rng and sel (0) of the fine state.
+void pmo_upsample(const pmo_scene* sc, const pmo_params* pr, const float* cdepth, const float* cnormal, int src_w,
+                  int src_h, pmo_state* st) {
+  Ctx c(sc, pr);
+  const int W = sc->width, H = sc->height;
+  const pmo_camera& c0 = c.cams[0];
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      const int idx = y * W + x;
+      uint32_t rs[2] = {static_cast<uint32_t>(idx) * 1099087573u + 2654435769u,
+                        static_cast<uint32_t>(idx) * 2246822519u + 13u};
+      const float scale_x = static_cast<float>(src_w) / static_cast<float>(W);
+      const float scale_y = static_cast<float>(src_h) / static_cast<float>(H);
+      const int sx = std::min(std::max(static_cast<int>(static_cast<float>(x) * scale_x), 0), src_w - 1);
+      const int sy = std::min(std::max(static_cast<int>(static_cast<float>(y) * scale_y), 0), src_h - 1);
+      const int sidx = sy * src_w + sx;
+      const float d = cdepth[sidx];
+      const V3 cn{cnormal[3 * sidx], cnormal[3 * sidx + 1], cnormal[3 * sidx + 2]};
+      const float fx = c0.K[0] * static_cast<float>(src_w) / static_cast<float>(W);
+      const float fy = c0.K[4] * static_cast<float>(src_h) / static_cast<float>(H);
+      const float cxs = c0.K[2] * static_cast<float>(src_w) / static_cast<float>(W);
+      const float cys = c0.K[5] * static_cast<float>(src_h) / static_cast<float>(H);
+      const V3 pt{d * (static_cast<float>(sx) - cxs) / fx, d * (static_cast<float>(sy) - cys) / fy, d};
+      const Plane cp{cn.x, cn.y, cn.z, -(cn.x * pt.x + cn.y * pt.y + cn.z * pt.z)};
+      const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cp, c0);
+      Plane out;
+      if (depth > 0.0f) {
+        out = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth,
+                                      normalize3(V3{cp.x, cp.y, cp.z}), c0);
+      } else {
+        const uint32_t key = static_cast<uint32_t>(idx + 7919);
+        const float ldmin = pm_logf(std::fmax(pr->depth_min, 1e-6f));
+        const float ldmax = pm_logf(std::fmax(pr->depth_max, 1e-5f));
+        const float rd = pm_expf(ldmin + rand_float(rs, key) * (ldmax - ldmin));
+        V3 n;
+        n.x = rand_float(rs, key + 3u) * 2.0f - 1.0f;
+        n.y = rand_float(rs, key + 5u) * 2.0f - 1.0f;
+        n.z = -1.0f;
+        n = normalize3(n);
+        out = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd, n, c0);
+      }
+      store_plane(st->planes, idx, out);
+      st->costs[idx] = 2.0f;
+      st->rng[2 * idx] = rs[0];
+      st->rng[2 * idx + 1] = rs[1];
+      st->sel[idx] = 0u;
+    }
+  }
+}
+
+// Host post-processing of a decoded map (ReadBackResults, estimator.cc:
+// 1194-1222): speckle removal (RemoveSmallSegments, :1231-1312: 4-connected
+// segments of depths within 1 % relative of their neighbour; smaller than
+// speckle_min pixels are cleared), gap filling (GapInterpolation,
+// :1319-1418: runs of at most gap_max empty pixels between two depths within
+// 2.5 % are filled linearly, rows first, then still-empty column runs), and
+// the confidence of the result.  Each step runs only when its size is > 0.
+void pmo_postprocess(float* depth, float* normal, const float* cost, float* confidence, int w, int h,
+                     int speckle_min, int gap_max) {
+  const int npix = w * h;
+  if (speckle_min > 0) {
+    const float thr = 0.01f;
+    std::vector<char> seen(static_cast<size_t>(npix), 0);
+    std::vector<int> comp;
+    const int ox[4] = {1, -1, 0, 0}, oy[4] = {0, 0, 1, -1};
+    for (int s = 0; s < npix; ++s) {
+      if (seen[s] || depth[s] <= 0.0f) { seen[s] = 1; continue; }
+      comp.assign(1, s);
+      seen[s] = 1;
+      for (size_t k = 0; k < comp.size(); ++k) {
+        const int c = comp[k], cy = c / w, cx = c % w;
+        for (int d = 0; d < 4; ++d) {
+          const int nx = cx + ox[d], ny = cy + oy[d];
+          if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+          const int ni = ny * w + nx;
+          if (seen[ni]) continue;
+          if (depth[ni] <= 0.0f) { seen[ni] = 1; continue; }
+          if (std::fabs(depth[c] - depth[ni]) / std::max(depth[c], depth[ni]) < thr) {
+            seen[ni] = 1;
+            comp.push_back(ni);
+          }
+        }
+      }
+      if (static_cast<int>(comp.size()) < speckle_min)
+        for (int c : comp) { depth[c] = 0.0f; normal[3 * c] = normal[3 * c + 1] = normal[3 * c + 2] = 0.0f; }
+    }
+  }
+  if (gap_max > 0) {
+    const float thr = 0.025f;
+    // fill pixel q from valid a to valid b at fraction t
+    auto fill = [&](int a, int b, int q, float t, bool only_empty) {
+      if (only_empty && depth[q] > 0.0f) return;
+      depth[q] = depth[a] + t * (depth[b] - depth[a]);
+      float v0 = (1.0f - t) * normal[3 * a] + t * normal[3 * b];
+      float v1 = (1.0f - t) * normal[3 * a + 1] + t * normal[3 * b + 1];
+      float v2 = (1.0f - t) * normal[3 * a + 2] + t * normal[3 * b + 2];
+      const float len = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+      if (len > 1e-6f) { v0 /= len; v1 /= len; v2 /= len; }
+      normal[3 * q] = v0; normal[3 * q + 1] = v1; normal[3 * q + 2] = v2;
+    };
+    // one pass over lines; idx(line, pos) maps to a pixel
+    auto pass = [&](int lines, int len, auto idx, bool only_empty) {
+      for (int l = 0; l < lines; ++l) {
+        int start = -1;
+        for (int i = 0; i < len; ++i) {
+          const int b = idx(l, i);
+          if (depth[b] <= 0.0f) { if (start < 0) start = i; continue; }
+          if (start >= 0 && start - 1 >= 0 && i - start <= gap_max) {
+            const int a = idx(l, start - 1);
+            // depth[a] is read before the run is filled (the run starts after a)
+            const float da = depth[a], db = depth[b];
+            if (std::fabs(da - db) / std::max(da, db) < thr)
+              for (int g = start; g < i; ++g)
+                fill(a, b, idx(l, g), static_cast<float>(g - (start - 1)) / static_cast<float>(i - (start - 1)),
+                     only_empty);
+          }
+          start = -1;
+        }
+      }
+    };
+    pass(h, w, [w](int l, int i) { return l * w + i; }, false);
+    pass(w, h, [w](int l, int i) { return i * w + l; }, true);
+  }
+  for (int i = 0; i < npix; ++i)
+    confidence[i] = depth[i] > 0.0f ? 1.0f / (1.0f + pm_expf((cost[i] - 0.5f) * 5.0f)) : 0.0f;
+}
+
+// Internal pieces, exposed so the tests can pin them one by one.
+float pmo_smoothness_cost(const pmo_scene* sc, const pmo_params* pr, const float* planes, const float* plane,
+                          int x, int y) {
+  Ctx c(sc, pr);
+  return smoothness_cost(c, planes, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
+}
+void pmo_surface_normal(const pmo_scene* sc, const pmo_params* pr, const float* planes, int x, int y,
+                        float center_depth, float* out) {
+  Ctx c(sc, pr);
+  const V3 n = surface_normal(c, planes, x, y, center_depth);
+  out[0] = n.x; out[1] = n.y; out[2] = n.z;
+}
+float pmo_geom_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, int sidx, const float* plane,
+                    int x, int y) {
+  Ctx c(sc, pr);
+  return geom_cost(c, st, sidx, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
+}
+float pmo_weighted_cost_geom(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const float* cv,
+                             const float* vw, float wnorm, const float* plane, int x, int y) {
+  Ctx c(sc, pr);
+  return weighted_cost_geom(c, st, cv, vw, wnorm, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
+}
+float pmo_mc_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const float* cv, const float* vw,
+                  float wnorm, const float* plane, int x, int y, uint32_t* rng, uint32_t key) {
+  Ctx c(sc, pr);
+  return mc_cost(c, st, cv, vw, wnorm, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, rng, key);
+}
+
+// Refinement hypotheses at (x, y) for the current plane (and an optional
+// prior plane); writes n planes (n, d) to out and returns n.  rng advances.
+int pmo_refinement_hypotheses(const pmo_scene* sc, const pmo_params* pr, const float* planes, int x, int y,
+                              const float* plane_now, const float* prior_plane, uint32_t* rng, uint32_t key,
+                              float* out) {
+  Ctx c(sc, pr);
+  const pmo_camera& c0 = c.cams[0];
+  const Plane pn{plane_now[0], plane_now[1], plane_now[2], plane_now[3]};
+  const float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pn, c0);
+  bool have_prior = prior_plane != nullptr;
+  float depth_prior = 0.0f;
+  V3 prior_n{0.0f, 0.0f, -1.0f};
+  if (have_prior) {
+    const Plane pp{prior_plane[0], prior_plane[1], prior_plane[2], prior_plane[3]};
+    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+    prior_n = normalize3(V3{pp.x, pp.y, pp.z});
+  }
+  const float depth_sigma = (pr->depth_max - pr->depth_min) / 32.0f;
+  const float angle_sigma = kPiF * (5.0f / 180.0f);
+  float rd[6];
+  V3 rn[6];
+  const int n = refinement_hypotheses(c, planes, x, y, pn, depth_now, have_prior, depth_prior, prior_n,
+                                      depth_sigma, angle_sigma, rng, key, rd, rn);
+  for (int h = 0; h < n; ++h) {
+    const Plane q = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[h], rn[h], c0);
+    out[4 * h] = q.x; out[4 * h + 1] = q.y; out[4 * h + 2] = q.z; out[4 * h + 3] = q.w;
+  }
+  return n;
 }
 
 // Score n (pixel, plane) pairs: per-view costs (n x num_src), top-k initial

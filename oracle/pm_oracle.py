@@ -82,6 +82,16 @@ def lib():
         L.pmo_checkerboard.argtypes = [vp, vp, vp, i]
         L.pmo_smooth_weight.restype = f; L.pmo_smooth_weight.argtypes = [vp, i]
         L.pmo_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.pmo_smoothness_cost.restype = f; L.pmo_smoothness_cost.argtypes = [vp, vp, vp, vp, i, i]
+        L.pmo_surface_normal.argtypes = [vp, vp, vp, i, i, f, vp]
+        L.pmo_geom_cost.restype = f; L.pmo_geom_cost.argtypes = [vp, vp, vp, i, vp, i, i]
+        L.pmo_weighted_cost_geom.restype = f
+        L.pmo_weighted_cost_geom.argtypes = [vp, vp, vp, vp, vp, f, vp, i, i]
+        L.pmo_mc_cost.restype = f; L.pmo_mc_cost.argtypes = [vp, vp, vp, vp, vp, f, vp, i, i, vp, u]
+        L.pmo_resize_area.argtypes = [vp, i, i, i, i, i, vp]
+        L.pmo_postprocess.argtypes = [vp, vp, vp, vp, i, i, i, i]
+        L.pmo_resize_linear.argtypes = [vp, i, i, i, i, vp]
+        L.pmo_upsample.argtypes = [vp, vp, vp, vp, i, i, vp]
         L.pmo_set_threads.argtypes = [i]
         L.pmo_max_threads.restype = i
         _lib = L
@@ -147,6 +157,8 @@ class Params:
     geom_weight: float = 0.0
     var_threshold: float = 1.5e-6
     checkerboard_filter: bool = False
+    speckle_min_size: int = 0
+    gap_max_size: int = 0
 
     def c(self) -> PmoParams:
         p = PmoParams()
@@ -291,7 +303,18 @@ def decode(scene: OracleScene, params: Params, state: OracleState):
     p = params.c()
     lib().pmo_decode(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c), _ptr(depth),
                      _ptr(normal), _ptr(cost), _ptr(conf))
+    if params.speckle_min_size > 0 or params.gap_max_size > 0:
+        postprocess(depth, normal, cost, conf, params.speckle_min_size, params.gap_max_size)
     return depth, normal, cost, conf
+
+
+def postprocess(depth: np.ndarray, normal: np.ndarray, cost: np.ndarray, conf: np.ndarray, speckle_min: int,
+                gap_max: int) -> None:
+    """Speckle removal, gap filling and confidence, in place (estimator.cc:1194-1222)."""
+    for a in (depth, normal, cost, conf):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    h, w = depth.shape
+    lib().pmo_postprocess(_ptr(depth), _ptr(normal), _ptr(cost), _ptr(conf), w, h, int(speckle_min), int(gap_max))
 
 
 def estimate(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params):
@@ -315,3 +338,195 @@ def rand_float(state: np.ndarray, key: int) -> float:
     """One draw; ``state`` (uint32[2]) is advanced in place."""
     assert state.dtype == np.uint32 and state.size == 2
     return float(lib().pmo_rand_float(state.ctypes.data, ctypes.c_uint32(key & 0xFFFFFFFF)))
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, np.float32)
+    assert n is None or a.size == n
+    return a
+
+
+def smoothness_cost(scene: OracleScene, params: Params, planes: np.ndarray, plane, x: int, y: int) -> float:
+    """4-neighbour smoothness of ``plane`` at (x, y) against ``planes`` (h, w, 4) (opencl_kernels.h:649-707)."""
+    p = params.c()
+    planes = _f32(planes, scene.width * scene.height * 4)
+    return float(lib().pmo_smoothness_cost(ctypes.byref(scene.c), ctypes.byref(p), _ptr(planes),
+                                           _ptr(_f32(plane, 4)), x, y))
+
+
+def surface_normal(scene: OracleScene, params: Params, planes: np.ndarray, x: int, y: int,
+                   center_depth: float) -> np.ndarray:
+    """Normal from the central differences of the 4-neighbour depths (opencl_kernels.h:725-772)."""
+    p = params.c()
+    planes = _f32(planes, scene.width * scene.height * 4)
+    out = np.zeros(3, np.float32)
+    lib().pmo_surface_normal(ctypes.byref(scene.c), ctypes.byref(p), _ptr(planes), x, y, float(center_depth),
+                             _ptr(out))
+    return out
+
+
+def geom_cost(scene: OracleScene, params: Params, state: OracleState, sidx: int, plane, x: int, y: int) -> float:
+    """Forward-backward reprojection cost against source sidx's previous depth (opencl_kernels.h:779-861)."""
+    p = params.c()
+    return float(lib().pmo_geom_cost(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c), sidx,
+                                     _ptr(_f32(plane, 4)), x, y))
+
+
+def weighted_cost_geom(scene: OracleScene, params: Params, state: OracleState, cv, vw, wnorm: float, plane,
+                       x: int, y: int) -> float:
+    p = params.c()
+    n = scene.n_views - 1
+    return float(lib().pmo_weighted_cost_geom(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c),
+                                              _ptr(_f32(cv, n)), _ptr(_f32(vw, n)), float(wnorm),
+                                              _ptr(_f32(plane, 4)), x, y))
+
+
+def mc_cost(scene: OracleScene, params: Params, state: OracleState, cv, vw, wnorm: float, plane, x: int, y: int,
+            rng: np.ndarray, key: int) -> float:
+    """32-draw view-sampled cost (opencl_kernels.h:910-969); ``rng`` (uint32[2]) advances in place."""
+    assert rng.dtype == np.uint32 and rng.size == 2
+    p = params.c()
+    n = scene.n_views - 1
+    return float(lib().pmo_mc_cost(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c),
+                                   _ptr(_f32(cv, n)), _ptr(_f32(vw, n)), float(wnorm), _ptr(_f32(plane, 4)),
+                                   x, y, rng.ctypes.data, ctypes.c_uint32(key & 0xFFFFFFFF)))
+
+
+# ---------------------------------------------------------------------------
+# Hierarchy levels (estimator.cc:134-255, 724-821)
+# ---------------------------------------------------------------------------
+def resize_area(img: np.ndarray, nw: int, nh: int, round_int: bool = False) -> np.ndarray:
+    src = np.ascontiguousarray(img, np.float32)
+    out = np.zeros((nh, nw), np.float32)
+    lib().pmo_resize_area(_ptr(src), src.shape[1], src.shape[0], nw, nh, 1 if round_int else 0, _ptr(out))
+    return out
+
+
+def resize_linear(img: np.ndarray, nw: int, nh: int) -> np.ndarray:
+    src = np.ascontiguousarray(img, np.float32)
+    out = np.zeros((nh, nw), np.float32)
+    lib().pmo_resize_linear(_ptr(src), src.shape[1], src.shape[0], nw, nh, _ptr(out))
+    return out
+
+
+def level_views(images: Sequence[np.ndarray], Ks, scale: float, quantize_half: bool = False):
+    """One pyramid level (BuildImagePyramid, estimator.cc:727-756).
+
+    ``images`` as given to the estimator: uint8 (resampled as u8, rounded,
+    then /255) or float in [0, 1].  Returns float32 images and scaled K."""
+    sc32 = np.float32(scale)
+    out_imgs, out_Ks = [], []
+    for im, K in zip(images, Ks):
+        h, w = im.shape
+        if scale < 0.999:
+            nw = max(int(np.float32(w) * sc32), 1)
+            nh = max(int(np.float32(h) * sc32), 1)
+        else:
+            nw, nh = w, h
+        if im.dtype == np.uint8:
+            u = im.astype(np.float32) if (nw, nh) == (w, h) else resize_area(im.astype(np.float32), nw, nh, True)
+            lv = u * np.float32(1.0 / 255.0)
+        else:
+            lv = np.asarray(im, np.float32) if (nw, nh) == (w, h) else resize_area(im, nw, nh, False)
+        if quantize_half:
+            lv = lv.astype(np.float16).astype(np.float32)
+        K2 = np.array(K, np.float64).copy()
+        if (nw, nh) != (w, h):
+            fx = float(np.float32(nw) / np.float32(w))
+            fy = float(np.float32(nh) / np.float32(h))
+            K2[0, 0] *= fx; K2[0, 2] *= fx; K2[1, 1] *= fy; K2[1, 2] *= fy
+        out_imgs.append(lv)
+        out_Ks.append(K2)
+    return out_imgs, out_Ks
+
+
+def upsample(scene: OracleScene, params: Params, state: OracleState, coarse_depth: np.ndarray,
+             coarse_normal: np.ndarray) -> None:
+    ch, cw = coarse_depth.shape
+    cd = np.ascontiguousarray(coarse_depth, np.float32)
+    cn = np.ascontiguousarray(coarse_normal, np.float32)
+    p = params.c()
+    lib().pmo_upsample(ctypes.byref(scene.c), ctypes.byref(p), _ptr(cd), _ptr(cn), cw, ch, ctypes.byref(state.c))
+
+
+def run_level(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params, level: int, levels: int, coarse=None,
+              prev_depths=None, quantize_half: bool = False):
+    """One pyramid level (RunLevel, estimator.cc:172-255).
+
+    ``coarse``: (depth, normal) of the previous level of the same view, or
+    None at level 0.  ``prev_depths``: {source slot: depth map of any size},
+    resized to the level bilinearly (UploadPreviousDepths, estimator.cc:
+    66-116).  Returns (depth, normal, cost, confidence, state)."""
+    scale = 1.0 / 2.0 ** (levels - 1 - level)
+    lv_images, lv_Ks = level_views(images, Ks, scale, quantize_half)
+    sc = OracleScene(lv_images, lv_Ks, Rs, ts)
+    h, w = sc.height, sc.width
+    stack, mask = None, 0
+    if prev_depths and params.geom_weight > 0:
+        stack = np.zeros((sc.n_views - 1, h, w), np.float32)
+        for slot, d in prev_depths.items():
+            if d is None or not (0 <= slot < sc.n_views - 1):
+                continue
+            d = np.asarray(d, np.float32)
+            stack[slot] = d if d.shape == (h, w) else resize_linear(d, w, h)
+            mask |= 1 << slot
+    st = OracleState(w, h, prev_depths=stack, prev_depth_mask=mask)
+    if coarse is None:
+        random_init(sc, params, st)
+    else:
+        upsample(sc, params, st, coarse[0], coarse[1])
+    prior_reinit(sc, params, st)
+    for it in range(params.max_iterations):
+        sweep(sc, params, st, 0, it)
+        sweep(sc, params, st, 1, it)
+    if params.checkerboard_filter:
+        checkerboard(sc, params, st, 0)
+        checkerboard(sc, params, st, 1)
+    d, nrm, c, conf = decode(sc, params, st)
+    return d, nrm, c, conf, st
+
+
+def estimate_levels(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params, levels: int,
+                    quantize_half: bool = False):
+    """Coarse-to-fine run over ``levels`` pyramid levels (Run / RunLevel,
+    estimator.cc:134-255); returns (depth, normal, cost, confidence, state)
+    of the full-resolution level."""
+    n = max(1, int(levels))
+    coarse = None
+    for level in range(n):
+        d, nrm, c, conf, st = run_level(images, Ks, Rs, ts, params, level, n, coarse,
+                                        quantize_half=quantize_half)
+        coarse = (d, nrm)
+    return d, nrm, c, conf, st
+
+
+def estimate_cluster(refs, params: Params, levels: int, geom_weight: float):
+    """DepthmapClusterEstimator::Run (cluster_estimator.cc:82-156).
+
+    ``refs``: list of dicts with images, Ks, Rs, ts (reference first),
+    depth_min, depth_max and links [(source slot, from reference)].  Level
+    by level, every reference runs with the previous level's depth maps of
+    the references its links name.  Returns [(depth, normal, cost,
+    confidence)] per reference."""
+    import dataclasses
+    n = max(1, int(levels))
+    k = len(refs)
+    coarse = [None] * k
+    prev = [None] * k
+    out = [None] * k
+    for level in range(n):
+        cur = [None] * k
+        for i, r in enumerate(refs):
+            p = dataclasses.replace(params, depth_min=r["depth_min"], depth_max=r["depth_max"],
+                                    geom_weight=geom_weight)
+            pd = {}
+            if level > 0 and geom_weight > 0:
+                for slot, frm in r.get("links", []):
+                    if 0 <= frm < k and prev[frm] is not None:
+                        pd[slot] = prev[frm]
+            d, nrm, c, conf, _ = run_level(r["images"], r["Ks"], r["Rs"], r["ts"], p, level, n, coarse[i], pd)
+            coarse[i] = (d, nrm)
+            cur[i] = d
+            out[i] = (d, nrm, c, conf)
+        prev = cur
+    return out

@@ -27,6 +27,8 @@ def gpu_estimator(pydense, scene, params, storage="f32", images=None):
     est.set_anchor_views(params.anchor_views)
     est.set_checkerboard_filter(params.checkerboard_filter)
     est.set_var_threshold(params.var_threshold)
+    est.set_speckle_min_size(params.speckle_min_size)
+    est.set_gap_max_size(params.gap_max_size)
     est.set_image_storage(storage)
     return est
 

@@ -108,3 +108,77 @@ def test_compute_patch_match_surface(pydense, scene_small):
     nz = depth > 0
     assert np.allclose(np.linalg.norm(normal[nz], axis=1), 1.0, atol=1e-5)
     assert np.all((depth[nz] >= p.depth_min) & (depth[nz] <= p.depth_max))
+
+
+def _run_both(pydense, scene, p, prior=None, prev=None):
+    """Full single-level run on the GPU and in the oracle; returns both depth maps."""
+    est = gpu_estimator(pydense, scene, p)
+    h, w = scene.gt_depth.shape
+    kw = {}
+    if prior is not None:
+        est.set_prior(prior[0], prior[1])
+        kw.update(prior_planes=prior[0], plane_masks=prior[1])
+    if prev is not None:
+        est.set_geom_consistency(p.geom_weight, prev)
+        stack = np.zeros((scene.num_src, h, w), np.float32)
+        mask = 0
+        for i, d in enumerate(prev):
+            if d is not None:
+                stack[i] = d
+                mask |= 1 << i
+        kw.update(prev_depths=stack, prev_depth_mask=mask)
+    out = est.compute()
+    sc = po.OracleScene(scene.images, scene.Ks, scene.Rs, scene.ts)
+    st = po.OracleState(w, h, **kw)
+    po.run(sc, p, st)
+    d_o, _, _, _ = po.decode(sc, p, st)
+    return out["depth"], d_o
+
+
+def _assert_parity(tag, d_g, d_o):
+    valid = d_o > 0
+    within = np.mean(np.abs(d_g[valid] - d_o[valid]) < 0.01 * d_o[valid])
+    print(f"[{tag}] bit-equal {np.mean(d_g == d_o):.6f} within1% {within:.6f}", flush=True)
+    assert within >= 0.99
+
+
+def test_prior_run_matches_oracle(pydense, scene_small):
+    # planar prior from the ground truth on every third row (estimator.cc:574-614)
+    s = scene_small
+    h, w = s.gt_depth.shape
+    K = np.asarray(s.Ks[0])
+    ys, xs = np.mgrid[0:h, 0:w]
+    d = np.where(s.gt_depth > 0, s.gt_depth, 10.0).astype(np.float64)
+    n = np.array([0.05, -0.05, -1.0]) / np.linalg.norm([0.05, -0.05, -1.0])
+    pt = np.stack([d * (xs - K[0, 2]) / K[0, 0], d * (ys - K[1, 2]) / K[1, 1], d], -1)
+    planes = np.concatenate([np.broadcast_to(n, (h, w, 3)), -(pt @ n)[..., None]], -1).astype(np.float32)
+    masks = np.zeros((h, w), np.uint32)
+    masks[::3] = 1
+    p = params_for(s, patch_size=5)
+    d_g, d_o = _run_both(pydense, s, p, prior=(planes, masks))
+    _assert_parity("prior", d_g, d_o)
+
+
+def test_geom_consistency_run_matches_oracle(pydense, scene_small):
+    # previous depth maps in source slots 0 and 2 (SetPreviousDepths, estimator.cc:59-116)
+    s = scene_small
+    rng = np.random.default_rng(7)
+    base = np.where(s.gt_depth > 0, s.gt_depth, 0.0).astype(np.float32)
+    prev = [base * np.float32(1.0 + 0.02 * rng.standard_normal()), None,
+            (base * (1.0 + 0.01 * rng.standard_normal(base.shape))).astype(np.float32), None]
+    p = params_for(s, patch_size=5, geom_weight=0.1)
+    d_g, d_o = _run_both(pydense, s, p, prev=prev)
+    _assert_parity("geom", d_g, d_o)
+
+
+@pytest.mark.parametrize("variant", ["checkerboard", "no_census", "patch3_topk2", "no_smooth_no_anchor",
+                                     "sigmas"])
+def test_parameter_variants_match_oracle(pydense, scene_tiny, variant):
+    kw = {"checkerboard": dict(checkerboard_filter=True),
+          "no_census": dict(use_census=False),
+          "patch3_topk2": dict(patch_size=3, top_k=2),
+          "no_smooth_no_anchor": dict(smooth_weight=0.0, anchor_views=0),
+          "sigmas": dict(sigma_spatial=3.0, sigma_color=3.0 / 255.0, max_iterations=4)}[variant]
+    p = params_for(scene_tiny, **kw)
+    d_g, d_o = _run_both(pydense, scene_tiny, p)
+    _assert_parity(variant, d_g, d_o)

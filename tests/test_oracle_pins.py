@@ -1,0 +1,531 @@
+"""Closed-form and special-case checks of the oracle's inner functions.
+
+Each test fixes a result by the mathematics or by a rule stated in the
+reference source (cited per test), so that a dropped term, a wrong sign, a
+swapped index or a wrong offset in the oracle fails.  All CPU.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pm_oracle as po
+
+
+def _cam_scene(w=64, h=48, fx=300.0, fy=250.0, n_src=1, src_ts=None, seed=0):
+    """Reference camera at the origin (R = I, t = 0) + sources; random images."""
+    rng = np.random.default_rng(seed)
+    K = np.array([[fx, 0.0, (w - 1) / 2.0], [0.0, fy, (h - 1) / 2.0], [0.0, 0.0, 1.0]])
+    Ks, Rs, ts = [K], [np.eye(3)], [np.zeros(3)]
+    for i in range(n_src):
+        Ks.append(K.copy())
+        Rs.append(np.eye(3))
+        ts.append(np.asarray(src_ts[i], float) if src_ts else np.array([-0.3 * (i + 1), 0.0, 0.0]))
+    images = [rng.random((h, w)).astype(np.float32) for _ in range(n_src + 1)]
+    return po.OracleScene(images, Ks, Rs, ts), K
+
+
+def _plane_through(n, depth, x, y, K):
+    """(n, d) of the plane with normal n through the ray of pixel (x, y) at ``depth``."""
+    n = np.asarray(n, float) / np.linalg.norm(n)
+    X = depth * np.array([(x - K[0, 2]) / K[0, 0], (y - K[1, 2]) / K[1, 1], 1.0])
+    return np.array([n[0], n[1], n[2], -n @ X], np.float32)
+
+
+def _depth64(plane, x, y, K):
+    r = np.array([(x - K[0, 2]) / K[0, 0], (y - K[1, 2]) / K[1, 1], 1.0])
+    return -float(plane[3]) / float(np.asarray(plane[:3], float) @ r)
+
+
+# ---------------------------------------------------------------------------
+# compute_surface_normal (opencl_kernels.h:725-772)
+# ---------------------------------------------------------------------------
+def test_surface_normal_special_cases_and_symmetry():
+    sc, K = _cam_scene()
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    w, h = sc.width, sc.height
+    x, y = 20, 17
+
+    def field(plane):
+        return np.broadcast_to(np.asarray(plane, np.float32), (h, w, 4)).copy()
+
+    # fronto-parallel plane: zero depth gradient -> exactly (0, 0, -1)
+    fp = np.array([0, 0, -1, 7.0], np.float32)
+    assert np.array_equal(po.surface_normal(sc, p, field(fp), x, y, 7.0), np.float32([0, 0, -1]))
+    # a plane tilted about the image y axis only has n_y == 0; one tilted
+    # about x only has n_x == 0 (no fx/fy or dx/dy swap)
+    px = _plane_through([0.4, 0.0, -1.0], 8.0, x, y, K)
+    nx = po.surface_normal(sc, p, field(px), x, y, _depth64(px, x, y, K))
+    assert nx[1] == 0.0 and abs(nx[0]) > 0.05
+    py = _plane_through([0.0, 0.4, -1.0], 8.0, x, y, K)
+    ny = po.surface_normal(sc, p, field(py), x, y, _depth64(py, x, y, K))
+    assert ny[0] == 0.0 and abs(ny[1]) > 0.05
+    # depth growing to the right (n_x > 0 for this convention) gives n_x > 0,
+    # the camera-facing sign n_z < 0, and unit length
+    dx = _depth64(px, x + 1, y, K) - _depth64(px, x - 1, y, K)
+    assert dx > 0 and nx[0] > 0 and nx[2] < 0
+    assert abs(np.linalg.norm(nx) - 1.0) < 1e-6
+    # border pixels and invalid centre depth -> zero vector
+    assert not po.surface_normal(sc, p, field(px), 0, y, 8.0).any()
+    assert not po.surface_normal(sc, p, field(px), x, h - 1, 8.0).any()
+    assert not po.surface_normal(sc, p, field(px), x, y, 0.0).any()
+
+
+def test_surface_normal_matches_the_documented_formula_on_a_plane():
+    # The reference documents n = normalize(fx*dx, fy*dy, (cx-x)*dx + (cy-y)*dy - depth)
+    # with dx = d(x+1) - d(x-1), dy = d(y+1) - d(y-1) (opencl_kernels.h:715-721).
+    # Evaluate it in float64 on the analytic plane depths and compare.
+    sc, K = _cam_scene()
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    w, h = sc.width, sc.height
+    for n, (x, y) in [([0.3, -0.2, -1.0], (20, 17)), ([-0.5, 0.25, -1.0], (41, 30)), ([0.1, 0.6, -1.0], (9, 38))]:
+        pl = _plane_through(n, 9.0, x, y, K)
+        planes = np.broadcast_to(pl, (h, w, 4)).copy()
+        d0 = _depth64(pl, x, y, K)
+        got = po.surface_normal(sc, p, planes, x, y, d0)
+        ddx = _depth64(pl, x + 1, y, K) - _depth64(pl, x - 1, y, K)
+        ddy = _depth64(pl, x, y + 1, K) - _depth64(pl, x, y - 1, K)
+        v = np.array([K[0, 0] * ddx, K[1, 1] * ddy, (K[0, 2] - x) * ddx + (K[1, 2] - y) * ddy - d0])
+        v /= np.linalg.norm(v)
+        if v[2] > 0:
+            v = -v
+        assert np.max(np.abs(got - v)) < 2e-4, (got, v)
+
+
+# ---------------------------------------------------------------------------
+# compute_smoothness_cost (opencl_kernels.h:649-707)
+# ---------------------------------------------------------------------------
+def _smooth_setup():
+    sc, K = _cam_scene(fx=100.0, fy=100.0)
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    return sc, K, p
+
+
+def test_smoothness_zero_on_the_same_plane():
+    sc, K, p = _smooth_setup()
+    x, y = 30, 20
+    cur = _plane_through([0.2, -0.3, -1.0], 6.0, x, y, K)
+    planes = np.broadcast_to(cur, (sc.height, sc.width, 4)).copy()
+    assert abs(po.smoothness_cost(sc, p, planes, cur, x, y)) < 1e-6
+
+
+@pytest.mark.parametrize("delta", [0.01, 0.03, 0.08, 0.2])
+def test_smoothness_depth_deviation_term(delta):
+    # neighbours fronto-parallel at D(1 + delta): normal term 0, depth term
+    # 0.5 * min(10 * delta, 1); equal bilateral weights cancel.
+    sc, K, p = _smooth_setup()
+    x, y, D = 30, 20, 6.0
+    cur = np.float32([0, 0, -1, D])
+    planes = np.broadcast_to(np.float32([0, 0, -1, D * (1 + delta)]), (sc.height, sc.width, 4)).copy()
+    # bilateral weight exp(-delta^2 / (2 * 0.02^2)) per neighbour; a total
+    # weight below 1e-6 makes the cost 0
+    wsum = 4 * math.exp(-delta * delta / (2 * 0.02 * 0.02))
+    expect = 0.5 * min(10 * delta, 1.0) if wsum >= 1e-6 else 0.0
+    got = po.smoothness_cost(sc, p, planes, cur, x, y)
+    assert abs(got - expect) < 1e-5
+    # border pixel: three neighbours, same value
+    if 0.75 * wsum >= 1e-6:
+        assert abs(po.smoothness_cost(sc, p, planes, cur, 0, y) - expect) < 1e-5
+
+
+@pytest.mark.parametrize("deg", [30.0, 60.0, 90.0])
+def test_smoothness_normal_term(deg):
+    # neighbours at the same depth with normals rotated by deg: depth term 0,
+    # normal term 0.5 * (1 - cos(deg)).
+    sc, K, p = _smooth_setup()
+    x, y, D = 30, 20, 6.0
+    cur = np.float32([0, 0, -1, D])
+    planes = np.zeros((sc.height, sc.width, 4), np.float32)
+    t = math.radians(deg)
+    for nx, ny in [(x - 1, y), (x + 1, y), (x, y - 1), (x, y + 1)]:
+        planes[ny, nx] = _plane_through([math.sin(t), 0.0, -math.cos(t)], D, nx, ny, K)
+    got = po.smoothness_cost(sc, p, planes, cur, x, y)
+    assert abs(got - 0.5 * (1.0 - math.cos(t))) < 2e-5
+
+
+def test_smoothness_without_valid_neighbours_is_zero():
+    sc, K, p = _smooth_setup()
+    planes = np.zeros((sc.height, sc.width, 4), np.float32)  # depth 0 everywhere
+    assert po.smoothness_cost(sc, p, planes, np.float32([0, 0, -1, 6.0]), 30, 20) == 0.0
+
+
+# ---------------------------------------------------------------------------
+# compute_geom_consistency_cost (opencl_kernels.h:779-861)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("err_px,expect", [(0.0, 0.0), (2.9, 0.0), (4.5, 0.5), (5.4, 0.8), (7.0, 1.0)])
+def test_geom_cost_parallel_stereo_closed_form(err_px, expect):
+    # Source displaced by b along x (R = I): a fronto-parallel plane at D
+    # maps pixel x to x - f b / D.  If the source's previous depth is D',
+    # the round trip lands at x - f b / D + f b / D', so the error is
+    # f b |1/D' - 1/D| pixels: 0 below 3 px, 1 above 6 px, linear between.
+    f, b, D = 100.0, 0.5, 5.0
+    sc, K = _cam_scene(w=96, h=64, fx=f, fy=f, n_src=1, src_ts=[[-b, 0.0, 0.0]])
+    p = po.Params(depth_min=1.0, depth_max=50.0, geom_weight=1.0)
+    Dp = 1.0 / (1.0 / D + err_px / (f * b))
+    prev = np.full((1, sc.height, sc.width), Dp, np.float32)
+    st = po.OracleState(sc.width, sc.height, prev_depths=prev, prev_depth_mask=1)
+    got = po.geom_cost(sc, p, st, 0, np.float32([0, 0, -1, D]), 60, 30)
+    assert abs(got - expect) < 2e-3, got
+
+
+def test_geom_cost_invalid_cases_cost_one():
+    f, b, D = 100.0, 0.5, 5.0
+    sc, K = _cam_scene(w=96, h=64, fx=f, fy=f, n_src=1, src_ts=[[-b, 0.0, 0.0]])
+    p = po.Params(depth_min=1.0, depth_max=50.0, geom_weight=1.0)
+    prev = np.full((1, sc.height, sc.width), D, np.float32)
+    st = po.OracleState(sc.width, sc.height, prev_depths=prev, prev_depth_mask=1)
+    plane = np.float32([0, 0, -1, D])
+    assert po.geom_cost(sc, p, st, 0, plane, 60, 30) == 0.0
+    assert po.geom_cost(sc, p, st, 0, plane, 3, 30) == 1.0          # projects left of the source
+    assert po.geom_cost(sc, p, st, 0, np.float32([0, 0, 0, 0]), 60, 30) == 1.0  # no depth
+    prev[:] = 0.0
+    assert po.geom_cost(sc, p, st, 0, plane, 60, 30) == 1.0          # no previous depth
+
+
+# ---------------------------------------------------------------------------
+# compute_weighted_cost_geom (opencl_kernels.h:871-900), compute_mc_cost (:910-969)
+# ---------------------------------------------------------------------------
+def _mc_setup(n_src=6):
+    sc, K = _cam_scene(n_src=n_src)
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    st = po.OracleState(sc.width, sc.height)
+    return sc, p, st
+
+
+def test_weighted_cost_is_the_weighted_mean_over_positive_weights():
+    sc, p, st = _mc_setup()
+    rng = np.random.default_rng(5)
+    cv = rng.random(6).astype(np.float32) * 2
+    vw = np.float32([0.5, 0.0, 1.5, 0.25, 0.0, 2.0])
+    cv[1] = np.nan  # a zero-weight view is never read
+    wn = float(vw.sum())
+    got = po.weighted_cost_geom(sc, p, st, cv, vw, wn, np.float32([0, 0, -1, 5]), 10, 10)
+    pos = vw > 0
+    assert abs(got - float(np.sum(vw[pos] * cv[pos]) / wn)) < 1e-6
+    assert po.weighted_cost_geom(sc, p, st, cv, vw * 0, 0.0, np.float32([0, 0, -1, 5]), 10, 10) == 2.0
+
+
+def test_mc_cost_zero_weight_sum_is_two_and_draws_nothing():
+    sc, p, st = _mc_setup()
+    r = np.array([123, 456], np.uint32)
+    got = po.mc_cost(sc, p, st, np.zeros(6, np.float32), np.zeros(6, np.float32), 0.0,
+                     np.float32([0, 0, -1, 5]), 3, 3, r, 77)
+    assert got == 2.0 and r.tolist() == [123, 456]
+
+
+def test_mc_cost_single_view_returns_its_cost():
+    sc, p, st = _mc_setup()
+    cv = np.float32([0.9, 0.3, 1.7, 0.123, 0.5, 1.1])
+    for i in range(6):
+        vw = np.zeros(6, np.float32)
+        vw[i] = 0.7
+        r = np.array([99 + i, 7 * i + 1], np.uint32)
+        got = po.mc_cost(sc, p, st, cv, vw, 0.7, np.float32([0, 0, -1, 5]), 3, 3, r, 1000)
+        assert abs(got - cv[i]) <= 2e-7 * 32 * max(cv[i], 1.0), (i, got)
+
+
+def test_mc_cost_draw_frequencies_follow_the_weights():
+    # one-hot cost vectors turn the MC mean into the share of draws that hit
+    # that view; over 4000 calls x 32 draws the shares must match w_i / sum(w)
+    sc, p, st = _mc_setup()
+    vw = np.float32([0.5, 0.0, 2.0, 1.0, 0.25, 1.25])
+    probs = vw / vw.sum()
+    n_calls = 4000
+    rng = np.random.default_rng(11)
+    states = rng.integers(0, 2 ** 32, size=(n_calls, 2), dtype=np.uint64).astype(np.uint32)
+    for i in range(6):
+        cv = np.zeros(6, np.float32)
+        cv[i] = 1.0
+        share = 0.0
+        for k in range(n_calls):
+            r = states[k].copy()
+            share += po.mc_cost(sc, p, st, cv, vw, float(vw.sum()), np.float32([0, 0, -1, 5]), 3, 3, r, 4242)
+        share /= n_calls
+        if vw[i] == 0.0:
+            assert share == 0.0
+        else:
+            sigma = math.sqrt(probs[i] * (1 - probs[i]) / (n_calls * 32))
+            assert abs(share - probs[i]) < 5 * sigma, (i, share, probs[i])
+
+
+# ---------------------------------------------------------------------------
+# acmmp_checkerboard_filter (opencl_kernels.h:1883-2039)
+# ---------------------------------------------------------------------------
+# the 20 opposite-colour positions the filter reads around a pixel
+_CB_OFFSETS = [(0, -1), (0, -3), (0, -5), (0, 1), (0, 3), (0, 5), (-1, 0), (-3, 0), (-5, 0), (1, 0), (3, 0),
+               (5, 0), (2, -1), (2, 1), (-2, -1), (-2, 1), (-1, -2), (1, -2), (-1, 2), (1, 2)]
+
+
+def _cb_state(sc, D):
+    st = po.OracleState(sc.width, sc.height)
+    st.planes[:] = np.float32([0, 0, -1, D])
+    st.costs[:] = 1.0
+    return st
+
+
+def _depth_at(st, K, x, y):
+    return _depth64(st.planes[y, x], x, y, K)
+
+
+def test_checkerboard_median_and_gate():
+    sc, K = _cam_scene(w=40, h=40, fx=50.0, fy=50.0)
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    D, x, y = 6.0, 20, 20
+    # an outlier inside the 3 % gate is replaced by the median (the neighbours' depth)
+    st = _cb_state(sc, D)
+    st.planes[y, x] = _plane_through([0.1, 0.0, -1.0], D * 1.02, x, y, K)
+    po.checkerboard(sc, p, st, (x + y) % 2)
+    assert abs(_depth_at(st, K, x, y) - D) < 1e-4
+    n = st.planes[y, x, :3] / np.linalg.norm(st.planes[y, x, :3])
+    assert np.allclose(n, np.array([0.1, 0, -1]) / np.linalg.norm([0.1, 0, -1]), atol=1e-6)  # normal kept
+    # beyond the gate the neighbours are dropped and the pixel keeps its depth
+    st = _cb_state(sc, D)
+    st.planes[y, x] = _plane_through([0, 0, -1.0], D * 1.10, x, y, K)
+    po.checkerboard(sc, p, st, (x + y) % 2)
+    assert abs(_depth_at(st, K, x, y) - D * 1.10) < 1e-4
+    # pixels with cost < 0.001 are left alone
+    st = _cb_state(sc, D)
+    st.planes[y, x] = _plane_through([0, 0, -1.0], D * 1.02, x, y, K)
+    st.costs[y, x] = 0.0
+    po.checkerboard(sc, p, st, (x + y) % 2)
+    assert abs(_depth_at(st, K, x, y) - D * 1.02) < 1e-4
+
+
+def test_checkerboard_reads_exactly_its_twenty_positions():
+    # centre at D, a subset S of positions at D * 1.01: the median of the 21
+    # values is D * 1.01 iff at least 11 of the read positions are in S.
+    sc, K = _cam_scene(w=40, h=40, fx=50.0, fy=50.0)
+    p = po.Params(depth_min=1.0, depth_max=50.0)
+    D, x, y = 6.0, 20, 20
+    rng = np.random.default_rng(3)
+    others = [(dx, dy) for dx in range(-6, 7) for dy in range(-6, 7)
+              if (dx + dy) % 2 == 1 and (dx, dy) not in _CB_OFFSETS]
+    for trial in range(12):
+        pick = [_CB_OFFSETS[i] for i in rng.choice(20, 10 + (trial % 2), replace=False)]
+        if trial % 2 == 0:  # 10 read positions + 1 unread one: median stays D
+            pick = pick + [others[rng.integers(len(others))]]
+        st = _cb_state(sc, D)
+        for dx, dy in pick:
+            st.planes[y + dy, x + dx] = np.float32([0, 0, -1, D * 1.01])
+        po.checkerboard(sc, p, st, (x + y) % 2)
+        expect = D * 1.01 if trial % 2 == 1 else D
+        assert abs(_depth_at(st, K, x, y) - expect) < 1e-4, (trial, pick)
+
+
+# ---------------------------------------------------------------------------
+# ReadBackResults decode (estimator.cc:1112-1224)
+# ---------------------------------------------------------------------------
+def test_decode_depth_normal_cost_confidence():
+    sc, K = _cam_scene(w=16, h=12, fx=20.0, fy=20.0)
+    p = po.Params(depth_min=2.0, depth_max=10.0)
+    st = po.OracleState(sc.width, sc.height)
+    nrm = np.array([0.2, -0.1, -1.0])
+    for yy in range(sc.height):
+        for xx in range(sc.width):
+            st.planes[yy, xx] = _plane_through(nrm, 5.0, xx, yy, K)
+    st.costs[:] = np.linspace(0, 2, sc.width * sc.height, dtype=np.float32).reshape(sc.height, sc.width)
+    st.planes[0, 0] = _plane_through(nrm, 12.0, 0, 0, K)    # beyond depth_max
+    st.planes[0, 1] = np.float32([np.nan, 0, -1, 5])         # not finite
+    st.costs[0, 2] = np.nan
+    depth, normal, cost, conf = po.decode(sc, p, st)
+    assert abs(depth[5, 7] - 5.0) < 1e-5 and abs(depth[0, 2] - 5.0) < 1e-5
+    assert np.allclose(normal[5, 7], nrm / np.linalg.norm(nrm), atol=1e-6)
+    assert depth[0, 0] == 0.0 and conf[0, 0] == 0.0 and not normal[0, 0].any()
+    assert depth[0, 1] == 0.0 and cost[0, 1] == 0.0
+    assert cost[0, 2] == 0.0                                  # NaN cost -> 0
+    c = cost[1:].ravel()
+    f = conf[1:].ravel()
+    assert np.all(np.diff(f[np.argsort(c)]) <= 0)             # confidence falls with cost
+    i = np.argmin(np.abs(c - 0.5))
+    assert abs(f[i] - 1.0 / (1.0 + math.exp((c[i] - 0.5) * 5.0))) < 1e-6
+    assert abs(conf[0, 2] - 1.0 / (1.0 + math.exp(-2.5))) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# acmmp_prior_reinit (opencl_kernels.h:1056-1182)
+# ---------------------------------------------------------------------------
+def test_prior_reinit_stays_within_one_percent_of_the_prior(scene_tiny):
+    s = scene_tiny
+    p = po.Params(depth_min=s.depth_min, depth_max=s.depth_max, patch_size=5)
+    sc = po.OracleScene(s.images, s.Ks, s.Rs, s.ts)
+    h, w = s.gt_depth.shape
+    K = np.asarray(s.Ks[0])
+    prior = np.zeros((h, w, 4), np.float32)
+    nrm = np.array([0.1, 0.05, -1.0])
+    for yy in range(h):
+        for xx in range(w):
+            prior[yy, xx] = _plane_through(nrm, float(s.gt_depth[yy, xx]) or 8.0, xx, yy, K)
+    masks = np.zeros((h, w), np.uint32)
+    masks[::3, ::2] = 1
+    st = po.OracleState(w, h, prior_planes=prior, plane_masks=masks)
+    po.random_init(sc, p, st)
+    before = st.planes.copy()
+    po.prior_reinit(sc, p, st)
+    changed = np.any(st.planes != before, axis=2)
+    assert not np.any(changed & (masks == 0))
+    assert changed.sum() > 0.2 * masks.sum()
+    for yy, xx in zip(*np.nonzero(changed)):
+        dp = _depth64(prior[yy, xx], xx, yy, K)
+        d = _depth64(st.planes[yy, xx], xx, yy, K)
+        assert dp * 0.99 * (1 - 1e-5) <= d <= dp * 1.01 * (1 + 1e-5)
+        n = st.planes[yy, xx, :3] / np.linalg.norm(st.planes[yy, xx, :3])
+        assert np.allclose(n, nrm / np.linalg.norm(nrm), atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# Candidate search of acmmp_patchmatch (opencl_kernels.h:1262-1430)
+# ---------------------------------------------------------------------------
+def _pattern_offsets():
+    """Offsets o from a pixel to the positions its 8 candidate groups probe."""
+    offs = set()
+    for i in range(11):  # far up / down / left / right: 3, 5, ..., 23 along an axis
+        k = 3 + 2 * i
+        offs |= {(0, -k), (0, k), (-k, 0), (k, 0)}
+    offs |= {(0, -1), (0, 1), (-1, 0), (1, 0)}
+    for i in range(3):  # near groups: V shapes around the 4-neighbours
+        offs |= {(-(i + 1), -(2 + i)), (i + 1, -(2 + i)), (-(i + 1), 2 + i), (i + 1, 2 + i)}
+        offs |= {(-(2 + i), -(i + 1)), (-(2 + i), i + 1), (2 + i, -(i + 1)), (2 + i, i + 1)}
+    return offs
+
+
+def _one_plane_scene(w=80, h=80, seed=4):
+    from opensfm_amd import synthetic
+    f = 0.9 * w
+    K = synthetic.intrinsics(w, h, f)
+    n = np.array([0.15, -0.1, -1.0])
+    n /= np.linalg.norm(n)
+    c = np.array([0.0, 0.0, 8.0])
+    u = np.cross(n, [0.0, 1.0, 0.0]); v = np.cross(n, u)
+    quad = synthetic._rect(c, u, v, 20.0, 20.0)
+    Ks, Rs, ts = [K], [np.eye(3)], [np.zeros(3)]
+    for C in ([0.6, 0.0, 0.0], [-0.4, 0.5, 0.0]):
+        Ks.append(K.copy()); Rs.append(np.eye(3)); ts.append(-np.asarray(C))
+    images, gt = synthetic._render_views([quad], synthetic.ValueNoise3D(seed), 0.25, Ks, Rs, ts,
+                                         [(w, h)] * 3, 0.005, seed, supersample=2)
+    return images, Ks, Rs, ts, gt, n, K
+
+
+def test_one_pass_propagates_a_seed_exactly_to_the_probed_positions():
+    # Every pixel holds the same wrong plane with cost 2, one seed pixel S
+    # holds the true plane with cost 0.  After one pass of the other colour,
+    # a pixel p takes the true plane iff one of its candidate groups probes
+    # S, i.e. S - p is one of the pattern offsets; elsewhere only a lucky
+    # random refinement can land within 1 % of the truth.
+    images, Ks, Rs, ts, gt, n, K = _one_plane_scene()
+    sc = po.OracleScene(images, Ks, Rs, ts)
+    p = po.Params(depth_min=3.0, depth_max=30.0, patch_size=5, smooth_weight=0.0, anchor_views=0)
+    h, w = gt.shape
+    st = po.OracleState(w, h)
+    po.random_init(sc, p, st)
+    sx, sy = 40, 40
+    wrong = np.float32([0, 0, -1, 0.6 * float(gt[sy, sx])])
+    st.planes[:] = wrong
+    st.costs[:] = 2.0
+    st.sel[:] = 0b11
+    st.planes[sy, sx] = _plane_through(n, float(gt[sy, sx]), sx, sy, K)
+    st.costs[sy, sx] = 0.0
+    po.sweep(sc, p, st, (sx + sy + 1) % 2, 0)
+    depth, _, _, _ = po.decode(sc, p, st)
+    took = np.abs(depth - gt) < 0.01 * gt
+    expected = {(sx - ox, sy - oy) for ox, oy in _pattern_offsets()}
+    assert len(expected) == 72
+    got = {(int(x), int(y)) for y, x in zip(*np.nonzero(took)) if (x, y) != (sx, sy)}
+    missing = expected - got
+    assert not missing, sorted(missing)
+    extra = got - expected
+    assert len(extra) <= 0.02 * (w * h / 2), len(extra)
+
+
+def _census_popcount(img):
+    """Per pixel: how many of the 24 neighbours of its 5x5 window (clamped at
+    the edges) are brighter than the pixel itself."""
+    h, w = img.shape
+    pad = np.pad(img, 2, mode="edge")
+    cnt = np.zeros((h, w), np.int32)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            if dx or dy:
+                cnt += pad[2 + dy:2 + dy + h, 2 + dx:2 + dx + w] > img
+    return cnt
+
+
+def test_validation_threshold_through_a_constant_source():
+    # A constant source has zero patch variance, so every stride falls back
+    # to census (opencl_kernels.h:544-550); its census is all zeros, so its
+    # cost at a pixel is popcount(ref census) / 24 for every candidate plane
+    # (:270-315), or 1.0 when both censuses have fewer than 3 bits set.  A
+    # view gets weight only from candidates with cost < 0.8 (:1450-1470), so
+    # the constant view is selected iff 3..19 of the 24 neighbours are
+    # brighter than the pixel (19/24 < 0.8 <= 20/24), wherever another view
+    # validates the pixel (else the all-fail fallback of :1552-1563 applies).
+    images, Ks, Rs, ts, gt, n, K = _one_plane_scene(w=48, h=40, seed=6)
+    images = list(images) + [np.full_like(images[0], 0.5)]
+    Ks = list(Ks) + [Ks[0].copy()]; Rs = list(Rs) + [np.eye(3)]; ts = list(ts) + [np.array([0.0, -0.5, 0.0])]
+    sc = po.OracleScene(images, Ks, Rs, ts)
+    p = po.Params(depth_min=3.0, depth_max=30.0, patch_size=5, anchor_views=0, max_iterations=2)
+    st = po.OracleState(sc.width, sc.height)
+    po.run(sc, p, st)
+    sel = st.sel
+    real = (sel & 0b011) != 0
+    assert real.mean() > 0.9
+    pc = _census_popcount(images[0])
+    want = (pc >= 3) & (pc <= 19)
+    assert (pc < 3).mean() > 0.03 and (pc > 19).mean() > 0.03   # all three cases occur
+    got = (sel & 0b100) != 0
+    agree = (got == want)[real]
+    assert agree.mean() > 0.98, agree.mean()
+
+
+# ---------------------------------------------------------------------------
+# ReadBackResults post-processing (estimator.cc:1194-1418)
+# ---------------------------------------------------------------------------
+def _pp(depth, normal=None, cost=None, speckle=0, gap=0):
+    depth = np.ascontiguousarray(depth, np.float32).copy()
+    h, w = depth.shape
+    normal = np.zeros((h, w, 3), np.float32) if normal is None else np.ascontiguousarray(normal, np.float32).copy()
+    cost = np.full((h, w), 0.3, np.float32) if cost is None else np.ascontiguousarray(cost, np.float32)
+    conf = np.zeros((h, w), np.float32)
+    po.postprocess(depth, normal, cost, conf, speckle, gap)
+    return depth, normal, conf
+
+
+def test_speckle_removal_drops_small_segments_only():
+    d = np.full((20, 30), 10.0, np.float32)
+    d[:, 15:] = np.linspace(12.0, 12.4, 15, dtype=np.float32)  # a ramp of < 1 % steps: one segment
+    d[5:7, 3:6] = 13.0                                         # 6-pixel island (30 % off)
+    d[15, 20] = 0.0                                            # invalid pixel inside the ramp
+    n = np.broadcast_to(np.float32([0, 0, -1]), (20, 30, 3))
+    for min_size, island_kept in [(7, False), (6, True), (1, True)]:
+        out, nout, conf = _pp(d, n, speckle=min_size)
+        assert np.all(out[5:7, 3:6] == (13.0 if island_kept else 0.0))
+        assert np.all(nout[5:7, 3:6, 2] == (-1.0 if island_kept else 0.0))
+        assert np.all(out[:, 15:][d[:, 15:] > 0] == d[:, 15:][d[:, 15:] > 0])   # 299 px ramp kept
+        assert np.all((conf > 0) == (out > 0))
+    # a 2 % step splits a region into two segments: the 4-pixel side goes
+    d2 = np.full((10, 10), 10.0, np.float32)
+    d2[:2, :2] = 10.25
+    out, _, _ = _pp(d2, speckle=5)
+    assert np.all(out[:2, :2] == 0.0) and np.all(out[2:, :] == 10.0)
+
+
+def test_gap_interpolation_fills_short_similar_runs_linearly():
+    # rows 10 apart, so that no column gap is short enough to join them
+    d = np.zeros((25, 24), np.float32)
+    d[2, :10] = 10.0; d[2, 13:] = 10.1                  # 3-pixel gap, 1 % apart: filled
+    d[12, :5] = 10.0; d[12, 13:] = 10.0                 # 8-pixel gap: too long for max 7
+    d[22, :10] = 10.0; d[22, 13:] = 10.5                # 3-pixel gap, 5 % apart: kept empty
+    n = np.zeros((25, 24, 3), np.float32)
+    n[2, :10] = [0, 0, -1]; n[2, 13:] = [0.6, 0, -0.8]
+    out, nout, conf = _pp(d, n, gap=7)
+    t = (np.arange(10, 13) - 9) / 4.0
+    assert np.allclose(out[2, 10:13], 10.0 + t * 0.1, atol=1e-6)
+    v = (1 - t)[:, None] * np.array([0, 0, -1]) + t[:, None] * np.array([0.6, 0, -0.8])
+    assert np.allclose(nout[2, 10:13], v / np.linalg.norm(v, axis=1, keepdims=True), atol=1e-6)
+    assert np.all(out[12, 5:13] == 0) and np.all(out[22, 10:13] == 0)
+    assert np.all((conf > 0) == (out > 0))
+    # columns: a vertical 2-pixel gap is filled, after the rows
+    dc = np.zeros((8, 3), np.float32)
+    dc[:3, 1] = 5.0; dc[5:, 1] = 5.05
+    out, _, _ = _pp(dc, gap=2)
+    assert np.allclose(out[3:5, 1], [5.0 + 0.05 / 3, 5.0 + 0.1 / 3], atol=1e-6)
+    assert np.all(out[:, 0] == 0) and np.all(out[:, 2] == 0)
