@@ -7,11 +7,12 @@ random init + 3 x (red pass + black pass) + decode, all on the device with
 the views already resident in HBM.  Metric: Mpix*views*iters/s =
 (W*H/1e6) * N_src * iterations / seconds.
 
-Also reported: the roofline of the dominant kernel (pass_candidates, timed
-with HIP events on the estimator's stream inside the timed steps) against
-the FP32 vector peak, per-stage device time, accuracy against the synthetic
-ground truth, and the CPU oracle timed on the host cores on a bounded sample
-of the same scene.  Each est.run() returns only after its HIP stream is idle
+Also reported: the roofline of the candidate-scoring kernel
+(pass_candidates, fixed algorithmic work per launch, timed with HIP events
+on the estimator's stream inside the timed steps) and of the whole colour
+pass against the FP32 vector peak, per-stage device time, accuracy against
+the synthetic ground truth, and the CPU oracle timed on the host cores on a
+bounded sample of the same scene.  Each est.run() returns only after its HIP stream is idle
 (hipStreamSynchronize), so the wall clock brackets complete GPU work.
 """
 from __future__ import annotations
@@ -35,7 +36,7 @@ FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap 
 # HBM bytes per pass_candidates launch: rocprofv3 --pmc FETCH_SIZE (x 1.9999)
 # + WRITE_SIZE (x 1.0) on this workload, each factor read off the same counter
 # on a 1 GiB device-to-device copy (profiles/*_hbm_traffic.txt).
-TRAFFIC_PER_LAUNCH = 8.661e9
+TRAFFIC_PER_LAUNCH = 4.774e9
 
 
 def log(msg: str) -> None:
@@ -101,7 +102,8 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--storage", choices=["f32", "f16"], default="f32")
-    ap.add_argument("--cpu-rows", type=int, default=96, help="reference rows in the CPU-oracle sample (0 = skip)")
+    ap.add_argument("--cpu-rows", type=int, default=384,
+                    help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -167,6 +169,13 @@ def main() -> int:
                 "kernel": f"pass_candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'}>",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     stage_ms = {k: round(v[0] / args.steps, 3) for k, v in stages.items()}
+    # The whole iteration path (all colour-pass kernels, fallbacks included)
+    # against the same peak: FLOP_unit x units / device time of the sweeps.
+    sweep_ms = sum(v[0] for k, v in stages.items() if k != "init") / args.steps
+    path_tflops = units_per_step * 1e6 * FLOP_PER_UNIT / (sweep_ms / 1e3) / 1e12
+    roofline_path = {"scope": "all colour-pass kernels of one step", "achieved": round(path_tflops, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(path_tflops / PEAK_FP32_TFLOPS, 4),
+                     "sweep_ms": round(sweep_ms, 3)}
 
     cpu = None
     if args.cpu_rows > 0:
@@ -185,6 +194,7 @@ def main() -> int:
                    "width": w, "height": h, "sources": n_src, "iterations": args.iters, "patch": PATCH,
                    "units_per_step": units_per_step},
         "roofline": roofline,
+        "roofline_path": roofline_path,
         "cpu_baseline": cpu,
         "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
         "stage_ms_per_step": stage_ms,

@@ -27,14 +27,17 @@
 //     coordinates to OpenCL's texel-corner convention, so the oracle samples
 //     at the pixel-centre coordinate directly.
 //
-// Parity unpinned: the parts below have no closed form, no golden vectors in
-// the reference's tests, and the reference cannot run in this environment,
-// so they rest on this restatement alone (tests/test_oracle.py pins the
-// rest): the view-weight heuristics (opencl_kernels.h:1452-1563), the
-// Monte-Carlo view sampling of mc_cost (:910-969), the candidate search
-// patterns (:1262-1430), the refinement recipe (:1669-1772), the smoothness
-// cost and surface normal (:649-772), geometric consistency (:779-900),
-// prior re-init (:1056-1182) and the checkerboard median (:1883-2039).
+// Every function here is pinned by tests/ against closed forms, special
+// cases, brute force or the reference's documented rules (test_oracle.py,
+// test_oracle_pins.py, test_levels.py): geometry, RNG, bilinear, cost
+// vectors, top-k, reference patch, surface normal, smoothness, geometric
+// consistency, view weights (validation sum, anchoring, angle weight,
+// incident-angle factor, all-fail fallback), weighted and Monte-Carlo costs
+// (draw frequencies), candidate search pattern (one-pass propagation of a
+// seed), refinement hypotheses (structure and draw distributions),
+// checkerboard median, prior re-init bounds, decode, pyramid resampling,
+// coarse-to-fine transfer, speckle removal and gap filling.  The whole
+// algorithm is pinned by the plane-sweep argmin and ground-truth tests.
 
 #include <cmath>
 #include <cstdint>
@@ -772,6 +775,77 @@ int refinement_hypotheses(const Ctx& c, const float* planes, int x, int y, Plane
   return have_sn ? 6 : 5;
 }
 
+// View weights of acmmp_patchmatch (opencl_kernels.h:1452-1563): per view
+// the sum of (1 - cost) over the flagged candidates it validates (cost <
+// 0.8); up to anchor_views views of the previous selection without support
+// get 1; positive weights are scaled by the angle weight and by the
+// incident-angle factor exp(-(1 - |n . ray|)^2 / (2 * 0.6^2)) of the
+// current plane; if nothing is left, every view with some candidate cost
+// below 2 gets 1.  ca[j * kMaxSrc + i] = cost of candidate j in view i.
+// Returns the weight sum; writes the weights and the selection mask.
+float view_weights(const Ctx& c, const float* ca, const bool* flags, uint32_t prev_sel, Plane cur, int x, int y,
+                   float vw[kMaxSrc], uint32_t* sel_out) {
+  const pmo_params& P = *c.pr;
+  const pmo_camera& c0 = c.cams[0];
+  const int n_src = c.n_src;
+  const float vthr = 0.8f;
+  for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
+  for (int i = 0; i < n_src; ++i) {
+    float vc = 0.0f;
+    for (int j = 0; j < 8; ++j) {
+      const float cc = ca[j * kMaxSrc + i];
+      if (flags[j] && cc < vthr) vc += (1.0f - cc);
+    }
+    if (vc > 0.0f) vw[i] = vc;
+  }
+  if (P.anchor_views > 0) {
+    int anchored = 0;
+    for (int i = 0; i < n_src && anchored < P.anchor_views; ++i) {
+      if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
+    }
+  }
+  {
+    V3 cur_n{cur.x, cur.y, cur.z};
+    const float nl = len3(cur_n);
+    const bool have_normal = nl > 0.1f;
+    if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
+    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
+    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
+    const float inc_sigma = 0.6f;
+    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
+    for (int i = 0; i < n_src; ++i) {
+      if (vw[i] < 1e-6f) continue;
+      vw[i] *= c.sc->angle_weights[i];
+      if (have_normal && cur_depth > 0.0f) {
+        const pmo_camera& ci = c.cams[i + 1];
+        V3 C;
+        C.x = -(ci.R[0] * ci.t[0] + ci.R[3] * ci.t[1] + ci.R[6] * ci.t[2]);
+        C.y = -(ci.R[1] * ci.t[0] + ci.R[4] * ci.t[1] + ci.R[7] * ci.t[2]);
+        C.z = -(ci.R[2] * ci.t[0] + ci.R[5] * ci.t[1] + ci.R[8] * ci.t[2]);
+        const V3 ray = normalize3(V3{C.x - p3.x, C.y - p3.y, C.z - p3.z});
+        const float cos_inc = std::fabs(dot3(cur_n, ray));
+        const float xi = 1.0f - cos_inc;
+        vw[i] *= pm_expf(xi * xi * inv2s2);
+      }
+    }
+  }
+  uint32_t new_sel = 0u;
+  float wnorm = 0.0f;
+  for (int i = 0; i < n_src; ++i)
+    if (vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
+  if (wnorm < 1e-6f) {
+    for (int i = 0; i < n_src; ++i) {
+      for (int j = 0; j < 8; ++j) {
+        if (flags[j] && ca[j * kMaxSrc + i] < 2.0f) {
+          vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
+        }
+      }
+    }
+  }
+  *sel_out = new_sel;
+  return wnorm;
+}
+
 // acmmp_patchmatch, one pixel of one colour: opencl_kernels.h:1197-1778.
 // Updates in place: every neighbour the pixel reads has the other colour
 // (odd Manhattan offsets), so this equals the reference's snapshot reads.
@@ -866,63 +940,9 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
   }
 
   // Step 2: view weights (:1452-1563).
-  const float vthr = 0.8f;
   float vw[kMaxSrc];
-  for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
-  for (int i = 0; i < n_src; ++i) {
-    float vc = 0.0f;
-    for (int j = 0; j < 8; ++j) {
-      const float cc = ca[j * kMaxSrc + i];
-      if (flags[j] && cc < vthr) vc += (1.0f - cc);
-    }
-    if (vc > 0.0f) vw[i] = vc;
-  }
-  const uint32_t prev_sel = st->sel[idx];
-  if (P.anchor_views > 0) {
-    int anchored = 0;
-    for (int i = 0; i < n_src && anchored < P.anchor_views; ++i) {
-      if ((prev_sel & (1u << i)) && vw[i] < 1e-6f) { vw[i] = 1.0f; ++anchored; }
-    }
-  }
-  {
-    const Plane cur = load_plane(planes, idx);
-    V3 cur_n{cur.x, cur.y, cur.z};
-    const float nl = len3(cur_n);
-    const bool have_normal = nl > 0.1f;
-    if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
-    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
-    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
-    const float inc_sigma = 0.6f;
-    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
-    for (int i = 0; i < n_src; ++i) {
-      if (vw[i] < 1e-6f) continue;
-      vw[i] *= c.sc->angle_weights[i];
-      if (have_normal && cur_depth > 0.0f) {
-        const pmo_camera& ci = c.cams[i + 1];
-        V3 C;
-        C.x = -(ci.R[0] * ci.t[0] + ci.R[3] * ci.t[1] + ci.R[6] * ci.t[2]);
-        C.y = -(ci.R[1] * ci.t[0] + ci.R[4] * ci.t[1] + ci.R[7] * ci.t[2]);
-        C.z = -(ci.R[2] * ci.t[0] + ci.R[5] * ci.t[1] + ci.R[8] * ci.t[2]);
-        const V3 ray = normalize3(V3{C.x - p3.x, C.y - p3.y, C.z - p3.z});
-        const float cos_inc = std::fabs(dot3(cur_n, ray));
-        const float xi = 1.0f - cos_inc;
-        vw[i] *= pm_expf(xi * xi * inv2s2);
-      }
-    }
-  }
   uint32_t new_sel = 0u;
-  float wnorm = 0.0f;
-  for (int i = 0; i < n_src; ++i)
-    if (vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
-  if (wnorm < 1e-6f) {
-    for (int i = 0; i < n_src; ++i) {
-      for (int j = 0; j < 8; ++j) {
-        if (flags[j] && ca[j * kMaxSrc + i] < 2.0f) {
-          vw[i] = 1.0f; new_sel |= (1u << i); wnorm += 1.0f; break;
-        }
-      }
-    }
-  }
+  const float wnorm = view_weights(c, ca, flags, st->sel[idx], load_plane(planes, idx), x, y, vw, &new_sel);
 
   // Step 3: candidate costs (:1570-1619).
   float fc[8];
@@ -1327,6 +1347,24 @@ float pmo_mc_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st
                   float wnorm, const float* plane, int x, int y, uint32_t* rng, uint32_t key) {
   Ctx c(sc, pr);
   return mc_cost(c, st, cv, vw, wnorm, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, rng, key);
+}
+
+// View weights at (x, y): cand_costs is (8, num_src) row-major, flag bit j
+// marks candidate j present.  Writes num_src weights and the selection mask;
+// returns the weight sum.
+float pmo_view_weights(const pmo_scene* sc, const pmo_params* pr, const float* cand_costs, uint32_t flag_bits,
+                       uint32_t prev_sel, const float* plane, int x, int y, float* weights, uint32_t* sel) {
+  Ctx c(sc, pr);
+  float ca[8 * kMaxSrc];
+  bool flags[8];
+  for (int j = 0; j < 8; ++j) {
+    flags[j] = (flag_bits >> j) & 1u;
+    for (int i = 0; i < kMaxSrc; ++i) ca[j * kMaxSrc + i] = i < c.n_src ? cand_costs[j * c.n_src + i] : 2.0f;
+  }
+  float vw[kMaxSrc];
+  const float wn = view_weights(c, ca, flags, prev_sel, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, vw, sel);
+  for (int i = 0; i < c.n_src; ++i) weights[i] = vw[i];
+  return wn;
 }
 
 // Refinement hypotheses at (x, y) for the current plane (and an optional

@@ -90,6 +90,10 @@ def lib():
         L.pmo_mc_cost.restype = f; L.pmo_mc_cost.argtypes = [vp, vp, vp, vp, vp, f, vp, i, i, vp, u]
         L.pmo_resize_area.argtypes = [vp, i, i, i, i, i, vp]
         L.pmo_postprocess.argtypes = [vp, vp, vp, vp, i, i, i, i]
+        L.pmo_view_weights.restype = f
+        L.pmo_view_weights.argtypes = [vp, vp, vp, u, u, vp, i, i, vp, vp]
+        L.pmo_refinement_hypotheses.restype = i
+        L.pmo_refinement_hypotheses.argtypes = [vp, vp, vp, i, i, vp, vp, vp, u, vp]
         L.pmo_resize_linear.argtypes = [vp, i, i, i, i, vp]
         L.pmo_upsample.argtypes = [vp, vp, vp, vp, i, i, vp]
         L.pmo_set_threads.argtypes = [i]
@@ -350,8 +354,8 @@ def smoothness_cost(scene: OracleScene, params: Params, planes: np.ndarray, plan
     """4-neighbour smoothness of ``plane`` at (x, y) against ``planes`` (h, w, 4) (opencl_kernels.h:649-707)."""
     p = params.c()
     planes = _f32(planes, scene.width * scene.height * 4)
-    return float(lib().pmo_smoothness_cost(ctypes.byref(scene.c), ctypes.byref(p), _ptr(planes),
-                                           _ptr(_f32(plane, 4)), x, y))
+    pl = _f32(plane, 4)
+    return float(lib().pmo_smoothness_cost(ctypes.byref(scene.c), ctypes.byref(p), _ptr(planes), _ptr(pl), x, y))
 
 
 def surface_normal(scene: OracleScene, params: Params, planes: np.ndarray, x: int, y: int,
@@ -368,17 +372,47 @@ def surface_normal(scene: OracleScene, params: Params, planes: np.ndarray, x: in
 def geom_cost(scene: OracleScene, params: Params, state: OracleState, sidx: int, plane, x: int, y: int) -> float:
     """Forward-backward reprojection cost against source sidx's previous depth (opencl_kernels.h:779-861)."""
     p = params.c()
+    pl = _f32(plane, 4)
     return float(lib().pmo_geom_cost(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c), sidx,
-                                     _ptr(_f32(plane, 4)), x, y))
+                                     _ptr(pl), x, y))
 
 
 def weighted_cost_geom(scene: OracleScene, params: Params, state: OracleState, cv, vw, wnorm: float, plane,
                        x: int, y: int) -> float:
     p = params.c()
     n = scene.n_views - 1
+    cv, vw, pl = _f32(cv, n), _f32(vw, n), _f32(plane, 4)
     return float(lib().pmo_weighted_cost_geom(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c),
-                                              _ptr(_f32(cv, n)), _ptr(_f32(vw, n)), float(wnorm),
-                                              _ptr(_f32(plane, 4)), x, y))
+                                              _ptr(cv), _ptr(vw), float(wnorm), _ptr(pl), x, y))
+
+
+def view_weights(scene: OracleScene, params: Params, cand_costs, flags: int, prev_sel: int, plane, x: int, y: int):
+    """View weights of acmmp_patchmatch (opencl_kernels.h:1452-1563) for candidate costs
+    (8, num_src); returns (weights, selection mask, weight sum)."""
+    p = params.c()
+    n = scene.n_views - 1
+    cc = _f32(cand_costs, 8 * n)
+    pl = _f32(plane, 4)
+    w = np.zeros(n, np.float32)
+    sel = np.zeros(1, np.uint32)
+    wn = lib().pmo_view_weights(ctypes.byref(scene.c), ctypes.byref(p), _ptr(cc), ctypes.c_uint32(flags),
+                                ctypes.c_uint32(prev_sel), _ptr(pl), x, y, _ptr(w), _ptr(sel))
+    return w, int(sel[0]), float(wn)
+
+
+def refinement_hypotheses(scene: OracleScene, params: Params, planes: np.ndarray, x: int, y: int, plane_now,
+                          rng: np.ndarray, key: int, prior_plane=None) -> np.ndarray:
+    """The refinement hypotheses of acmmp_patchmatch at (x, y) (opencl_kernels.h:1669-1742):
+    (n, 4) planes, n = 6 or 5; ``rng`` advances in place."""
+    assert rng.dtype == np.uint32 and rng.size == 2
+    p = params.c()
+    planes = _f32(planes, scene.width * scene.height * 4)
+    out = np.zeros((6, 4), np.float32)
+    pp = None if prior_plane is None else _f32(prior_plane, 4)
+    pn = _f32(plane_now, 4)
+    n = lib().pmo_refinement_hypotheses(ctypes.byref(scene.c), ctypes.byref(p), _ptr(planes), x, y, _ptr(pn),
+                                        _ptr(pp), rng.ctypes.data, ctypes.c_uint32(key & 0xFFFFFFFF), _ptr(out))
+    return out[:n]
 
 
 def mc_cost(scene: OracleScene, params: Params, state: OracleState, cv, vw, wnorm: float, plane, x: int, y: int,
@@ -387,9 +421,10 @@ def mc_cost(scene: OracleScene, params: Params, state: OracleState, cv, vw, wnor
     assert rng.dtype == np.uint32 and rng.size == 2
     p = params.c()
     n = scene.n_views - 1
-    return float(lib().pmo_mc_cost(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c),
-                                   _ptr(_f32(cv, n)), _ptr(_f32(vw, n)), float(wnorm), _ptr(_f32(plane, 4)),
-                                   x, y, rng.ctypes.data, ctypes.c_uint32(key & 0xFFFFFFFF)))
+    cv, vw, pl = _f32(cv, n), _f32(vw, n), _f32(plane, 4)
+    return float(lib().pmo_mc_cost(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c), _ptr(cv),
+                                   _ptr(vw), float(wnorm), _ptr(pl), x, y, rng.ctypes.data,
+                                   ctypes.c_uint32(key & 0xFFFFFFFF)))
 
 
 # ---------------------------------------------------------------------------

@@ -529,3 +529,156 @@ def test_gap_interpolation_fills_short_similar_runs_linearly():
     out, _, _ = _pp(dc, gap=2)
     assert np.allclose(out[3:5, 1], [5.0 + 0.05 / 3, 5.0 + 0.1 / 3], atol=1e-6)
     assert np.all(out[:, 0] == 0) and np.all(out[:, 2] == 0)
+
+
+# ---------------------------------------------------------------------------
+# Refinement hypotheses of acmmp_patchmatch (opencl_kernels.h:1669-1742)
+# ---------------------------------------------------------------------------
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def test_refinement_hypotheses_structure():
+    sc, K = _cam_scene(w=40, h=30, fx=60.0, fy=62.0)
+    p = po.Params(depth_min=2.0, depth_max=30.0)
+    x, y = 17, 12
+    n_cur = _unit([0.2, -0.1, -1.0])
+    cur = _plane_through(n_cur, 7.0, x, y, K)
+    nb = _plane_through([0.1, 0.3, -1.0], 7.0, x, y, K)       # the neighbours' plane
+    planes = np.broadcast_to(nb, (30, 40, 4)).copy()
+    d_now = _depth64(cur, x, y, K)
+    for seed in range(20):
+        r = np.array([1234 + seed, 99 * seed + 7], np.uint32)
+        hyp = po.refinement_hypotheses(sc, p, planes, x, y, cur, r, 5000 + seed)
+        assert hyp.shape == (6, 4)
+        d = np.array([_depth64(h, x, y, K) for h in hyp])
+        n = np.array([_unit(h[:3]) for h in hyp])
+        for k in (1, 3, 5):                                       # current depth
+            assert abs(d[k] - d_now) < 1e-4 * d_now
+        assert abs(d[0] - d[2]) < 1e-4 * d[0] and 2.0 * (1 - 1e-5) <= d[0] <= 30.0 * (1 + 1e-5)
+        assert 0.995 * d_now * (1 - 1e-5) <= d[4] <= 1.005 * d_now * (1 + 1e-5)
+        assert np.allclose(n[0], n_cur, atol=1e-5) and np.allclose(n[4], n_cur, atol=1e-5)
+        assert np.allclose(n[1], n[2], atol=1e-5)
+        # perturbed normal: +-0.005 pi added to x and y before normalising
+        assert np.degrees(np.arccos(min(1.0, n[3] @ n_cur))) <= np.degrees(np.arctan(0.005 * np.pi * np.sqrt(2))) + 0.01
+        sn = po.surface_normal(sc, p, planes, x, y, np.float32(d_now))
+        assert np.allclose(n[5], sn, atol=1e-5)
+    # no surface normal on the border: five hypotheses
+    r = np.array([5, 6], np.uint32)
+    assert po.refinement_hypotheses(sc, p, planes, 0, y, cur, r, 1).shape == (5, 4)
+
+
+def test_refinement_random_draws_are_log_uniform_and_uniform_slopes():
+    sc, K = _cam_scene(w=40, h=30, fx=60.0, fy=62.0)
+    p = po.Params(depth_min=2.0, depth_max=30.0)
+    x, y = 17, 12
+    cur = _plane_through([0.0, 0.0, -1.0], 7.0, x, y, K)
+    planes = np.broadcast_to(cur, (30, 40, 4)).copy()
+    rng = np.random.default_rng(9)
+    u, sx, sy = [], [], []
+    for k in range(3000):
+        r = rng.integers(0, 2 ** 32, 2, dtype=np.uint64).astype(np.uint32)
+        hyp = po.refinement_hypotheses(sc, p, planes, x, y, cur, r, k)
+        d0 = _depth64(hyp[0], x, y, K)
+        u.append((np.log(d0) - np.log(2.0)) / (np.log(30.0) - np.log(2.0)))
+        n1 = hyp[1][:3].astype(np.float64)
+        sx.append(n1[0] / -n1[2]); sy.append(n1[1] / -n1[2])
+    u, sx, sy = map(np.asarray, (u, sx, sy))
+    n = len(u)
+    assert u.min() >= -1e-5 and u.max() <= 1 + 1e-5
+    assert abs(u.mean() - 0.5) < 4 * np.sqrt(1 / 12 / n) and abs(u.var() - 1 / 12) < 0.01
+    for s in (sx, sy):  # (u, v, -1) normalised: slopes uniform in [-1, 1)
+        assert s.min() >= -1 - 1e-5 and s.max() <= 1 + 1e-5
+        assert abs(s.mean()) < 4 * np.sqrt(1 / 3 / n) and abs(s.var() - 1 / 3) < 0.03
+
+
+def test_refinement_random_draws_follow_the_prior():
+    sc, K = _cam_scene(w=40, h=30, fx=60.0, fy=62.0)
+    p = po.Params(depth_min=2.0, depth_max=30.0)
+    x, y = 17, 12
+    cur = _plane_through([0.0, 0.0, -1.0], 7.0, x, y, K)
+    planes = np.broadcast_to(cur, (30, 40, 4)).copy()
+    n_prior = _unit([0.3, 0.1, -1.0])
+    prior = _plane_through(n_prior, 12.0, x, y, K)
+    sigma = (30.0 - 2.0) / 32.0
+    rng = np.random.default_rng(10)
+    for k in range(300):
+        r = rng.integers(0, 2 ** 32, 2, dtype=np.uint64).astype(np.uint32)
+        hyp = po.refinement_hypotheses(sc, p, planes, x, y, cur, r, k, prior_plane=prior)
+        d0 = _depth64(hyp[0], x, y, K)
+        assert 12.0 - 3 * sigma - 1e-3 <= d0 <= 12.0 + 3 * sigma + 1e-3
+        ang = np.degrees(np.arccos(min(1.0, _unit(hyp[1][:3]) @ n_prior)))
+        assert ang <= np.degrees(np.arctan(np.radians(2.5) * np.sqrt(2))) + 0.05
+
+
+# ---------------------------------------------------------------------------
+# Angle weights (estimator.cc:380-407) and view anchoring (opencl_kernels.h:1470-1490)
+# ---------------------------------------------------------------------------
+def test_angle_weights_closed_form():
+    # max(sin(angle between baseline and the reference optical axis), 0.01)
+    Rs = [np.eye(3)] * 4
+    ts = [np.zeros(3), -np.array([0.0, 0.0, 1.0]), -np.array([1.0, 0.0, 0.0]), -np.array([1.0, 0.0, 1.0])]
+    w = po.angle_weights(Rs, ts)
+    assert np.allclose(w, [0.01, 1.0, np.sqrt(0.5)], atol=1e-7)
+
+
+def test_anchored_view_keeps_weight_without_support():
+    # source 0 looks away from the scene (every warp invalid, cost 2.0), so it
+    # validates nothing; with anchor_views = 1 and source 0 in the previous
+    # selection it gets weight 1 and stays selected, with 0 it never is.  A
+    # pass writes the selection only where it adopts a candidate
+    # (opencl_kernels.h:1643-1656), so with 0 the changed selections are the
+    # ones to check.
+    images, Ks, Rs, ts, gt, n, K = _one_plane_scene(w=40, h=32, seed=8)
+    away = np.array([[-1.0, 0, 0], [0, 1.0, 0], [0, 0, -1.0]])   # optical axis along -z
+    images = [images[0], images[1], images[1], images[2]]
+    Ks = [Ks[0], Ks[0], Ks[1], Ks[2]]
+    Rs = [Rs[0], away, Rs[1], Rs[2]]
+    ts = [ts[0], np.zeros(3), ts[1], ts[2]]
+    sc = po.OracleScene(images, Ks, Rs, ts)
+    for anchor in (0, 1):
+        p = po.Params(depth_min=3.0, depth_max=30.0, patch_size=5, anchor_views=anchor)
+        st = po.OracleState(sc.width, sc.height)
+        po.random_init(sc, p, st)
+        st.sel[:] = 0b001
+        po.sweep(sc, p, st, 0, 0)
+        swept = (np.add.outer(np.arange(sc.height), np.arange(sc.width)) % 2) == 0
+        bit0 = (st.sel & 1) != 0
+        if anchor:
+            assert np.all(bit0[swept])
+        else:
+            changed = swept & (st.sel != 0b001)
+            assert changed.mean() > 0.2
+            assert not np.any(bit0[changed])
+
+
+def test_view_weights_validation_sum_angle_and_incidence():
+    # weight_i = sum over flagged candidates with cost < 0.8 of (1 - cost),
+    # times the angle weight, times exp(-(1 - |n . ray_i|)^2 / (2 * 0.6^2))
+    # with ray_i from the surface point to source i's centre
+    # (opencl_kernels.h:1452-1545).
+    srcs = [np.array([0.5, 0.0, 0.0]), np.array([0.0, 2.0, 1.0]), np.array([-3.0, 1.0, 2.0])]
+    sc, K = _cam_scene(w=48, h=40, fx=50.0, fy=55.0, n_src=3, src_ts=[-c for c in srcs])
+    p = po.Params(depth_min=1.0, depth_max=50.0, anchor_views=0)
+    x, y = 30, 12
+    n_cur = _unit([0.4, -0.2, -1.0])
+    plane = _plane_through(n_cur, 6.0, x, y, K)
+    costs = np.full((8, 3), 0.9, np.float32)
+    costs[[0, 2, 5], 0] = [0.1, 0.5, 0.79]         # view 0: 0.9 + 0.5 + 0.21
+    costs[[1, 3], 1] = [0.2, 0.35]                # view 1: 0.8 + 0.65
+    costs[4, 2] = 0.3                             # view 2: only candidate 4, which is not flagged
+    flags = 0xFF & ~(1 << 4)
+    w, sel, wn = po.view_weights(sc, p, costs, flags, 0, plane, x, y)
+    aw = po.angle_weights([np.eye(3)] * 4, [np.zeros(3)] + [-c for c in srcs])
+    X = _depth64(plane, x, y, K) * np.array([(x - K[0, 2]) / K[0, 0], (y - K[1, 2]) / K[1, 1], 1.0])
+    want = []
+    for i, (vc, C) in enumerate(zip([1.61, 1.45, 0.0], srcs)):
+        ray = _unit(C - X)
+        xi = 1.0 - abs(n_cur @ ray)
+        want.append(vc * aw[i] * np.exp(-xi * xi / (2 * 0.36)))
+    assert np.allclose(w, want, rtol=2e-6, atol=0)
+    assert sel == 0b011 and abs(wn - sum(want)) < 1e-5
+    # nothing validated: every view with a candidate cost below 2 gets 1
+    w2, sel2, wn2 = po.view_weights(sc, p, np.full((8, 3), 1.5, np.float32), 0xFF, 0, plane, x, y)
+    assert np.all(w2 == 1.0) and sel2 == 0b111 and wn2 == 3.0
