@@ -182,6 +182,26 @@ int pm_write_state(pm_handle* h, const float* planes, const float* costs, const 
 int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs,
                     float* init_cost, uint32_t* sel);
 
+/* Differentiable photometric cost of per-pixel planes (no counterpart in the
+ * reference; the refinement after PatchMatch).  Per reference pixel: depth
+ * d, normal slopes (a, b) of the camera-facing normal (a, b, -1), and a
+ * view mask; the cost is the mean over the masked views valid at the
+ * pixel of 1 - NCC (the stride-1 bilateral NCC of compute_cost_vector,
+ * opencl_kernels.h:471-499, on the same local affine warp, :401-461, and no
+ * textureless fallback); grad = d cost / d(d, a, b) (3 per pixel, NULL =
+ * cost only).  depth, cost: H*W; ab: 2*H*W; views: H*W; after pm_upload.
+ * device_pointers = 1: every pointer is device memory of the handle's
+ * device and the call returns when the results are written. */
+int pm_ncc_cost_grad(pm_handle* h, const float* depth, const float* ab, const uint32_t* views, float* cost,
+                     float* grad, int device_pointers);
+/* Gauss-Newton refinement of the PatchMatch state (after pm_run): `steps`
+ * damped steps (Levenberg-Marquardt factor `damping`) on the residuals
+ * 1 - NCC of each pixel's selected views, a step kept only where it lowers
+ * the pixel's cost; then decode again (pm_download reads the result). */
+int pm_gauss_newton(pm_handle* h, int steps, float damping);
+/* Device time of the last pm_gauss_newton's steps in milliseconds. */
+int pm_last_refine_timing(pm_handle* h, float* ms);
+
 /* Device time of the last pm_run in milliseconds (HIP events on the
  * handle's stream): the whole run, and the iterations (all colour passes). */
 int pm_last_timing(pm_handle* h, float* run_ms, float* sweep_ms, int* sweep_launches);

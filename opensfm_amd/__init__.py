@@ -17,8 +17,24 @@ def library_path() -> str:
     return os.path.join(_HERE, "libopensfm_pm.so")
 
 
+def _share_torch_runtime() -> None:
+    """Load PyTorch's HIP runtime before the library's.
+
+    torch ships its own libamdhip64.so.7; the dynamic loader keeps the first
+    library of that name for the whole process.  Loading torch's first
+    makes the library and torch share one runtime, so torch tensors and the
+    library can be used together (the autograd wrapper); the other order
+    leaves torch without a device.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is optional for the library itself
+        pass
+
+
 def native():
     """Import the compiled ``pydense`` module (RuntimeError if it was not built)."""
+    _share_torch_runtime()
     try:
         from . import pydense  # noqa: F401
     except ImportError as e:  # pragma: no cover - depends on the build

@@ -144,6 +144,8 @@ struct pm_handle {
   DevBuf<unsigned char> quads;  // 2 x 2 quad copy of the padded views
   DevBuf<PmView> gviews;       // view table (index 0 = reference)
   DevBuf<PmCam> gcams;         // cameras
+  DevBuf<float> vgeom;         // per source: reference camera frame -> source pixels (12 floats)
+  float last_gn_ms = 0.0f;
   DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
@@ -173,6 +175,7 @@ struct pm_handle {
 
   ~pm_handle() {
     images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
+    vgeom.release();
     ca.release(); vwb.release(); wnb.release(); cpos.release(); nref.release(); refp.release();
     fb_masks.release();
     planes.release(); prior.release();
@@ -819,6 +822,23 @@ int upload_level(pm_handle* h, int level, int nlevels) {
       vt[i].angle_w = static_cast<float>(std::max(sa, 0.01));
     }
   }
+  // Composite maps of the differentiable cost: p = A (camera-frame point) + b
+  // with A = K_i R_i R0^T and b = K_i (t_i - R_i R0^T t0), in double.
+  std::vector<float> vgeom(static_cast<size_t>(12) * std::max(nv - 1, 1), 0.0f);
+  for (int i = 1; i < nv; ++i) {
+    const HostView& v = views[static_cast<size_t>(i)];
+    double M[9], T[3];  // R_i R0^T and t_i - M t0
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        M[3 * r + c] = v.R[3 * r] * v0.R[3 * c] + v.R[3 * r + 1] * v0.R[3 * c + 1] + v.R[3 * r + 2] * v0.R[3 * c + 2];
+    for (int r = 0; r < 3; ++r) T[r] = v.t[r] - (M[3 * r] * v0.t[0] + M[3 * r + 1] * v0.t[1] + M[3 * r + 2] * v0.t[2]);
+    float* g = &vgeom[static_cast<size_t>(12) * (i - 1)];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c)
+        g[3 * r + c] = static_cast<float>(v.K[3 * r] * M[c] + v.K[3 * r + 1] * M[3 + c] + v.K[3 * r + 2] * M[6 + c]);
+      g[9 + r] = static_cast<float>(v.K[3 * r] * T[0] + v.K[3 * r + 1] * T[1] + v.K[3 * r + 2] * T[2]);
+    }
+  }
   const size_t npix = static_cast<size_t>(h->W) * h->H;
   if (int r = h->images.alloc(host.size())) return r;
   if (int r = h->quads.alloc(static_cast<size_t>(qtotal) * 4 * esz)) return r;
@@ -848,6 +868,9 @@ int upload_level(pm_handle* h, int level, int nlevels) {
   PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
   PM_HIP(hipMemcpyAsync(h->gviews.p, vt.data(), vt.size() * sizeof(PmView), hipMemcpyHostToDevice, h->stream),
          "upload view table");
+  if (int r = h->vgeom.alloc(vgeom.size())) return r;
+  PM_HIP(hipMemcpyAsync(h->vgeom.p, vgeom.data(), vgeom.size() * sizeof(float), hipMemcpyHostToDevice, h->stream),
+         "upload view geometry");
   PM_HIP(hipMemcpyAsync(h->gcams.p, cams.data(), cams.size() * sizeof(PmCam), hipMemcpyHostToDevice, h->stream),
          "upload cameras");
   {
@@ -1139,7 +1162,7 @@ int pm_release_device(pm_handle* h) {
   if (int r = check_handle(h)) return r;
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   PM_HIP(hipStreamSynchronize(h->stream), "pm_release_device");
-  h->images.release(); h->quads.release(); h->gviews.release(); h->gcams.release();
+  h->images.release(); h->quads.release(); h->gviews.release(); h->gcams.release(); h->vgeom.release();
   h->ref_taps.release(); h->ref_sums.release();
   h->ca.release(); h->vwb.release(); h->wnb.release(); h->cpos.release(); h->nref.release(); h->refp.release();
   h->fb_masks.release();
@@ -1149,6 +1172,66 @@ int pm_release_device(pm_handle* h) {
   h->uploaded = false;
   h->state_ready = false;
   h->ref_ready = false;
+  return PM_OK;
+}
+
+int pm_ncc_cost_grad(pm_handle* h, const float* depth, const float* ab, const uint32_t* views, float* cost,
+                     float* grad, int device_pointers) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_ncc_cost_grad: views are not uploaded (call pm_upload)");
+  if (!depth || !ab || !views || !cost) return fail(PM_E_INVALID, "pm_ncc_cost_grad: null argument");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  if (int r = ensure_refpatch(h)) return r;
+  const PmArgs a = make_args(h);
+  const int hp = h->params.patch_size / 2;
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  if (device_pointers) {
+    PM_HIP(launch_ncc_energy(a, hp, h->f16, h->vgeom.p, depth, ab, views, cost, grad, h->stream),
+           "energy kernel launch");
+    PM_HIP(hipStreamSynchronize(h->stream), "pm_ncc_cost_grad");
+    return PM_OK;
+  }
+  DevBuf<float> dd, dab, dc, dg;
+  DevBuf<uint32_t> dv;
+  if (int r = dd.alloc(npix)) return r;
+  if (int r = dab.alloc(2 * npix)) return r;
+  if (int r = dv.alloc(npix)) return r;
+  if (int r = dc.alloc(npix)) return r;
+  if (grad)
+    if (int r = dg.alloc(3 * npix)) return r;
+  PM_HIP(hipMemcpyAsync(dd.p, depth, npix * 4, hipMemcpyHostToDevice, h->stream), "upload");
+  PM_HIP(hipMemcpyAsync(dab.p, ab, 2 * npix * 4, hipMemcpyHostToDevice, h->stream), "upload");
+  PM_HIP(hipMemcpyAsync(dv.p, views, npix * 4, hipMemcpyHostToDevice, h->stream), "upload");
+  PM_HIP(launch_ncc_energy(a, hp, h->f16, h->vgeom.p, dd.p, dab.p, dv.p, dc.p, grad ? dg.p : nullptr, h->stream),
+         "energy kernel launch");
+  PM_HIP(hipMemcpyAsync(cost, dc.p, npix * 4, hipMemcpyDeviceToHost, h->stream), "download");
+  if (grad) PM_HIP(hipMemcpyAsync(grad, dg.p, 3 * npix * 4, hipMemcpyDeviceToHost, h->stream), "download");
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_ncc_cost_grad");
+  return PM_OK;
+}
+
+int pm_gauss_newton(pm_handle* h, int steps, float damping) {
+  if (int r = check_handle(h)) return r;
+  if (!h->state_ready) return fail(PM_E_STATE, "pm_gauss_newton: no state to refine (call pm_run)");
+  if (steps < 0 || !(damping >= 0.0f)) return fail(PM_E_INVALID, "pm_gauss_newton: bad steps or damping");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  if (int r = ensure_refpatch(h)) return r;
+  const PmArgs a = make_args(h);
+  const int hp = h->params.patch_size / 2;
+  PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
+  for (int i = 0; i < steps; ++i)
+    PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, h->stream), "Gauss-Newton kernel launch");
+  PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
+  if (int r = pm_decode(h)) return r;
+  if (int r = postprocess(h)) return r;
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_gauss_newton");
+  PM_HIP(hipEventElapsedTime(&h->last_gn_ms, h->ev[0], h->ev[1]), "event time");
+  return PM_OK;
+}
+
+int pm_last_refine_timing(pm_handle* h, float* ms) {
+  if (int r = check_handle(h)) return r;
+  if (ms) *ms = h->last_gn_ms;
   return PM_OK;
 }
 

@@ -239,4 +239,23 @@ __device__ __forceinline__ float sample(const Sampler& v, float sx, float sy) {
   return fmaf(ay, v1 - v0, v0);
 }
 
+// The same bilinear sample and its gradient d/d(sx, sy).  Outside the
+// clamp range the padded border repeats the edge texel, so the gradient
+// along a clamped axis comes out zero.
+template <typename T>
+__device__ __forceinline__ float sample_grad(const Sampler& v, float sx, float sy, float& gx, float& gy) {
+  sx = __builtin_amdgcn_fmed3f(sx, -1.0f, v.xmax);
+  sy = __builtin_amdgcn_fmed3f(sy, -1.0f, v.ymax);
+  const float fx0 = floorf(sx), fy0 = floorf(sy);
+  const float ax = sx - fx0, ay = sy - fy0;
+  const uint32_t qi = static_cast<uint32_t>(fmaf(fy0, v.pitch_f, fx0 + v.origin_f));
+  float t00, t01, t10, t11;
+  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, t01, t10, t11, T{});
+  const float v0 = fmaf(ax, t01 - t00, t00);
+  const float v1 = fmaf(ax, t11 - t10, t10);
+  gx = fmaf(ay, (t11 - t10) - (t01 - t00), t01 - t00);
+  gy = v1 - v0;
+  return fmaf(ay, v1 - v0, v0);
+}
+
 }  // namespace pmhip

@@ -79,6 +79,16 @@ hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, hipStre
 hipError_t launch_refine(const PmArgs& a, hipStream_t stream);
 hipError_t launch_filter(const PmArgs& a, hipStream_t stream);
 hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stream);
+
+// Differentiable photometric cost (pm_refine.hip).  vgeom: 12 floats per
+// source view (A = K_v R_v R0^T row-major, b = K_v (t_v - R_v R0^T t0)).
+// depth: H*W, ab: 2*H*W normal slopes (a, b) of u = (a, b, -1), views:
+// H*W view masks; cost: H*W mean residual, grad: 3*H*W (may be null).
+hipError_t launch_ncc_energy(const PmArgs& a, int half_patch, bool f16, const float* vgeom, const float* depth,
+                             const float* ab, const uint32_t* views, float* cost, float* grad, hipStream_t stream);
+// One damped Gauss-Newton step on a.planes over the views of a.sel.
+hipError_t launch_gauss_newton(const PmArgs& a, int half_patch, bool f16, const float* vgeom, float lambda,
+                               hipStream_t stream);
 // conf = logistic(cost) where depth > 0, else 0 (n pixels).
 hipError_t launch_confidence(int64_t n, const float* depth, const float* cost, float* conf, hipStream_t stream);
 hipError_t launch_score(const PmArgs& a, int half_patch, bool f16, int n, const int32_t* xy,

@@ -9,6 +9,7 @@
 #include <pybind11/stl.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <initializer_list>
 #include <stdexcept>
@@ -202,6 +203,42 @@ class Estimator {
                             init.mutable_data(), sel.mutable_data()));
     }
     return py::make_tuple(costs, init, sel);
+  }
+
+  // Photometric cost and d cost / d(depth, a, b) of per-pixel planes
+  // (pm_ncc_cost_grad) on host arrays.
+  py::tuple ncc_cost_grad(py::array_t<float, py::array::c_style | py::array::forcecast> depth,
+                          py::array_t<float, py::array::c_style | py::array::forcecast> ab,
+                          py::array_t<uint32_t, py::array::c_style | py::array::forcecast> views) {
+    const auto [w, h] = ref_size();
+    const py::ssize_t n = w * h;
+    if (depth.size() != n || ab.size() != 2 * n || views.size() != n)
+      throw std::invalid_argument("ncc_cost_grad: arrays do not match the reference view size");
+    py::array_t<float> cost(shape({h, w})), grad(shape({h, w, 3}));
+    {
+      py::gil_scoped_release nogil;
+      check(pm_ncc_cost_grad(h_, depth.data(), ab.data(), views.data(), cost.mutable_data(), grad.mutable_data(), 0));
+    }
+    return py::make_tuple(cost, grad);
+  }
+
+  // The same on device buffers given by address (torch tensors' data_ptr()).
+  void ncc_cost_grad_device(uintptr_t depth, uintptr_t ab, uintptr_t views, uintptr_t cost, uintptr_t grad) {
+    py::gil_scoped_release nogil;
+    check(pm_ncc_cost_grad(h_, reinterpret_cast<const float*>(depth), reinterpret_cast<const float*>(ab),
+                           reinterpret_cast<const uint32_t*>(views), reinterpret_cast<float*>(cost),
+                           reinterpret_cast<float*>(grad), 1));
+  }
+
+  void gauss_newton(int steps, float damping) {
+    py::gil_scoped_release nogil;
+    check(pm_gauss_newton(h_, steps, damping));
+  }
+
+  float last_refine_ms() {
+    float ms = 0.0f;
+    check(pm_last_refine_timing(h_, &ms));
+    return ms;
   }
 
   py::dict stage_timing() {
@@ -472,6 +509,13 @@ PYBIND11_MODULE(pydense, m) {
       .def("write_state", &Estimator::write_state)
       .def("score_planes", &Estimator::score_planes, py::arg("xy"), py::arg("planes"))
       .def("last_timing", &Estimator::last_timing)
+      .def("ncc_cost_grad", &Estimator::ncc_cost_grad, py::arg("depth"), py::arg("ab"), py::arg("views"),
+           "Photometric cost (H,W) and its gradient (H,W,3) w.r.t. (depth, a, b) of the normal (a, b, -1)")
+      .def("ncc_cost_grad_device", &Estimator::ncc_cost_grad_device, py::arg("depth"), py::arg("ab"),
+           py::arg("views"), py::arg("cost"), py::arg("grad"))
+      .def("gauss_newton", &Estimator::gauss_newton, py::arg("steps") = 10, py::arg("damping") = 1e-3f,
+           "Gauss-Newton refinement of the PatchMatch result; read it with download()")
+      .def("last_refine_ms", &Estimator::last_refine_ms)
       .def("stage_timing", &Estimator::stage_timing,
            "Per-stage device time of the last run: {stage: (milliseconds, launches)}")
       .def_static("is_available", []() {

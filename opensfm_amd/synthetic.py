@@ -296,3 +296,82 @@ def facet_heightfield(seed: int = 2, width: int = 1600, height: int = 1200, num_
     noise = ValueNoise3D(seed)
     images, gt = _render_views(quads, noise, 0.5, Ks, Rs, ts, [(width, height)] * len(Ks), sigma, seed)
     return Scene(images, Ks, Rs, ts, gt, 4.0, 40.0, f"facet_heightfield(seed={seed},{width}x{height},N={num_src})")
+
+
+def _render_surface(zf, K, R, t, width, height, noise, base_cell, sigma, rng, supersample=1, iters=40):
+    """Ray-cast the graph z = zf(x, y) (world frame; returns (z, dz/dx, dz/dy))
+    by Newton's method along each pixel ray, started on the plane z = 10.
+    Returns (image, depth along the camera's optical axis)."""
+    s = max(1, int(supersample))
+    Ks = K.copy()
+    if s > 1:
+        Ks[0, 0] *= s; Ks[1, 1] *= s
+        Ks[0, 2] = (K[0, 2] + 0.5) * s - 0.5
+        Ks[1, 2] = (K[1, 2] + 0.5) * s - 0.5
+
+    def cast(Kc, w, h):
+        C = -R.T @ t
+        ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+        pix = np.stack([xs.ravel(), ys.ravel(), np.ones(xs.size)], 0)
+        dirs = R.T @ (np.linalg.inv(Kc) @ pix)       # world directions with z_cam = 1
+        lam = (10.0 - C[2]) / dirs[2]
+        for _ in range(iters):
+            X = C[:, None] + dirs * lam
+            zz, gx, gy = zf(X[0], X[1])
+            g = X[2] - zz
+            dg = dirs[2] - gx * dirs[0] - gy * dirs[1]
+            lam = lam - g / np.where(np.abs(dg) > 1e-9, dg, 1e-9)
+            if np.max(np.abs(g)) < 1e-10:
+                break
+        X = C[:, None] + dirs * lam
+        ok = (lam > 0) & (np.abs(X[2] - zf(X[0], X[1])[0]) < 1e-6)
+        return X, lam, ok
+
+    X, lam, ok = cast(Ks, width * s, height * s)
+    img = np.full(X.shape[1], 0.5)
+    img[ok] = noise.fbm(X[:, ok].T, base_cell)
+    img = img.reshape(height * s, width * s)
+    if s > 1:
+        img = img.reshape(height, s, width, s).mean(axis=(1, 3))
+    img = img + rng.normal(0.0, sigma, img.shape)
+    _, lam1, ok1 = cast(K, width, height)
+    depth = np.where(ok1, lam1, 0.0).reshape(height, width)
+    return img.astype(np.float32), depth.astype(np.float32)
+
+
+def sinusoidal_bumps(seed: int = 4, width: int = 1024, height: int = 768, num_src: int = 6,
+                     amplitude: float = 0.5, sigma: float = 0.01, supersample: int = 2) -> Scene:
+    """A plane 10 m from the reference camera with smooth sinusoidal bumps of
+    ``amplitude`` m (z = 10 + A sin(2 pi x / lx + p1) sin(2 pi y / ly + p2),
+    seeded wavelengths 2-4 m, i.e. bumps over a 10 m span), seen by the
+    reference and ``num_src`` neighbours on a 0.8 m ring.  Exact ray casting of
+    the smooth surface; focal 0.9 * width; depth range [6, 14] m."""
+    rng = np.random.default_rng(seed)
+    f = 0.9 * width
+    K = intrinsics(width, height, f)
+    lx, ly = rng.uniform(2.0, 4.0, 2)
+    p1, p2 = rng.uniform(0, 2 * np.pi, 2)
+    kx, ky = 2 * np.pi / lx, 2 * np.pi / ly
+
+    def zf(x, y):
+        sx, sy = np.sin(kx * x + p1), np.sin(ky * y + p2)
+        return (10.0 + amplitude * sx * sy, amplitude * kx * np.cos(kx * x + p1) * sy,
+                amplitude * ky * sx * np.cos(ky * y + p2))
+
+    target = np.array([0.0, 0.0, 10.0])
+    Ks, Rs, ts = [K.copy()], [np.eye(3)], [np.zeros(3)]
+    for k in range(num_src):
+        ang = 2 * np.pi * k / num_src + rng.uniform(-0.2, 0.2)
+        C = np.array([0.8 * np.cos(ang), 0.8 * np.sin(ang), rng.uniform(-0.1, 0.1)])
+        R, t = look_at(C, target + np.append(rng.uniform(-0.3, 0.3, 2), 0.0))
+        Ks.append(K.copy()); Rs.append(R); ts.append(t)
+    noise = ValueNoise3D(seed)
+    rr = np.random.default_rng(seed + 1000)
+    images, gt = [], None
+    for i, (Kc, R, t) in enumerate(zip(Ks, Rs, ts)):
+        img, depth = _render_surface(zf, Kc, R, t, width, height, noise, 0.25, sigma, rr, supersample)
+        images.append(img)
+        if i == 0:
+            gt = depth
+    return Scene(images, Ks, Rs, ts, gt, 6.0, 14.0,
+                 f"sinusoidal_bumps(seed={seed},{width}x{height},N={num_src},A={amplitude})")

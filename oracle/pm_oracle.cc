@@ -856,7 +856,6 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
   const pmo_params& P = *c.pr;
   uint32_t rs[2] = {st->rng[2 * idx], st->rng[2 * idx + 1]};
   const uint32_t key = static_cast<uint32_t>(idx + iteration * 7919 + 42);
-  const int n_src = c.n_src;
   const float* planes = st->planes;
   const float* costs = st->costs;
 
@@ -1319,6 +1318,92 @@ void pmo_postprocess(float* depth, float* normal, const float* cost, float* conf
   }
   for (int i = 0; i < npix; ++i)
     confidence[i] = depth[i] > 0.0f ? 1.0f / (1.0f + pm_expf((cost[i] - 0.5f) * 5.0f)) : 0.0f;
+}
+
+// Float64 evaluation of the differentiable photometric cost of one pixel
+// (the checker of pm_ncc_cost_grad's gradient by finite differences): mean
+// over the views of `views` valid at theta of 1 - NCC, stride-1 bilateral
+// NCC on the local affine warp of the plane through the pixel's point at
+// depth d with normal (a, b, -1).  Returns the cost and sets *used to the
+// number of valid views.
+double pmo_ncc_energy_f64(const pmo_scene* sc, const pmo_params* pr, int x, int y, double d, double a, double b,
+                          uint32_t views, int* used) {
+  Ctx c(sc, pr);
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+  const pmo_camera& c0 = c.cams[0];
+  const double fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
+  const double r0x = (x - cx) / fx, r0y = (y - cy) / fy, r1x = (x + 1 - cx) / fx, r2y = (y + 1 - cy) / fy;
+  const double ur0 = a * r0x + b * r0y - 1.0, ur1 = a * r1x + b * r0y - 1.0, ur2 = a * r0x + b * r2y - 1.0;
+  *used = 0;
+  if (std::fabs(ur0) < 1e-6 || std::fabs(ur1) < 1e-6 || std::fabs(ur2) < 1e-6) return 0.0;
+  const double dk[3] = {d, d * ur0 / ur1, d * ur0 / ur2};
+  if (!(dk[0] > 0 && dk[1] > 0 && dk[2] > 0)) return 0.0;
+  const double rx[3] = {r0x, r1x, r0x}, ry[3] = {r0y, r0y, r2y};
+  const int hp = pr->half_patch, D = 2 * hp + 1;
+  const double W = rp.sum_w;
+  if (!(W >= 1e-6)) return 0.0;
+  const double mean_r = rp.sum_wr / W, var_r = rp.sum_wrr / W - mean_r * mean_r;
+  if (var_r < pr->var_threshold) return 0.0;
+  double e = 0.0;
+  for (int s = 0; s < c.n_src; ++s) {
+    if (!((views >> s) & 1u)) continue;
+    const pmo_camera& ci = c.cams[s + 1];
+    double M[9], T[3], A[9], B[3];
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q)
+        M[3 * r + q] = static_cast<double>(ci.R[3 * r]) * c0.R[3 * q] + static_cast<double>(ci.R[3 * r + 1]) * c0.R[3 * q + 1] +
+                       static_cast<double>(ci.R[3 * r + 2]) * c0.R[3 * q + 2];
+    for (int r = 0; r < 3; ++r) T[r] = ci.t[r] - (M[3 * r] * c0.t[0] + M[3 * r + 1] * c0.t[1] + M[3 * r + 2] * c0.t[2]);
+    for (int r = 0; r < 3; ++r) {
+      for (int q = 0; q < 3; ++q)
+        A[3 * r + q] = static_cast<double>(ci.K[3 * r]) * M[q] + static_cast<double>(ci.K[3 * r + 1]) * M[3 + q] +
+                       static_cast<double>(ci.K[3 * r + 2]) * M[6 + q];
+      B[r] = ci.K[3 * r] * T[0] + ci.K[3 * r + 1] * T[1] + ci.K[3 * r + 2] * T[2];
+    }
+    double u[3], v[3];
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) {
+      const double px = dk[k] * (A[0] * rx[k] + A[1] * ry[k] + A[2]) + B[0];
+      const double py = dk[k] * (A[3] * rx[k] + A[4] * ry[k] + A[5]) + B[1];
+      const double pz = dk[k] * (A[6] * rx[k] + A[7] * ry[k] + A[8]) + B[2];
+      if (!(pz > 1e-6)) { ok = false; break; }
+      u[k] = px / pz;
+      v[k] = py / pz;
+    }
+    if (!ok) continue;
+    const double jxx = u[1] - u[0], jxy = v[1] - v[0], jyx = u[2] - u[0], jyy = v[2] - v[0];
+    const double det = jxx * jyy - jxy * jyx;
+    if (det < 0.02 || det > 50.0) continue;
+    const Img& im = c.src[s];
+    auto bil = [&](double sx, double sy) {
+      sx = std::min(std::max(sx, -1.0), static_cast<double>(im.w - 1));
+      sy = std::min(std::max(sy, -1.0), static_cast<double>(im.h - 1));
+      const double fx0 = std::floor(sx), fy0 = std::floor(sy), ax = sx - fx0, ay = sy - fy0;
+      const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+      const double t00 = im.texel(x0, y0), t01 = im.texel(x0 + 1, y0);
+      const double t10 = im.texel(x0, y0 + 1), t11 = im.texel(x0 + 1, y0 + 1);
+      const double v0 = t00 + ax * (t01 - t00), v1 = t10 + ax * (t11 - t10);
+      return v0 + ay * (v1 - v0);
+    };
+    const double scen = bil(u[0], v[0]);
+    double ss = 0, sss = 0, srs = 0;
+    int p = 0;
+    for (int iy = -hp; iy <= hp; ++iy)
+      for (int ix = -hp; ix <= hp; ++ix, ++p) {
+        const double sd = bil(u[0] + ix * jxx + iy * jyx, v[0] + ix * jxy + iy * jyy) - scen;
+        ss += rp.w[p] * sd;
+        sss += rp.w[p] * sd * sd;
+        srs += rp.wr[p] * sd;
+      }
+    (void)D;
+    const double mean_s = ss / W, var_s = sss / W - mean_s * mean_s;
+    if (var_s < pr->var_threshold) continue;
+    const double ncc = (srs / W - mean_r * mean_s) / std::sqrt(var_r * var_s);
+    e += 1.0 - ncc;
+    ++*used;
+  }
+  return *used ? e / *used : 0.0;
 }
 
 // Internal pieces, exposed so the tests can pin them one by one.

@@ -90,6 +90,8 @@ def lib():
         L.pmo_mc_cost.restype = f; L.pmo_mc_cost.argtypes = [vp, vp, vp, vp, vp, f, vp, i, i, vp, u]
         L.pmo_resize_area.argtypes = [vp, i, i, i, i, i, vp]
         L.pmo_postprocess.argtypes = [vp, vp, vp, vp, i, i, i, i]
+        L.pmo_ncc_energy_f64.restype = ctypes.c_double
+        L.pmo_ncc_energy_f64.argtypes = [vp, vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_double, u, vp]
         L.pmo_view_weights.restype = f
         L.pmo_view_weights.argtypes = [vp, vp, vp, u, u, vp, i, i, vp, vp]
         L.pmo_refinement_hypotheses.restype = i
@@ -565,3 +567,13 @@ def estimate_cluster(refs, params: Params, levels: int, geom_weight: float):
             out[i] = (d, nrm, c, conf)
         prev = cur
     return out
+
+
+def ncc_energy_f64(scene: OracleScene, params: Params, x: int, y: int, d: float, a: float, b: float,
+                   views: int):
+    """Float64 photometric cost of one pixel (the model of pm_ncc_cost_grad); returns (cost, views used)."""
+    p = params.c()
+    used = ctypes.c_int(0)
+    e = lib().pmo_ncc_energy_f64(ctypes.byref(scene.c), ctypes.byref(p), int(x), int(y), float(d), float(a), float(b),
+                                 ctypes.c_uint32(int(views)), ctypes.byref(used))
+    return float(e), int(used.value)

@@ -1,0 +1,138 @@
+"""The differentiable photometric cost and its Gauss-Newton refinement.
+
+The gradient of the HIP adjoint (pm_ncc_cost_grad) is checked against
+central finite differences (h = 1e-4) of a float64 evaluation of the same
+cost in the oracle (pmo_ncc_energy_f64) on 4096 random pixels; the torch
+wrapper must return the kernel's gradient; Gauss-Newton must lower the
+photometric cost and keep the depth accuracy.
+"""
+import numpy as np
+import pytest
+
+from oracle import pm_oracle as po
+from tests.helpers import gpu_estimator, params_for
+
+
+@pytest.fixture(scope="module")
+def bumps():
+    from opensfm_amd import synthetic
+    return synthetic.sinusoidal_bumps(4, 256, 192, 4)
+
+
+def test_f64_cost_vanishes_for_an_identical_view():
+    # a source with the reference's camera and image warps every tap onto
+    # itself (identity warp, det 1): NCC = 1 for any plane, cost 0 up to rounding
+    rng = np.random.default_rng(0)
+    img = rng.random((40, 48)).astype(np.float32)
+    K = np.array([[60.0, 0, 23.5], [0, 60.0, 19.5], [0, 0, 1]])
+    sc = po.OracleScene([img, img], [K, K], [np.eye(3)] * 2, [np.zeros(3)] * 2)
+    p = po.Params(depth_min=1.0, depth_max=50.0, patch_size=5)
+    for (d, a, b) in [(5.0, 0.0, 0.0), (7.0, 0.3, -0.2)]:
+        e, used = po.ncc_energy_f64(sc, p, 20, 15, d, a, b, 1)
+        assert used == 1 and abs(e) < 1e-6   # the reference moments are float sums
+
+
+def _planes_to_theta(planes, K):
+    h, w = planes.shape[:2]
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    rx, ry = (xs - K[0, 2]) / K[0, 0], (ys - K[1, 2]) / K[1, 1]
+    n = planes[..., :3].astype(np.float64)
+    depth = -planes[..., 3] / (n[..., 0] * rx + n[..., 1] * ry + n[..., 2])
+    ab = n[..., :2] / -n[..., 2:3]
+    return depth.astype(np.float32), ab.astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_fp64_finite_differences(pydense, bumps):
+    s = bumps
+    p = params_for(s, patch_size=5)
+    est = gpu_estimator(pydense, s, p)
+    out = est.compute()
+    depth, ab = _planes_to_theta(out["plane"], np.asarray(s.Ks[0]))
+    views = out["views"]
+    rng = np.random.default_rng(1)
+    # move away from the optimum so that the gradient is not ~0
+    depth = (depth * (1.0 + 0.01 * rng.standard_normal(depth.shape))).astype(np.float32)
+    ab = (ab + 0.05 * rng.standard_normal(ab.shape)).astype(np.float32)
+    cost, grad = est.ncc_cost_grad(depth, ab, views)
+    sc = po.OracleScene(s.images, s.Ks, s.Rs, s.ts)
+    h, w = depth.shape
+    cand = np.argwhere((views > 0) & (depth > 0))
+    cand = cand[(cand[:, 0] > 3) & (cand[:, 0] < h - 4) & (cand[:, 1] > 3) & (cand[:, 1] < w - 4)]
+    pick = cand[rng.choice(len(cand), 4096, replace=False)]
+    hstep = 1e-4
+    ok_cost, ok_grad, n = 0, 0, 0
+    for y, x in pick:
+        th = [float(depth[y, x]), float(ab[y, x, 0]), float(ab[y, x, 1])]
+        e0, u0 = po.ncc_energy_f64(sc, p, x, y, *th, int(views[y, x]))
+        fd = []
+        stable = u0 > 0
+        for j in range(3):
+            tp, tm = list(th), list(th)
+            tp[j] += hstep
+            tm[j] -= hstep
+            ep, up = po.ncc_energy_f64(sc, p, x, y, *tp, int(views[y, x]))
+            em, um = po.ncc_energy_f64(sc, p, x, y, *tm, int(views[y, x]))
+            stable &= up == u0 and um == u0
+            fd.append((ep - em) / (2 * hstep))
+        if not stable:
+            continue
+        n += 1
+        ok_cost += abs(cost[y, x] - e0) <= 1e-4
+        g = grad[y, x].astype(np.float64)
+        fd = np.asarray(fd)
+        ok_grad += np.all(np.abs(g - fd) <= 2e-3 + 2e-2 * np.abs(fd))
+    print(f"adjoint check: {n} pixels, cost within 1e-4 {ok_cost / n:.4f}, gradient within tolerance "
+          f"{ok_grad / n:.4f}", flush=True)
+    assert n > 3000
+    assert ok_cost / n >= 0.99
+    assert ok_grad / n >= 0.98
+
+
+@pytest.mark.gpu
+def test_torch_autograd_returns_the_kernel_gradient(pydense, bumps):
+    import torch
+    from opensfm_amd.autograd import ncc_cost
+
+    s = bumps
+    p = params_for(s, patch_size=5)
+    est = gpu_estimator(pydense, s, p)
+    out = est.compute()
+    depth, ab = _planes_to_theta(out["plane"], np.asarray(s.Ks[0]))
+    views = out["views"]
+    cost_ref, grad_ref = est.ncc_cost_grad(depth, ab, views)
+    dev = torch.device("cuda:0")
+    td = torch.tensor(depth, device=dev, requires_grad=True)
+    tab = torch.tensor(ab, device=dev, requires_grad=True)
+    tv = torch.tensor(views.view(np.int32), device=dev)
+    cost = ncc_cost(est, td, tab, tv)
+    weights = torch.linspace(0.5, 1.5, cost.numel(), device=dev).reshape(cost.shape)
+    (cost * weights).sum().backward()
+    wn = weights.cpu().numpy()
+    assert np.array_equal(cost.detach().cpu().numpy(), cost_ref)
+    assert np.allclose(td.grad.cpu().numpy(), wn * grad_ref[..., 0], rtol=1e-6, atol=0)
+    assert np.allclose(tab.grad.cpu().numpy(), wn[..., None] * grad_ref[..., 1:], rtol=1e-6, atol=0)
+
+
+@pytest.mark.gpu
+def test_gauss_newton_lowers_the_cost(pydense, bumps):
+    s = bumps
+    p = params_for(s, patch_size=5)
+    est = gpu_estimator(pydense, s, p)
+    before = est.compute()
+    d0, ab0 = _planes_to_theta(before["plane"], np.asarray(s.Ks[0]))
+    c0, _ = est.ncc_cost_grad(d0, ab0, before["views"])
+    est.gauss_newton(10, 1e-3)
+    after = est.download()
+    d1, ab1 = _planes_to_theta(after["plane"], np.asarray(s.Ks[0]))
+    c1, _ = est.ncc_cost_grad(d1, ab1, before["views"])
+    gt = s.gt_depth
+    acc0 = np.mean(np.abs(before["depth"] - gt) < 0.01 * gt)
+    acc1 = np.mean(np.abs(after["depth"] - gt) < 0.01 * gt)
+    m = before["views"] > 0
+    print(f"Gauss-Newton: mean cost {c0[m].mean():.5f} -> {c1[m].mean():.5f}, GT within 1% {acc0:.4f} -> "
+          f"{acc1:.4f}, {est.last_refine_ms():.2f} ms", flush=True)
+    # a step is kept only where it lowers the cost (up to the plane <-> (d, a, b) round trip)
+    assert np.mean(c1[m] <= c0[m] + 1e-4) >= 0.999
+    assert c1[m].mean() < c0[m].mean()
+    assert acc1 >= acc0 - 0.01
