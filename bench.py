@@ -43,8 +43,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_scene(seed: int):
+WORKLOADS = {
+    # name: (scene factory, description)
+    "c2": ("facet_heightfield", "plain PatchMatch, one 1600x1200 {storage} reference + 8 neighbours, 64-facet "
+                                "heightfield (seed {seed}), patch 7x7, {iters} iterations, init+sweeps+decode per step"),
+    "c4": ("sinusoidal_bumps", "PatchMatch + 10 Gauss-Newton steps, one 1024x768 {storage} reference + 6 neighbours, "
+                               "sinusoidal bumps 0.5 m over a 10 m plane (seed {seed}), patch 7x7, {iters} iterations"),
+}
+
+
+def make_scene(seed: int, workload: str = "c2"):
     from opensfm_amd import synthetic
+    if workload == "c4":
+        return synthetic.sinusoidal_bumps(seed=seed, width=1024, height=768, num_src=6)
     return synthetic.facet_heightfield(seed=seed, width=1600, height=1200, num_src=8)
 
 
@@ -104,6 +115,9 @@ def main() -> int:
     ap.add_argument("--storage", choices=["f32", "f16"], default="f32")
     ap.add_argument("--cpu-rows", type=int, default=384,
                     help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
+                    help="c2: the headline single-view workload; c4: PatchMatch + Gauss-Newton refinement")
+    ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 workload")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -118,8 +132,10 @@ def main() -> int:
     if not pydense.DepthmapEstimator.is_available():
         raise RuntimeError("no HIP device visible")
 
+    if args.workload == "c4" and args.seed == 2:
+        args.seed = 4
     log("rendering scene ...")
-    scene = make_scene(args.seed)
+    scene = make_scene(args.seed, args.workload)
     h, w = scene.gt_depth.shape
     n_src = scene.num_src
     units_per_step = w * h / 1e6 * n_src * args.iters
@@ -165,7 +181,8 @@ def main() -> int:
     flop_per_launch = (w * h / 2.0) * 9 * n_src * (21 * PATCH * PATCH + 75)
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": TRAFFIC_PER_LAUNCH,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": TRAFFIC_PER_LAUNCH if args.workload == "c2" and args.storage == "f32" else None,
                 "kernel": f"pass_candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'}>",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     stage_ms = {k: round(v[0] / args.steps, 3) for k, v in stages.items()}
@@ -176,6 +193,25 @@ def main() -> int:
     roofline_path = {"scope": "all colour-pass kernels of one step", "achieved": round(path_tflops, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(path_tflops / PEAK_FP32_TFLOPS, 4),
                      "sweep_ms": round(sweep_ms, 3)}
+
+    refine = None
+    if args.workload == "c4":
+        # Gauss-Newton after PatchMatch (device time of the steps, HIP events)
+        gn_ms, acc_b, acc_a = [], [], []
+        for i in range(args.steps):
+            est.run()
+            before = est.download()["depth"]
+            est.gauss_newton(args.refine_steps, 1e-3)
+            gn_ms.append(est.last_refine_ms())
+            after = est.download()["depth"]
+            acc_b.append(float(np.mean(np.abs(before - gt) < 0.01 * gt)))
+            acc_a.append(float(np.mean(np.abs(after - gt) < 0.01 * gt)))
+        ms = float(np.mean(gn_ms))
+        refine = {"steps": args.refine_steps, "ms": round(ms, 3),
+                  "value": round(w * h / 1e6 * n_src * args.refine_steps / (ms / 1e3), 3),
+                  "unit": "Mpx·views·steps/s", "gt_within_1pct_before": round(acc_b[-1], 4),
+                  "gt_within_1pct_after": round(acc_a[-1], 4)}
+        log(f"Gauss-Newton {args.refine_steps} steps: {ms:.2f} ms")
 
     cpu = None
     if args.cpu_rows > 0:
@@ -188,9 +224,8 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else "f16-storage/f32",
         "data": "synthetic",
-        "config": {"workload": f"plain PatchMatch, one {w}x{h} {args.storage} reference + {n_src} neighbours, "
-                               f"64-facet heightfield (seed {args.seed}), patch {PATCH}x{PATCH}, "
-                               f"{args.iters} iterations, init+sweeps+decode per step",
+        "config": {"workload": WORKLOADS[args.workload][1].format(storage=args.storage, seed=args.seed,
+                                                                   iters=args.iters),
                    "width": w, "height": h, "sources": n_src, "iterations": args.iters, "patch": PATCH,
                    "units_per_step": units_per_step},
         "roofline": roofline,
@@ -200,6 +235,8 @@ def main() -> int:
         "stage_ms_per_step": stage_ms,
         "pcie_inclusive_value": round(pcie_value, 3),
     }
+    if refine is not None:
+        line["refine"] = refine
     if cpu and cpu["value"] > 0:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)

@@ -150,7 +150,7 @@ struct pm_handle {
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
   DevBuf<uint64_t> fb_masks;   // pending textureless fallbacks per (tile, slot, view)
-  DevBuf<float2> ref_taps;     // reference-side patch data, strides 1..3
+  DevBuf<float2> ref_taps;     // reference-side stride-1 taps (w, w*r)
   DevBuf<float4> ref_sums;
   int64_t ref_plane = 0;
   bool ref_ready = false;
@@ -846,7 +846,7 @@ int upload_level(pm_handle* h, int level, int nlevels) {
   if (int r = h->gcams.alloc(cams.size())) return r;
   h->ref_plane = 2 * static_cast<int64_t>(h->H) * ((h->W + 1) / 2);
   const int P = (p.patch_size / 2 * 2 + 1) * (p.patch_size / 2 * 2 + 1);
-  if (int r = h->ref_taps.alloc(static_cast<size_t>(3 * P) * h->ref_plane)) return r;
+  if (int r = h->ref_taps.alloc(static_cast<size_t>(P) * h->ref_plane)) return r;
   if (int r = h->ref_sums.alloc(static_cast<size_t>(3) * h->ref_plane)) return r;
   const size_t rp = static_cast<size_t>(h->ref_plane);
   if (int r = h->ca.alloc(9 * static_cast<size_t>(h->n_src) * rp)) return r;

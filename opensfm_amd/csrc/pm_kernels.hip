@@ -100,13 +100,15 @@ __device__ __forceinline__ const T* images_of(const PmArgs& a) { return static_c
 // Reference-side patch data.  The bilateral weights w and the products w*r
 // of every tap, and the reference moments, depend only on the reference
 // image, the pixel and the sampling stride, so they are computed once per
-// run for strides 1, 2 and 3 and kept in HBM (about 1.2 KB per pixel):
-//   ref_taps[(s * P + p) * ref_plane + cidx] = (w, w*r)     s = stride - 1
-//   ref_sums[s * ref_plane + cidx] = (centre, sum_w, sum_wr, sum_wrr)
+// upload and kept in HBM: the stride-1 taps (392 B per pixel at patch 7)
+// and the moments of strides 1, 2 and 3:
+//   ref_taps[p * ref_plane + cidx] = (w, w*r)
+//   ref_sums[s * ref_plane + cidx] = (centre, sum_w, sum_wr, sum_wrr)   s = stride - 1
 // cidx is the colour-major pixel index, so the 64 lanes of a colour-pass
-// wave read consecutive entries.  Values are exactly those the reference
-// recomputes for every hypothesis (opencl_kernels.h:328-366 for stride 1,
-// the reference half of :216-244 for strides 2 and 3).
+// wave read consecutive entries.  The strided taps are recomputed where
+// they are used with this same arithmetic.  Values are exactly those the
+// reference recomputes for every hypothesis (opencl_kernels.h:328-366 for
+// stride 1, the reference half of :216-244 for strides 2 and 3).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int colour_index(const PmArgs& a, int x, int y) {
   const int hw = (a.width + 1) / 2;
@@ -146,7 +148,6 @@ __global__ __launch_bounds__(kWave) void refpatch_kernel(PmArgs a) {
   for (int si = 1; si < 3; ++si) {  // strides 2 and 3 (opencl_kernels.h:216-244)
     const float stride = static_cast<float>(si + 1);
     float sum_w = 0.0f, sum_r = 0.0f, sum_rr = 0.0f;
-    int p = 0;
     for (int iy = -HP; iy <= HP; ++iy) {
       for (int ix = -HP; ix <= HP; ++ix) {
         const float fdx = stride * static_cast<float>(ix);
@@ -159,8 +160,6 @@ __global__ __launch_bounds__(kWave) void refpatch_kernel(PmArgs a) {
         sum_r += wr;
         sum_rr += wr * r;
         sum_w += w;
-        taps[(static_cast<int64_t>(si) * P + p) * a.ref_plane + cidx] = make_float2(w, wr);
-        ++p;
       }
     }
     sums[static_cast<int64_t>(si) * a.ref_plane + cidx] = make_float4(center, sum_w, sum_r, sum_rr);
