@@ -183,7 +183,7 @@ def main() -> int:
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": TRAFFIC_PER_LAUNCH if args.workload == "c2" and args.storage == "f32" else None,
-                "kernel": f"pass_candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'}>",
+                "kernel": f"candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'},{est.tiles()[0]}>",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     stage_ms = {k: round(v[0] / args.steps, 3) for k, v in stages.items()}
     # The whole iteration path (all colour-pass kernels, fallbacks included)

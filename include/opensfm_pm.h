@@ -217,6 +217,59 @@ int pm_last_timing(pm_handle* h, float* run_ms, float* sweep_ms, int* sweep_laun
 #define PM_STAGES 6
 int pm_last_stage_timing(pm_handle* h, int stage, float* ms, int* launches);
 
+/* Colour-tile widths of the kernels (tiles are width x 64/width pixels of
+ * one colour): cost_tile_x for the cost kernels and the fallback (4, 8 or
+ * 16), pixel_tile_x for the per-pixel kernels (16 or 32); 0 = the default
+ * (16 for float32 images, 4 for float16; 32).  Results do not depend on
+ * them.  pm_get_tiles returns the widths in effect.  No counterpart in the
+ * reference (its work-group shape is fixed by the OpenCL launch,
+ * estimator.cc:619-680). */
+int pm_set_tiles(pm_handle* h, int cost_tile_x, int pixel_tile_x);
+int pm_get_tiles(pm_handle* h, int* cost_tile_x, int* pixel_tile_x);
+
+/* ---- Batched runs ----------------------------------------------------
+ * A batch runs many references (each a pm_handle with its views, parameters
+ * and depth range) through one launch per kernel of the schedule, captured
+ * once into a HIP graph.  It replaces the sequential reference loop of
+ * DepthmapClusterEstimator::Run (cluster_estimator.cc:117-151) at one
+ * pyramid level; every reference's result equals pm_run on that handle
+ * alone, and lands in the handle (pm_download).  References may differ in
+ * size, source count and depth range; patch size, image storage, iterations
+ * and the checkerboard filter must be the same for all, hierarchy_levels 1.
+ * The batch does not own the handles: they must outlive its runs.  A
+ * change to a handle (views, parameters, tiles) is noticed by the next
+ * pm_batch_run, which stages and captures again. */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(int device, pm_batch** out);
+int pm_batch_destroy(pm_batch* b);
+int pm_batch_add(pm_batch* b, pm_handle* ref);
+int pm_batch_clear(pm_batch* b);
+int pm_batch_size(pm_batch* b, int* n);
+/* cost_tile_x / pixel_tile_x: 0 = chosen by the autotuner; persistent: -1 =
+ * autotuner, 0 = one workgroup per tile, 1 = a resident grid that strides
+ * over the tiles; use_graph: 0 = enqueue the launches on every run. */
+int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int persistent, int use_graph);
+/* Configuration in use after pm_batch_prepare / pm_batch_run, and the time
+ * the autotuner took (0 when its result came from the per-process cache). */
+int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, int* persistent, float* tune_ms);
+/* The autotuner's trials, 9 floats each: cost_tile_x, pixel_tile_x,
+ * persistent, then the device ms of candidates, fallback, select,
+ * refine_cost, fallback, refine.  *count = number of floats. */
+int pm_batch_tune_log(pm_batch* b, float* out, int capacity, int* count);
+/* Forget the autotuner's per-process results. */
+void pm_batch_reset_tuning(void);
+/* Stage all references, tune, capture the graph (pm_batch_run does this
+ * when needed). */
+int pm_batch_prepare(pm_batch* b);
+int pm_batch_run(pm_batch* b);
+/* Device time of the last pm_batch_run's graph (ms). */
+int pm_batch_timing(pm_batch* b, float* run_ms);
+/* Device memory (bytes) that staging and running this handle at its finest
+ * level allocates; for admitting references to a batch within a memory
+ * budget (the reference frees each reference's buffers after its run,
+ * cluster_estimator.cc:150). */
+int pm_device_bytes(pm_handle* h, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,10 +42,11 @@ def build_hip(out: str = LIB, defines=(), sources=None, extra=()) -> str:
     """Compile the HIP sources (in parallel) and link ``out``.
 
     ``defines``: extra -D flags (tuning knobs); ``sources``: override the
-    source paths (default: csrc/pm_kernels.hip, pm_refine.hip, pm_api.hip);
+    source paths (default: csrc/pm_kernels.hip, pm_cost_f32.hip, pm_cost_f16.hip,
+    pm_refine.hip, pm_api.hip);
     ``extra``: further compiler flags.
     """
-    srcs = sources or [os.path.join(CSRC, n) for n in ("pm_kernels.hip", "pm_refine.hip", "pm_api.hip")]
+    srcs = sources or [os.path.join(CSRC, n) for n in ("pm_kernels.hip", "pm_cost_f32.hip", "pm_cost_f16.hip", "pm_refine.hip", "pm_api.hip")]
     objs = []
     procs = []
     for src in srcs:

@@ -172,6 +172,13 @@ struct pm_handle {
   float stage_ms[PM_STAGES] = {};
   int stage_launches[PM_STAGES] = {};
   bool timing = false;
+  // Colour-tile widths of the cost kernels and of the per-pixel kernels
+  // (0 = by image storage: see default_cost_tile).
+  int cost_tile_x = 0;
+  int pixel_tile_x = 0;
+  // Bumped by every change of views, parameters or device buffers, so a
+  // batch knows when its argument tables are stale.
+  uint64_t gen = 0;
 
   ~pm_handle() {
     images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
@@ -194,6 +201,16 @@ int check_handle(pm_handle* h) {
   if (!h) return fail(PM_E_INVALID, "null pm_handle");
   return PM_OK;
 }
+
+// Default colour-tile widths, measured on MI355X at 1600x1200 with 8
+// sources: the cost kernels and the fallback run fastest with 16-wide
+// tiles on float32 images and 4-wide tiles on float16 images; the
+// per-pixel kernels with 32-wide tiles.
+int cost_tile(const pm_handle* h) {
+  if (h->cost_tile_x) return h->cost_tile_x;
+  return h->params.image_storage == PM_STORE_F16 ? 4 : 16;
+}
+int pixel_tile(const pm_handle* h) { return h->pixel_tile_x ? h->pixel_tile_x : 32; }
 
 PmArgs make_args(pm_handle* h) {
   PmArgs a{};
@@ -405,6 +422,7 @@ int pm_destroy(pm_handle* h) {
 int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t, const void* image,
                 int format, int width, int height, int64_t row_stride, const uint8_t* /*mask*/) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   if (!K || !R || !t || !image) return fail(PM_E_INVALID, "add_view: null argument");
   if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "add_view: empty image");
   int esize = format == PM_U8 ? 1 : format == PM_F32 ? 4 : format == PM_F16 ? 2 : 0;
@@ -444,6 +462,7 @@ int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t,
 
 int pm_clear_views(pm_handle* h) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   h->views.clear();
   h->uploaded = false;
   h->state_ready = false;
@@ -459,6 +478,7 @@ int pm_num_views(pm_handle* h, int* n) {
 
 int pm_set_depth_range(pm_handle* h, float dmin, float dmax) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   h->params.depth_min = dmin;
   h->params.depth_max = dmax;
   h->state_ready = false;
@@ -467,6 +487,7 @@ int pm_set_depth_range(pm_handle* h, float dmin, float dmax) {
 
 int pm_set_params(pm_handle* h, const pm_params* p) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   if (!p) return fail(PM_E_INVALID, "null params");
   const bool restage = p->image_storage != h->params.image_storage || p->quantize_half != h->params.quantize_half ||
                        p->patch_size != h->params.patch_size;
@@ -486,6 +507,7 @@ int pm_get_params(pm_handle* h, pm_params* p) {
 
 int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev_depths, int n) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   if (n < 0 || n > PM_MAX_SOURCES) return fail(PM_E_INVALID, "too many previous depth maps");
   if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before previous depths");
   h->geom_weight = weight;
@@ -502,6 +524,7 @@ int pm_set_geom_consistency(pm_handle* h, float weight, const float* const* prev
 
 int pm_set_previous_depth(pm_handle* h, int slot, const float* depth, int width, int height) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   if (slot < 0 || slot >= PM_MAX_SOURCES) return fail(PM_E_INVALID, "previous depth: source slot out of range");
   if (!depth) {
     h->prev_depths[slot] = pm_handle::PrevDepth{};
@@ -516,6 +539,7 @@ int pm_set_previous_depth(pm_handle* h, int slot, const float* depth, int width,
 
 int pm_clear_previous_depths(pm_handle* h) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   for (auto& e : h->prev_depths) e = pm_handle::PrevDepth{};
   h->uploaded = false;
   return PM_OK;
@@ -523,6 +547,7 @@ int pm_clear_previous_depths(pm_handle* h) {
 
 int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before the prior");
   const size_t npix = static_cast<size_t>(h->views[0].w) * h->views[0].h;
   if (!prior_planes || !masks) {
@@ -748,6 +773,7 @@ int level_count(const pm_params& p) { return std::max(1, p.hierarchy_levels); }
 // sizes the per-pixel state for it (UploadData, estimator.cc:299-510).
 int upload_level(pm_handle* h, int level, int nlevels) {
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  ++h->gen;
   // Level views: level l of N is scaled by 1 / 2^(N-1-l) (estimator.cc:182-201).
   std::vector<HostView> scaled;
   const double scale = 1.0 / std::pow(2.0, nlevels - 1 - level);
@@ -1018,15 +1044,15 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
     if (color >= 0 && c != color) continue;
     a.color = c;
     if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
-                      [&] { return launch_candidates(a, hp, f16, st); })) return r;
+                      [&] { return launch_candidates(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
-                      [&] { return launch_fallback(a, hp, f16, 0, st); })) return r;
-    if (int r = timed(h, PM_STAGE_SELECT, "select kernel launch", [&] { return launch_select(a, st); })) return r;
+                      [&] { return launch_fallback(a, hp, f16, 0, cost_tile(h), st); })) return r;
+    if (int r = timed(h, PM_STAGE_SELECT, "select kernel launch", [&] { return launch_select(a, pixel_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_REFINE_COST, "refine-cost kernel launch",
-                      [&] { return launch_refine_cost(a, hp, f16, st); })) return r;
+                      [&] { return launch_refine_cost(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
-                      [&] { return launch_fallback(a, hp, f16, 1, st); })) return r;
-    if (int r = timed(h, PM_STAGE_REFINE, "refine kernel launch", [&] { return launch_refine(a, st); })) return r;
+                      [&] { return launch_fallback(a, hp, f16, 1, cost_tile(h), st); })) return r;
+    if (int r = timed(h, PM_STAGE_REFINE, "refine kernel launch", [&] { return launch_refine(a, pixel_tile(h), st); })) return r;
   }
   return PM_OK;
 }
@@ -1037,7 +1063,7 @@ int pm_filter(pm_handle* h) {
   PmArgs a = make_args(h);
   for (int c = 0; c < 2; ++c) {
     a.color = c;
-    PM_HIP(launch_filter(a, h->stream), "filter kernel launch");
+    PM_HIP(launch_filter(a, pixel_tile(h), h->stream), "filter kernel launch");
   }
   return PM_OK;
 }
@@ -1160,6 +1186,7 @@ int pm_level_size(pm_handle* h, int* width, int* height) {
 
 int pm_release_device(pm_handle* h) {
   if (int r = check_handle(h)) return r;
+  ++h->gen;
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   PM_HIP(hipStreamSynchronize(h->stream), "pm_release_device");
   h->images.release(); h->quads.release(); h->gviews.release(); h->gcams.release(); h->vgeom.release();
@@ -1352,6 +1379,516 @@ int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes,
   } while (0);
   dxy.release(); dpl.release(); dcost.release(); dinit.release(); dsel.release();
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Colour-tile widths of a single estimator.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int pm_set_tiles(pm_handle* h, int cost_tile_x, int pixel_tile_x) {
+  if (int r = check_handle(h)) return r;
+  if (cost_tile_x != 0 && !valid_cost_tile(cost_tile_x))
+    return fail(PM_E_INVALID, "cost tile width must be 0 (default), 4, 8 or 16");
+  if (pixel_tile_x != 0 && !valid_pixel_tile(pixel_tile_x))
+    return fail(PM_E_INVALID, "pixel tile width must be 0 (default), 16 or 32");
+  ++h->gen;
+  h->cost_tile_x = cost_tile_x;
+  h->pixel_tile_x = pixel_tile_x;
+  return PM_OK;
+}
+
+int pm_get_tiles(pm_handle* h, int* cost_tile_x, int* pixel_tile_x) {
+  if (int r = check_handle(h)) return r;
+  if (cost_tile_x) *cost_tile_x = cost_tile(h);
+  if (pixel_tile_x) *pixel_tile_x = pixel_tile(h);
+  return PM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched runs: the references of a pm_batch go through one launch per
+// kernel of the schedule (init, 6 kernels per colour pass, decode), every
+// launch covering the workgroups of all references, and the whole schedule
+// is captured once into a HIP graph.  DepthmapClusterEstimator::Run
+// (cluster_estimator.cc:82-156) runs the references one after the other;
+// the results here are bit-identical to running each reference alone
+// (same kernels, same per-reference arguments and workgroup mapping).
+// ---------------------------------------------------------------------------
+struct pm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<pm_handle*> refs;
+  bool use_graph = true;
+  // Requested configuration: 0 (tiles) / -1 (persistence) = autotune.
+  int want_cost_x = 0, want_pixel_x = 0, want_persistent = -1;
+  // Configuration in use.
+  int cost_x = 4, pixel_x = 32, persistent = 0;
+  float tune_ms = 0.0f, last_ms = 0.0f;
+  std::vector<float> tune_log;  // per trial: cost_x, pixel_x, persistent, 6 stage times (ms)
+  // Prepared state.
+  bool prepared = false;
+  std::vector<uint64_t> gens;
+  int hp = 0, max_src = 0, iters = 0, cus = 0, nslices = 0;
+  bool f16 = false;
+  DevBuf<PmArgs> args;  // [slice][reference]
+  DevBuf<int> bases;    // block offsets per (grid kind, tile width), nrefs + 1 each
+  std::vector<int> host_bases;
+  std::vector<std::pair<int, int>> base_keys;  // (grid kind, tile width) of each base array
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~pm_batch() {
+    drop_graph();
+    args.release();
+    bases.release();
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// The grid kind of a kernel: kernels of one kind share a workgroup layout.
+int grid_kind(int kernel) {
+  switch (kernel) {
+    case PM_K_REFPATCH: case PM_K_INIT: case PM_K_PRIOR: return PM_K_INIT;
+    case PM_K_CANDIDATES: case PM_K_REFINE_COST: return PM_K_CANDIDATES;
+    case PM_K_SELECT: case PM_K_REFINE: case PM_K_FILTER: return PM_K_SELECT;
+    default: return kernel;  // PM_K_FALLBACK, PM_K_DECODE
+  }
+}
+
+int waves_per_group(int kernel) {
+  switch (kernel) {
+    case PM_K_CANDIDATES: return 9;
+    case PM_K_REFINE_COST: return 6;
+    case PM_K_DECODE: return 4;
+    default: return 1;
+  }
+}
+
+// Argument slice of (iteration, colour, fallback phase); slice 0 holds the
+// init / decode arguments.
+int batch_slice(int it, int color, int phase) { return 1 + ((it * 2 + color) * 2 + phase); }
+
+// Launch view of the first `nrefs` references for `kernel` with tile width tile_x.
+PmBatch batch_view(const pm_batch* b, int kernel, int tile_x, int slice, int nrefs) {
+  const int kind = grid_kind(kernel);
+  const int tile = (kind == PM_K_INIT || kind == PM_K_DECODE) ? 0 : tile_x;
+  const int n = static_cast<int>(b->refs.size());
+  for (size_t i = 0; i < b->base_keys.size(); ++i) {
+    if (b->base_keys[i].first == kind && b->base_keys[i].second == tile) {
+      const int off = static_cast<int>(i) * (n + 1);
+      return PmBatch{b->args.p + static_cast<size_t>(slice) * n, b->bases.p + off, nrefs,
+                     b->host_bases[static_cast<size_t>(off + nrefs)]};
+    }
+  }
+  return PmBatch{nullptr, nullptr, 0, 0};
+}
+
+int batch_launch(pm_batch* b, int kernel, int slice, int nrefs, int tile_x, int persistent) {
+  const PmBatch v = batch_view(b, kernel, tile_x, slice, nrefs);
+  if (!v.args) return fail(PM_E_INVALID, "batch: no launch layout for this tile width");
+  unsigned grid = static_cast<unsigned>(v.total);
+  if (persistent > 0) {
+    // one resident round: 32 waves per CU
+    int64_t g = static_cast<int64_t>(b->cus) * 32 / waves_per_group(kernel);
+    g = 8 * ((g + 7) / 8);
+    if (g < v.total) grid = static_cast<unsigned>(g);
+  }
+  PM_HIP(launch_batch(kernel, v, grid, b->hp, b->f16, b->max_src, tile_x, b->stream), "batched kernel launch");
+  return PM_OK;
+}
+
+// One colour pass of iteration `it` over the first nrefs references;
+// stage_ev (optional): 7 events bracketing the 6 kernels.
+int batch_pass(pm_batch* b, int it, int c, int nrefs, int cx, int px, int persistent, hipEvent_t* stage_ev) {
+  const int s0 = batch_slice(it, c, 0), s1 = batch_slice(it, c, 1);
+  const int kernels[6] = {PM_K_CANDIDATES, PM_K_FALLBACK, PM_K_SELECT, PM_K_REFINE_COST, PM_K_FALLBACK, PM_K_REFINE};
+  const int slices[6] = {s0, s0, s0, s1, s1, s1};
+  for (int k = 0; k < 6; ++k) {
+    if (stage_ev) PM_HIP(hipEventRecord(stage_ev[k], b->stream), "event");
+    const int tx = (kernels[k] == PM_K_SELECT || kernels[k] == PM_K_REFINE) ? px : cx;
+    if (int r = batch_launch(b, kernels[k], slices[k], nrefs, tx, persistent)) return r;
+  }
+  if (stage_ev) PM_HIP(hipEventRecord(stage_ev[6], b->stream), "event");
+  return PM_OK;
+}
+
+// Prior re-init of the references that have a prior (PriorReinit,
+// estimator.cc:574-614), one launch each.
+int batch_priors(pm_batch* b) {
+  for (pm_handle* h : b->refs) {
+    if (!h->masks.p) continue;
+    PM_HIP(hipMemcpyAsync(h->snapshot.p, h->planes.p, h->planes.n * sizeof(float4), hipMemcpyDeviceToDevice,
+                          b->stream), "prior snapshot");
+    PM_HIP(launch_prior(make_args(h), b->hp, b->f16, b->stream), "prior kernel launch");
+  }
+  return PM_OK;
+}
+
+// The whole schedule of one run (Run, estimator.cc:134-255, one level).
+int batch_schedule(pm_batch* b) {
+  const int n = static_cast<int>(b->refs.size());
+  if (int r = batch_launch(b, PM_K_INIT, 0, n, 0, b->persistent)) return r;
+  if (int r = batch_priors(b)) return r;
+  for (int it = 0; it < b->iters; ++it)
+    for (int c = 0; c < 2; ++c)
+      if (int r = batch_pass(b, it, c, n, b->cost_x, b->pixel_x, b->persistent, nullptr)) return r;
+  if (b->refs[0]->params.checkerboard_filter)
+    for (int c = 0; c < 2; ++c)
+      if (int r = batch_launch(b, PM_K_FILTER, batch_slice(0, c, 0), n, b->pixel_x, b->persistent)) return r;
+  return batch_launch(b, PM_K_DECODE, 0, n, 0, b->persistent);
+}
+
+struct TuneResult {
+  int cost_x = 4, pixel_x = 32, persistent = 0;
+};
+std::vector<std::pair<std::vector<int>, TuneResult>> g_tune_cache;  // key: device, half patch, f16
+
+// Autotuner: on a prefix of the batch (about 8 Mpx of references), time
+// one colour pass per candidate configuration, each from a fresh random
+// init, with events around every kernel; keep the cost-tile width with the
+// least cost-kernel time, the pixel-tile width with the least select +
+// refine time, then persistence if it is faster.  All configurations give
+// bit-identical results, so only time is compared.
+int batch_tune(pm_batch* b) {
+  const int n = static_cast<int>(b->refs.size());
+  const std::vector<int> key = {b->device, b->hp, b->f16 ? 1 : 0};
+  TuneResult best;
+  bool cached = false;
+  for (const auto& e : g_tune_cache)
+    if (e.first == key) { best = e.second; cached = true; }
+  const bool need = (b->want_cost_x == 0 || b->want_pixel_x == 0 || b->want_persistent < 0) && !cached;
+  b->tune_ms = 0.0f;
+  if (need) {
+    int k = 0;
+    int64_t px = 0;
+    while (k < n && px < 8000000) { px += static_cast<int64_t>(b->refs[k]->W) * b->refs[k]->H; ++k; }
+    hipEvent_t ev[9];
+    for (auto& e : ev) PM_HIP(hipEventCreate(&e), "hipEventCreate");
+    auto trial = [&](int cx, int pxw, int pers, float* st) -> int {
+      if (int r = batch_launch(b, PM_K_INIT, 0, k, 0, 0)) return r;
+      if (int r = batch_pass(b, 0, 0, k, cx, pxw, pers, ev)) return r;
+      PM_HIP(hipEventSynchronize(ev[6]), "tune sync");
+      for (int j = 0; j < 6; ++j) PM_HIP(hipEventElapsedTime(&st[j], ev[j], ev[j + 1]), "event time");
+      b->tune_log.insert(b->tune_log.end(), {static_cast<float>(cx), static_cast<float>(pxw), static_cast<float>(pers),
+                                             st[0], st[1], st[2], st[3], st[4], st[5]});
+      return PM_OK;
+    };
+    PM_HIP(hipEventRecord(ev[7], b->stream), "event");
+    b->tune_log.clear();
+    float st[6];
+    int rc = trial(kCostTileWidths[0], kPixelTileWidths[1], 0, st);  // warm-up
+    float best_cost = 1e30f, best_pix = 1e30f, best_total = 1e30f;
+    for (int cx : kCostTileWidths) {
+      if (rc) break;
+      rc = trial(cx, kPixelTileWidths[1], 0, st);
+      const float c = st[0] + st[1] + st[3] + st[4];
+      if (c < best_cost) { best_cost = c; best.cost_x = cx; }
+    }
+    for (int pw : kPixelTileWidths) {
+      if (rc) break;
+      rc = trial(best.cost_x, pw, 0, st);
+      const float p = st[2] + st[5];
+      if (p < best_pix) { best_pix = p; best.pixel_x = pw; }
+    }
+    for (int pers = 0; pers < 2 && !rc; ++pers) {
+      rc = trial(best.cost_x, best.pixel_x, pers, st);
+      const float t = st[0] + st[1] + st[2] + st[3] + st[4] + st[5];
+      if (t < best_total) { best_total = t; best.persistent = pers; }
+    }
+    if (!rc) {
+      PM_HIP(hipEventRecord(ev[8], b->stream), "event");
+      PM_HIP(hipEventSynchronize(ev[8]), "tune sync");
+      PM_HIP(hipEventElapsedTime(&b->tune_ms, ev[7], ev[8]), "event time");
+      g_tune_cache.emplace_back(key, best);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+  }
+  b->cost_x = b->want_cost_x ? b->want_cost_x : best.cost_x;
+  b->pixel_x = b->want_pixel_x ? b->want_pixel_x : best.pixel_x;
+  b->persistent = b->want_persistent >= 0 ? b->want_persistent : best.persistent;
+  return PM_OK;
+}
+
+int batch_validate(pm_batch* b) {
+  if (b->refs.empty()) return fail(PM_E_INVALID, "batch: no references added");
+  const pm_params& p0 = b->refs[0]->params;
+  for (pm_handle* h : b->refs) {
+    if (h->device != b->device) return fail(PM_E_INVALID, "batch: every reference must be on the batch's device");
+    if (int r = validate(h)) return r;
+    const pm_params& p = h->params;
+    if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "batch: references run at one pyramid level");
+    if (p.patch_size != p0.patch_size || p.image_storage != p0.image_storage ||
+        p.max_iterations != p0.max_iterations || p.checkerboard_filter != p0.checkerboard_filter)
+      return fail(PM_E_INVALID,
+                  "batch: patch size, image storage, iterations and checkerboard filter must be the same for all "
+                  "references");
+  }
+  return PM_OK;
+}
+
+// Stage every reference, build the argument slices and block offsets, tune,
+// capture the schedule.
+int batch_prepare(pm_batch* b) {
+  b->prepared = false;
+  b->drop_graph();
+  if (int r = batch_validate(b)) return r;
+  PM_HIP(hipSetDevice(b->device), "hipSetDevice");
+  const int n = static_cast<int>(b->refs.size());
+  for (pm_handle* h : b->refs) {
+    if (!h->uploaded || h->W != h->views[0].w || h->H != h->views[0].h)
+      if (int r = upload_level(h, 0, 1)) return r;
+    if (int r = ensure_refpatch(h)) return r;
+    PM_HIP(hipStreamSynchronize(h->stream), "batch staging");
+  }
+  const pm_params& p0 = b->refs[0]->params;
+  b->hp = p0.patch_size / 2;
+  b->f16 = p0.image_storage == PM_STORE_F16;
+  b->iters = p0.max_iterations;
+  b->max_src = 0;
+  for (pm_handle* h : b->refs) b->max_src = std::max(b->max_src, h->n_src);
+  hipDeviceProp_t prop;
+  PM_HIP(hipGetDeviceProperties(&prop, b->device), "hipGetDeviceProperties");
+  b->cus = prop.multiProcessorCount;
+  // Argument slices.
+  const int iter_slices = std::max(b->iters, 1);
+  b->nslices = 1 + iter_slices * 4;
+  std::vector<PmArgs> tab(static_cast<size_t>(b->nslices) * n);
+  for (int r = 0; r < n; ++r) {
+    pm_handle* h = b->refs[r];
+    PmArgs a = make_args(h);
+    a.out = PmDecodeOut{h->out_depth.p, h->out_normal.p, h->out_cost.p, h->out_conf.p};
+    tab[static_cast<size_t>(r)] = a;
+    for (int it = 0; it < iter_slices; ++it)
+      for (int c = 0; c < 2; ++c)
+        for (int ph = 0; ph < 2; ++ph) {
+          PmArgs s = a;
+          s.iteration = it;
+          s.smooth_w = smooth_weight_at(h->params, it);
+          s.color = c;
+          s.phase = ph;
+          tab[static_cast<size_t>(batch_slice(it, c, ph)) * n + r] = s;
+        }
+  }
+  // Block offsets per grid kind and tile width.
+  b->base_keys.clear();
+  b->base_keys.emplace_back(PM_K_INIT, 0);
+  b->base_keys.emplace_back(PM_K_DECODE, 0);
+  for (int w : kCostTileWidths) {
+    b->base_keys.emplace_back(PM_K_CANDIDATES, w);
+    b->base_keys.emplace_back(PM_K_FALLBACK, w);
+  }
+  for (int w : kPixelTileWidths) b->base_keys.emplace_back(PM_K_SELECT, w);
+  b->host_bases.assign(b->base_keys.size() * (n + 1), 0);
+  for (size_t k = 0; k < b->base_keys.size(); ++k) {
+    int64_t acc = 0;
+    int* row = &b->host_bases[k * (n + 1)];
+    for (int r = 0; r < n; ++r) {
+      row[r] = static_cast<int>(acc);
+      const pm_handle* h = b->refs[r];
+      acc += kernel_blocks(b->base_keys[k].first, h->W, h->H, h->n_src, b->base_keys[k].second);
+      if (acc > 0x7fffffff) return fail(PM_E_INVALID, "batch: too many workgroups for one launch");
+    }
+    row[n] = static_cast<int>(acc);
+  }
+  if (int r = b->args.alloc(tab.size())) return r;
+  if (int r = b->bases.alloc(b->host_bases.size())) return r;
+  PM_HIP(hipMemcpyAsync(b->args.p, tab.data(), tab.size() * sizeof(PmArgs), hipMemcpyHostToDevice, b->stream),
+         "upload batch arguments");
+  PM_HIP(hipMemcpyAsync(b->bases.p, b->host_bases.data(), b->host_bases.size() * sizeof(int),
+                        hipMemcpyHostToDevice, b->stream), "upload batch offsets");
+  PM_HIP(hipStreamSynchronize(b->stream), "batch upload");
+  if (int r = batch_tune(b)) return r;
+  if (b->use_graph) {
+    PM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    const int rc = batch_schedule(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    PM_HIP(e, "end graph capture");
+    b->graph = g;
+    PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+  }
+  b->gens.clear();
+  for (pm_handle* h : b->refs) b->gens.push_back(h->gen);
+  b->prepared = true;
+  return PM_OK;
+}
+
+bool batch_stale(const pm_batch* b) {
+  if (!b->prepared || b->gens.size() != b->refs.size()) return true;
+  for (size_t i = 0; i < b->refs.size(); ++i)
+    if (!b->refs[i]->uploaded || b->refs[i]->gen != b->gens[i]) return true;
+  return false;
+}
+
+int check_batch(pm_batch* b) { return b ? PM_OK : fail(PM_E_INVALID, "null pm_batch"); }
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_create(int device, pm_batch** out) {
+  if (!out) return fail(PM_E_INVALID, "null output batch");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PM_E_DEVICE, "no HIP device is available");
+  if (device < 0 || device >= n) return fail(PM_E_INVALID, "device index out of range");
+  PM_HIP(hipSetDevice(device), "hipSetDevice");
+  pm_batch* b = new pm_batch();
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(PM_E_DEVICE, "hipStreamCreate failed");
+  }
+  for (auto& e : b->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete b;
+      return fail(PM_E_DEVICE, "hipEventCreate failed");
+    }
+  }
+  *out = b;
+  return PM_OK;
+}
+
+int pm_batch_destroy(pm_batch* b) {
+  if (!b) return PM_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return PM_OK;
+}
+
+int pm_batch_add(pm_batch* b, pm_handle* h) {
+  if (int r = check_batch(b)) return r;
+  if (int r = check_handle(h)) return r;
+  for (pm_handle* o : b->refs)
+    if (o == h) return fail(PM_E_INVALID, "batch: reference added twice");
+  b->refs.push_back(h);
+  b->prepared = false;
+  return PM_OK;
+}
+
+int pm_batch_clear(pm_batch* b) {
+  if (int r = check_batch(b)) return r;
+  b->refs.clear();
+  b->prepared = false;
+  b->drop_graph();
+  return PM_OK;
+}
+
+int pm_batch_size(pm_batch* b, int* n) {
+  if (int r = check_batch(b)) return r;
+  if (!n) return fail(PM_E_INVALID, "null output");
+  *n = static_cast<int>(b->refs.size());
+  return PM_OK;
+}
+
+int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int persistent, int use_graph) {
+  if (int r = check_batch(b)) return r;
+  if (cost_tile_x != 0 && !valid_cost_tile(cost_tile_x)) return fail(PM_E_INVALID, "cost tile width: 0, 4, 8 or 16");
+  if (pixel_tile_x != 0 && !valid_pixel_tile(pixel_tile_x)) return fail(PM_E_INVALID, "pixel tile width: 0, 16 or 32");
+  if (persistent < -1 || persistent > 1) return fail(PM_E_INVALID, "persistent: -1 (tune), 0 or 1");
+  b->want_cost_x = cost_tile_x;
+  b->want_pixel_x = pixel_tile_x;
+  b->want_persistent = persistent;
+  b->use_graph = use_graph != 0;
+  b->prepared = false;
+  return PM_OK;
+}
+
+int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, int* persistent, float* tune_ms) {
+  if (int r = check_batch(b)) return r;
+  if (cost_tile_x) *cost_tile_x = b->cost_x;
+  if (pixel_tile_x) *pixel_tile_x = b->pixel_x;
+  if (persistent) *persistent = b->persistent;
+  if (tune_ms) *tune_ms = b->tune_ms;
+  return PM_OK;
+}
+
+int pm_batch_tune_log(pm_batch* b, float* out, int capacity, int* count) {
+  if (int r = check_batch(b)) return r;
+  const int n = static_cast<int>(b->tune_log.size());
+  if (count) *count = n;
+  if (out) std::memcpy(out, b->tune_log.data(), sizeof(float) * static_cast<size_t>(std::min(n, capacity)));
+  return PM_OK;
+}
+
+void pm_batch_reset_tuning(void) { g_tune_cache.clear(); }
+
+int pm_batch_prepare(pm_batch* b) {
+  if (int r = check_batch(b)) return r;
+  return batch_prepare(b);
+}
+
+int pm_batch_run(pm_batch* b) {
+  if (int r = check_batch(b)) return r;
+  if (batch_stale(b))
+    if (int r = batch_prepare(b)) return r;
+  PM_HIP(hipSetDevice(b->device), "hipSetDevice");
+  PM_HIP(hipEventRecord(b->ev[0], b->stream), "event");
+  if (b->use_graph) {
+    PM_HIP(hipGraphLaunch(b->exec, b->stream), "graph launch");
+  } else {
+    if (int r = batch_schedule(b)) return r;
+  }
+  PM_HIP(hipEventRecord(b->ev[1], b->stream), "event");
+  PM_HIP(hipStreamSynchronize(b->stream), "batch run");
+  PM_HIP(hipEventElapsedTime(&b->last_ms, b->ev[0], b->ev[1]), "event time");
+  for (pm_handle* h : b->refs) {
+    h->state_ready = true;
+    if (int r = postprocess(h)) return r;
+  }
+  return PM_OK;
+}
+
+int pm_batch_timing(pm_batch* b, float* run_ms) {
+  if (int r = check_batch(b)) return r;
+  if (run_ms) *run_ms = b->last_ms;
+  return PM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int pm_device_bytes(pm_handle* h, uint64_t* bytes) {
+  if (int r = check_handle(h)) return r;
+  if (!bytes) return fail(PM_E_INVALID, "null output");
+  if (int r = validate(h)) return r;
+  const pm_params& p = h->params;
+  const uint64_t esz = p.image_storage == PM_STORE_F16 ? 2 : 4;
+  uint64_t total = 0;
+  for (const HostView& v : h->views) total += static_cast<uint64_t>(v.w + 2) * (v.h + 2) * esz * 5 + 64 * esz;
+  const uint64_t w = h->views[0].w, hh = h->views[0].h, npix = w * hh;
+  const uint64_t rp = 2 * hh * ((w + 1) / 2);
+  const uint64_t ns = h->views.size() - 1;
+  const uint64_t P = static_cast<uint64_t>(p.patch_size / 2 * 2 + 1) * (p.patch_size / 2 * 2 + 1);
+  total += rp * (P * 8 + 3 * 16 + 9 * ns * 4 + ns * 4 + 4 + 8 * 4 + 4 + 6 * 16);
+  total += static_cast<uint64_t>(fallback_mask_count(static_cast<int>(w), static_cast<int>(hh), static_cast<int>(ns))) * 8;
+  total += npix * (16 + 4 + 8 + 4 + 4 + 12 + 4 + 4);
+  if (!h->prior_masks.empty()) total += npix * (16 + 4 + 16);
+  for (const auto& e : h->prev_depths)
+    if (!e.d.empty()) { total += ns * npix * 4; break; }
+  *bytes = total;
+  return PM_OK;
 }
 
 }  // extern "C"

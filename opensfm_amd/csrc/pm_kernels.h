@@ -1,4 +1,6 @@
-// Host-side launchers of the PatchMatch kernels (implemented in pm_kernels.hip).
+// Host-side launchers of the PatchMatch kernels (pm_kernels.hip, and
+// pm_cost_f32.hip / pm_cost_f16.hip for the kernels templated on the image
+// storage type).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,8 +10,17 @@
 
 namespace pmhip {
 
-// Everything a kernel of the path needs, passed by value as the kernel
-// argument.  Per-pixel state is row-major over the reference view.
+struct PmDecodeOut {
+  float* depth;
+  float* normal;  // 3 per pixel
+  float* cost;
+  float* confidence;
+};
+
+// Everything a kernel of the path needs about one reference: passed by
+// value as the kernel argument of a single-reference launch, or read from
+// a device table by a batched launch.  Per-pixel state is row-major over
+// the reference view.
 struct PmArgs {
   float4* planes;              // (nx, ny, nz, d)
   float* costs;
@@ -50,19 +61,58 @@ struct PmArgs {
   int anchor_views;
   int color;
   int iteration;
+  PmDecodeOut out;             // decode targets
 };
 
-struct PmDecodeOut {
-  float* depth;
-  float* normal;  // 3 per pixel
-  float* cost;
-  float* confidence;
+// One launch over many references: workgroups [base[r], base[r+1]) of the
+// launch's item space belong to reference r, whose arguments are args[r]
+// (both arrays in device memory).  Every base is a multiple of 8 (the XCD
+// count), so a workgroup's index within its reference keeps the XCD that
+// the tile mapping assumes.
+struct PmBatch {
+  const PmArgs* args;
+  const int* base;   // nrefs + 1 entries
+  int nrefs;
+  int total;         // base[nrefs]
 };
+
+// Kernels of the path, for the batched launcher and the grid sizes.
+enum PmKernel {
+  PM_K_REFPATCH = 0,
+  PM_K_INIT,
+  PM_K_PRIOR,
+  PM_K_CANDIDATES,
+  PM_K_FALLBACK,
+  PM_K_SELECT,
+  PM_K_REFINE_COST,
+  PM_K_REFINE,
+  PM_K_FILTER,
+  PM_K_DECODE,
+  PM_K_COUNT
+};
+
+// Colour-tile widths (tiles are width x 64/width same-colour pixels).  The
+// cost kernels and the fallback share one width (the fallback masks are per
+// tile); the per-pixel kernels (select, refine, filter) take their own.
+constexpr int kCostTileWidths[] = {4, 8, 16};
+constexpr int kPixelTileWidths[] = {16, 32};
+bool valid_cost_tile(int tile_x);
+bool valid_pixel_tile(int tile_x);
+
+// Workgroups of one w x h reference with n_src sources for `kernel`
+// (tile_x: its colour-tile width; ignored by the full-resolution kernels).
+int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x);
+
+// Batched launch of `kernel` over bt with `grid` workgroups (grid <= bt.total;
+// smaller = persistent).  max_src: the largest n_src of the batch.
+hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int half_patch, bool f16, int max_src,
+                        int tile_x, hipStream_t stream);
 
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream);
-// Length of the fallback mask array for a w x h reference.
+// Length of the fallback mask array for a w x h reference (enough for
+// every cost-tile width).
 int64_t fallback_mask_count(int width, int height, int n_src);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
@@ -72,13 +122,21 @@ hipError_t launch_upsample(const PmArgs& a, const float* cdepth, const float* cn
                            hipStream_t stream);
 // One colour pass (a.color) of iteration a.iteration, in this order:
 // candidates, fallback(0), select, refine_cost, fallback(1), refine.
-hipError_t launch_candidates(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
-hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, hipStream_t stream);
-hipError_t launch_select(const PmArgs& a, hipStream_t stream);
-hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
-hipError_t launch_refine(const PmArgs& a, hipStream_t stream);
-hipError_t launch_filter(const PmArgs& a, hipStream_t stream);
+// cost_x / pixel_x: colour-tile widths (kCostTileWidths / kPixelTileWidths).
+hipError_t launch_candidates(const PmArgs& a, int half_patch, bool f16, int cost_x, hipStream_t stream);
+hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, int cost_x, hipStream_t stream);
+hipError_t launch_select(const PmArgs& a, int pixel_x, hipStream_t stream);
+hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, int cost_x, hipStream_t stream);
+hipError_t launch_refine(const PmArgs& a, int pixel_x, hipStream_t stream);
+hipError_t launch_filter(const PmArgs& a, int pixel_x, hipStream_t stream);
 hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stream);
+
+// The kernels templated on the image storage type, one translation unit per
+// type: a single-reference launch (a != null) or a batched one (bt != null).
+hipError_t launch_storage_f32(int kernel, const PmArgs* a, const PmBatch* bt, unsigned grid, size_t lds,
+                              int half_patch, int tile_x, hipStream_t stream);
+hipError_t launch_storage_f16(int kernel, const PmArgs* a, const PmBatch* bt, unsigned grid, size_t lds,
+                              int half_patch, int tile_x, hipStream_t stream);
 
 // Differentiable photometric cost (pm_refine.hip).  vgeom: 12 floats per
 // source view (A = K_v R_v R0^T row-major, b = K_v (t_v - R_v R0^T t0)).

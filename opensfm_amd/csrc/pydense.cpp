@@ -306,9 +306,77 @@ class Estimator {
     ref_w_ = ref_h_ = -1;
   }
 
+  pm_handle* handle() const { return h_; }
+  void set_tiles(int cost_x, int pixel_x) { check(pm_set_tiles(h_, cost_x, pixel_x)); }
+  py::tuple tiles() {
+    int c = 0, p = 0;
+    check(pm_get_tiles(h_, &c, &p));
+    return py::make_tuple(c, p);
+  }
+  uint64_t device_bytes() {
+    uint64_t b = 0;
+    check(pm_device_bytes(h_, &b));
+    return b;
+  }
+
  private:
   pm_handle* h_ = nullptr;
   py::ssize_t ref_w_ = -1, ref_h_ = -1;
+};
+
+// Many reference estimators run through one set of launches (pm_batch_*).
+// Results land in each estimator (download() / compute-style dicts).
+class Batch {
+ public:
+  explicit Batch(int device) { check(pm_batch_create(device, &b_)); }
+  ~Batch() { pm_batch_destroy(b_); }
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  void add(Estimator& e) { check(pm_batch_add(b_, e.handle())); }
+  void clear() { check(pm_batch_clear(b_)); }
+  int size() {
+    int n = 0;
+    check(pm_batch_size(b_, &n));
+    return n;
+  }
+  void set_config(int cost_x, int pixel_x, int persistent, bool use_graph) {
+    check(pm_batch_set_config(b_, cost_x, pixel_x, persistent, use_graph ? 1 : 0));
+  }
+  py::dict config() {
+    int c = 0, p = 0, pers = 0;
+    float ms = 0.0f;
+    check(pm_batch_get_config(b_, &c, &p, &pers, &ms));
+    py::dict d;
+    d["cost_tile_x"] = c;
+    d["pixel_tile_x"] = p;
+    d["persistent"] = pers;
+    d["tune_ms"] = ms;
+    return d;
+  }
+  py::array_t<float> tune_log() {
+    int n = 0;
+    check(pm_batch_tune_log(b_, nullptr, 0, &n));
+    py::array_t<float> out(shape({static_cast<py::ssize_t>(n / 9), 9}));
+    check(pm_batch_tune_log(b_, out.mutable_data(), n, &n));
+    return out;
+  }
+  void prepare() {
+    py::gil_scoped_release nogil;
+    check(pm_batch_prepare(b_));
+  }
+  float run() {
+    {
+      py::gil_scoped_release nogil;
+      check(pm_batch_run(b_));
+    }
+    float ms = 0.0f;
+    check(pm_batch_timing(b_, &ms));
+    return ms;
+  }
+
+ private:
+  pm_batch* b_ = nullptr;
 };
 
 // Per-device memory reservations of reserve_device_memory.
@@ -381,7 +449,46 @@ class ClusterEstimator {
     std::vector<Map> prev(static_cast<size_t>(K));
     std::vector<std::vector<float>> out(static_cast<size_t>(K));
     std::vector<std::pair<int, int>> size(static_cast<size_t>(K));
-    {
+    if (levels == 1) {
+      // One level: every reference runs in one batch (in groups that fit the
+      // free device memory), instead of one after the other.
+      py::gil_scoped_release nogil;
+      uint64_t total_mem = 0, free_mem = 0;
+      check(pm_device_memory(device_, &total_mem, &free_mem));
+      const uint64_t budget = free_mem / 10 * 9;
+      int i = 0;
+      while (i < K) {
+        pm_batch* b = nullptr;
+        check(pm_batch_create(device_, &b));
+        std::vector<int> group;
+        uint64_t used = 0;
+        try {
+          for (; i < K; ++i) {
+            uint64_t need = 0;
+            check(pm_device_bytes(refs_[i].h, &need));
+            if (!group.empty() && used + need > budget) break;
+            check(pm_batch_add(b, refs_[i].h));
+            group.push_back(i);
+            used += need;
+          }
+          check(pm_batch_run(b));
+          for (int j : group) {
+            int w = 0, h = 0;
+            check(pm_level_size(refs_[j].h, &w, &h));
+            const size_t n = static_cast<size_t>(w) * h;
+            std::vector<float>& o = out[j];
+            o.resize(6 * n);
+            check(pm_download(refs_[j].h, o.data(), nullptr, o.data() + n, o.data() + 4 * n, nullptr, o.data() + 5 * n));
+            size[j] = {w, h};
+            check(pm_release_device(refs_[j].h));
+          }
+        } catch (...) {
+          pm_batch_destroy(b);
+          throw;
+        }
+        pm_batch_destroy(b);
+      }
+    } else {
       py::gil_scoped_release nogil;
       for (int level = 0; level < levels; ++level) {
         const bool last = level == levels - 1;
@@ -518,6 +625,10 @@ PYBIND11_MODULE(pydense, m) {
       .def("last_refine_ms", &Estimator::last_refine_ms)
       .def("stage_timing", &Estimator::stage_timing,
            "Per-stage device time of the last run: {stage: (milliseconds, launches)}")
+      .def("set_tiles", &Estimator::set_tiles, py::arg("cost_tile_x"), py::arg("pixel_tile_x"),
+           "Colour-tile widths of the cost kernels (4, 8, 16) and the per-pixel kernels (16, 32)")
+      .def("tiles", &Estimator::tiles)
+      .def("device_bytes", &Estimator::device_bytes, "Device memory a run of this estimator allocates")
       .def_static("is_available", []() {
         int n = 0;
         return pm_device_count(&n) == PM_OK && n > 0;
@@ -538,6 +649,21 @@ PYBIND11_MODULE(pydense, m) {
   auto set = [](auto field) {
     return [field](C& c, decltype(std::declval<pm_params&>().*field) v) { c.params().*field = v; };
   };
+  py::class_<Batch>(m, "DepthmapBatch",
+                    "Many DepthmapEstimator references in one launch per kernel, captured in a HIP graph")
+      .def(py::init<int>(), py::arg("device") = 0)
+      .def("add", &Batch::add, py::arg("estimator"), py::keep_alive<1, 2>())
+      .def("clear", &Batch::clear)
+      .def("size", &Batch::size)
+      .def("set_config", &Batch::set_config, py::arg("cost_tile_x") = 0, py::arg("pixel_tile_x") = 0,
+           py::arg("persistent") = -1, py::arg("use_graph") = true,
+           "0 / -1 = chosen by the autotuner")
+      .def("config", &Batch::config)
+      .def("tune_log", &Batch::tune_log)
+      .def("prepare", &Batch::prepare)
+      .def("run", &Batch::run, "Run every reference; returns the device time in ms")
+      .def_static("reset_tuning", []() { pm_batch_reset_tuning(); });
+
   py::class_<C>(m, "DepthmapClusterEstimator")
       .def(py::init<>())
       .def("set_max_iterations", set(&pm_params::max_iterations))

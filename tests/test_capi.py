@@ -72,6 +72,20 @@ def test_no_device_is_reported_without_a_gpu(lib):
     assert not h.value
 
 
+def test_batch_without_a_gpu(lib):
+    n = ctypes.c_int(-1)
+    assert lib.pm_device_count(ctypes.byref(n)) == 0
+    if n.value > 0:
+        pytest.skip("a HIP device is visible; the no-device path is not reachable here")
+    b = ctypes.c_void_p()
+    assert lib.pm_batch_create(0, ctypes.byref(b)) == -2
+    assert not b.value
+    assert lib.pm_batch_destroy(None) == 0
+    assert lib.pm_batch_run(None) == -1
+    assert b"null pm_batch" in lib.pm_last_error()
+    assert lib.pm_batch_add(None, None) == -1
+
+
 def test_pydense_surface_names():
     from opensfm_amd import native
     pd = native()
@@ -79,6 +93,8 @@ def test_pydense_surface_names():
     for m in ("add_view", "set_depth_range", "set_patchmatch_iterations", "set_patch_size",
               "set_min_patch_sd", "compute_patch_match", "score_planes", "is_available", "num_devices"):
         assert hasattr(cls, m), m
+    for m in ("add", "set_config", "config", "tune_log", "prepare", "run", "reset_tuning"):
+        assert hasattr(pd.DepthmapBatch, m), m
 
 
 @pytest.mark.gpu
