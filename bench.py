@@ -49,6 +49,9 @@ WORKLOADS = {
                                 "heightfield (seed {seed}), patch 7x7, {iters} iterations, init+sweeps+decode per step"),
     "c4": ("sinusoidal_bumps", "PatchMatch + 10 Gauss-Newton steps, one 1024x768 {storage} reference + 6 neighbours, "
                                "sinusoidal bumps 0.5 m over a 10 m plane (seed {seed}), patch 7x7, {iters} iterations"),
+    "c3": ("camera_walk", "batched PatchMatch, {views} references of 640x480..2000x1500 with 4-12 neighbours each "
+                          "(camera walk around a 20x20 m box scene, seed {seed}), {storage} images, patch 7x7, "
+                          "{iters} iterations, one HIP-graph launch per step"),
 }
 
 
@@ -86,6 +89,110 @@ def cpu_baseline(scene, iters: int, rows: int):
                       f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
 
 
+def run_c3(args, pydense) -> int:
+    """Many references of mixed resolution in one batched, graph-captured run."""
+    import torch
+    from opensfm_amd import synthetic
+    from oracle import pm_oracle as po
+
+    seed = 3 if args.seed == 2 else args.seed
+    storage = args.storage if args.storage_set else "f16"
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    log(f"rendering camera walk ({args.views} views) on {dev} ...")
+    t0 = time.perf_counter()
+    ws = synthetic.camera_walk(seed=seed, num_views=args.views, device=dev)
+    torch.cuda.empty_cache()
+    log(f"rendered in {time.perf_counter() - t0:.1f} s")
+    units = ws.units(args.iters)
+    ests = []
+    for i in range(len(ws.refs)):
+        est = pydense.DepthmapEstimator()
+        configure(est, ws.reference(i), args.iters, storage)
+        ests.append(est)
+    batch = pydense.DepthmapBatch()
+    for e in ests:
+        batch.add(e)
+    t0 = time.perf_counter()
+    batch.prepare()
+    prep_s = time.perf_counter() - t0
+    cfg = batch.config()
+    log(f"staged + tuned + captured in {prep_s:.1f} s; config {cfg}")
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}: {batch.run():.1f} ms")
+    t0 = time.perf_counter()
+    dev_ms = []
+    for i in range(args.steps):
+        dev_ms.append(batch.run())
+        log(f"step {i + 1}/{args.steps}: {dev_ms[-1]:.1f} ms")
+    elapsed = time.perf_counter() - t0
+    value = units * args.steps / elapsed
+
+    # accuracy against the ground truth, pooled over all references
+    hit = tot = hit5 = 0
+    outs = []
+    for i, e in enumerate(ests):
+        d = e.download()["depth"]
+        outs.append(d)
+        gt = ws.gt_depth[ws.refs[i][0]]
+        m = gt > 0
+        hit += int(np.sum(np.abs(d[m] - gt[m]) < 0.01 * gt[m]))
+        hit5 += int(np.sum(np.abs(d[m] - gt[m]) < 0.05 * gt[m]))
+        tot += int(m.sum())
+    # the same references one after the other (single-reference path)
+    seq_ms = 0.0
+    for e in ests:
+        e.run()
+        seq_ms += e.last_timing()["run_ms"]
+    log(f"sequential single-reference runs: {seq_ms:.1f} ms")
+
+    # CPU oracle on the smallest reference, and parity of the batched result there
+    small = min(range(len(ws.refs)), key=lambda i: ws.images[ws.refs[i][0]].size * len(ws.refs[i][1]))
+    sc = ws.reference(small)
+    cpu, parity = None, None
+    if args.cpu_rows > 0:
+        log(f"CPU oracle on reference {small} ...")
+        p = po.Params(patch_size=PATCH, depth_min=sc.depth_min, depth_max=sc.depth_max, max_iterations=args.iters)
+        imgs = [im.astype(np.float16).astype(np.float32) for im in sc.images] if storage == "f16" else sc.images
+        t1 = time.perf_counter()
+        d_o, _, _, _, _ = po.estimate(imgs, sc.Ks, sc.Rs, sc.ts, p)
+        dt = time.perf_counter() - t1
+        hh, ww = sc.gt_depth.shape
+        u = ww * hh / 1e6 * sc.num_src * args.iters
+        cpu = {"value": round(u / dt, 5), "unit": UNIT, "cores": po.max_threads(), "kind": "port",
+               "sample": f"reference {small} of the walk ({ww}x{hh}, {sc.num_src} sources, {args.iters} iterations, "
+                         f"init+iterations+decode), {dt:.1f} s"}
+        m = d_o > 0
+        parity = {"reference": small, "within_1pct_of_oracle": round(float(np.mean(
+            np.abs(outs[small][m] - d_o[m]) < 0.01 * d_o[m])), 5),
+            "bit_equal": round(float(np.mean(outs[small] == d_o)), 5)}
+
+    sweep_tflops = units * 1e6 * FLOP_PER_UNIT / (float(np.mean(dev_ms)) / 1e3) / 1e12
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if storage == "f32" else "f16-storage/f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["c3"][1].format(views=len(ws.refs), seed=seed, storage=storage,
+                                                         iters=args.iters),
+                   "references": len(ws.refs), "megapixels": round(sum(ws.images[r[0]].size for r in ws.refs) / 1e6, 3),
+                   "sources_per_reference": [len(r[1]) for r in ws.refs], "iterations": args.iters, "patch": PATCH,
+                   "units_per_step": round(units, 3)},
+        "roofline_path": {"scope": "whole batched run (init, colour passes, decode) of one step",
+                          "achieved": round(sweep_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(sweep_tflops / PEAK_FP32_TFLOPS, 4), "device_ms": round(float(np.mean(dev_ms)), 3)},
+        "batch": {"config": cfg, "prepare_s": round(prep_s, 2), "sequential_ms": round(seq_ms, 3),
+                  "sequential_value": round(units / (seq_ms / 1e3), 3),
+                  "tune_log": batch.tune_log().round(3).tolist()},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "accuracy": {"gt_within_1pct": round(hit / max(tot, 1), 4), "gt_within_5pct": round(hit5 / max(tot, 1), 4)},
+    }
+    if cpu and cpu["value"] > 0:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def copy_probe(mib: int) -> None:
     """One device-to-device copy of ``mib`` MiB through the HIP runtime (a
     plain copy kernel of known size, to scale the HBM traffic counters)."""
@@ -112,15 +219,21 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--storage", choices=["f32", "f16"], default="f32")
+    ap.add_argument("--storage", choices=["f32", "f16"], default=None,
+                    help="image storage (default f32; f16 for the c3 workload)")
     ap.add_argument("--cpu-rows", type=int, default=384,
                     help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
-                    help="c2: the headline single-view workload; c4: PatchMatch + Gauss-Newton refinement")
+                    help="c2: the headline single-view workload; c3: many references in one batched launch; "
+                         "c4: PatchMatch + Gauss-Newton refinement")
     ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 workload")
+    ap.add_argument("--views", type=int, default=64, help="references of the c3 workload")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
+    args.storage_set = args.storage is not None
+    if args.storage is None:
+        args.storage = "f32"
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by reference "
               f"view with no cross-GPU traffic at one hierarchy level, so N GPUs would run N independent "
@@ -132,6 +245,8 @@ def main() -> int:
     if not pydense.DepthmapEstimator.is_available():
         raise RuntimeError("no HIP device visible")
 
+    if args.workload == "c3":
+        return run_c3(args, pydense)
     if args.workload == "c4" and args.seed == 2:
         args.seed = 4
     log("rendering scene ...")

@@ -245,16 +245,15 @@ int pm_batch_destroy(pm_batch* b);
 int pm_batch_add(pm_batch* b, pm_handle* ref);
 int pm_batch_clear(pm_batch* b);
 int pm_batch_size(pm_batch* b, int* n);
-/* cost_tile_x / pixel_tile_x: 0 = chosen by the autotuner; persistent: -1 =
- * autotuner, 0 = one workgroup per tile, 1 = a resident grid that strides
- * over the tiles; use_graph: 0 = enqueue the launches on every run. */
-int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int persistent, int use_graph);
+/* cost_tile_x / pixel_tile_x: 0 = chosen by the autotuner; use_graph: 0 =
+ * enqueue the launches on every run (per-stage timing available). */
+int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int use_graph);
 /* Configuration in use after pm_batch_prepare / pm_batch_run, and the time
  * the autotuner took (0 when its result came from the per-process cache). */
-int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, int* persistent, float* tune_ms);
-/* The autotuner's trials, 9 floats each: cost_tile_x, pixel_tile_x,
- * persistent, then the device ms of candidates, fallback, select,
- * refine_cost, fallback, refine.  *count = number of floats. */
+int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, float* tune_ms);
+/* The autotuner's trials, 8 floats each: cost_tile_x, pixel_tile_x, then
+ * the device ms of candidates, fallback, select, refine_cost, fallback,
+ * refine.  *count = number of floats. */
 int pm_batch_tune_log(pm_batch* b, float* out, int capacity, int* count);
 /* Forget the autotuner's per-process results. */
 void pm_batch_reset_tuning(void);
@@ -262,8 +261,11 @@ void pm_batch_reset_tuning(void);
  * when needed). */
 int pm_batch_prepare(pm_batch* b);
 int pm_batch_run(pm_batch* b);
-/* Device time of the last pm_batch_run's graph (ms). */
+/* Device time of the last pm_batch_run (ms). */
 int pm_batch_timing(pm_batch* b, float* run_ms);
+/* Per-stage device time of the last pm_batch_run without a graph
+ * (use_graph 0; HIP events around every launch); zero after a graph run. */
+int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches);
 /* Device memory (bytes) that staging and running this handle at its finest
  * level allocates; for admitting references to a batch within a memory
  * budget (the reference frees each reference's buffers after its run,

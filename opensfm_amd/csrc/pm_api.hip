@@ -10,10 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/opensfm_pm.h"
@@ -1424,12 +1427,12 @@ struct pm_batch {
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<pm_handle*> refs;
   bool use_graph = true;
-  // Requested configuration: 0 (tiles) / -1 (persistence) = autotune.
-  int want_cost_x = 0, want_pixel_x = 0, want_persistent = -1;
+  // Requested tile widths: 0 = autotune.
+  int want_cost_x = 0, want_pixel_x = 0;
   // Configuration in use.
-  int cost_x = 4, pixel_x = 32, persistent = 0;
+  int cost_x = 4, pixel_x = 32;
   float tune_ms = 0.0f, last_ms = 0.0f;
-  std::vector<float> tune_log;  // per trial: cost_x, pixel_x, persistent, 6 stage times (ms)
+  std::vector<float> tune_log;  // per trial: cost_x, pixel_x, 6 stage times (ms)
   // Prepared state.
   bool prepared = false;
   std::vector<uint64_t> gens;
@@ -1441,6 +1444,14 @@ struct pm_batch {
   std::vector<std::pair<int, int>> base_keys;  // (grid kind, tile width) of each base array
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Per-stage device time of the last eager run (no graph): events around
+  // every launch, as for a single handle.
+  bool timing = false;
+  std::vector<hipEvent_t> kev;
+  std::vector<std::pair<int, int>> kev_marks;  // (stage, index of the start event)
+  size_t kev_used = 0;
+  float stage_ms[PM_STAGES] = {};
+  int stage_launches[PM_STAGES] = {};
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1453,11 +1464,25 @@ struct pm_batch {
     args.release();
     bases.release();
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : kev) if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
+
+// Stage of a kernel for the per-stage timing (-1: not attributed).
+int stage_of(int kernel) {
+  switch (kernel) {
+    case PM_K_INIT: return PM_STAGE_INIT;
+    case PM_K_CANDIDATES: return PM_STAGE_CANDIDATES;
+    case PM_K_FALLBACK: return PM_STAGE_FALLBACK;
+    case PM_K_SELECT: return PM_STAGE_SELECT;
+    case PM_K_REFINE_COST: return PM_STAGE_REFINE_COST;
+    case PM_K_REFINE: return PM_STAGE_REFINE;
+    default: return -1;
+  }
+}
 
 // The grid kind of a kernel: kernels of one kind share a workgroup layout.
 int grid_kind(int kernel) {
@@ -1466,15 +1491,6 @@ int grid_kind(int kernel) {
     case PM_K_CANDIDATES: case PM_K_REFINE_COST: return PM_K_CANDIDATES;
     case PM_K_SELECT: case PM_K_REFINE: case PM_K_FILTER: return PM_K_SELECT;
     default: return kernel;  // PM_K_FALLBACK, PM_K_DECODE
-  }
-}
-
-int waves_per_group(int kernel) {
-  switch (kernel) {
-    case PM_K_CANDIDATES: return 9;
-    case PM_K_REFINE_COST: return 6;
-    case PM_K_DECODE: return 4;
-    default: return 1;
   }
 }
 
@@ -1497,30 +1513,38 @@ PmBatch batch_view(const pm_batch* b, int kernel, int tile_x, int slice, int nre
   return PmBatch{nullptr, nullptr, 0, 0};
 }
 
-int batch_launch(pm_batch* b, int kernel, int slice, int nrefs, int tile_x, int persistent) {
+int batch_launch(pm_batch* b, int kernel, int slice, int nrefs, int tile_x) {
   const PmBatch v = batch_view(b, kernel, tile_x, slice, nrefs);
   if (!v.args) return fail(PM_E_INVALID, "batch: no launch layout for this tile width");
-  unsigned grid = static_cast<unsigned>(v.total);
-  if (persistent > 0) {
-    // one resident round: 32 waves per CU
-    int64_t g = static_cast<int64_t>(b->cus) * 32 / waves_per_group(kernel);
-    g = 8 * ((g + 7) / 8);
-    if (g < v.total) grid = static_cast<unsigned>(g);
+  const unsigned grid = static_cast<unsigned>(v.total);
+  const int stage = b->timing ? stage_of(kernel) : -1;
+  if (stage >= 0) {
+    while (b->kev.size() < b->kev_used + 2) {
+      hipEvent_t e;
+      PM_HIP(hipEventCreate(&e), "hipEventCreate");
+      b->kev.push_back(e);
+    }
+    PM_HIP(hipEventRecord(b->kev[b->kev_used], b->stream), "event");
   }
   PM_HIP(launch_batch(kernel, v, grid, b->hp, b->f16, b->max_src, tile_x, b->stream), "batched kernel launch");
+  if (stage >= 0) {
+    PM_HIP(hipEventRecord(b->kev[b->kev_used + 1], b->stream), "event");
+    b->kev_marks.emplace_back(stage, static_cast<int>(b->kev_used));
+    b->kev_used += 2;
+  }
   return PM_OK;
 }
 
 // One colour pass of iteration `it` over the first nrefs references;
 // stage_ev (optional): 7 events bracketing the 6 kernels.
-int batch_pass(pm_batch* b, int it, int c, int nrefs, int cx, int px, int persistent, hipEvent_t* stage_ev) {
+int batch_pass(pm_batch* b, int it, int c, int nrefs, int cx, int px, hipEvent_t* stage_ev) {
   const int s0 = batch_slice(it, c, 0), s1 = batch_slice(it, c, 1);
   const int kernels[6] = {PM_K_CANDIDATES, PM_K_FALLBACK, PM_K_SELECT, PM_K_REFINE_COST, PM_K_FALLBACK, PM_K_REFINE};
   const int slices[6] = {s0, s0, s0, s1, s1, s1};
   for (int k = 0; k < 6; ++k) {
     if (stage_ev) PM_HIP(hipEventRecord(stage_ev[k], b->stream), "event");
     const int tx = (kernels[k] == PM_K_SELECT || kernels[k] == PM_K_REFINE) ? px : cx;
-    if (int r = batch_launch(b, kernels[k], slices[k], nrefs, tx, persistent)) return r;
+    if (int r = batch_launch(b, kernels[k], slices[k], nrefs, tx)) return r;
   }
   if (stage_ev) PM_HIP(hipEventRecord(stage_ev[6], b->stream), "event");
   return PM_OK;
@@ -1541,36 +1565,40 @@ int batch_priors(pm_batch* b) {
 // The whole schedule of one run (Run, estimator.cc:134-255, one level).
 int batch_schedule(pm_batch* b) {
   const int n = static_cast<int>(b->refs.size());
-  if (int r = batch_launch(b, PM_K_INIT, 0, n, 0, b->persistent)) return r;
+  if (int r = batch_launch(b, PM_K_INIT, 0, n, 0)) return r;
   if (int r = batch_priors(b)) return r;
   for (int it = 0; it < b->iters; ++it)
     for (int c = 0; c < 2; ++c)
-      if (int r = batch_pass(b, it, c, n, b->cost_x, b->pixel_x, b->persistent, nullptr)) return r;
+      if (int r = batch_pass(b, it, c, n, b->cost_x, b->pixel_x, nullptr)) return r;
   if (b->refs[0]->params.checkerboard_filter)
     for (int c = 0; c < 2; ++c)
-      if (int r = batch_launch(b, PM_K_FILTER, batch_slice(0, c, 0), n, b->pixel_x, b->persistent)) return r;
-  return batch_launch(b, PM_K_DECODE, 0, n, 0, b->persistent);
+      if (int r = batch_launch(b, PM_K_FILTER, batch_slice(0, c, 0), n, b->pixel_x)) return r;
+  return batch_launch(b, PM_K_DECODE, 0, n, 0);
 }
 
 struct TuneResult {
-  int cost_x = 4, pixel_x = 32, persistent = 0;
+  int cost_x = 4, pixel_x = 32;
 };
 std::vector<std::pair<std::vector<int>, TuneResult>> g_tune_cache;  // key: device, half patch, f16
+std::mutex g_tune_mutex;
 
 // Autotuner: on a prefix of the batch (about 8 Mpx of references), time
 // one colour pass per candidate configuration, each from a fresh random
 // init, with events around every kernel; keep the cost-tile width with the
-// least cost-kernel time, the pixel-tile width with the least select +
-// refine time, then persistence if it is faster.  All configurations give
-// bit-identical results, so only time is compared.
+// least cost-kernel time (candidates, refine_cost and both fallbacks) and
+// the pixel-tile width with the least select + refine time.  All
+// configurations give bit-identical results, so only time is compared.
 int batch_tune(pm_batch* b) {
   const int n = static_cast<int>(b->refs.size());
   const std::vector<int> key = {b->device, b->hp, b->f16 ? 1 : 0};
   TuneResult best;
   bool cached = false;
-  for (const auto& e : g_tune_cache)
-    if (e.first == key) { best = e.second; cached = true; }
-  const bool need = (b->want_cost_x == 0 || b->want_pixel_x == 0 || b->want_persistent < 0) && !cached;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    for (const auto& e : g_tune_cache)
+      if (e.first == key) { best = e.second; cached = true; }
+  }
+  const bool need = (b->want_cost_x == 0 || b->want_pixel_x == 0) && !cached;
   b->tune_ms = 0.0f;
   if (need) {
     int k = 0;
@@ -1578,41 +1606,37 @@ int batch_tune(pm_batch* b) {
     while (k < n && px < 8000000) { px += static_cast<int64_t>(b->refs[k]->W) * b->refs[k]->H; ++k; }
     hipEvent_t ev[9];
     for (auto& e : ev) PM_HIP(hipEventCreate(&e), "hipEventCreate");
-    auto trial = [&](int cx, int pxw, int pers, float* st) -> int {
-      if (int r = batch_launch(b, PM_K_INIT, 0, k, 0, 0)) return r;
-      if (int r = batch_pass(b, 0, 0, k, cx, pxw, pers, ev)) return r;
+    auto trial = [&](int cx, int pxw, float* st) -> int {
+      if (int r = batch_launch(b, PM_K_INIT, 0, k, 0)) return r;
+      if (int r = batch_pass(b, 0, 0, k, cx, pxw, ev)) return r;
       PM_HIP(hipEventSynchronize(ev[6]), "tune sync");
       for (int j = 0; j < 6; ++j) PM_HIP(hipEventElapsedTime(&st[j], ev[j], ev[j + 1]), "event time");
-      b->tune_log.insert(b->tune_log.end(), {static_cast<float>(cx), static_cast<float>(pxw), static_cast<float>(pers),
-                                             st[0], st[1], st[2], st[3], st[4], st[5]});
+      b->tune_log.insert(b->tune_log.end(), {static_cast<float>(cx), static_cast<float>(pxw), st[0], st[1], st[2],
+                                             st[3], st[4], st[5]});
       return PM_OK;
     };
     PM_HIP(hipEventRecord(ev[7], b->stream), "event");
     b->tune_log.clear();
     float st[6];
-    int rc = trial(kCostTileWidths[0], kPixelTileWidths[1], 0, st);  // warm-up
-    float best_cost = 1e30f, best_pix = 1e30f, best_total = 1e30f;
+    int rc = trial(kCostTileWidths[0], kPixelTileWidths[1], st);  // warm-up
+    float best_cost = 1e30f, best_pix = 1e30f;
     for (int cx : kCostTileWidths) {
       if (rc) break;
-      rc = trial(cx, kPixelTileWidths[1], 0, st);
+      rc = trial(cx, kPixelTileWidths[1], st);
       const float c = st[0] + st[1] + st[3] + st[4];
       if (c < best_cost) { best_cost = c; best.cost_x = cx; }
     }
     for (int pw : kPixelTileWidths) {
       if (rc) break;
-      rc = trial(best.cost_x, pw, 0, st);
+      rc = trial(best.cost_x, pw, st);
       const float p = st[2] + st[5];
       if (p < best_pix) { best_pix = p; best.pixel_x = pw; }
-    }
-    for (int pers = 0; pers < 2 && !rc; ++pers) {
-      rc = trial(best.cost_x, best.pixel_x, pers, st);
-      const float t = st[0] + st[1] + st[2] + st[3] + st[4] + st[5];
-      if (t < best_total) { best_total = t; best.persistent = pers; }
     }
     if (!rc) {
       PM_HIP(hipEventRecord(ev[8], b->stream), "event");
       PM_HIP(hipEventSynchronize(ev[8]), "tune sync");
       PM_HIP(hipEventElapsedTime(&b->tune_ms, ev[7], ev[8]), "event time");
+      std::lock_guard<std::mutex> lock(g_tune_mutex);
       g_tune_cache.emplace_back(key, best);
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -1620,7 +1644,6 @@ int batch_tune(pm_batch* b) {
   }
   b->cost_x = b->want_cost_x ? b->want_cost_x : best.cost_x;
   b->pixel_x = b->want_pixel_x ? b->want_pixel_x : best.pixel_x;
-  b->persistent = b->want_persistent >= 0 ? b->want_persistent : best.persistent;
   return PM_OK;
 }
 
@@ -1649,11 +1672,31 @@ int batch_prepare(pm_batch* b) {
   if (int r = batch_validate(b)) return r;
   PM_HIP(hipSetDevice(b->device), "hipSetDevice");
   const int n = static_cast<int>(b->refs.size());
-  for (pm_handle* h : b->refs) {
-    if (!h->uploaded || h->W != h->views[0].w || h->H != h->views[0].h)
-      if (int r = upload_level(h, 0, 1)) return r;
-    if (int r = ensure_refpatch(h)) return r;
-    PM_HIP(hipStreamSynchronize(h->stream), "batch staging");
+  {
+    // Stage the references on host threads (image conversion and padding
+    // are host work; each handle uploads on its own stream).
+    std::vector<int> rcs(static_cast<size_t>(n), PM_OK);
+    std::vector<std::string> msgs(static_cast<size_t>(n));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      if (hipSetDevice(b->device) != hipSuccess) return;
+      for (int i = next++; i < n; i = next++) {
+        pm_handle* h = b->refs[static_cast<size_t>(i)];
+        int r = PM_OK;
+        if (!h->uploaded || h->W != h->views[0].w || h->H != h->views[0].h) r = upload_level(h, 0, 1);
+        if (r == PM_OK) r = ensure_refpatch(h);
+        if (r == PM_OK && hipStreamSynchronize(h->stream) != hipSuccess) r = fail(PM_E_DEVICE, "batch staging");
+        rcs[static_cast<size_t>(i)] = r;
+        if (r != PM_OK) msgs[static_cast<size_t>(i)] = g_last_error;
+      }
+    };
+    const int nt = std::max(1, std::min({n, 16, static_cast<int>(std::thread::hardware_concurrency())}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (int i = 0; i < n; ++i)
+      if (rcs[static_cast<size_t>(i)] != PM_OK) return fail(rcs[static_cast<size_t>(i)], msgs[static_cast<size_t>(i)]);
   }
   const pm_params& p0 = b->refs[0]->params;
   b->hp = p0.patch_size / 2;
@@ -1801,24 +1844,21 @@ int pm_batch_size(pm_batch* b, int* n) {
   return PM_OK;
 }
 
-int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int persistent, int use_graph) {
+int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int use_graph) {
   if (int r = check_batch(b)) return r;
   if (cost_tile_x != 0 && !valid_cost_tile(cost_tile_x)) return fail(PM_E_INVALID, "cost tile width: 0, 4, 8 or 16");
   if (pixel_tile_x != 0 && !valid_pixel_tile(pixel_tile_x)) return fail(PM_E_INVALID, "pixel tile width: 0, 16 or 32");
-  if (persistent < -1 || persistent > 1) return fail(PM_E_INVALID, "persistent: -1 (tune), 0 or 1");
   b->want_cost_x = cost_tile_x;
   b->want_pixel_x = pixel_tile_x;
-  b->want_persistent = persistent;
   b->use_graph = use_graph != 0;
   b->prepared = false;
   return PM_OK;
 }
 
-int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, int* persistent, float* tune_ms) {
+int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, float* tune_ms) {
   if (int r = check_batch(b)) return r;
   if (cost_tile_x) *cost_tile_x = b->cost_x;
   if (pixel_tile_x) *pixel_tile_x = b->pixel_x;
-  if (persistent) *persistent = b->persistent;
   if (tune_ms) *tune_ms = b->tune_ms;
   return PM_OK;
 }
@@ -1831,7 +1871,10 @@ int pm_batch_tune_log(pm_batch* b, float* out, int capacity, int* count) {
   return PM_OK;
 }
 
-void pm_batch_reset_tuning(void) { g_tune_cache.clear(); }
+void pm_batch_reset_tuning(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tune_cache.clear();
+}
 
 int pm_batch_prepare(pm_batch* b) {
   if (int r = check_batch(b)) return r;
@@ -1844,14 +1887,27 @@ int pm_batch_run(pm_batch* b) {
     if (int r = batch_prepare(b)) return r;
   PM_HIP(hipSetDevice(b->device), "hipSetDevice");
   PM_HIP(hipEventRecord(b->ev[0], b->stream), "event");
+  for (int k = 0; k < PM_STAGES; ++k) { b->stage_ms[k] = 0.0f; b->stage_launches[k] = 0; }
   if (b->use_graph) {
     PM_HIP(hipGraphLaunch(b->exec, b->stream), "graph launch");
   } else {
-    if (int r = batch_schedule(b)) return r;
+    b->timing = true;
+    b->kev_used = 0;
+    b->kev_marks.clear();
+    const int rc = batch_schedule(b);
+    b->timing = false;
+    if (rc) return rc;
   }
   PM_HIP(hipEventRecord(b->ev[1], b->stream), "event");
   PM_HIP(hipStreamSynchronize(b->stream), "batch run");
   PM_HIP(hipEventElapsedTime(&b->last_ms, b->ev[0], b->ev[1]), "event time");
+  for (const auto& m : b->kev_marks) {
+    float ms = 0.0f;
+    PM_HIP(hipEventElapsedTime(&ms, b->kev[m.second], b->kev[m.second + 1]), "event time");
+    b->stage_ms[m.first] += ms;
+    b->stage_launches[m.first] += 1;
+  }
+  b->kev_marks.clear();
   for (pm_handle* h : b->refs) {
     h->state_ready = true;
     if (int r = postprocess(h)) return r;
@@ -1862,6 +1918,14 @@ int pm_batch_run(pm_batch* b) {
 int pm_batch_timing(pm_batch* b, float* run_ms) {
   if (int r = check_batch(b)) return r;
   if (run_ms) *run_ms = b->last_ms;
+  return PM_OK;
+}
+
+int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches) {
+  if (int r = check_batch(b)) return r;
+  if (stage < 0 || stage >= PM_STAGES) return fail(PM_E_INVALID, "unknown stage");
+  if (ms) *ms = b->stage_ms[stage];
+  if (launches) *launches = b->stage_launches[stage];
   return PM_OK;
 }
 

@@ -492,7 +492,8 @@ __device__ __forceinline__ void fallback_body(const PmArgs& a, int b) {
   const int n_src = a.n_src;
   if (s >= n_src || tile >= ntiles) return;
   const int nslots = a.phase == 0 ? 9 : 6;
-  const PM_GLOBAL uint64_t* mk = gp(a.fb_masks) + static_cast<int64_t>(tile) * 9 * n_src + s;
+  // written by the cost kernel before this launch: read-only here
+  const PM_CONST uint64_t* mk = (const PM_CONST uint64_t*)(a.fb_masks) + static_cast<int64_t>(tile) * 9 * n_src + s;
   uint64_t m[9];
   int cum[9];
   int total = 0;
@@ -1367,11 +1368,13 @@ __device__ __forceinline__ void decode_body(const PmArgs& a, int b) {
 // ---------------------------------------------------------------------------
 // Kernel entry points.  X_kernel runs one reference (arguments by value,
 // workgroup index = blockIdx.x).  X_batch runs the references of a PmBatch
-// in one launch: workgroups stride over the items of all references (a
-// persistent grid when the grid is smaller than the item count), find
-// their reference by binary search over the block offsets and load its
+// in one launch, one workgroup per item of any reference: the workgroup
+// finds its reference by binary search over the block offsets and loads its
 // arguments through the constant address space (scalar loads: the item is
-// uniform over the workgroup).
+// uniform over the workgroup).  A workgroup does one item only: in a loop
+// over items, the stores of one item would precede the next item's loads,
+// and uniform loads from global memory that may follow a store cannot be
+// scalar loads (this made a grid-stride variant 1.5x slower).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int batch_ref(const PmBatch& bt, int item) {
   const PM_CONST int* base = (const PM_CONST int*)(bt.base);
@@ -1383,16 +1386,14 @@ __device__ __forceinline__ int batch_ref(const PmBatch& bt, int item) {
   return lo;
 }
 
-// kLds: the body stages LDS, so the workgroup waits before the next item.
-template <bool kLds, typename F>
-__device__ __forceinline__ void for_each_item(const PmBatch& bt, F&& body) {
+template <typename F>
+__device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
   const PM_CONST int* base = (const PM_CONST int*)(bt.base);
-  for (int item = static_cast<int>(blockIdx.x); item < bt.total; item += static_cast<int>(gridDim.x)) {
-    const int r = batch_ref(bt, item);
-    const PmArgs a = ((const PM_CONST PmArgs*)(bt.args))[r];
-    body(a, item - base[r]);
-    if (kLds) __syncthreads();
-  }
+  const int item = static_cast<int>(blockIdx.x);
+  if (item >= bt.total) return;
+  const int r = batch_ref(bt, item);
+  const PmArgs a = ((const PM_CONST PmArgs*)(bt.args))[r];
+  body(a, item - base[r]);
 }
 
 #define PM_BLOCK static_cast<int>(blockIdx.x)
@@ -1401,68 +1402,68 @@ template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void refpatch_kernel(PmArgs a) { refpatch_body<HP, T>(a, PM_BLOCK); }
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void refpatch_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { refpatch_body<HP, T>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { refpatch_body<HP, T>(a, b); });
 }
 
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) { init_body<HP, T>(a, PM_BLOCK); }
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void init_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { init_body<HP, T>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { init_body<HP, T>(a, b); });
 }
 
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) { prior_body<HP, T>(a, PM_BLOCK); }
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void prior_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { prior_body<HP, T>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { prior_body<HP, T>(a, b); });
 }
 
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave * 9) void candidates_kernel(PmArgs a) { candidates_body<HP, T, TX>(a, PM_BLOCK); }
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave * 9) void candidates_batch(PmBatch bt) {
-  for_each_item<true>(bt, [](const PmArgs& a, int b) { candidates_body<HP, T, TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { candidates_body<HP, T, TX>(a, b); });
 }
 
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave * 6) void refine_cost_kernel(PmArgs a) { refine_cost_body<HP, T, TX>(a, PM_BLOCK); }
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave * 6) void refine_cost_batch(PmBatch bt) {
-  for_each_item<true>(bt, [](const PmArgs& a, int b) { refine_cost_body<HP, T, TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { refine_cost_body<HP, T, TX>(a, b); });
 }
 
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave) void fallback_kernel(PmArgs a) { fallback_body<HP, T, TX>(a, PM_BLOCK); }
 template <int HP, typename T, int TX>
 __global__ __launch_bounds__(kWave) void fallback_batch(PmBatch bt) {
-  for_each_item<true>(bt, [](const PmArgs& a, int b) { fallback_body<HP, T, TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { fallback_body<HP, T, TX>(a, b); });
 }
 
 template <int NV, int TX>
 __global__ __launch_bounds__(kWave) void select_kernel(PmArgs a) { select_body<NV, TX>(a, PM_BLOCK); }
 template <int NV, int TX>
 __global__ __launch_bounds__(kWave) void select_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { select_body<NV, TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { select_body<NV, TX>(a, b); });
 }
 
 template <int NV, int TX>
 __global__ __launch_bounds__(kWave) void refine_kernel(PmArgs a) { refine_body<NV, TX>(a, PM_BLOCK); }
 template <int NV, int TX>
 __global__ __launch_bounds__(kWave) void refine_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { refine_body<NV, TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { refine_body<NV, TX>(a, b); });
 }
 
 template <int TX>
 __global__ __launch_bounds__(kWave) void filter_kernel(PmArgs a) { filter_body<TX>(a, PM_BLOCK); }
 template <int TX>
 __global__ __launch_bounds__(kWave) void filter_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { filter_body<TX>(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { filter_body<TX>(a, b); });
 }
 
 __global__ __launch_bounds__(kDecodeThreads) void decode_kernel(PmArgs a) { decode_body(a, PM_BLOCK); }
 __global__ __launch_bounds__(kDecodeThreads) void decode_batch(PmBatch bt) {
-  for_each_item<false>(bt, [](const PmArgs& a, int b) { decode_body(a, b); });
+  batch_item(bt, [](const PmArgs& a, int b) { decode_body(a, b); });
 }
 
 #undef PM_BLOCK

@@ -103,8 +103,8 @@ bool valid_pixel_tile(int tile_x);
 // (tile_x: its colour-tile width; ignored by the full-resolution kernels).
 int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x);
 
-// Batched launch of `kernel` over bt with `grid` workgroups (grid <= bt.total;
-// smaller = persistent).  max_src: the largest n_src of the batch.
+// Batched launch of `kernel` over bt with `grid` (= bt.total) workgroups,
+// one per item.  max_src: the largest n_src of the batch.
 hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int half_patch, bool f16, int max_src,
                         int tile_x, hipStream_t stream);
 

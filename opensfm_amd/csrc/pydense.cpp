@@ -340,24 +340,23 @@ class Batch {
     check(pm_batch_size(b_, &n));
     return n;
   }
-  void set_config(int cost_x, int pixel_x, int persistent, bool use_graph) {
-    check(pm_batch_set_config(b_, cost_x, pixel_x, persistent, use_graph ? 1 : 0));
+  void set_config(int cost_x, int pixel_x, bool use_graph) {
+    check(pm_batch_set_config(b_, cost_x, pixel_x, use_graph ? 1 : 0));
   }
   py::dict config() {
-    int c = 0, p = 0, pers = 0;
+    int c = 0, p = 0;
     float ms = 0.0f;
-    check(pm_batch_get_config(b_, &c, &p, &pers, &ms));
+    check(pm_batch_get_config(b_, &c, &p, &ms));
     py::dict d;
     d["cost_tile_x"] = c;
     d["pixel_tile_x"] = p;
-    d["persistent"] = pers;
     d["tune_ms"] = ms;
     return d;
   }
   py::array_t<float> tune_log() {
     int n = 0;
     check(pm_batch_tune_log(b_, nullptr, 0, &n));
-    py::array_t<float> out(shape({static_cast<py::ssize_t>(n / 9), 9}));
+    py::array_t<float> out(shape({static_cast<py::ssize_t>(n / 8), 8}));
     check(pm_batch_tune_log(b_, out.mutable_data(), n, &n));
     return out;
   }
@@ -373,6 +372,17 @@ class Batch {
     float ms = 0.0f;
     check(pm_batch_timing(b_, &ms));
     return ms;
+  }
+  py::dict stage_timing() {
+    static const char* names[PM_STAGES] = {"init", "candidates", "fallback", "select", "refine_cost", "refine"};
+    py::dict d;
+    for (int k = 0; k < PM_STAGES; ++k) {
+      float ms = 0;
+      int n = 0;
+      check(pm_batch_stage_timing(b_, k, &ms, &n));
+      d[names[k]] = py::make_tuple(ms, n);
+    }
+    return d;
   }
 
  private:
@@ -656,12 +666,13 @@ PYBIND11_MODULE(pydense, m) {
       .def("clear", &Batch::clear)
       .def("size", &Batch::size)
       .def("set_config", &Batch::set_config, py::arg("cost_tile_x") = 0, py::arg("pixel_tile_x") = 0,
-           py::arg("persistent") = -1, py::arg("use_graph") = true,
-           "0 / -1 = chosen by the autotuner")
+           py::arg("use_graph") = true, "tile widths: 0 = chosen by the autotuner")
       .def("config", &Batch::config)
       .def("tune_log", &Batch::tune_log)
       .def("prepare", &Batch::prepare)
       .def("run", &Batch::run, "Run every reference; returns the device time in ms")
+      .def("stage_timing", &Batch::stage_timing,
+           "Per-stage device time of the last run without a graph: {stage: (milliseconds, launches)}")
       .def_static("reset_tuning", []() { pm_batch_reset_tuning(); });
 
   py::class_<C>(m, "DepthmapClusterEstimator")

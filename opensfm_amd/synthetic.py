@@ -375,3 +375,180 @@ def sinusoidal_bumps(seed: int = 4, width: int = 1024, height: int = 768, num_sr
             gt = depth
     return Scene(images, Ks, Rs, ts, gt, 6.0, 14.0,
                  f"sinusoidal_bumps(seed={seed},{width}x{height},N={num_src},A={amplitude})")
+
+
+# ---------------------------------------------------------------------------
+# Many references: a camera walk around a piecewise-planar scene
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class WalkScene:
+    """Views of a camera walk and the references built on them.
+
+    ``refs[i] = (view index, [source view indices], depth_min, depth_max)``;
+    ``gt_depth[v]`` is the ground-truth depth of view v (0 = no hit).
+    """
+    images: List[np.ndarray]
+    Ks: List[np.ndarray]
+    Rs: List[np.ndarray]
+    ts: List[np.ndarray]
+    gt_depth: List[np.ndarray]
+    refs: List[Tuple[int, List[int], float, float]]
+    name: str = ""
+
+    def reference(self, i: int) -> Scene:
+        """Reference i as a single-reference Scene (view 0 = the reference)."""
+        v, srcs, dmin, dmax = self.refs[i]
+        order = [v] + list(srcs)
+        return Scene([self.images[j] for j in order], [self.Ks[j] for j in order], [self.Rs[j] for j in order],
+                     [self.ts[j] for j in order], self.gt_depth[v], dmin, dmax, f"{self.name}[ref {i}]")
+
+    def units(self, iterations: int = 3) -> float:
+        """Mpix x source views x iterations of all references."""
+        return sum(self.images[v].size / 1e6 * len(s) * iterations for v, s, _, _ in self.refs)
+
+
+def _box_quads(cx, cy, hx, hy, height, angle):
+    """Top and four sides of a box standing on z = 0 (world z up)."""
+    c, s = np.cos(angle), np.sin(angle)
+    ax, ay = np.array([c, s, 0.0]) * hx, np.array([-s, c, 0.0]) * hy
+    base = np.array([cx, cy, 0.0])
+    up = np.array([0.0, 0.0, height])
+    corners = [base - ax - ay, base + ax - ay, base + ax + ay, base - ax + ay]
+    quads = [np.array([p + up for p in corners])]
+    for k in range(4):
+        a, b = corners[k], corners[(k + 1) % 4]
+        quads.append(np.array([a, b, b + up, a + up]))
+    return quads
+
+
+def _render_torch(quads, noise, base_cell, K, R, t, width, height, sigma, gen, device):
+    """Ray-cast convex planar quads with torch (float64) on ``device``; the
+    same model as ``render`` (z-buffer, 4-octave value noise, sensor noise).
+    Returns (image float32, depth float32) as numpy arrays."""
+    import torch
+
+    dt = torch.float64
+    Kt = torch.as_tensor(K, dtype=dt, device=device)
+    Rt = torch.as_tensor(R, dtype=dt, device=device)
+    tt = torch.as_tensor(t, dtype=dt, device=device)
+    C = -Rt.T @ tt
+    ys, xs = torch.meshgrid(torch.arange(height, dtype=dt, device=device),
+                            torch.arange(width, dtype=dt, device=device), indexing="ij")
+    pix = torch.stack([xs.reshape(-1), ys.reshape(-1), torch.ones(width * height, dtype=dt, device=device)], 0)
+    dirs = Rt.T @ (torch.linalg.inv(Kt) @ pix)         # z_cam = 1 per ray
+    zbuf = torch.full((width * height,), float("inf"), dtype=dt, device=device)
+    hit = torch.zeros((3, width * height), dtype=dt, device=device)
+    for quad in quads:
+        q = torch.as_tensor(np.asarray(quad, float), dtype=dt, device=device)
+        n = torch.linalg.cross(q[1] - q[0], q[2] - q[0])
+        n = n / torch.linalg.norm(n)
+        d = -(n @ q[0])
+        denom = n @ dirs
+        lam = -(n @ C + d) / denom
+        X = C[:, None] + dirs * lam[None, :]
+        inside = (lam > 1e-6) & torch.isfinite(lam)
+        for k in range(4):
+            a, b = q[k], q[(k + 1) % 4]
+            e = torch.linalg.cross((b - a).expand(X.shape[1], 3), (X.T - a), dim=1) @ n
+            inside &= e >= -1e-9
+        take = inside & (lam < zbuf)
+        zbuf = torch.where(take, lam, zbuf)
+        hit = torch.where(take[None, :], X, hit)
+    valid = torch.isfinite(zbuf)
+    table = torch.as_tensor(noise.values, dtype=dt, device=device)
+    nt = noise.n
+    p = hit.T
+    total = torch.zeros(p.shape[0], dtype=dt, device=device)
+    amp, cell, norm = 1.0, base_cell, 0.0
+    for o in range(4):
+        q = p / cell + 17.31 * o
+        fl = torch.floor(q)
+        f = q - fl
+        i = fl.to(torch.int64) % nt
+        f = f * f * (3.0 - 2.0 * f)
+        acc = torch.zeros_like(total)
+        for dx in (0, 1):
+            wx = f[:, 0] if dx else 1.0 - f[:, 0]
+            for dy in (0, 1):
+                wy = f[:, 1] if dy else 1.0 - f[:, 1]
+                for dz in (0, 1):
+                    wz = f[:, 2] if dz else 1.0 - f[:, 2]
+                    acc += wx * wy * wz * table[(i[:, 0] + dx) % nt, (i[:, 1] + dy) % nt, (i[:, 2] + dz) % nt]
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        cell *= 0.5
+    img = torch.where(valid, total / norm, torch.full_like(total, 0.5))
+    img = img + sigma * torch.randn(img.shape, generator=gen, dtype=dt, device=device)
+    depth = torch.where(valid, zbuf, torch.zeros_like(zbuf))
+    return (img.reshape(height, width).to(torch.float32).cpu().numpy(),
+            depth.reshape(height, width).to(torch.float32).cpu().numpy())
+
+
+WALK_SIZES = ((640, 480), (1024, 768), (1600, 1200), (2000, 1500))
+
+
+def camera_walk(seed: int = 3, num_views: int = 64, sizes: Sequence[Tuple[int, int]] = WALK_SIZES,
+                neighbours: Tuple[int, int] = (4, 12), num_boxes: int = 14, sigma: float = 0.01,
+                focal_frac: float = 0.85, device: str = "cpu") -> WalkScene:
+    """A seeded random camera walk around a 20 x 20 m piecewise-planar scene.
+
+    The scene (world z up) is a 20 x 20 m ground plane with ``num_boxes``
+    rotated boxes 0.5-4 m tall.  ``num_views`` cameras walk an arc 12-15 m
+    from the centre at 10-13 m height, each looking at the centre with some
+    jitter (so that the scene fills the views); each view draws its resolution from ``sizes`` (focal
+    ``focal_frac`` x width).  Every view is a reference; its sources are
+    4-12 (``neighbours``, drawn per reference) views nearest along the walk
+    whose viewing directions are within 35 degrees and whose baseline is
+    0.2-8 m.  The depth range of a reference is the depth span of the
+    scene's bounding box in its camera, widened by 10 %.
+    """
+    import torch
+
+    rng = np.random.default_rng(seed)
+    quads = [_rect([0.0, 0.0, 0.0], [1, 0, 0], [0, 1, 0], 10.0, 10.0)]
+    for _ in range(num_boxes):
+        cx, cy = rng.uniform(-7.5, 7.5, 2)
+        hx, hy = rng.uniform(0.4, 1.8, 2)
+        quads += _box_quads(cx, cy, hx, hy, rng.uniform(0.5, 4.0), rng.uniform(0, np.pi / 2))
+    bbox = np.array([[x, y, z] for x in (-10, 10) for y in (-10, 10) for z in (0.0, 4.0)])
+
+    phi = rng.uniform(0, 2 * np.pi)
+    radius, height = 13.5, 11.5
+    Ks, Rs, ts, centres, axes, sz = [], [], [], [], [], []
+    for _ in range(num_views):
+        phi += rng.uniform(0.03, 0.08)
+        radius = float(np.clip(radius + rng.normal(0, 0.3), 12.0, 15.0))
+        height = float(np.clip(height + rng.normal(0, 0.3), 10.0, 13.0))
+        C = np.array([radius * np.cos(phi), radius * np.sin(phi), height])
+        target = np.array([rng.normal(0, 0.8), rng.normal(0, 0.8), 0.5])
+        R, t = look_at(C, target, up=(0.0, 0.0, -1.0))
+        w, h = sizes[rng.integers(len(sizes))]
+        Ks.append(intrinsics(w, h, focal_frac * w)); Rs.append(R); ts.append(t)
+        centres.append(C); axes.append(R[2]); sz.append((w, h))
+
+    refs = []
+    for i in range(num_views):
+        want = int(rng.integers(neighbours[0], neighbours[1] + 1))
+        cand = []
+        for j in range(num_views):
+            if j == i:
+                continue
+            base = np.linalg.norm(centres[j] - centres[i])
+            ang = np.degrees(np.arccos(np.clip(axes[i] @ axes[j], -1.0, 1.0)))
+            if 0.2 <= base <= 8.0 and ang <= 35.0:
+                cand.append((abs(j - i), j))
+        srcs = [j for _, j in sorted(cand)[:want]]
+        z = (Rs[i] @ bbox.T).T[:, 2] + ts[i][2]
+        refs.append((i, srcs, float(max(0.5, 0.9 * z.min())), float(1.1 * z.max())))
+
+    noise = ValueNoise3D(seed)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed + 1000)
+    images, gts = [], []
+    for K, R, t, (w, h) in zip(Ks, Rs, ts, sz):
+        img, depth = _render_torch(quads, noise, 0.4, K, R, t, w, h, sigma, gen, device)
+        images.append(img)
+        gts.append(depth)
+    return WalkScene(images, Ks, Rs, ts, gts, refs,
+                     f"camera_walk(seed={seed},{num_views} views,{len(quads)} quads)")
