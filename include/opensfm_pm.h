@@ -272,6 +272,35 @@ int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches);
  * cluster_estimator.cc:150). */
 int pm_device_bytes(pm_handle* h, uint64_t* bytes);
 
+/* ---- Depth-map cleaner ----------------------------------------------
+ * Cross-view depth consistency with space carving and a grazing-angle rule:
+ * GPUDepthmapCleaner (opensfm/src/lib/dense/cleaner.h:21-102, cleaner.cc)
+ * and its kernel acmmp_clean_depthmap (opencl_kernels.h:2116-2279), bound
+ * to Python as GPUDepthmapCleaner (python/pybind.cc:125-143).  Depth maps
+ * stay in device memory from pm_cleaner_add_view until pm_cleaner_clear. */
+typedef struct pm_cleaner pm_cleaner;
+int pm_cleaner_create(int device, pm_cleaner** out);
+int pm_cleaner_destroy(pm_cleaner* c);
+/* SetSameDepthThreshold, SetMinConsistentViews, SetCarvingThreshold,
+ * SetMaxCarvedViews, SetGrazingCosThreshold (cleaner.h:27-43; defaults
+ * 0.01, 3, 0.05, 2, 0.2). */
+int pm_cleaner_set_params(pm_cleaner* c, float same_depth_threshold, int min_consistent_views,
+                          float carving_threshold, int max_carved_views, float grazing_cos_threshold);
+/* AddView (cleaner.h:46-51): K, R row-major float64, t float64, depth
+ * (height x width float32, copied); normal: camera-frame normals, 3 floats
+ * per pixel interleaved, or NULL.  *index = the view's index. */
+int pm_cleaner_add_view(pm_cleaner* c, const double* K, const double* R, const double* t, const float* depth,
+                        int width, int height, const float* normal, int* index);
+int pm_cleaner_num_views(pm_cleaner* c, int* n);
+int pm_cleaner_view_size(pm_cleaner* c, int index, int* width, int* height);
+/* Clean (cleaner.cc:95-242): the cleaned depth of view ref_idx (its size)
+ * against up to 16 neighbours, into out (caller-owned, width x height). */
+int pm_cleaner_clean(pm_cleaner* c, int ref_idx, const int* neighbor_ids, int num_neighbors, float* out);
+/* Device time of the last pm_cleaner_clean's kernel (ms). */
+int pm_cleaner_last_timing(pm_cleaner* c, float* ms);
+/* Clear (cleaner.cc:244-256): drop every view and its device memory. */
+int pm_cleaner_clear(pm_cleaner* c);
+
 #ifdef __cplusplus
 }
 #endif

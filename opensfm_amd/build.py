@@ -28,6 +28,10 @@ HIP_FLAGS = [
 ]
 
 
+# HIP translation units of the library, compiled in parallel.
+SOURCES = ("pm_kernels.hip", "pm_cost_f32.hip", "pm_cost_f16.hip", "pm_refine.hip", "pm_clean.hip", "pm_api.hip")
+
+
 def pydense_path() -> str:
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     return os.path.join(HERE, "pydense" + suffix)
@@ -42,11 +46,10 @@ def build_hip(out: str = LIB, defines=(), sources=None, extra=()) -> str:
     """Compile the HIP sources (in parallel) and link ``out``.
 
     ``defines``: extra -D flags (tuning knobs); ``sources``: override the
-    source paths (default: csrc/pm_kernels.hip, pm_cost_f32.hip, pm_cost_f16.hip,
-    pm_refine.hip, pm_api.hip);
+    source paths (default: SOURCES under csrc/);
     ``extra``: further compiler flags.
     """
-    srcs = sources or [os.path.join(CSRC, n) for n in ("pm_kernels.hip", "pm_cost_f32.hip", "pm_cost_f16.hip", "pm_refine.hip", "pm_api.hip")]
+    srcs = sources or [os.path.join(CSRC, n) for n in SOURCES]
     objs = []
     procs = []
     for src in srcs:

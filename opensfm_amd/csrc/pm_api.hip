@@ -121,6 +121,11 @@ struct DevBuf {
 
 }  // namespace
 
+namespace pmhip {
+// Error reporting for the other translation units of the C ABI.
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace pmhip
+
 struct pm_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -623,6 +628,16 @@ void resize_linear(const float* src, int w, int h, int nw, int nh, float* dst) {
     }
   }
 }
+
+}  // namespace
+
+namespace pmhip {
+void host_resize_linear(const float* src, int w, int h, int nw, int nh, float* dst) {
+  resize_linear(src, w, h, nw, nh, dst);
+}
+}  // namespace pmhip
+
+namespace {
 
 // One pyramid level of a view (BuildImagePyramid, estimator.cc:727-756):
 // size int(w * scale), area-resampled pixels, fx, cx scaled by new_w / w

@@ -389,6 +389,90 @@ class Batch {
   pm_batch* b_ = nullptr;
 };
 
+// GPUDepthmapCleaner (depthmap_bind.h:120-162, pybind.cc:125-143).
+class Cleaner {
+ public:
+  Cleaner() { check(pm_cleaner_create(0, &c_)); }
+  ~Cleaner() { pm_cleaner_destroy(c_); }
+  Cleaner(const Cleaner&) = delete;
+  Cleaner& operator=(const Cleaner&) = delete;
+
+  void set_device(int d) {
+    int n = 0;
+    check(pm_cleaner_num_views(c_, &n));
+    if (n > 0) throw std::runtime_error("GPUDepthmapCleaner.set_device: call before add_view");
+    pm_cleaner* c = nullptr;
+    check(pm_cleaner_create(d, &c));
+    pm_cleaner_destroy(c_);
+    c_ = c;
+    push();
+  }
+  void set_same_depth_threshold(float v) { same_ = v; push(); }
+  void set_min_consistent_views(int v) { min_views_ = v; push(); }
+  void set_carving_threshold(float v) { carve_ = v; push(); }
+  void set_max_carved_views(int v) { max_carved_ = v; push(); }
+  void set_grazing_cos_threshold(float v) { grazing_ = v; push(); }
+
+  int add_view(py::handle K, py::handle R, py::handle t, py::handle depth, py::object normal) {
+    auto k = as_f64(K, 9, "K");
+    auto r = as_f64(R, 9, "R");
+    auto tt = as_f64(t, 3, "t");
+    auto d = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(depth);
+    if (!d || d.ndim() != 2) throw std::invalid_argument("depth must be a 2-D (H, W) array");
+    const py::ssize_t h = d.shape(0), w = d.shape(1);
+    std::vector<float> nrm;
+    if (!normal.is_none()) {
+      // (3, H*W) as PixelData3f (depthmap_bind.h:135-139), or (H, W, 3)
+      py::array a = py::array::ensure(normal);
+      if (a.ndim() == 2 && a.shape(0) == 3 && a.shape(1) == h * w) {
+        auto f = py::array_t<float, py::array::f_style | py::array::forcecast>::ensure(a);
+        nrm.assign(f.data(), f.data() + 3 * h * w);  // column-major (3, N) = interleaved
+      } else if (a.ndim() == 3 && a.shape(0) == h && a.shape(1) == w && a.shape(2) == 3) {
+        auto cc = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(a);
+        nrm.assign(cc.data(), cc.data() + 3 * h * w);
+      } else {
+        throw std::invalid_argument("normal must be (3, H*W) or (H, W, 3)");
+      }
+    }
+    int idx = -1;
+    check(pm_cleaner_add_view(c_, k.data(), r.data(), tt.data(), d.data(), static_cast<int>(w), static_cast<int>(h),
+                              nrm.empty() ? nullptr : nrm.data(), &idx));
+    return idx;
+  }
+
+  py::array_t<float> clean(int ref_idx, py::handle neighbor_ids) {
+    auto ids = py::array_t<int, py::array::c_style | py::array::forcecast>::ensure(neighbor_ids);
+    if (!ids) throw std::invalid_argument("neighbor_ids must be an integer array");
+    int n = 0;
+    check(pm_cleaner_num_views(c_, &n));
+    if (ref_idx < 0 || ref_idx >= n)
+      throw std::out_of_range("Clean: ref_idx " + std::to_string(ref_idx) + " out of range [0, " + std::to_string(n) +
+                              ")");
+    int w = 0, h = 0;
+    check(pm_cleaner_view_size(c_, ref_idx, &w, &h));
+    py::array_t<float> out(shape({h, w}));
+    std::vector<int> v(ids.data(), ids.data() + ids.size());
+    float* o = out.mutable_data();
+    {
+      py::gil_scoped_release nogil;
+      check(pm_cleaner_clean(c_, ref_idx, v.data(), static_cast<int>(v.size()), o));
+    }
+    return out;
+  }
+  float last_ms() {
+    float ms = 0.0f;
+    check(pm_cleaner_last_timing(c_, &ms));
+    return ms;
+  }
+  void clear() { check(pm_cleaner_clear(c_)); }
+
+ private:
+  void push() { check(pm_cleaner_set_params(c_, same_, min_views_, carve_, max_carved_, grazing_)); }
+  pm_cleaner* c_ = nullptr;
+  float same_ = 0.01f, carve_ = 0.05f, grazing_ = 0.2f;
+  int min_views_ = 3, max_carved_ = 2;
+};
+
 // Per-device memory reservations of reserve_device_memory.
 uint64_t& reserved_bytes(int device) {
   static std::vector<uint64_t> r(64, 0);
@@ -659,6 +743,27 @@ PYBIND11_MODULE(pydense, m) {
   auto set = [](auto field) {
     return [field](C& c, decltype(std::declval<pm_params&>().*field) v) { c.params().*field = v; };
   };
+  py::class_<Cleaner>(m, "GPUDepthmapCleaner")
+      .def(py::init<>())
+      .def("set_same_depth_threshold", &Cleaner::set_same_depth_threshold)
+      .def("set_min_consistent_views", &Cleaner::set_min_consistent_views)
+      .def("set_device", &Cleaner::set_device)
+      .def("set_carving_threshold", &Cleaner::set_carving_threshold)
+      .def("set_max_carved_views", &Cleaner::set_max_carved_views)
+      .def("set_grazing_cos_threshold", &Cleaner::set_grazing_cos_threshold)
+      .def("add_view", [](Cleaner& c, py::handle K, py::handle R, py::handle t, py::handle depth) {
+        return c.add_view(K, R, t, depth, py::none());
+      })
+      .def("add_view_with_normal", [](Cleaner& c, py::handle K, py::handle R, py::handle t, py::handle depth,
+                                      py::object normal) { return c.add_view(K, R, t, depth, normal); })
+      .def("clean", &Cleaner::clean, py::arg("ref_idx"), py::arg("neighbor_ids"))
+      .def("clear", &Cleaner::clear)
+      .def("last_ms", &Cleaner::last_ms, "Device time of the last clean's kernel")
+      .def_static("is_available", []() {
+        int n = 0;
+        return pm_device_count(&n) == PM_OK && n > 0;
+      });
+
   py::class_<Batch>(m, "DepthmapBatch",
                     "Many DepthmapEstimator references in one launch per kernel, captured in a HIP graph")
       .def(py::init<int>(), py::arg("device") = 0)

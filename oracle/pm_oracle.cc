@@ -1599,4 +1599,105 @@ void pmo_run(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Depth-map cleaning: acmmp_clean_depthmap (opencl_kernels.h:2116-2279) as
+// run by GPUDepthmapCleaner::Clean (cleaner.cc:95-242).  src_depths are the
+// neighbours' depth maps already at the reference size (Clean resizes them
+// with cv::resize INTER_LINEAR, cleaner.cc:148-158); cams[0] is the
+// reference, cams[1..n_src] the neighbours (CLCamera of each view's own K,
+// R, t, cleaner.cc:196-201).  Neighbour depths are read bilinearly at the
+// projected pixel-centre coordinate (read_src_depth, :2090-2112); the
+// reference depth at the pixel (nearest sampler).  normal: the reference's
+// camera-frame normals (3 per pixel) or null (has_normal = 0).
+// min_consistent_views is the effective minimum Clean passes,
+// min(min_consistent_views, num_views) (cleaner.cc:222).
+// ---------------------------------------------------------------------------
+void pmo_clean_depthmap(const float* ref_depth, const float* normal, int w, int h, const float* const* src_depths,
+                        const pmo_camera* cams, int n_src_in, float same_depth_threshold, int min_consistent_views,
+                        float carving_threshold, int max_carved_views, float grazing_cos_threshold, float* out) {
+  const int n_src = std::min(n_src_in, kMaxSrc);  // MAX_CLEAN_SOURCES 16 (:2057)
+  const pmo_camera& rc = cams[0];
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < h; ++y) {
+    for (int x = 0; x < w; ++x) {
+      const int idx = y * w + x;
+      const float ref_d = ref_depth[idx];
+      if (!(ref_d > 0.0f)) {
+        out[idx] = 0.0f;
+        continue;
+      }
+      // grazing angle from the normal (:2142-2160)
+      int is_grazing = 0;
+      if (normal) {
+        const float nnx = normal[3 * idx], nny = normal[3 * idx + 1], nnz = normal[3 * idx + 2];
+        const float nlen = std::sqrt(nnx * nnx + nny * nny + nnz * nnz);
+        if (nlen > 1e-6f) {
+          const float ray_x = (static_cast<float>(x) - rc.K[2]) / rc.K[0];
+          const float ray_y = (static_cast<float>(y) - rc.K[5]) / rc.K[4];
+          const float ray_z = 1.0f;
+          const float rlen = std::sqrt(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
+          const float cos_angle = std::fabs((nnx * ray_x + nny * ray_y + nnz * ray_z) / (nlen * rlen));
+          is_grazing = (cos_angle < grazing_cos_threshold) ? 1 : 0;
+        }
+      }
+      const int is_suspicious = is_grazing;
+      const float fx = rc.K[0], cx = rc.K[2], fy = rc.K[4], cy = rc.K[5];
+      // back-project into the world (:2164-2177)
+      const float cam_x = (static_cast<float>(x) - cx) / fx * ref_d;
+      const float cam_y = (static_cast<float>(y) - cy) / fy * ref_d;
+      const float cam_z = ref_d;
+      const float pcx = cam_x - rc.t[0], pcy = cam_y - rc.t[1], pcz = cam_z - rc.t[2];
+      const float wx = rc.R[0] * pcx + rc.R[3] * pcy + rc.R[6] * pcz;
+      const float wy = rc.R[1] * pcx + rc.R[4] * pcy + rc.R[7] * pcz;
+      const float wz = rc.R[2] * pcx + rc.R[5] * pcy + rc.R[8] * pcz;
+      int consistent = 1;
+      int carved = 0;
+      for (int v = 0; v < n_src; ++v) {
+        const pmo_camera& sc = cams[v + 1];
+        const float sx = sc.R[0] * wx + sc.R[1] * wy + sc.R[2] * wz + sc.t[0];
+        const float sy = sc.R[3] * wx + sc.R[4] * wy + sc.R[5] * wz + sc.t[1];
+        const float sz = sc.R[6] * wx + sc.R[7] * wy + sc.R[8] * wz + sc.t[2];
+        if (sz <= 1e-6f) continue;
+        const float su = sc.K[0] * sx / sz + sc.K[2];
+        const float sv = sc.K[4] * sy / sz + sc.K[5];
+        if (su < 0.0f || sv < 0.0f || su >= static_cast<float>(w - 1) || sv >= static_cast<float>(h - 1)) continue;
+        const float src_d = Img{src_depths[v], w, h}.bilinear(su, sv);
+        if (!(src_d > 0.0f)) continue;
+        // the source surface point back in the world (:2212-2226, 2239-2252)
+        const float scx = (su - sc.K[2]) / sc.K[0] * src_d;
+        const float scy = (sv - sc.K[5]) / sc.K[4] * src_d;
+        const float scz = src_d;
+        const float qx = scx - sc.t[0], qy = scy - sc.t[1], qz = scz - sc.t[2];
+        const float wx2 = sc.R[0] * qx + sc.R[3] * qy + sc.R[6] * qz;
+        const float wy2 = sc.R[1] * qx + sc.R[4] * qy + sc.R[7] * qz;
+        const float wz2 = sc.R[2] * qx + sc.R[5] * qy + sc.R[8] * qz;
+        // space carving: the source sees further (:2210-2235)
+        if (src_d > sz * (1.0f + carving_threshold)) {
+          const float rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
+          if (rz > 1e-6f && rz > ref_d * (1.0f + same_depth_threshold)) carved++;
+          continue;
+        }
+        // forward depth check, then back into the reference (:2238-2267)
+        if (std::fabs(src_d - sz) >= sz * same_depth_threshold) continue;
+        const float rx = rc.R[0] * wx2 + rc.R[1] * wy2 + rc.R[2] * wz2 + rc.t[0];
+        const float ry = rc.R[3] * wx2 + rc.R[4] * wy2 + rc.R[5] * wz2 + rc.t[1];
+        const float rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
+        if (rz <= 1e-6f) continue;
+        const float ru = fx * rx / rz + cx;
+        const float rv = fy * ry / rz + cy;
+        if (std::fabs(ru - static_cast<float>(x)) + std::fabs(rv - static_cast<float>(y)) > 2.0f) continue;
+        if (std::fabs(rz - ref_d) >= ref_d * same_depth_threshold) continue;
+        consistent++;
+      }
+      // adaptive decision (:2270-2276)
+      const int eff_min = is_suspicious ? (min_consistent_views + 1) : min_consistent_views;
+      const int eff_max_carve = is_suspicious ? std::max(1, max_carved_views - 1) : max_carved_views;
+      const int carve_budget = eff_max_carve + std::max(0, consistent - eff_min);
+      const bool keep = (consistent >= eff_min) && (carved <= carve_budget);
+      out[idx] = keep ? ref_d : 0.0f;
+    }
+  }
+}
+
 }  // extern "C"

@@ -98,6 +98,7 @@ def lib():
         L.pmo_refinement_hypotheses.argtypes = [vp, vp, vp, i, i, vp, vp, vp, u, vp]
         L.pmo_resize_linear.argtypes = [vp, i, i, i, i, vp]
         L.pmo_upsample.argtypes = [vp, vp, vp, vp, i, i, vp]
+        L.pmo_clean_depthmap.argtypes = [vp, vp, i, i, vp, vp, i, f, i, f, i, f, vp]
         L.pmo_set_threads.argtypes = [i]
         L.pmo_max_threads.restype = i
         _lib = L
@@ -577,3 +578,39 @@ def ncc_energy_f64(scene: OracleScene, params: Params, x: int, y: int, d: float,
     e = lib().pmo_ncc_energy_f64(ctypes.byref(scene.c), ctypes.byref(p), int(x), int(y), float(d), float(a), float(b),
                                  ctypes.c_uint32(int(views)), ctypes.byref(used))
     return float(e), int(used.value)
+
+
+def clean_depthmap(ref_depth: np.ndarray, ref_normal, views, same_depth_threshold: float = 0.01,
+                   min_consistent_views: int = 3, carving_threshold: float = 0.05, max_carved_views: int = 2,
+                   grazing_cos_threshold: float = 0.2) -> np.ndarray:
+    """GPUDepthmapCleaner::Clean for one reference (cleaner.cc:95-242).
+
+    ``views``: [(K, R, t)] of the reference then [(K, R, t, depth)] of each
+    neighbour -- given as a list whose first entry is (K, R, t) and the rest
+    (K, R, t, depth).  ``ref_normal``: (H, W, 3) camera-frame normals or None.
+    Neighbour depth maps (and the normal) of another size are resized to the
+    reference size with the INTER_LINEAR restatement, as Clean does.
+    """
+    ref = np.ascontiguousarray(ref_depth, np.float32)
+    h, w = ref.shape
+    normal = None
+    if ref_normal is not None:
+        nrm = np.asarray(ref_normal, np.float32).reshape(ref_normal.shape[0], ref_normal.shape[1], 3)
+        if nrm.shape[:2] != (h, w):
+            nrm = np.stack([resize_linear(nrm[..., c], w, h) for c in range(3)], -1)
+        normal = np.ascontiguousarray(nrm, np.float32)
+    K0, R0, t0 = views[0][:3]
+    cams = (PmoCamera * len(views))()
+    cams[0] = make_camera(K0, R0, t0, w, h)
+    srcs = []
+    for j, (K, R, t, d) in enumerate(views[1:]):
+        cams[j + 1] = make_camera(K, R, t, w, h)
+        d = np.asarray(d, np.float32)
+        srcs.append(np.ascontiguousarray(d) if d.shape == (h, w) else resize_linear(d, w, h))
+    ptrs = (ctypes.c_void_p * max(len(srcs), 1))(*[a.ctypes.data for a in srcs])
+    n = len(srcs)
+    out = np.zeros((h, w), np.float32)
+    lib().pmo_clean_depthmap(_ptr(ref), _ptr(normal), w, h, ptrs, cams, n, float(same_depth_threshold),
+                             int(min(min_consistent_views, n + 1)), float(carving_threshold), int(max_carved_views),
+                             float(grazing_cos_threshold), _ptr(out))
+    return out

@@ -95,6 +95,10 @@ def test_pydense_surface_names():
         assert hasattr(cls, m), m
     for m in ("add", "set_config", "config", "tune_log", "prepare", "run", "reset_tuning"):
         assert hasattr(pd.DepthmapBatch, m), m
+    for m in ("set_same_depth_threshold", "set_min_consistent_views", "set_device", "set_carving_threshold",
+              "set_max_carved_views", "set_grazing_cos_threshold", "add_view", "add_view_with_normal", "clean",
+              "clear", "is_available"):     # python/pybind.cc:125-143
+        assert hasattr(pd.GPUDepthmapCleaner, m), m
 
 
 @pytest.mark.gpu
