@@ -77,7 +77,7 @@ def build_pydense() -> str:
     cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
            f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
            f"-I{os.path.join(ROOT, 'include')}",
-           os.path.join(CSRC, "pydense.cpp"), "-o", tmp,
+           os.path.join(CSRC, "pydense.cpp"), os.path.join(CSRC, "lz4_frame.cpp"), "-o", tmp,
            f"-L{HERE}", "-lopensfm_pm", "-Wl,-rpath,$ORIGIN"]
     _run(cmd)
     os.replace(tmp, out)

@@ -17,6 +17,7 @@
 #include <utility>
 #include <vector>
 
+#include "lz4_frame.h"
 #include "opensfm_pm.h"
 
 namespace py = pybind11;
@@ -743,6 +744,47 @@ PYBIND11_MODULE(pydense, m) {
   auto set = [](auto field) {
     return [field](C& c, decltype(std::declval<pm_params&>().*field) v) { c.params().*field = v; };
   };
+  // LZ4 frames of the depth-map files (dataset.py:1416-1452).
+  m.def(
+      "lz4_frame_compress",
+      [](py::buffer data, int block_max_id, bool content_checksum) {
+        py::buffer_info bi = data.request();
+        const auto* p = static_cast<const uint8_t*>(bi.ptr);
+        const size_t n = static_cast<size_t>(bi.size * bi.itemsize);
+        std::vector<uint8_t> out;
+        {
+          py::gil_scoped_release nogil;
+          out = lz4f::compress_frame(p, n, block_max_id, content_checksum);
+        }
+        return py::bytes(reinterpret_cast<const char*>(out.data()), out.size());
+      },
+      py::arg("data"), py::arg("block_max_id") = 7, py::arg("content_checksum") = false,
+      "One LZ4 frame of data (independent blocks of 64 KB << 2*(block_max_id - 4), content size stored)");
+  m.def(
+      "lz4_frame_decompress",
+      [](py::buffer data) {
+        py::buffer_info bi = data.request();
+        const auto* p = static_cast<const uint8_t*>(bi.ptr);
+        const size_t n = static_cast<size_t>(bi.size * bi.itemsize);
+        std::vector<uint8_t> out;
+        std::string err;
+        bool ok;
+        {
+          py::gil_scoped_release nogil;
+          ok = lz4f::decompress_frames(p, n, out, err);
+        }
+        if (!ok) throw std::runtime_error(err);
+        return py::bytes(reinterpret_cast<const char*>(out.data()), out.size());
+      },
+      py::arg("data"), "Content of one or more concatenated LZ4 frames");
+  m.def(
+      "xxh32",
+      [](py::buffer data, uint32_t seed) {
+        py::buffer_info bi = data.request();
+        return lz4f::xxh32(static_cast<const uint8_t*>(bi.ptr), static_cast<size_t>(bi.size * bi.itemsize), seed);
+      },
+      py::arg("data"), py::arg("seed") = 0);
+
   py::class_<Cleaner>(m, "GPUDepthmapCleaner")
       .def(py::init<>())
       .def("set_same_depth_threshold", &Cleaner::set_same_depth_threshold)
