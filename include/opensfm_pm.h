@@ -272,6 +272,42 @@ int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches);
  * cluster_estimator.cc:150). */
 int pm_device_bytes(pm_handle* h, uint64_t* bytes);
 
+/* ---- SfM planar prior and neighbour selection --------------------------
+ * Host-side steps around the PatchMatch path. */
+
+/* SfM points (num_points x 3 float64, world frame) of the reference:
+ * SetSfMPoints (estimator.cc:118-132).  Each pyramid level then builds its
+ * planar prior from them (ComputeSfMPlanarPrior, estimator.cc:838-1107) and
+ * PatchMatch re-seeds the masked pixels from it, as with pm_set_prior
+ * (which this replaces; points = NULL / num_points = 0 clears them). */
+int pm_set_sfm_points(pm_handle* h, const double* points, int num_points);
+/* ComputeSfMPlanarPrior for a w x h image with intrinsics K (float64
+ * row-major, of that image size), R, t: prior_planes (4 floats per pixel,
+ * (0, 0, -1, 0) where unmasked) and masks (1 where a triangle's plane gives
+ * a depth in [depth_min, depth_max]); *num_triangles = triangles kept after
+ * the depth-ratio (1.5) and area (20000 px) gates.  The triangulation is a
+ * Delaunay triangulation in canonical order (cv::Subdiv2D in the
+ * reference). */
+int pm_sfm_planar_prior(const double* K, const double* R, const double* t, int width, int height, float depth_min,
+                        float depth_max, const double* points, int num_points, float* prior_planes, uint32_t* masks,
+                        int* num_triangles);
+/* Delaunay triangulation of n 2-D points (xy interleaved), canonical order
+ * (vertex indices ascending, triangles sorted); writes min(capacity,
+ * count) triangles and sets *count.  No counterpart in the reference (it
+ * calls cv::Subdiv2D); exposed for testing the prior. */
+int pm_delaunay(const double* xy, int n, int* triangles, int capacity, int* count);
+/* SelectNeighbors (opensfm/src/lib/sfm/src/dense_helpers.cc:331-568):
+ * pair scores of num_shots cameras (origins 3 x float64 each) over
+ * num_points super-points (centroid, track count, the shots that see it:
+ * vis_shots[vis_offsets[p] .. vis_offsets[p + 1])).  best: num_shots x
+ * (1 + num_neighbors) shot indices (self first, then by score), all_out:
+ * num_shots x num_shots (best, then every other shot sharing more than
+ * min_point_all tracks), both -1 padded, with counts. */
+int pm_select_neighbors(int num_shots, const double* origins, int num_points, const double* coords,
+                        const int* vis_offsets, const int* vis_shots, const int* track_counts, int num_neighbors,
+                        int min_point_best, int min_point_all, double theta_min_deg, double theta_max_deg, int* best,
+                        int* best_counts, int* all_out, int* all_counts);
+
 /* ---- Depth-map cleaner ----------------------------------------------
  * Cross-view depth consistency with space carving and a grazing-angle rule:
  * GPUDepthmapCleaner (opensfm/src/lib/dense/cleaner.h:21-102, cleaner.cc)

@@ -29,7 +29,8 @@ HIP_FLAGS = [
 
 
 # HIP translation units of the library, compiled in parallel.
-SOURCES = ("pm_kernels.hip", "pm_cost_f32.hip", "pm_cost_f16.hip", "pm_refine.hip", "pm_clean.hip", "pm_api.hip")
+SOURCES = ("pm_kernels.hip", "pm_cost_f32.hip", "pm_cost_f16.hip", "pm_refine.hip", "pm_clean.hip", "pm_prior.hip",
+           "pm_api.hip")
 
 
 def pydense_path() -> str:

@@ -142,6 +142,7 @@ struct pm_handle {
   int next_level = 0;        // next level pm_run_level accepts
   std::vector<float> prior_planes;
   std::vector<uint32_t> prior_masks;
+  std::vector<double> sfm_points;  // 3 per point, world frame (SetSfMPoints)
 
   // device state
   bool uploaded = false;
@@ -558,6 +559,7 @@ int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks)
   ++h->gen;
   if (h->views.empty()) return fail(PM_E_INVALID, "add the reference view before the prior");
   const size_t npix = static_cast<size_t>(h->views[0].w) * h->views[0].h;
+  h->sfm_points.clear();
   if (!prior_planes || !masks) {
     h->prior_planes.clear();
     h->prior_masks.clear();
@@ -565,6 +567,17 @@ int pm_set_prior(pm_handle* h, const float* prior_planes, const uint32_t* masks)
     h->prior_planes.assign(prior_planes, prior_planes + 4 * npix);
     h->prior_masks.assign(masks, masks + npix);
   }
+  h->uploaded = false;
+  return PM_OK;
+}
+
+int pm_set_sfm_points(pm_handle* h, const double* points, int num_points) {
+  if (int r = check_handle(h)) return r;
+  if (num_points < 0 || (num_points > 0 && !points)) return fail(PM_E_INVALID, "set_sfm_points: bad arguments");
+  ++h->gen;
+  h->prior_planes.clear();
+  h->prior_masks.clear();
+  h->sfm_points.assign(points, points + 3 * static_cast<size_t>(num_points));
   h->uploaded = false;
   return PM_OK;
 }
@@ -925,22 +938,31 @@ int upload_level(pm_handle* h, int level, int nlevels) {
     PM_HIP(launch_quads(qa, nv, h->f16, h->stream), "quad kernel launch");
   }
   // Prior (estimator.cc:480-486) and previous depths (:499-508, 66-116).
-  if (!h->prior_masks.empty()) {
+  if (!h->prior_masks.empty() || !h->sfm_points.empty()) {
     if (int r = h->prior.alloc(npix)) return r;
     if (int r = h->masks.alloc(npix)) return r;
     if (int r = h->snapshot.alloc(npix)) return r;
-    // Planes are in the reference camera frame, so a coarse level takes the
-    // prior of the full-resolution pixel at the centre of its footprint.
     std::vector<float> pp(4 * npix);
     std::vector<uint32_t> pm(npix);
-    const int fw = h->views[0].w, fh = h->views[0].h;
-    for (int y = 0; y < h->H; ++y) {
-      const int fy = std::min(static_cast<int>((y + 0.5) * fh / h->H), fh - 1);
-      for (int x = 0; x < h->W; ++x) {
-        const int fx = std::min(static_cast<int>((x + 0.5) * fw / h->W), fw - 1);
-        const size_t si = static_cast<size_t>(fy) * fw + fx, di = static_cast<size_t>(y) * h->W + x;
-        std::memcpy(&pp[4 * di], &h->prior_planes[4 * si], 4 * sizeof(float));
-        pm[di] = h->prior_masks[si];
+    if (!h->sfm_points.empty()) {
+      // built for this level from the SfM points (ComputeSfMPlanarPrior per
+      // level, estimator.cc:214-218)
+      const HostView& v0 = views[0];
+      if (int r = pm_sfm_planar_prior(v0.K, v0.R, v0.t, h->W, h->H, p.depth_min, p.depth_max, h->sfm_points.data(),
+                                      static_cast<int>(h->sfm_points.size() / 3), pp.data(), pm.data(), nullptr))
+        return r;
+    } else {
+      // Planes are in the reference camera frame, so a coarse level takes the
+      // prior of the full-resolution pixel at the centre of its footprint.
+      const int fw = h->views[0].w, fh = h->views[0].h;
+      for (int y = 0; y < h->H; ++y) {
+        const int fy = std::min(static_cast<int>((y + 0.5) * fh / h->H), fh - 1);
+        for (int x = 0; x < h->W; ++x) {
+          const int fx = std::min(static_cast<int>((x + 0.5) * fw / h->W), fw - 1);
+          const size_t si = static_cast<size_t>(fy) * fw + fx, di = static_cast<size_t>(y) * h->W + x;
+          std::memcpy(&pp[4 * di], &h->prior_planes[4 * si], 4 * sizeof(float));
+          pm[di] = h->prior_masks[si];
+        }
       }
     }
     PM_HIP(hipMemcpyAsync(h->prior.p, pp.data(), npix * sizeof(float4), hipMemcpyHostToDevice, h->stream),
@@ -1963,7 +1985,7 @@ int pm_device_bytes(pm_handle* h, uint64_t* bytes) {
   total += rp * (P * 8 + 3 * 16 + 9 * ns * 4 + ns * 4 + 4 + 8 * 4 + 4 + 6 * 16);
   total += static_cast<uint64_t>(fallback_mask_count(static_cast<int>(w), static_cast<int>(hh), static_cast<int>(ns))) * 8;
   total += npix * (16 + 4 + 8 + 4 + 4 + 12 + 4 + 4);
-  if (!h->prior_masks.empty()) total += npix * (16 + 4 + 16);
+  if (!h->prior_masks.empty() || !h->sfm_points.empty()) total += npix * (16 + 4 + 16);
   for (const auto& e : h->prev_depths)
     if (!e.d.empty()) { total += ns * npix * 4; break; }
   *bytes = total;

@@ -64,6 +64,14 @@ void add_view_to(pm_handle* h, py::handle K, py::handle R, py::handle t, py::arr
                     static_cast<int>(img.shape(0)), 0, m.empty() ? nullptr : m.data()));
 }
 
+// (N, 3) float64 world points -> pm_set_sfm_points.
+void set_points(pm_handle* h, py::handle points) {
+  auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(points);
+  if (!a || (a.size() > 0 && (a.ndim() != 2 || a.shape(1) != 3)))
+    throw std::invalid_argument("SfM points must be an (N, 3) array");
+  check(pm_set_sfm_points(h, a.size() ? a.data() : nullptr, static_cast<int>(a.size() / 3)));
+}
+
 class Estimator {
  public:
   explicit Estimator(int device) { check(pm_create(device, &h_)); }
@@ -511,12 +519,11 @@ class ClusterEstimator {
     add_view_to(refs_.back().h, K, R, t, image, py::none());
   }
 
+  // SetSfMPoints of the current reference (cluster_estimator.cc; called
+  // per reference by depthmaps.py:235-242).
   void set_sfm_points(py::array points) {
     if (refs_.empty()) throw std::runtime_error("DepthmapClusterEstimator: no ref view started");
-    if (points.size() > 0)
-      throw std::runtime_error(
-          "DepthmapClusterEstimator: the SfM planar prior is not available; pass prior planes to "
-          "DepthmapEstimator.set_prior instead");
+    set_points(refs_.back().h, points);
   }
 
   void add_geom_link(int ref_idx, int source_view_idx, int from_ref_idx) {
@@ -696,6 +703,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("set_gap_max_size", [](Estimator& e, int v) { e.update([v](pm_params& p) { p.gap_max_size = v; }); })
       .def("set_geom_consistency", &Estimator::set_geom_consistency, py::arg("weight"), py::arg("prev_depths"))
       .def("set_prior", &Estimator::set_prior, py::arg("prior_planes"), py::arg("masks"))
+      .def("set_sfm_points", [](Estimator& e, py::handle pts) { set_points(e.handle(), pts); }, py::arg("points"),
+           "SfM points (N, 3) of the reference: planar prior per level (ComputeSfMPlanarPrior)")
       .def("params", &Estimator::params)
       .def("compute_patch_match", &Estimator::compute_patch_match,
            "Run PatchMatch; returns [depth (H,W), plane normal (H,W,3), score (H,W), selected-view bitmask (H,W)]")
@@ -744,6 +753,78 @@ PYBIND11_MODULE(pydense, m) {
   auto set = [](auto field) {
     return [field](C& c, decltype(std::declval<pm_params&>().*field) v) { c.params().*field = v; };
   };
+  m.def(
+      "sfm_planar_prior",
+      [](py::handle K, py::handle R, py::handle t, int w, int h, float dmin, float dmax, py::handle points) {
+        auto k = as_f64(K, 9, "K");
+        auto r = as_f64(R, 9, "R");
+        auto tt = as_f64(t, 3, "t");
+        auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(points);
+        if (!a || (a.size() > 0 && (a.ndim() != 2 || a.shape(1) != 3)))
+          throw std::invalid_argument("points must be an (N, 3) array");
+        py::array_t<float> planes(shape({h, w, 4}));
+        py::array_t<uint32_t> masks(shape({h, w}));
+        int ntri = 0;
+        check(pm_sfm_planar_prior(k.data(), r.data(), tt.data(), w, h, dmin, dmax, a.size() ? a.data() : nullptr,
+                                  static_cast<int>(a.size() / 3), planes.mutable_data(), masks.mutable_data(), &ntri));
+        return py::make_tuple(planes, masks, ntri);
+      },
+      py::arg("K"), py::arg("R"), py::arg("t"), py::arg("width"), py::arg("height"), py::arg("depth_min"),
+      py::arg("depth_max"), py::arg("points"),
+      "ComputeSfMPlanarPrior of one image: (prior planes (H, W, 4), masks (H, W), triangles kept)");
+  m.def(
+      "delaunay",
+      [](py::handle xy) {
+        auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(xy);
+        if (!a || (a.size() > 0 && (a.ndim() != 2 || a.shape(1) != 2)))
+          throw std::invalid_argument("points must be an (N, 2) array");
+        const int n = static_cast<int>(a.size() / 2);
+        int count = 0;
+        check(pm_delaunay(a.data(), n, nullptr, 0, &count));
+        py::array_t<int> tris(shape({count, 3}));
+        check(pm_delaunay(a.data(), n, tris.mutable_data(), count, &count));
+        return tris;
+      },
+      py::arg("xy"), "Delaunay triangles of (N, 2) points, canonical order");
+  m.def(
+      "select_neighbors",
+      [](py::handle origins, py::handle coords, const std::vector<std::vector<int>>& vis,
+         const std::vector<int>& track_counts, int num_neighbors, int min_point_best, int min_point_all,
+         double theta_min_deg, double theta_max_deg) {
+        auto o = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(origins);
+        auto c = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(coords);
+        if (!o || !c) throw std::invalid_argument("origins / coords must be float arrays");
+        const int N = static_cast<int>(o.size() / 3), P = static_cast<int>(c.size() / 3);
+        if (static_cast<int>(vis.size()) != P || static_cast<int>(track_counts.size()) != P)
+          throw std::invalid_argument("one visibility list and one track count per super-point");
+        std::vector<int> off(1, 0), shots;
+        for (const auto& v : vis) {
+          shots.insert(shots.end(), v.begin(), v.end());
+          off.push_back(static_cast<int>(shots.size()));
+        }
+        std::vector<int> best(static_cast<size_t>(N) * (1 + num_neighbors)), bc(N), all(static_cast<size_t>(N) * N),
+            ac(N);
+        {
+          py::gil_scoped_release nogil;
+          check(pm_select_neighbors(N, o.data(), P, c.data(), off.data(), shots.data(), track_counts.data(),
+                                    num_neighbors, min_point_best, min_point_all, theta_min_deg, theta_max_deg,
+                                    best.data(), bc.data(), all.data(), ac.data()));
+        }
+        py::list b, a;
+        for (int i = 0; i < N; ++i) {
+          py::list bi, ai;
+          for (int k = 0; k < bc[i]; ++k) bi.append(best[static_cast<size_t>(i) * (1 + num_neighbors) + k]);
+          for (int k = 0; k < ac[i]; ++k) ai.append(all[static_cast<size_t>(i) * N + k]);
+          b.append(bi);
+          a.append(ai);
+        }
+        return py::make_tuple(b, a);
+      },
+      py::arg("origins"), py::arg("coords"), py::arg("visibility"), py::arg("track_counts"),
+      py::arg("num_neighbors"), py::arg("min_point_best"), py::arg("min_point_all"), py::arg("theta_min_deg"),
+      py::arg("theta_max_deg"),
+      "SelectNeighbors: (best, all) neighbour index lists per shot, self first");
+
   // LZ4 frames of the depth-map files (dataset.py:1416-1452).
   m.def(
       "lz4_frame_compress",

@@ -614,3 +614,149 @@ def clean_depthmap(ref_depth: np.ndarray, ref_normal, views, same_depth_threshol
                              int(min(min_consistent_views, n + 1)), float(carving_threshold), int(max_carved_views),
                              float(grazing_cos_threshold), _ptr(out))
     return out
+
+
+def sfm_planar_prior(K, R, t, w: int, h: int, depth_min: float, depth_max: float, points):
+    """ComputeSfMPlanarPrior (estimator.cc:838-1107) restated with numpy:
+    Delaunay by Qhull (scipy.spatial), planes from the float64 SVD null
+    vector of [X 1], the parametric rasteriser in float32.  Triangles are
+    rasterised in canonical order (vertex indices ascending, sorted) --
+    cv::Subdiv2D's order is not reproducible.  Returns (planes (h, w, 4),
+    masks (h, w) uint32, triangles kept)."""
+    from scipy.spatial import Delaunay
+    K = np.asarray(K, np.float64); R = np.asarray(R, np.float64); t = np.asarray(t, np.float64)
+    f32 = np.float32
+    fx, fy, cx, cy = f32(K[0, 0]), f32(K[1, 1]), f32(K[0, 2]), f32(K[1, 2])
+    planes = np.zeros((h, w, 4), np.float32)
+    planes[..., 2] = -1.0
+    masks = np.zeros((h, w), np.uint32)
+    proj = []
+    for X in np.asarray(points, np.float64).reshape(-1, 3):
+        xc, yc, zc = R @ X + t
+        if zc <= 0.0:
+            continue
+        depth = f32(zc)
+        if depth < depth_min or depth > depth_max:
+            continue
+        u = f32(float(fx) * (xc / zc) + float(cx))
+        v = f32(float(fy) * (yc / zc) + float(cy))
+        if u < 0 or u >= w or v < 0 or v >= h:
+            continue
+        proj.append((u, v, depth, np.array([xc, yc, zc], np.float32)))
+    if len(proj) < 4:
+        return planes, masks, 0
+    uv = np.array([(p[0], p[1]) for p in proj], np.float64)
+    key = {int(p[1]) * w + int(p[0]): i for i, p in enumerate(proj)}
+    simp = Delaunay(uv).simplices
+    tris = sorted(tuple(sorted(int(i) for i in s)) for s in simp)
+    kept = []
+    for tri in tris:
+        pts = [(int(uv[i, 0]), int(uv[i, 1])) for i in tri]
+        if any(not (0 <= x < w and 0 <= y < h) for x, y in pts):
+            continue
+        if any(y * w + x not in key for x, y in pts):
+            continue
+        pi = [key[y * w + x] for x, y in pts]
+        ds = [proj[i][2] for i in pi]
+        if min(ds) > 0 and f32(max(ds)) / f32(min(ds)) > f32(1.5):
+            continue
+        ax, ay = f32(pts[1][0] - pts[0][0]), f32(pts[1][1] - pts[0][1])
+        bx, by = f32(pts[2][0] - pts[0][0]), f32(pts[2][1] - pts[0][1])
+        if f32(0.5) * abs(ax * by - ay * bx) > f32(20000.0):
+            continue
+        if len(set(pi)) < 3:
+            # two vertices in one pixel map to one SfM point (the key map
+            # keeps the last point of a pixel, :914-917): the reference takes
+            # an arbitrary null vector of a rank-2 matrix there; skipped here
+            # and in the library
+            continue
+        A = np.array([[*proj[i][3].astype(np.float64), 1.0] for i in pi])
+        B = np.linalg.svd(A)[2][3]
+        nrm = np.linalg.norm(B[:3])
+        if nrm < 1e-8:
+            continue
+        B = B / (-nrm if B[3] < 0 else nrm)
+        kept.append((pts, B.astype(np.float32)))
+    owner = np.zeros((h, w), np.int64)
+    for ti, (pts, _) in enumerate(kept):
+        (x1, y1), (x2, y2), (x3, y3) = [(f32(x), f32(y)) for x, y in pts]
+        L = [np.hypot(f32(pts[a][0] - pts[b][0]), f32(pts[a][1] - pts[b][1])).astype(np.float32)
+             for a, b in ((0, 1), (0, 2), (1, 2))]
+        max_edge = max(L)
+        if max_edge < 1.0:
+            continue
+        step = f32(1.0) / f32(max_edge)
+        p = f32(0.0)
+        qs = [f32(0.0)]
+        while qs[-1] < 1.0:
+            qs.append(f32(qs[-1] + step))
+        qs = np.array(qs, np.float32)
+        while p < f32(1.0):
+            q = qs[qs < f32(f32(1.0) - p)]
+            r = (f32(1.0) - p) - q
+            px = ((p * x1 + q * x2) + r * x3).astype(np.float32).astype(np.int64)
+            py = ((p * y1 + q * y2) + r * y3).astype(np.float32).astype(np.int64)
+            ok = (px >= 0) & (px < w) & (py >= 0) & (py < h)
+            owner[py[ok], px[ok]] = ti + 1
+            p = f32(p + step)
+    ys, xs = np.nonzero(owner)
+    for y, x in zip(ys, xs):
+        pl = kept[owner[y, x] - 1][1]
+        rx = (f32(x) - cx) / fx
+        ry = (f32(y) - cy) / fy
+        denom = f32(f32(pl[0] * rx) + f32(pl[1] * ry)) + f32(pl[2] * f32(1.0))
+        d = -pl[3] / denom if abs(denom) > 1e-6 else f32(0.0)
+        if depth_min <= d <= depth_max:
+            planes[y, x] = pl
+            masks[y, x] = 1
+    return planes, masks, len(kept)
+
+
+def select_neighbors(origins, coords, visibility, track_counts, num_neighbors: int, min_point_best: int,
+                     min_point_all: int, theta_min_deg: float, theta_max_deg: float):
+    """SelectNeighbors (dense_helpers.cc:331-568) in plain Python; ties of
+    equal score ordered by shot index, the 'all' extras by index."""
+    import math
+    N = len(origins)
+    tmin, tmax = math.radians(theta_min_deg), math.radians(theta_max_deg)
+    cos_lo, cos_hi = math.cos(tmax), math.cos(tmin)
+    per = [dict() for _ in range(N)]
+    for P, vis, nt in zip(coords, visibility, track_counts):
+        cams = [c for c in vis if 0 <= c < N]
+        if len(cams) < 2:
+            continue
+        vs = [np.asarray(origins[c], float) - np.asarray(P, float) for c in cams]
+        ns = [float(np.linalg.norm(v)) for v in vs]
+        for i in range(len(cams)):
+            if ns[i] < 1e-12:
+                continue
+            for j in range(i + 1, len(cams)):
+                if ns[j] < 1e-12:
+                    continue
+                c = min(1.0, max(-1.0, float(vs[i] @ vs[j]) / (ns[i] * ns[j])))
+                good = cos_lo < c < cos_hi
+                for a, b in ((cams[i], cams[j]), (cams[j], cams[i])):
+                    s = per[a].setdefault(b, [0, 0, 0.0])
+                    s[0] += nt if good else 0
+                    s[1] += nt
+                    s[2] += c * nt
+    best, allv = [], []
+    for i in range(N):
+        nb, na = [], []
+        for o, (sb, cnt, sc) in per[i].items():
+            if cnt == 0:
+                continue
+            if cnt > min_point_all:
+                na.append(o)
+            if cnt <= min_point_best:
+                continue
+            avg = math.acos(min(1.0, max(-1.0, sc / cnt)))
+            if avg < tmin or avg > tmax:
+                continue
+            if sb > min_point_best:
+                nb.append((-sb, o))
+        nb.sort()
+        b = [i] + [o for _, o in nb[:num_neighbors]]
+        best.append(b)
+        allv.append(b + [o for o in sorted(na) if o not in set(b)])
+    return best, allv

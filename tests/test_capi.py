@@ -99,6 +99,9 @@ def test_pydense_surface_names():
               "set_max_carved_views", "set_grazing_cos_threshold", "add_view", "add_view_with_normal", "clean",
               "clear", "is_available"):     # python/pybind.cc:125-143
         assert hasattr(pd.GPUDepthmapCleaner, m), m
+    for f in ("sfm_planar_prior", "delaunay", "select_neighbors", "lz4_frame_compress", "lz4_frame_decompress"):
+        assert hasattr(pd, f), f
+    assert hasattr(pd.DepthmapEstimator, "set_sfm_points") and hasattr(pd.DepthmapClusterEstimator, "set_sfm_points")
 
 
 @pytest.mark.gpu
