@@ -85,7 +85,8 @@ def cpu_baseline(scene, iters: int, rows: int):
     dt = time.perf_counter() - t0
     units = sc.width * sc.height / 1e6 * band.num_src * iters
     return {"value": units / dt, "unit": UNIT, "cores": po.max_threads(), "kind": "port",
-            "sample": f"reference rows {y0}..{y0 + rows} of the same 1600x1200 scene ({sc.width}x{rows} px, "
+            "sample": f"reference rows {y0}..{y0 + rows} of the same {sc.width}x{scene.gt_depth.shape[0]} scene "
+                      f"({sc.width}x{rows} px, "
                       f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
 
 
