@@ -220,12 +220,17 @@ int pm_last_stage_timing(pm_handle* h, int stage, float* ms, int* launches);
 /* Colour-tile widths of the kernels (tiles are width x 64/width pixels of
  * one colour): cost_tile_x for the cost kernels and the fallback (4, 8 or
  * 16), pixel_tile_x for the per-pixel kernels (16 or 32); 0 = the default
- * (16 for float32 images, 4 for float16; 32).  Results do not depend on
+ * (4; 32).  Results do not depend on
  * them.  pm_get_tiles returns the widths in effect.  No counterpart in the
  * reference (its work-group shape is fixed by the OpenCL launch,
  * estimator.cc:619-680). */
 int pm_set_tiles(pm_handle* h, int cost_tile_x, int pixel_tile_x);
 int pm_get_tiles(pm_handle* h, int* cost_tile_x, int* pixel_tile_x);
+/* At most waves_per_simd waves of the textureless-fallback kernel per SIMD
+ * (1..8; 0 or -1 = no cap, the default).  Results do
+ * not depend on it; a batch takes its first reference's setting.  No
+ * counterpart in the reference. */
+int pm_set_fallback_occupancy(pm_handle* h, int waves_per_simd);
 
 /* ---- Batched runs ----------------------------------------------------
  * A batch runs many references (each a pm_handle with its views, parameters

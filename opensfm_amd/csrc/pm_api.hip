@@ -185,6 +185,7 @@ struct pm_handle {
   // (0 = by image storage: see default_cost_tile).
   int cost_tile_x = 0;
   int pixel_tile_x = 0;
+  int fallback_waves = -1;  // occupancy cap of the fallback kernel (-1: default, 0: none)
   // Bumped by every change of views, parameters or device buffers, so a
   // batch knows when its argument tables are stale.
   uint64_t gen = 0;
@@ -212,14 +213,14 @@ int check_handle(pm_handle* h) {
 }
 
 // Default colour-tile widths, measured on MI355X at 1600x1200 with 8
-// sources: the cost kernels and the fallback run fastest with 16-wide
-// tiles on float32 images and 4-wide tiles on float16 images; the
-// per-pixel kernels with 32-wide tiles.
-int cost_tile(const pm_handle* h) {
-  if (h->cost_tile_x) return h->cost_tile_x;
-  return h->params.image_storage == PM_STORE_F16 ? 4 : 16;
-}
+// sources: the cost kernels and the fallback run fastest with 4-wide tiles
+// (fallback 33.5 ms against 46.2 ms at 16 on float32 images, about equal on
+// float16), the per-pixel kernels with 32-wide tiles.
+int cost_tile(const pm_handle* h) { return h->cost_tile_x ? h->cost_tile_x : 4; }
 int pixel_tile(const pm_handle* h) { return h->pixel_tile_x ? h->pixel_tile_x : 32; }
+// Occupancy cap of the fallback (waves per SIMD); by default none: capping
+// at 2 or 3 waves was slower in every measured configuration.
+int fallback_cap(const pm_handle* h) { return h->fallback_waves > 0 ? h->fallback_waves : 0; }
 
 PmArgs make_args(pm_handle* h) {
   PmArgs a{};
@@ -1086,12 +1087,12 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
     if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
                       [&] { return launch_candidates(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
-                      [&] { return launch_fallback(a, hp, f16, 0, cost_tile(h), st); })) return r;
+                      [&] { return launch_fallback(a, hp, f16, 0, cost_tile(h), fallback_cap(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_SELECT, "select kernel launch", [&] { return launch_select(a, pixel_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_REFINE_COST, "refine-cost kernel launch",
                       [&] { return launch_refine_cost(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
-                      [&] { return launch_fallback(a, hp, f16, 1, cost_tile(h), st); })) return r;
+                      [&] { return launch_fallback(a, hp, f16, 1, cost_tile(h), fallback_cap(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_REFINE, "refine kernel launch", [&] { return launch_refine(a, pixel_tile(h), st); })) return r;
   }
   return PM_OK;
@@ -1440,6 +1441,14 @@ int pm_set_tiles(pm_handle* h, int cost_tile_x, int pixel_tile_x) {
   return PM_OK;
 }
 
+int pm_set_fallback_occupancy(pm_handle* h, int waves_per_simd) {
+  if (int r = check_handle(h)) return r;
+  if (waves_per_simd < -1 || waves_per_simd > 8) return fail(PM_E_INVALID, "fallback occupancy: -1, 0 or 1..8");
+  ++h->gen;
+  h->fallback_waves = waves_per_simd;
+  return PM_OK;
+}
+
 int pm_get_tiles(pm_handle* h, int* cost_tile_x, int* pixel_tile_x) {
   if (int r = check_handle(h)) return r;
   if (cost_tile_x) *cost_tile_x = cost_tile(h);
@@ -1468,6 +1477,7 @@ struct pm_batch {
   int want_cost_x = 0, want_pixel_x = 0;
   // Configuration in use.
   int cost_x = 4, pixel_x = 32;
+  int fallback_waves = 0;  // the first reference's fallback occupancy cap
   float tune_ms = 0.0f, last_ms = 0.0f;
   std::vector<float> tune_log;  // per trial: cost_x, pixel_x, 6 stage times (ms)
   // Prepared state.
@@ -1563,7 +1573,8 @@ int batch_launch(pm_batch* b, int kernel, int slice, int nrefs, int tile_x) {
     }
     PM_HIP(hipEventRecord(b->kev[b->kev_used], b->stream), "event");
   }
-  PM_HIP(launch_batch(kernel, v, grid, b->hp, b->f16, b->max_src, tile_x, b->stream), "batched kernel launch");
+  PM_HIP(launch_batch(kernel, v, grid, b->hp, b->f16, b->max_src, tile_x, b->fallback_waves, b->stream),
+         "batched kernel launch");
   if (stage >= 0) {
     PM_HIP(hipEventRecord(b->kev[b->kev_used + 1], b->stream), "event");
     b->kev_marks.emplace_back(stage, static_cast<int>(b->kev_used));
@@ -1739,6 +1750,7 @@ int batch_prepare(pm_batch* b) {
   b->hp = p0.patch_size / 2;
   b->f16 = p0.image_storage == PM_STORE_F16;
   b->iters = p0.max_iterations;
+  b->fallback_waves = fallback_cap(b->refs[0]);
   b->max_src = 0;
   for (pm_handle* h : b->refs) b->max_src = std::max(b->max_src, h->n_src);
   hipDeviceProp_t prop;

@@ -14,6 +14,8 @@
 // 664-668, 740-746), which makes an in-place pass equal to the reference's
 // snapshot pass.
 #pragma once
+#include <type_traits>
+
 #include "pm_kernels.h"
 
 namespace pmhip {
@@ -194,12 +196,14 @@ __device__ __forceinline__ bool plane_points(const PmCam& c0, int x, int y, Plan
 
 // Stride-1 per-view costs of one plane (opencl_kernels.h:381-557 without
 // the textureless fallback).  Views run in chunks of kViewChunk inside the
-// tap loop, so each tap's (w, w*r) is loaded once per chunk; each view
-// still accumulates its taps in row-major order.  out[s * os] gets the
+// tap loop, so each tap's (w, w*r) is read once per chunk; each view still
+// accumulates its taps in row-major order.  With the taps in LDS, one view
+// per chunk is fastest on MI355X (fewer live registers, more waves: 21.7 vs
+// 31.5 ms per candidates launch at 1600x1200 with 8 sources).  out[s * os] gets the
 // cost, or -- for views whose NCC is <= 0.1 -- the stride-1 NCC, with bit s
 // set in the returned pending mask for resolve_fallbacks().
 #ifndef PM_VIEW_CHUNK
-#define PM_VIEW_CHUNK 2
+#define PM_VIEW_CHUNK 1
 #endif
 constexpr int kViewChunk = PM_VIEW_CHUNK;
 
@@ -350,31 +354,46 @@ __device__ __forceinline__ float census_inner(const Ref& ref, const Sampler& sm,
 // view pending work on it together: the view is wave-uniform (scalar
 // descriptors, one source image per gather instruction, neighbouring
 // pixels), which keeps the gathers coherent.
-// Strided NCC of the textureless fallback (opencl_kernels.h:188-258).  The
-// reference-side (w, w*r) of each tap are recomputed from the reference
-// texels with the arithmetic of refpatch_kernel (bit-identical to the
-// stored strides 2 and 3), which is cheaper than reading them back for
-// scattered jobs; the moments come from ref_sums.
-template <int HP, typename T, typename Ref>
-__device__ __forceinline__ float ncc_at_stride_u(const PmArgs& a, const Ref& ref, const Sampler& sm, int si, int x,
-                                                 int y, int cidx, float src_center, const Warp& w) {
-  const float stride = static_cast<float>(si + 1);
-  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(si) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+// Strided NCC of the textureless fallback (opencl_kernels.h:188-258) at
+// stride S.  The reference-side (w, w*r) of each tap are recomputed from the
+// reference texels with the arithmetic of refpatch_kernel (bit-identical to
+// the stored strides 2 and 3), which is cheaper than reading them back for
+// scattered jobs; the moments come from ref_sums.  The tap at (ix, iy) reads
+// the reference texel (x + S*ix, y + S*iy) -- the int conversion of
+// x + float(S*ix) is exact -- so an LDS copy of the reference is read at a
+// per-row base plus a constant offset.  The weight's argument is <= 0.
+// S = 0: the stride is the runtime value rs (one loop for both strides:
+// smaller code where the fallback runs inline, as in the init kernel).
+template <int HP, typename T, int S, typename Ref>
+__device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const Ref& ref, const Sampler& sm, int x, int y,
+                                               int cidx, float src_center, const Warp& w, int rs_ = S) {
+  constexpr bool kLds = std::is_same<Ref, LdsRef>::value && S > 0;
+  const int SS = S > 0 ? S : rs_;
+  const float stride = static_cast<float>(SS);
+  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(SS - 1) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
   const float sf = a.spatial_factor, cf = a.color_factor;
   float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
 #pragma unroll 1
   for (int iy = -HP; iy <= HP; ++iy) {
+    const float fdy = stride * static_cast<float>(iy);
+    const float* rrow = nullptr;
+    if constexpr (kLds) rrow = ref.t + (y + SS * iy - ref.y0) * ref.pitch + (x - ref.x0);
 #pragma unroll
     for (int ix = -HP; ix <= HP; ++ix) {
       const float fdx = stride * static_cast<float>(ix);
-      const float fdy = stride * static_cast<float>(iy);
       const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
       const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
       const float sd = sample<T>(sm, sx, sy) - src_center;
-      const float rx = static_cast<float>(x) + fdx;
-      const float ry = static_cast<float>(y) + fdy;
-      const float r = ref(static_cast<int>(rx), static_cast<int>(ry)) - rs.x;
-      const float wt = pm_expf(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
+      float rv;
+      if constexpr (kLds) {
+        rv = rrow[SS * ix];
+      } else {
+        const float rx = static_cast<float>(x) + fdx;
+        const float ry = static_cast<float>(y) + fdy;
+        rv = ref(static_cast<int>(rx), static_cast<int>(ry));
+      }
+      const float r = rv - rs.x;
+      const float wt = pm_expf_nonpos(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
       const float wr = wt * r;
       sum_s += wt * sd;
       sum_ss += wt * sd * sd;
@@ -400,11 +419,20 @@ __device__ __forceinline__ float fallback_cost(const PmArgs& a, const Ref& ref, 
                                                int cidx, float ref_center, const Warp& w, float ncc1) {
   const float scen = sample<T>(sm, w.scx, w.scy);
   float best = ncc1;
+  if constexpr (std::is_same<Ref, LdsRef>::value) {  // strides 2 then 3 (:533-542)
+    const float ncc2 = ncc_at_stride<HP, T, 2>(a, ref, sm, x, y, cidx, scen, w);
+    if (ncc2 > best) best = ncc2;
+    if (!(best > 0.1f)) {
+      const float ncc3 = ncc_at_stride<HP, T, 3>(a, ref, sm, x, y, cidx, scen, w);
+      if (ncc3 > best) best = ncc3;
+    }
+  } else {
 #pragma unroll 1
-  for (int si = 1; si < 3; ++si) {
-    const float ncc = ncc_at_stride_u<HP, T>(a, ref, sm, si, x, y, cidx, scen, w);
-    if (ncc > best) best = ncc;
-    if (best > 0.1f) break;
+    for (int s = 2; s <= 3; ++s) {
+      const float ncc = ncc_at_stride<HP, T, 0>(a, ref, sm, x, y, cidx, scen, w, s);
+      if (ncc > best) best = ncc;
+      if (best > 0.1f) break;
+    }
   }
   if (best <= -0.5f) return (a.census_weight > 1e-6f) ? census_inner<T>(ref, sm, ref_center, scen, x, y, w) : 2.0f;
   return 1.0f - best;
@@ -1398,6 +1426,25 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 
 #define PM_BLOCK static_cast<int>(blockIdx.x)
 
+// Occupancy targets (waves per SIMD) of the cost kernels and the fallback;
+// 0 = the compiler's choice.
+#ifndef PM_COST_WAVES_PER_EU
+#define PM_COST_WAVES_PER_EU 0
+#endif
+#ifndef PM_FALLBACK_WAVES_PER_EU
+#define PM_FALLBACK_WAVES_PER_EU 0
+#endif
+#if PM_COST_WAVES_PER_EU
+#define PM_COST_ATTR __attribute__((amdgpu_waves_per_eu(PM_COST_WAVES_PER_EU)))
+#else
+#define PM_COST_ATTR
+#endif
+#if PM_FALLBACK_WAVES_PER_EU
+#define PM_FALLBACK_ATTR __attribute__((amdgpu_waves_per_eu(PM_FALLBACK_WAVES_PER_EU)))
+#else
+#define PM_FALLBACK_ATTR
+#endif
+
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void refpatch_kernel(PmArgs a) { refpatch_body<HP, T>(a, PM_BLOCK); }
 template <int HP, typename T>
@@ -1420,23 +1467,29 @@ __global__ __launch_bounds__(kWave) void prior_batch(PmBatch bt) {
 }
 
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 9) void candidates_kernel(PmArgs a) { candidates_body<HP, T, TX>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave * 9) PM_COST_ATTR void candidates_kernel(PmArgs a) {
+  candidates_body<HP, T, TX>(a, PM_BLOCK);
+}
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 9) void candidates_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave * 9) PM_COST_ATTR void candidates_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { candidates_body<HP, T, TX>(a, b); });
 }
 
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 6) void refine_cost_kernel(PmArgs a) { refine_cost_body<HP, T, TX>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave * 6) PM_COST_ATTR void refine_cost_kernel(PmArgs a) {
+  refine_cost_body<HP, T, TX>(a, PM_BLOCK);
+}
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 6) void refine_cost_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave * 6) PM_COST_ATTR void refine_cost_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { refine_cost_body<HP, T, TX>(a, b); });
 }
 
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave) void fallback_kernel(PmArgs a) { fallback_body<HP, T, TX>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave) PM_FALLBACK_ATTR void fallback_kernel(PmArgs a) {
+  fallback_body<HP, T, TX>(a, PM_BLOCK);
+}
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave) void fallback_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave) PM_FALLBACK_ATTR void fallback_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { fallback_body<HP, T, TX>(a, b); });
 }
 

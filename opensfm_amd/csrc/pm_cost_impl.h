@@ -10,7 +10,7 @@ namespace pmhip {
 namespace {
 
 template <int HP, typename T, int TX>
-void launch_tiled(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, hipStream_t s) {
+void launch_tiled(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, size_t lds, hipStream_t s) {
   switch (kernel) {
     case PM_K_CANDIDATES:
       if (a) hipLaunchKernelGGL((candidates_kernel<HP, T, TX>), grid, dim3(kWave, 9), 0, s, *a);
@@ -20,9 +20,9 @@ void launch_tiled(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, hip
       if (a) hipLaunchKernelGGL((refine_cost_kernel<HP, T, TX>), grid, dim3(kWave, 6), 0, s, *a);
       else hipLaunchKernelGGL((refine_cost_batch<HP, T, TX>), grid, dim3(kWave, 6), 0, s, *bt);
       break;
-    default:  // PM_K_FALLBACK
-      if (a) hipLaunchKernelGGL((fallback_kernel<HP, T, TX>), grid, dim3(kWave), 0, s, *a);
-      else hipLaunchKernelGGL((fallback_batch<HP, T, TX>), grid, dim3(kWave), 0, s, *bt);
+    default:  // PM_K_FALLBACK; lds: dynamic LDS that only caps its occupancy
+      if (a) hipLaunchKernelGGL((fallback_kernel<HP, T, TX>), grid, dim3(kWave), lds, s, *a);
+      else hipLaunchKernelGGL((fallback_batch<HP, T, TX>), grid, dim3(kWave), lds, s, *bt);
       break;
   }
 }
@@ -46,9 +46,9 @@ hipError_t launch_hp(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, 
     case PM_K_REFINE_COST:
     case PM_K_FALLBACK:
       switch (tx) {
-        case 4: launch_tiled<HP, T, 4>(kernel, a, bt, grid, s); break;
-        case 8: launch_tiled<HP, T, 8>(kernel, a, bt, grid, s); break;
-        case 16: launch_tiled<HP, T, 16>(kernel, a, bt, grid, s); break;
+        case 4: launch_tiled<HP, T, 4>(kernel, a, bt, grid, lds, s); break;
+        case 8: launch_tiled<HP, T, 8>(kernel, a, bt, grid, lds, s); break;
+        case 16: launch_tiled<HP, T, 16>(kernel, a, bt, grid, lds, s); break;
         default: return hipErrorInvalidValue;
       }
       break;

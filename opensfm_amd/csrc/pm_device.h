@@ -76,6 +76,27 @@ __device__ __forceinline__ float pm_expf(float x) {
   return (x != x) ? x : res;
 }
 
+// pm_expf for finite x <= 0 (bilateral weights): the same value as
+// pm_expf(x) for every such x -- the upper clamp and the overflow and NaN
+// cases cannot occur, so they are left out.
+__device__ __forceinline__ float pm_expf_nonpos(float x) {
+  const float xc = fmaxf(x, -87.0f);
+  const float kf = __builtin_rintf(xc * 1.44269502162933349609f);
+  float r = fmaf(-kf, 0.693145751953125f, xc);
+  r = fmaf(-kf, 1.42860682030941723212e-6f, r);
+  float p = 1.98412698412698412698e-4f;
+  p = fmaf(p, r, 1.38888888888888888889e-3f);
+  p = fmaf(p, r, 8.33333333333333333333e-3f);
+  p = fmaf(p, r, 4.16666666666666666667e-2f);
+  p = fmaf(p, r, 1.66666666666666666667e-1f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const int k = static_cast<int>(kf);
+  const float res = p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
+  return (x < -86.0f) ? 0.0f : res;
+}
+
 // ln x, x > 0: ln m = 2 atanh((m-1)/(m+1)) with m in [sqrt(1/2), sqrt(2)).
 __device__ __forceinline__ float pm_logf(float x) {
   if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");

@@ -106,7 +106,7 @@ int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x);
 // Batched launch of `kernel` over bt with `grid` (= bt.total) workgroups,
 // one per item.  max_src: the largest n_src of the batch.
 hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int half_patch, bool f16, int max_src,
-                        int tile_x, hipStream_t stream);
+                        int tile_x, int fallback_waves, hipStream_t stream);
 
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
@@ -124,7 +124,12 @@ hipError_t launch_upsample(const PmArgs& a, const float* cdepth, const float* cn
 // candidates, fallback(0), select, refine_cost, fallback(1), refine.
 // cost_x / pixel_x: colour-tile widths (kCostTileWidths / kPixelTileWidths).
 hipError_t launch_candidates(const PmArgs& a, int half_patch, bool f16, int cost_x, hipStream_t stream);
-hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, int cost_x, hipStream_t stream);
+// fallback_waves > 0: at most that many fallback waves per SIMD (an LDS
+// reservation per workgroup caps the workgroups per CU); 0 = no cap.
+hipError_t launch_fallback(const PmArgs& a, int half_patch, bool f16, int phase, int cost_x, int fallback_waves,
+                           hipStream_t stream);
+// Dynamic LDS bytes of a fallback launch capped at fallback_waves per SIMD.
+size_t fallback_cap_lds(int half_patch, int tile_x, int fallback_waves);
 hipError_t launch_select(const PmArgs& a, int pixel_x, hipStream_t stream);
 hipError_t launch_refine_cost(const PmArgs& a, int half_patch, bool f16, int cost_x, hipStream_t stream);
 hipError_t launch_refine(const PmArgs& a, int pixel_x, hipStream_t stream);

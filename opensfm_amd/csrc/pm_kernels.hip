@@ -262,11 +262,24 @@ hipError_t launch_refine_cost(const PmArgs& a, int hp, bool f16, int cost_x, hip
                         stream);
 }
 
-hipError_t launch_fallback(const PmArgs& a, int hp, bool f16, int phase, int cost_x, hipStream_t stream) {
+size_t fallback_cap_lds(int hp, int tile_x, int fallback_waves) {
+  if (fallback_waves <= 0 || tile_x <= 0) return 0;
+  constexpr size_t kLdsPerCu = 160 * 1024;  // MI355X
+  constexpr int kSimdsPerCu = 4;
+  const int r = 3 * hp > 2 ? 3 * hp : 2;
+  const size_t static_bytes = sizeof(float) * (2 * tile_x + 2 * r) * (kWave / tile_x + 2 * r);
+  // more than 1/(n+1) of the LDS per one-wave workgroup: at most n per CU
+  const size_t per_wg = kLdsPerCu / (static_cast<size_t>(kSimdsPerCu) * fallback_waves + 1) + 1;
+  return per_wg > static_bytes ? per_wg - static_bytes : 0;
+}
+
+hipError_t launch_fallback(const PmArgs& a, int hp, bool f16, int phase, int cost_x, int fallback_waves,
+                           hipStream_t stream) {
   if (!valid_cost_tile(cost_x)) return hipErrorInvalidValue;
   PmArgs b = a;
   b.phase = phase;
-  return launch_storage(f16, PM_K_FALLBACK, &b, nullptr, blocks_of(PM_K_FALLBACK, a, cost_x), 0, hp, cost_x, stream);
+  return launch_storage(f16, PM_K_FALLBACK, &b, nullptr, blocks_of(PM_K_FALLBACK, a, cost_x),
+                        fallback_cap_lds(hp, cost_x, fallback_waves), hp, cost_x, stream);
 }
 
 hipError_t launch_select(const PmArgs& a, int pixel_x, hipStream_t stream) {
@@ -295,7 +308,7 @@ hipError_t launch_decode(const PmArgs& a, const PmDecodeOut& o, hipStream_t stre
 }
 
 hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int hp, bool f16, int max_src, int tile_x,
-                        hipStream_t stream) {
+                        int fallback_waves, hipStream_t stream) {
   if (grid == 0 || bt.total == 0) return hipSuccess;
   switch (kernel) {
     case PM_K_REFPATCH:
@@ -307,7 +320,9 @@ hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int hp, bo
     case PM_K_REFINE_COST:
     case PM_K_FALLBACK:
       if (!valid_cost_tile(tile_x)) return hipErrorInvalidValue;
-      return launch_storage(f16, kernel, nullptr, &bt, grid, 0, hp, tile_x, stream);
+      return launch_storage(f16, kernel, nullptr, &bt, grid,
+                            kernel == PM_K_FALLBACK ? fallback_cap_lds(hp, tile_x, fallback_waves) : 0, hp, tile_x,
+                            stream);
     case PM_K_SELECT:
     case PM_K_REFINE:
     case PM_K_FILTER:
