@@ -8,7 +8,7 @@ the views already resident in HBM.  Metric: Mpix*views*iters/s =
 (W*H/1e6) * N_src * iterations / seconds.
 
 Also reported: the roofline of the candidate-scoring kernel
-(pass_candidates, fixed algorithmic work per launch, timed with HIP events
+(candidates_kernel, fixed algorithmic work per launch, timed with HIP events
 on the estimator's stream inside the timed steps) and of the whole colour
 pass against the FP32 vector peak, per-stage device time, accuracy against
 the synthetic ground truth, and the CPU oracle timed on the host cores on a
@@ -33,10 +33,12 @@ UNIT = "Mpix·views·iters/s"
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PATCH = 7
 FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap x P taps + 75)
-# HBM bytes per pass_candidates launch: rocprofv3 --pmc FETCH_SIZE (x 1.9999)
-# + WRITE_SIZE (x 1.0) on this workload, each factor read off the same counter
-# on a 1 GiB device-to-device copy (profiles/*_hbm_traffic.txt).
-TRAFFIC_PER_LAUNCH = 4.774e9
+# HBM bytes per candidates_kernel<3, float, 4> launch: rocprofv3 --pmc
+# FETCH_SIZE (x 1.9999) + WRITE_SIZE (x 1.0) on this workload, each factor
+# read off the same counter on a 1 GiB device-to-device copy
+# (profiles/*_hbm_traffic.txt).
+TRAFFIC_PER_LAUNCH = 2.135e9
+TRAFFIC_TILE = 4
 
 
 def log(msg: str) -> None:
@@ -289,7 +291,7 @@ def main() -> int:
     acc1 = float(np.mean(np.abs(d - gt) < 0.01 * gt))
     acc5 = float(np.mean(np.abs(d - gt) < 0.05 * gt))
 
-    # Roofline of the dominant kernel, pass_candidates: one launch scores the
+    # Roofline of the candidates kernel (fixed work per launch): one launch scores the
     # 8 propagated candidates + the current plane of every pixel of one
     # colour, i.e. (W*H/2) x 9 hypotheses x N_src x (21 P + 75) FLOP.
     cand_ms, cand_n = stages["candidates"]
@@ -298,7 +300,8 @@ def main() -> int:
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": TRAFFIC_PER_LAUNCH if args.workload == "c2" and args.storage == "f32" else None,
+                "traffic": (TRAFFIC_PER_LAUNCH if args.workload == "c2" and args.storage == "f32"
+                            and est.tiles()[0] == TRAFFIC_TILE else None),
                 "kernel": f"candidates_kernel<3,{'float' if args.storage == 'f32' else '_Float16'},{est.tiles()[0]}>",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     stage_ms = {k: round(v[0] / args.steps, 3) for k, v in stages.items()}
