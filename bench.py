@@ -54,6 +54,10 @@ WORKLOADS = {
     "c3": ("camera_walk", "batched PatchMatch, {views} references of 640x480..2000x1500 with 4-12 neighbours each "
                           "(camera walk around a 20x20 m box scene, seed {seed}), {storage} images, patch 7x7, "
                           "{iters} iterations, one HIP-graph launch per step"),
+    "c5": ("aerial_survey", "aerial batch, {views} nadir {width}x{height} {storage} views over a fractal terrain "
+                            "(2x2 km, 0-80 m relief, 1 m grid, seed {seed}), 80%/70% overlap at 100 m, "
+                            "{neighbours} neighbours each, depth [60,140] m, patch 7x7, {iters} iterations, "
+                            "groups admitted by device memory, {refine} Gauss-Newton steps per reference"),
 }
 
 
@@ -73,18 +77,21 @@ def configure(est, scene, iters: int, storage: str = "f32"):
     est.set_image_storage(storage)
 
 
-def cpu_baseline(scene, iters: int, rows: int):
-    """CPU oracle on a band of reference rows (all sources), init + iterations."""
+def cpu_baseline(scene, iters: int, rows: int, out: dict = None):
+    """CPU oracle on a band of reference rows (all sources), init + iterations.
+    With ``out``, the band scene and the oracle's depth go into it."""
     from oracle import pm_oracle as po
     y0 = scene.gt_depth.shape[0] // 2 - rows // 2
     band = scene.crop_rows(y0, y0 + rows)
     p = po.Params(patch_size=PATCH, depth_min=scene.depth_min, depth_max=scene.depth_max, max_iterations=iters)
-    sc = po.OracleScene(band.images, band.Ks, band.Rs, band.ts)
+    sc = po.OracleScene([np.asarray(im, np.float32) for im in band.images], band.Ks, band.Rs, band.ts)
     st = po.OracleState(sc.width, sc.height)
     t0 = time.perf_counter()
     po.run(sc, p, st)
-    po.decode(sc, p, st)
+    depth, _, _, _ = po.decode(sc, p, st)
     dt = time.perf_counter() - t0
+    if out is not None:
+        out["band"], out["depth"] = band, depth
     units = sc.width * sc.height / 1e6 * band.num_src * iters
     return {"value": units / dt, "unit": UNIT, "cores": po.max_threads(), "kind": "port",
             "sample": f"reference rows {y0}..{y0 + rows} of the same {sc.width}x{scene.gt_depth.shape[0]} scene "
@@ -196,6 +203,119 @@ def run_c3(args, pydense) -> int:
     return 0
 
 
+def run_c5(args, pydense) -> int:
+    """Nadir survey: every view a reference, in groups admitted by device
+    memory, each group one batched run followed by Gauss-Newton refinement."""
+    import torch
+    from opensfm_amd import synthetic
+    from opensfm_amd.admission import default_budget, run_admitted
+
+    seed = 5 if args.seed == 2 else args.seed
+    storage = args.storage if args.storage_set else "f16"
+    views = args.views or 200
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    log(f"rendering aerial survey ({views} views of {args.width}x{args.height}) on {dev} ...")
+    t0 = time.perf_counter()
+    ws = synthetic.aerial_survey(seed=seed, num_views=views, width=args.width, height=args.height,
+                                 neighbours=args.neighbours, device=dev)
+    torch.cuda.empty_cache()
+    log(f"rendered in {time.perf_counter() - t0:.1f} s")
+    units = ws.units(args.iters)
+    budget = int(args.hbm_budget_gb * 1e9) if args.hbm_budget_gb > 0 else default_budget(pydense)
+
+    def build(i):
+        est = pydense.DepthmapEstimator()
+        configure(est, ws.reference(i), args.iters, storage)
+        return est
+
+    def keep_depth(i, est):
+        return est.download(["depth"])["depth"]
+
+    def group_log(g):
+        log(f"group of {len(g.refs)} (refs {g.refs[0]}..{g.refs[-1]}, {g.device_bytes / 1e9:.1f} GB): build "
+            f"{g.build_s:.1f} s, prepare {g.prepare_s:.1f} s, PatchMatch {g.run_ms:.0f} ms, Gauss-Newton "
+            f"{g.refine_ms:.0f} ms, results {g.results_s:.1f} s, wall {g.wall_s:.1f} s")
+
+    log(f"device budget {budget / 1e9:.1f} GB; one reference needs {build(0).device_bytes() / 1e9:.2f} GB")
+    for i in range(args.warmup):        # first use: autotuner, code objects
+        run_admitted(pydense, min(2, views), build, budget=budget, refine_steps=args.refine_steps,
+                     on_result=lambda i, e: None)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    depths = None
+    groups = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        depths, groups = run_admitted(pydense, views, build, budget=budget, refine_steps=args.refine_steps,
+                                      on_result=keep_depth, on_group=group_log)
+        log(f"step {k + 1}/{args.steps}: {len(groups)} groups")
+    elapsed = time.perf_counter() - t0
+    value = units * args.steps / elapsed
+    pm_ms = sum(g.run_ms for g in groups)
+    gn_ms = sum(g.refine_ms for g in groups)
+
+    hit = hit5 = tot = 0
+    for i, d in depths.items():
+        gt = ws.gt_depth[ws.refs[i][0]]
+        m = gt > 0
+        hit += int(np.sum(np.abs(d[m] - gt[m]) < 0.01 * gt[m]))
+        hit5 += int(np.sum(np.abs(d[m] - gt[m]) < 0.05 * gt[m]))
+        tot += int(m.sum())
+    del depths
+
+    cpu, parity = None, None
+    if args.cpu_rows > 0:
+        log(f"CPU oracle on {args.cpu_rows} rows of reference 0 ...")
+        keep = {}
+        sc0 = ws.reference(0)
+        cpu = cpu_baseline(sc0, args.iters, args.cpu_rows, keep)
+        cpu["value"] = round(cpu["value"], 5)
+        est = pydense.DepthmapEstimator()
+        configure(est, keep["band"], args.iters, storage)
+        d = est.compute()["depth"]
+        m = keep["depth"] > 0
+        parity = {"sample": "the same rows of reference 0 on the GPU (no refinement)",
+                  "bit_equal": round(float(np.mean(d == keep["depth"])), 5),
+                  "within_1pct_of_oracle": round(float(np.mean(np.abs(d[m] - keep["depth"][m])
+                                                               < 0.01 * keep["depth"][m])), 5)}
+
+    pm_tflops = units * 1e6 * FLOP_PER_UNIT / (pm_ms / 1e3) / 1e12
+    w, h = args.width, args.height
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if storage == "f32" else "f16-storage/f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["c5"][1].format(views=views, width=w, height=h, storage=storage, seed=seed,
+                                                         neighbours=args.neighbours, iters=args.iters,
+                                                         refine=args.refine_steps),
+                   "references": views, "megapixels": round(views * w * h / 1e6, 3),
+                   "iterations": args.iters, "patch": PATCH, "units_per_step": round(units, 3),
+                   "value_scope": "whole job: building, staging, batched PatchMatch, Gauss-Newton and depth "
+                                  "download of every group"},
+        "roofline_path": {"scope": "batched PatchMatch device time of all groups (init, colour passes, decode)",
+                          "achieved": round(pm_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(pm_tflops / PEAK_FP32_TFLOPS, 4), "device_ms": round(pm_ms, 3)},
+        "patchmatch_device_value": round(units / (pm_ms / 1e3), 3),
+        "refine": {"steps": args.refine_steps, "ms": round(gn_ms, 3),
+                   "value": round(views * w * h / 1e6 * args.neighbours * args.refine_steps / (gn_ms / 1e3), 3)
+                   if gn_ms > 0 else None, "unit": "Mpx·views·steps/s"},
+        "admission": {"budget_bytes": budget, "groups": len(groups),
+                      "per_group": [{"refs": len(g.refs), "device_gb": round(g.device_bytes / 1e9, 2),
+                                     "build_s": round(g.build_s, 2), "prepare_s": round(g.prepare_s, 2),
+                                     "run_ms": round(g.run_ms, 1), "refine_ms": round(g.refine_ms, 1),
+                                     "results_s": round(g.results_s, 2), "wall_s": round(g.wall_s, 2)}
+                                    for g in groups]},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "accuracy": {"scope": "after refinement, all references", "gt_within_1pct": round(hit / max(tot, 1), 4),
+                     "gt_within_5pct": round(hit5 / max(tot, 1), 4)},
+    }
+    if cpu and cpu["value"] > 0:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def copy_probe(mib: int) -> None:
     """One device-to-device copy of ``mib`` MiB through the HIP runtime (a
     plain copy kernel of known size, to scale the HBM traffic counters)."""
@@ -218,7 +338,7 @@ def copy_probe(mib: int) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; 1 for c5)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
@@ -228,13 +348,20 @@ def main() -> int:
                     help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
                     help="c2: the headline single-view workload; c3: many references in one batched launch; "
-                         "c4: PatchMatch + Gauss-Newton refinement")
-    ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 workload")
-    ap.add_argument("--views", type=int, default=64, help="references of the c3 workload")
+                         "c4: PatchMatch + Gauss-Newton refinement; c5: aerial survey in memory-admitted groups")
+    ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 and c5 workloads")
+    ap.add_argument("--views", type=int, default=None, help="references of the c3 (64) and c5 (200) workloads")
+    ap.add_argument("--width", type=int, default=4000, help="c5 image width")
+    ap.add_argument("--height", type=int, default=3000, help="c5 image height")
+    ap.add_argument("--neighbours", type=int, default=10, help="c5 sources per reference")
+    ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
+                    help="c5 device budget per group in GB (default: 90%% of the free device memory)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed steps, run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
     args.storage_set = args.storage is not None
+    if args.steps is None:
+        args.steps = 1 if args.workload == "c5" else 5
     if args.storage is None:
         args.storage = "f32"
     if args.gpus != 1:
@@ -249,7 +376,11 @@ def main() -> int:
         raise RuntimeError("no HIP device visible")
 
     if args.workload == "c3":
+        if args.views is None:
+            args.views = 64
         return run_c3(args, pydense)
+    if args.workload == "c5":
+        return run_c5(args, pydense)
     if args.workload == "c4" and args.seed == 2:
         args.seed = 4
     log("rendering scene ...")

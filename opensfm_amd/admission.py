@@ -1,0 +1,141 @@
+"""Many references on one GPU, in groups admitted by device memory.
+
+The reference's dense driver cuts the views into clusters of at most
+``max_cluster_size`` views (opensfm/dense/clustering.py:164-216) and hands
+them to per-device workers that run each cluster's references one after the
+other (opensfm/dense/depthmaps.py:70-151; cluster_estimator.cc:82-156).
+Here the unit of admission is device memory: references are built one at a
+time, each reports the device bytes its run allocates
+(``DepthmapEstimator.device_bytes``), and a group takes references in order
+while their sum fits the budget (by default 90 % of the device memory free
+when the run starts).  A reference larger than the budget runs alone.
+
+A group runs as one ``DepthmapBatch`` (one launch per kernel for all its
+references, captured in a HIP graph), is refined by Gauss-Newton when asked,
+and hands each estimator to ``on_result`` while its results are on the
+device; then the group's device memory is released and its estimators are
+dropped before the next group is built, so the host holds one group's
+images at a time.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from typing import Any, Callable, Dict, List, Optional, Sequence
+
+
+@dataclasses.dataclass
+class GroupStats:
+    refs: List[int]            # reference indices of the group
+    device_bytes: int          # sum of the references' device_bytes()
+    build_s: float = 0.0       # host: building the estimators (views, parameters)
+    prepare_s: float = 0.0     # staging, first-use tuning and graph capture
+    run_ms: float = 0.0        # device time of the batched PatchMatch run
+    refine_ms: float = 0.0     # device time of the Gauss-Newton steps (sum over the group)
+    results_s: float = 0.0     # on_result calls and releasing the device memory
+    wall_s: float = 0.0        # the whole group, end to end
+
+
+def admit(sizes: Sequence[int], budget: int) -> List[List[int]]:
+    """Split indices 0..len(sizes)-1, in order, into groups whose sizes sum
+    to at most ``budget``; an item larger than the budget forms a group of
+    its own."""
+    if budget <= 0:
+        raise ValueError("admit: the budget must be positive")
+    groups: List[List[int]] = []
+    cur: List[int] = []
+    used = 0
+    for i, s in enumerate(sizes):
+        if s < 0:
+            raise ValueError("admit: negative size")
+        if cur and used + s > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += s
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def default_budget(pydense, device: int = 0, fraction: float = 0.9) -> int:
+    """``fraction`` of the device memory free now (release cached torch
+    blocks first if torch shares the device)."""
+    return int(pydense.DepthmapEstimator.device_free_memory(device) * fraction)
+
+
+def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optional[int] = None,
+                 refine_steps: int = 0, damping: float = 1e-3,
+                 on_result: Optional[Callable[[int, Any], Any]] = None, device: int = 0,
+                 batch_config: Optional[Dict[str, Any]] = None,
+                 on_group: Optional[Callable[[GroupStats], None]] = None):
+    """Run references 0..count-1 in memory-admitted groups.
+
+    ``build(i)`` returns a configured ``DepthmapEstimator`` of reference i
+    (views added, parameters set, on ``device``).  ``on_result(i, est)`` is
+    called once per reference after its group's run and refinement; its
+    return value is collected (default: ``est.download(["depth"])``).
+    ``batch_config`` goes to ``DepthmapBatch.set_config``.  ``on_group`` is
+    called with each finished group's timing.  Returns
+    ``(results: {i: value}, groups: [GroupStats])``.
+    """
+    if count < 0:
+        raise ValueError("run_admitted: negative count")
+    if budget is None:
+        budget = default_budget(pydense, device)
+    if budget <= 0:
+        raise ValueError("run_admitted: no device memory to admit references into")
+    if on_result is None:
+        on_result = lambda i, est: est.download(["depth"])  # noqa: E731
+    results: Dict[int, Any] = {}
+    groups: List[GroupStats] = []
+    pending = None            # an estimator built but not admitted into the previous group
+    i = 0
+    while i < count or pending is not None:
+        t_group = time.perf_counter()
+        ests, refs, used = [], [], 0
+        t0 = time.perf_counter()
+        while True:
+            if pending is not None:
+                idx, est, need = pending
+                pending = None
+            elif i < count:
+                idx = i
+                est = build(i)
+                need = int(est.device_bytes())
+                i += 1
+            else:
+                break
+            if refs and used + need > budget:
+                pending = (idx, est, need)
+                break
+            ests.append(est)
+            refs.append(idx)
+            used += need
+        st = GroupStats(refs=refs, device_bytes=used, build_s=time.perf_counter() - t0)
+        batch = pydense.DepthmapBatch(device)
+        for e in ests:
+            batch.add(e)
+        if batch_config:
+            batch.set_config(**batch_config)
+        t0 = time.perf_counter()
+        batch.prepare()
+        st.prepare_s = time.perf_counter() - t0
+        st.run_ms = batch.run()
+        for e in ests:
+            if refine_steps > 0:
+                e.gauss_newton(refine_steps, damping)
+                st.refine_ms += e.last_refine_ms()
+        t0 = time.perf_counter()
+        for idx, e in zip(refs, ests):
+            results[idx] = on_result(idx, e)
+        del batch
+        for e in ests:
+            e.release_device()
+        ests.clear()
+        st.results_s = time.perf_counter() - t0
+        st.wall_s = time.perf_counter() - t_group
+        groups.append(st)
+        if on_group is not None:
+            on_group(st)
+    return results, groups

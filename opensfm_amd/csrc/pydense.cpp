@@ -157,25 +157,45 @@ class Estimator {
   void filter() { check(pm_filter(h_)); }
   void decode() { check(pm_decode(h_)); }
 
-  py::dict download() {
+  // Every decoded map, or only the ones named in ``maps``.
+  py::dict download(py::object maps = py::none()) {
+    static const char* kNames[] = {"depth", "plane", "normal", "cost", "views", "confidence"};
+    bool want[6] = {true, true, true, true, true, true};
+    if (!maps.is_none()) {
+      for (bool& b : want) b = false;
+      for (py::handle m : maps) {
+        const std::string name = py::str(m);
+        int k = 0;
+        while (k < 6 && name != kNames[k]) ++k;
+        if (k == 6) throw std::invalid_argument("download: unknown map '" + name + "'");
+        want[k] = true;
+      }
+    }
     const auto [w, h] = ref_size();
-    py::array_t<float> depth(shape({h, w})), plane(shape({h, w, 4})), normal(shape({h, w, 3})),
-        cost(shape({h, w})), conf(shape({h, w}));
-    py::array_t<uint32_t> views(shape({h, w}));
+    py::array_t<float> depth, plane, normal, cost, conf;
+    py::array_t<uint32_t> views;
+    if (want[0]) depth = py::array_t<float>(shape({h, w}));
+    if (want[1]) plane = py::array_t<float>(shape({h, w, 4}));
+    if (want[2]) normal = py::array_t<float>(shape({h, w, 3}));
+    if (want[3]) cost = py::array_t<float>(shape({h, w}));
+    if (want[4]) views = py::array_t<uint32_t>(shape({h, w}));
+    if (want[5]) conf = py::array_t<float>(shape({h, w}));
     {
       py::gil_scoped_release nogil;
-      check(pm_download(h_, depth.mutable_data(), plane.mutable_data(), normal.mutable_data(),
-                        cost.mutable_data(), views.mutable_data(), conf.mutable_data()));
+      check(pm_download(h_, want[0] ? depth.mutable_data() : nullptr, want[1] ? plane.mutable_data() : nullptr,
+                        want[2] ? normal.mutable_data() : nullptr, want[3] ? cost.mutable_data() : nullptr,
+                        want[4] ? views.mutable_data() : nullptr, want[5] ? conf.mutable_data() : nullptr));
     }
     py::dict d;
-    d["depth"] = depth;
-    d["plane"] = plane;
-    d["normal"] = normal;
-    d["cost"] = cost;
-    d["views"] = views;
-    d["confidence"] = conf;
+    if (want[0]) d["depth"] = depth;
+    if (want[1]) d["plane"] = plane;
+    if (want[2]) d["normal"] = normal;
+    if (want[3]) d["cost"] = cost;
+    if (want[4]) d["views"] = views;
+    if (want[5]) d["confidence"] = conf;
     return d;
   }
+  void release_device() { check(pm_release_device(h_)); }
 
   py::tuple read_state() {
     const auto [w, h] = ref_size();
@@ -711,7 +731,11 @@ PYBIND11_MODULE(pydense, m) {
       .def("compute", &Estimator::compute, "Run PatchMatch; returns a dict of every decoded map")
       .def("upload", &Estimator::upload)
       .def("run", &Estimator::run)
-      .def("download", &Estimator::download)
+      .def("download", &Estimator::download, py::arg("maps") = py::none(),
+           "Decoded maps of the last run: all, or those named in maps (depth, plane, normal, cost, views, "
+           "confidence)")
+      .def("release_device", &Estimator::release_device,
+           "Free the device memory of this estimator (views stay on the host; the next run uploads again)")
       .def("init_state", &Estimator::init_state)
       .def("iterate", &Estimator::iterate, py::arg("iteration"), py::arg("color") = -1)
       .def("filter", &Estimator::filter)
@@ -737,6 +761,11 @@ PYBIND11_MODULE(pydense, m) {
         int n = 0;
         return pm_device_count(&n) == PM_OK && n > 0;
       })
+      .def_static("device_free_memory", [](int i) {
+        uint64_t total = 0, free_b = 0;
+        check(pm_device_memory(i, &total, &free_b));
+        return free_b;
+      }, py::arg("device") = 0, "Free memory of a device in bytes")
       .def_static("num_devices", []() {
         int n = 0;
         pm_device_count(&n);

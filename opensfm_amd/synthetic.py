@@ -421,6 +421,35 @@ def _box_quads(cx, cy, hx, hy, height, angle):
     return quads
 
 
+def _fbm_torch(noise: ValueNoise3D, p, base_cell: float, octaves: int = 4):
+    """ValueNoise3D.fbm of the (n, 3) float64 tensor ``p``, on its device."""
+    import torch
+
+    table = torch.as_tensor(noise.values, dtype=p.dtype, device=p.device)
+    nt = noise.n
+    total = torch.zeros(p.shape[0], dtype=p.dtype, device=p.device)
+    amp, cell, norm = 1.0, base_cell, 0.0
+    for o in range(octaves):
+        q = p / cell + 17.31 * o
+        fl = torch.floor(q)
+        f = q - fl
+        i = fl.to(torch.int64) % nt
+        f = f * f * (3.0 - 2.0 * f)
+        acc = torch.zeros_like(total)
+        for dx in (0, 1):
+            wx = f[:, 0] if dx else 1.0 - f[:, 0]
+            for dy in (0, 1):
+                wy = f[:, 1] if dy else 1.0 - f[:, 1]
+                for dz in (0, 1):
+                    wz = f[:, 2] if dz else 1.0 - f[:, 2]
+                    acc += wx * wy * wz * table[(i[:, 0] + dx) % nt, (i[:, 1] + dy) % nt, (i[:, 2] + dz) % nt]
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        cell *= 0.5
+    return total / norm
+
+
 def _render_torch(quads, noise, base_cell, K, R, t, width, height, sigma, gen, device):
     """Ray-cast convex planar quads with torch (float64) on ``device``; the
     same model as ``render`` (z-buffer, 4-octave value noise, sensor noise).
@@ -455,30 +484,8 @@ def _render_torch(quads, noise, base_cell, K, R, t, width, height, sigma, gen, d
         zbuf = torch.where(take, lam, zbuf)
         hit = torch.where(take[None, :], X, hit)
     valid = torch.isfinite(zbuf)
-    table = torch.as_tensor(noise.values, dtype=dt, device=device)
-    nt = noise.n
-    p = hit.T
-    total = torch.zeros(p.shape[0], dtype=dt, device=device)
-    amp, cell, norm = 1.0, base_cell, 0.0
-    for o in range(4):
-        q = p / cell + 17.31 * o
-        fl = torch.floor(q)
-        f = q - fl
-        i = fl.to(torch.int64) % nt
-        f = f * f * (3.0 - 2.0 * f)
-        acc = torch.zeros_like(total)
-        for dx in (0, 1):
-            wx = f[:, 0] if dx else 1.0 - f[:, 0]
-            for dy in (0, 1):
-                wy = f[:, 1] if dy else 1.0 - f[:, 1]
-                for dz in (0, 1):
-                    wz = f[:, 2] if dz else 1.0 - f[:, 2]
-                    acc += wx * wy * wz * table[(i[:, 0] + dx) % nt, (i[:, 1] + dy) % nt, (i[:, 2] + dz) % nt]
-        total += amp * acc
-        norm += amp
-        amp *= 0.5
-        cell *= 0.5
-    img = torch.where(valid, total / norm, torch.full_like(total, 0.5))
+    total = _fbm_torch(noise, hit.T, base_cell)
+    img = torch.where(valid, total, torch.full_like(total, 0.5))
     img = img + sigma * torch.randn(img.shape, generator=gen, dtype=dt, device=device)
     depth = torch.where(valid, zbuf, torch.zeros_like(zbuf))
     return (img.reshape(height, width).to(torch.float32).cpu().numpy(),
@@ -552,3 +559,177 @@ def camera_walk(seed: int = 3, num_views: int = 64, sizes: Sequence[Tuple[int, i
         gts.append(depth)
     return WalkScene(images, Ks, Rs, ts, gts, refs,
                      f"camera_walk(seed={seed},{num_views} views,{len(quads)} quads)")
+
+
+# ---------------------------------------------------------------------------
+# Aerial survey: nadir views over a fractal terrain
+# ---------------------------------------------------------------------------
+def fractal_dem(seed: int, extent: float = 2000.0, grid: float = 1.0, relief: float = 80.0,
+                beta: float = 3.8) -> np.ndarray:
+    """Seeded fractal elevation grid ((n, n) float64, n = extent / grid + 1,
+    heights in [0, relief] m): spectral synthesis with power spectrum
+    f^-beta on a power-of-two periodic grid, cropped.  beta = 3.8 keeps the
+    steepest 1 m step at about 0.7."""
+    n = int(round(extent / grid)) + 1
+    m = 1 << int(np.ceil(np.log2(n)))
+    rng = np.random.default_rng(seed)
+    fx = np.fft.fftfreq(m)[:, None]
+    fy = np.fft.rfftfreq(m)[None, :]
+    f = np.hypot(fx, fy)
+    f[0, 0] = 1.0
+    amp = f ** (-beta / 2.0)
+    amp[0, 0] = 0.0
+    spec = amp * (rng.standard_normal(f.shape) + 1j * rng.standard_normal(f.shape))
+    z = np.fft.irfft2(spec, s=(m, m))[:n, :n]
+    return (z - z.min()) / (z.max() - z.min()) * relief
+
+
+def _rot_xyz(rx: float, ry: float, rz: float) -> np.ndarray:
+    cx, sx, cy, sy, cz, sz = np.cos(rx), np.sin(rx), np.cos(ry), np.sin(ry), np.cos(rz), np.sin(rz)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def _render_dem_torch(dem, grid: float, noise: ValueNoise3D, base_cell: float, K, R, t, width: int, height: int,
+                      sigma: float, gen, device, iters: int = 48):
+    """Ray-cast the bilinear surface of ``dem`` (heights at x = j * grid,
+    y = i * grid for dem[i, j]) by bracketed Newton along each pixel ray.
+    The caller guarantees that the height along every ray is monotone (one
+    crossing), so the root is the first hit.  Returns (image float16, depth
+    float32) numpy arrays; rays leaving the grid get depth 0 and grey 0.5."""
+    import torch
+
+    dt = torch.float64
+    Z = torch.as_tensor(dem, dtype=dt, device=device)
+    ny, nx = Z.shape
+    Kt = torch.as_tensor(K, dtype=dt, device=device)
+    Rt = torch.as_tensor(R, dtype=dt, device=device)
+    tt = torch.as_tensor(t, dtype=dt, device=device)
+    C = -Rt.T @ tt
+    ys, xs = torch.meshgrid(torch.arange(height, dtype=dt, device=device),
+                            torch.arange(width, dtype=dt, device=device), indexing="ij")
+    pix = torch.stack([xs.reshape(-1), ys.reshape(-1), torch.ones(width * height, dtype=dt, device=device)], 0)
+    dirs = Rt.T @ (torch.linalg.inv(Kt) @ pix)         # z_cam = 1 per ray
+    del pix, xs, ys
+    dx, dy, dz = dirs[0], dirs[1], dirs[2]
+
+    def surface(lam):
+        gx = ((C[0] + lam * dx) / grid).clamp(0.0, nx - 1.0)
+        gy = ((C[1] + lam * dy) / grid).clamp(0.0, ny - 1.0)
+        ix = torch.floor(gx).clamp(max=nx - 2)
+        iy = torch.floor(gy).clamp(max=ny - 2)
+        fx, fy = gx - ix, gy - iy
+        k = iy.to(torch.int64) * nx + ix.to(torch.int64)
+        zf = Z.reshape(-1)
+        z00, z10, z01, z11 = zf[k], zf[k + 1], zf[k + nx], zf[k + nx + 1]
+        h = (1 - fy) * ((1 - fx) * z00 + fx * z10) + fy * ((1 - fx) * z01 + fx * z11)
+        hx = ((1 - fy) * (z10 - z00) + fy * (z11 - z01)) / grid
+        hy = ((1 - fx) * (z01 - z00) + fx * (z11 - z10)) / grid
+        return h, hx, hy
+
+    zmin, zmax = float(Z.min()), float(Z.max())
+    lo = (zmax + 1e-3 - C[2]) / dz                     # above the highest point
+    hi = (zmin - 1e-3 - C[2]) / dz                     # below the lowest point
+    lam = 0.5 * (lo + hi)
+    g_prev = torch.full_like(lam, float("inf"))
+    for _ in range(iters):
+        h, hx, hy = surface(lam)
+        g = C[2] + lam * dz - h
+        above = g > 0
+        lo = torch.where(above, lam, lo)
+        hi = torch.where(above, hi, lam)
+        dg = dz - hx * dx - hy * dy
+        nl = lam - g / dg
+        # Newton inside the bracket while |g| at least halves; bisection
+        # otherwise (Newton can cycle across the kinks between grid cells)
+        newton = (nl > lo) & (nl < hi) & (torch.abs(g) <= 0.5 * g_prev)
+        g_prev = torch.abs(g)
+        lam = torch.where(newton, nl, 0.5 * (lo + hi))
+    h, _, _ = surface(lam)
+    X = C[:, None] + dirs * lam[None, :]
+    inside = (X[0] >= 0) & (X[0] <= (nx - 1) * grid) & (X[1] >= 0) & (X[1] <= (ny - 1) * grid)
+    ok = inside & (torch.abs(X[2] - h) < 1e-6)
+    tex = _fbm_torch(noise, X.T, base_cell)
+    img = torch.where(ok, tex, torch.full_like(tex, 0.5))
+    img = img + sigma * torch.randn(img.shape, generator=gen, dtype=dt, device=device)
+    depth = torch.where(ok, lam, torch.zeros_like(lam))
+    return (img.reshape(height, width).to(torch.float16).cpu().numpy(),
+            depth.reshape(height, width).to(torch.float32).cpu().numpy())
+
+
+def aerial_survey(seed: int = 5, num_views: int = 200, width: int = 4000, height: int = 3000,
+                  neighbours: int = 10, extent: float = 2000.0, grid: float = 1.0, relief: float = 80.0,
+                  flight_height: float = 100.0, forward_overlap: float = 0.8, side_overlap: float = 0.7,
+                  focal_frac: float = 0.9, depth_range: Tuple[float, float] = (60.0, 140.0),
+                  sigma: float = 0.01, device: str = "cpu") -> WalkScene:
+    """Nadir views of a seeded fractal terrain, flown in a lawn-mower pattern.
+
+    Terrain: ``fractal_dem`` over ``extent`` x ``extent`` m at ``grid`` m,
+    heights 0..``relief`` m, textured by 4-octave value noise (0.8 m down to
+    0.1 m cells) in world coordinates.  Cameras (focal ``focal_frac`` x
+    width) fly ``flight_height`` m above the mid-relief level, looking
+    straight down with small seeded attitude (0.3 degree) and position
+    (0.3 m) jitter, along strips parallel to y: the image height lies along
+    the strip, consecutive photos overlap by ``forward_overlap`` and
+    neighbouring strips by ``side_overlap`` (at the mid-relief level), and
+    every other strip is flown in the opposite direction (images rotated by
+    180 degrees).  Strips: about sqrt(num_views / 2), the survey centred on
+    the terrain.  Every view is a reference; its sources are the
+    ``neighbours`` views with the nearest camera centres (ties by index),
+    and its depth range is ``depth_range``.  Images are float16.
+    """
+    import torch
+
+    rng = np.random.default_rng(seed)
+    dem = fractal_dem(seed, extent, grid, relief)
+    f = focal_frac * width
+    K = intrinsics(width, height, f)
+    along = height / f * flight_height * (1.0 - forward_overlap)
+    across = width / f * flight_height * (1.0 - side_overlap)
+    strips = max(1, int(round(np.sqrt(num_views / 2.0))))
+    per = int(np.ceil(num_views / strips))
+    z0 = relief / 2.0 + flight_height
+    base = np.array([[1.0, 0.0, 0.0], [0.0, -1.0, 0.0], [0.0, 0.0, -1.0]])   # image up = +y, looking down
+    flip = np.diag([-1.0, -1.0, 1.0])                                         # 180 degrees about the axis
+    Ks, Rs, ts, centres = [], [], [], []
+    for s in range(strips):
+        for k in range(per):
+            if len(Ks) == num_views:
+                break
+            kk = k if s % 2 == 0 else per - 1 - k
+            C = np.array([extent / 2 + (s - (strips - 1) / 2) * across,
+                          extent / 2 + (kk - (per - 1) / 2) * along, z0])
+            C = C + np.append(rng.normal(0.0, 0.3, 2), rng.normal(0.0, 0.3))
+            R0 = base if s % 2 == 0 else flip @ base
+            R = _rot_xyz(*np.radians(rng.normal(0.0, 0.3, 3))) @ R0
+            Ks.append(K.copy()); Rs.append(R); ts.append(-R @ C); centres.append(C)
+    centres = np.array(centres)
+
+    # one crossing per ray: the height change along a ray's horizontal step
+    # stays below its vertical step (bilinear cells: per-axis step bounds)
+    sx = np.abs(np.diff(dem, axis=1)).max() / grid
+    sy = np.abs(np.diff(dem, axis=0)).max() / grid
+    corners = np.linalg.inv(K) @ np.array([[0, width - 1, 0, width - 1], [0, 0, height - 1, height - 1], [1, 1, 1, 1.0]])
+    for R in Rs:
+        d = R.T @ corners
+        if np.any(sx * np.abs(d[0]) + sy * np.abs(d[1]) >= 0.95 * np.abs(d[2])):
+            raise ValueError("aerial_survey: terrain too steep for single-crossing rays")
+
+    refs = []
+    for i in range(num_views):
+        dist = np.linalg.norm(centres - centres[i], axis=1)
+        order = [j for j in np.lexsort((np.arange(num_views), dist)) if j != i]
+        refs.append((i, [int(j) for j in order[:neighbours]], float(depth_range[0]), float(depth_range[1])))
+
+    noise = ValueNoise3D(seed)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed + 1000)
+    images, gts = [], []
+    for K_, R, t in zip(Ks, Rs, ts):
+        img, depth = _render_dem_torch(dem, grid, noise, 0.8, K_, R, t, width, height, sigma, gen, device)
+        images.append(img)
+        gts.append(depth)
+    return WalkScene(images, Ks, Rs, ts, gts, refs,
+                     f"aerial_survey(seed={seed},{num_views} views,{width}x{height},{strips} strips)")

@@ -1,0 +1,162 @@
+"""Aerial survey workload: nadir views of a fractal terrain, run in groups
+admitted by device memory, with Gauss-Newton refinement.
+
+The scene generator is pinned by an independent ray march (numpy, fine
+steps) and the numpy value noise at the hit points, and by the flight
+geometry (strip and photo spacing from the overlaps).  Admission is pinned by
+its grouping rule; on the GPU, references run in admitted groups must equal
+the same references run alone (and refined alone) bit for bit, and one
+aerial reference must equal the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import pm_oracle as po
+from tests.helpers import gpu_estimator, half_round, params_for
+
+
+def test_admit_groups():
+    from opensfm_amd.admission import admit
+    assert admit([3, 3, 3, 3], 7) == [[0, 1], [2, 3]]
+    assert admit([3, 3, 3, 3], 6) == [[0, 1], [2, 3]]
+    assert admit([3, 3, 3, 3], 5) == [[0], [1], [2], [3]]
+    assert admit([10, 1, 1, 10, 1], 4) == [[0], [1, 2], [3], [4]]      # too large: alone
+    assert admit([], 4) == []
+    assert admit([0, 0, 5], 5) == [[0, 1, 2]]
+    with pytest.raises(ValueError):
+        admit([1], 0)
+    with pytest.raises(ValueError):
+        admit([-1], 3)
+
+
+def test_fractal_dem():
+    from opensfm_amd.synthetic import fractal_dem
+    z = fractal_dem(5)
+    assert z.shape == (2001, 2001)
+    assert z.min() == 0.0 and abs(z.max() - 80.0) < 1e-9
+    assert max(np.abs(np.diff(z, axis=0)).max(), np.abs(np.diff(z, axis=1)).max()) < 0.75
+    assert np.array_equal(z, fractal_dem(5)) and not np.array_equal(z, fractal_dem(6))
+
+
+W, H = 200, 150     # same footprint as 4000x3000 (focal 0.9 x width)
+
+
+@pytest.fixture(scope="module")
+def survey():
+    from opensfm_amd import synthetic
+    return synthetic.aerial_survey(seed=5, num_views=12, width=W, height=H, neighbours=5)
+
+
+def _bilinear(z, x, y):
+    j, i = int(np.floor(x)), int(np.floor(y))
+    fx, fy = x - j, y - i
+    return ((1 - fy) * ((1 - fx) * z[i, j] + fx * z[i, j + 1])
+            + fy * ((1 - fx) * z[i + 1, j] + fx * z[i + 1, j + 1]))
+
+
+def test_survey_layout(survey):
+    from opensfm_amd.synthetic import fractal_dem
+    s = survey
+    assert len(s.images) == 12 and all(im.dtype == np.float16 and im.shape == (H, W) for im in s.images)
+    C = np.array([-R.T @ t for R, t in zip(s.Rs, s.ts)])
+    # 2 strips of 6: photos 100 * 150/180 * 0.2 m apart along a strip, strips
+    # 100 * 200/180 * 0.3 m apart (jitter 0.3 m), at 140 m
+    along, across = 100 * H / (0.9 * W) * 0.2, 100 * W / (0.9 * W) * 0.3
+    assert np.allclose(np.diff(C[:6, 1]), along, atol=1.5)
+    assert np.allclose(np.diff(C[6:, 1]), -along, atol=1.5)          # flown back
+    assert abs(np.mean(C[6:, 0]) - np.mean(C[:6, 0]) - across) < 1.0
+    assert np.allclose(C[:, 2], 140.0, atol=1.5)
+    for R in s.Rs:
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12)
+        assert np.degrees(np.arccos(-R[2, 2])) < 2.0                   # looking down
+    assert s.Rs[0][0, 0] > 0.99 and s.Rs[6][0, 0] < -0.99              # second strip rotated 180 degrees
+    for v, srcs, dmin, dmax in s.refs:
+        assert (dmin, dmax) == (60.0, 140.0) and len(srcs) == 5 and v not in srcs
+        d = np.linalg.norm(C[srcs] - C[v], axis=1)
+        others = np.delete(np.linalg.norm(C - C[v], axis=1), [v] + srcs)
+        assert np.all(np.diff(d) >= 0) and d.max() <= others.min()     # the nearest, in order
+        g = s.gt_depth[v]
+        assert np.all(g > 60.0) and np.all(g < 140.0)
+    # forward overlap with the next photo of the strip, near 80 %
+    v, u = 2, 3
+    ys, xs = np.mgrid[0:H, 0:W]
+    d = s.gt_depth[v].astype(np.float64)
+    X = s.Rs[v].T @ (np.linalg.inv(s.Ks[v]) @ np.stack([xs.ravel(), ys.ravel(), np.ones(xs.size)]) * d.ravel()
+                     - s.ts[v][:, None])
+    p = s.Ks[u] @ (s.Rs[u] @ X + s.ts[u][:, None])
+    inside = (p[0] / p[2] >= 0) & (p[0] / p[2] <= W - 1) & (p[1] / p[2] >= 0) & (p[1] / p[2] <= H - 1)
+    assert 0.7 < inside.mean() < 0.9
+    # the same seed gives the same survey
+    from opensfm_amd import synthetic
+    again = synthetic.aerial_survey(seed=5, num_views=12, width=W, height=H, neighbours=5)
+    assert all(np.array_equal(a, b) for a, b in zip(s.images, again.images))
+    # ground truth against a fine march along random rays, and the image
+    # against the value noise at the hit point
+    z = fractal_dem(5)
+    noise = synthetic.ValueNoise3D(5)
+    rng = np.random.default_rng(0)
+    errs, texd = [], []
+    for v in (0, 7):
+        K, R, t = s.Ks[v], s.Rs[v], s.ts[v]
+        Cv = -R.T @ t
+        for _ in range(40):
+            x, y = rng.integers(0, W), rng.integers(0, H)
+            dvec = R.T @ np.linalg.solve(K, [x, y, 1.0])
+            lam = np.arange(60.0, 150.0, 0.01)
+            P = Cv[:, None] + dvec[:, None] * lam
+            g = P[2] - np.array([_bilinear(z, px, py) for px, py in zip(P[0], P[1])])
+            k = int(np.argmax(g <= 0))
+            l0, l1 = lam[k - 1], lam[k]
+            for _ in range(60):                     # bisection inside the fine step
+                m = 0.5 * (l0 + l1)
+                Pm = Cv + dvec * m
+                if Pm[2] - _bilinear(z, Pm[0], Pm[1]) > 0:
+                    l0 = m
+                else:
+                    l1 = m
+            errs.append(abs(s.gt_depth[v][y, x] - l0))
+            hit = Cv + dvec * l0
+            texd.append(abs(float(s.images[v][y, x]) - noise.fbm(hit[None, :], 0.8)[0]))
+    assert max(errs) < 2e-4                          # float32 depth of ~100 m
+    assert np.median(texd) < 0.012 and max(texd) < 0.05   # sensor noise sigma 0.01 + float16
+
+
+@pytest.mark.gpu
+def test_admitted_groups_equal_single_runs(pydense, survey):
+    from opensfm_amd.admission import run_admitted
+    s = survey
+    refs = [s.reference(i) for i in range(6)]
+
+    def build(i):
+        return gpu_estimator(pydense, refs[i], params_for(refs[i]), storage="f16")
+
+    sizes = [build(i).device_bytes() for i in range(len(refs))]
+    budget = int(2.5 * max(sizes))
+    for steps in (0, 3):
+        got, groups = run_admitted(pydense, len(refs), build, budget=budget, refine_steps=steps,
+                                   on_result=lambda i, e: e.download(["depth", "normal", "cost"]))
+        assert [g.refs for g in groups] == [[0, 1], [2, 3], [4, 5]]
+        assert all(g.device_bytes <= budget and g.run_ms > 0 for g in groups)
+        assert all((g.refine_ms > 0) == (steps > 0) for g in groups)
+        print(f"aerial groups (refine {steps}): " + ", ".join(f"{g.refs} {g.run_ms:.2f}+{g.refine_ms:.2f} ms"
+                                                             for g in groups), flush=True)
+        for i in range(len(refs)):
+            est = build(i)
+            est.compute()
+            if steps:
+                est.gauss_newton(steps, 1e-3)
+            want = est.download(["depth", "normal", "cost"])
+            assert set(got[i]) == {"depth", "normal", "cost"}
+            for k in want:
+                assert np.array_equal(got[i][k], want[k]), (steps, i, k)
+
+
+@pytest.mark.gpu
+def test_aerial_reference_matches_oracle(pydense, survey):
+    sc = survey.reference(4)
+    p = params_for(sc)
+    d = gpu_estimator(pydense, sc, p, storage="f16").compute()["depth"]
+    d_o, _, _, _, _ = po.estimate(half_round(sc.images), sc.Ks, sc.Rs, sc.ts, p)
+    assert np.array_equal(d, d_o)
+    gt = sc.gt_depth
+    print(f"aerial reference 4: GT within 1% {np.mean(np.abs(d - gt) < 0.01 * gt):.4f}", flush=True)

@@ -102,6 +102,8 @@ def test_pydense_surface_names():
     for f in ("sfm_planar_prior", "delaunay", "select_neighbors", "lz4_frame_compress", "lz4_frame_decompress"):
         assert hasattr(pd, f), f
     assert hasattr(pd.DepthmapEstimator, "set_sfm_points") and hasattr(pd.DepthmapClusterEstimator, "set_sfm_points")
+    for m in ("device_bytes", "release_device", "device_free_memory", "gauss_newton", "download"):
+        assert hasattr(cls, m), m
 
 
 @pytest.mark.gpu
