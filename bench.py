@@ -233,8 +233,8 @@ def run_c5(args, pydense) -> int:
 
     def group_log(g):
         log(f"group of {len(g.refs)} (refs {g.refs[0]}..{g.refs[-1]}, {g.device_bytes / 1e9:.1f} GB): build "
-            f"{g.build_s:.1f} s, prepare {g.prepare_s:.1f} s, PatchMatch {g.run_ms:.0f} ms, Gauss-Newton "
-            f"{g.refine_ms:.0f} ms, results {g.results_s:.1f} s, wall {g.wall_s:.1f} s")
+            f"{g.build_s:.1f} s (waited {g.wait_s:.1f} s), prepare {g.prepare_s:.1f} s, PatchMatch {g.run_ms:.0f} ms, Gauss-Newton "
+            f"{g.refine_ms:.0f} ms, results {g.results_s:.1f} s, release {g.release_s:.1f} s, wall {g.wall_s:.1f} s")
 
     log(f"device budget {budget / 1e9:.1f} GB; one reference needs {build(0).device_bytes() / 1e9:.2f} GB")
     for i in range(args.warmup):        # first use: autotuner, code objects
@@ -301,9 +301,11 @@ def run_c5(args, pydense) -> int:
                    if gn_ms > 0 else None, "unit": "Mpx·views·steps/s"},
         "admission": {"budget_bytes": budget, "groups": len(groups),
                       "per_group": [{"refs": len(g.refs), "device_gb": round(g.device_bytes / 1e9, 2),
-                                     "build_s": round(g.build_s, 2), "prepare_s": round(g.prepare_s, 2),
+                                     "build_s": round(g.build_s, 2), "wait_s": round(g.wait_s, 2),
+                                     "prepare_s": round(g.prepare_s, 2),
                                      "run_ms": round(g.run_ms, 1), "refine_ms": round(g.refine_ms, 1),
-                                     "results_s": round(g.results_s, 2), "wall_s": round(g.wall_s, 2)}
+                                     "results_s": round(g.results_s, 2), "release_s": round(g.release_s, 2),
+                                     "wall_s": round(g.wall_s, 2)}
                                     for g in groups]},
         "cpu_baseline": cpu,
         "parity": parity,

@@ -276,6 +276,17 @@ int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches);
  * budget (the reference frees each reference's buffers after its run,
  * cluster_estimator.cc:150). */
 int pm_device_bytes(pm_handle* h, uint64_t* bytes);
+/* Keep up to max_bytes of the device buffers that pm_release_device and
+ * pm_destroy give back, for reuse by later handles that need blocks of the
+ * same sizes (0: free them at once, the default; lowering the limit frees
+ * blocks).  No counterpart in the reference (which frees each reference's
+ * buffers, cluster_estimator.cc:150); for runs of many references of one
+ * shape in memory-admitted groups, where freeing and re-allocating the
+ * buffers between groups costs seconds.  Cached blocks count as used device
+ * memory; a failed allocation frees them all and tries again. */
+int pm_set_buffer_cache(int device, uint64_t max_bytes);
+/* Bytes held by the buffer cache of a device. */
+int pm_buffer_cache_bytes(int device, uint64_t* held);
 
 /* ---- SfM planar prior and neighbour selection --------------------------
  * Host-side steps around the PatchMatch path. */

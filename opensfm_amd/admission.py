@@ -14,13 +14,18 @@ A group runs as one ``DepthmapBatch`` (one launch per kernel for all its
 references, captured in a HIP graph), is refined by Gauss-Newton when asked,
 and hands each estimator to ``on_result`` while its results are on the
 device; then the group's device memory is released and its estimators are
-dropped before the next group is built, so the host holds one group's
-images at a time.
+dropped.  While a group runs on the device, a worker thread builds the next
+group's estimators (host work: views and parameters), so the host holds at
+most two groups' images.  During the run the library keeps released device
+buffers for reuse (``DepthmapEstimator.set_buffer_cache``, up to the
+budget): groups of same-shaped references then reuse the blocks of the
+previous group instead of freeing and re-allocating them.
 """
 from __future__ import annotations
 
 import dataclasses
 import time
+from concurrent.futures import ThreadPoolExecutor
 from typing import Any, Callable, Dict, List, Optional, Sequence
 
 
@@ -28,11 +33,13 @@ from typing import Any, Callable, Dict, List, Optional, Sequence
 class GroupStats:
     refs: List[int]            # reference indices of the group
     device_bytes: int          # sum of the references' device_bytes()
-    build_s: float = 0.0       # host: building the estimators (views, parameters)
+    build_s: float = 0.0       # host: building the estimators (views, parameters), on the worker
+    wait_s: float = 0.0        # time the run waited for that build (the part not overlapped)
     prepare_s: float = 0.0     # staging, first-use tuning and graph capture
     run_ms: float = 0.0        # device time of the batched PatchMatch run
     refine_ms: float = 0.0     # device time of the Gauss-Newton steps (sum over the group)
-    results_s: float = 0.0     # on_result calls and releasing the device memory
+    results_s: float = 0.0     # on_result calls (downloads)
+    release_s: float = 0.0     # destroying the batch, releasing the device memory
     wall_s: float = 0.0        # the whole group, end to end
 
 
@@ -68,15 +75,17 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                  refine_steps: int = 0, damping: float = 1e-3,
                  on_result: Optional[Callable[[int, Any], Any]] = None, device: int = 0,
                  batch_config: Optional[Dict[str, Any]] = None,
-                 on_group: Optional[Callable[[GroupStats], None]] = None):
+                 on_group: Optional[Callable[[GroupStats], None]] = None, reuse_buffers: bool = True):
     """Run references 0..count-1 in memory-admitted groups.
 
     ``build(i)`` returns a configured ``DepthmapEstimator`` of reference i
-    (views added, parameters set, on ``device``).  ``on_result(i, est)`` is
-    called once per reference after its group's run and refinement; its
-    return value is collected (default: ``est.download(["depth"])``).
-    ``batch_config`` goes to ``DepthmapBatch.set_config``.  ``on_group`` is
-    called with each finished group's timing.  Returns
+    (views added, parameters set, on ``device``); it is called in index
+    order on a worker thread.  ``on_result(i, est)`` is called once per
+    reference after its group's run and refinement; its return value is
+    collected (default: ``est.download(["depth"])``).  ``batch_config`` goes
+    to ``DepthmapBatch.set_config``.  ``on_group`` is called with each
+    finished group's timing.  ``reuse_buffers``: keep released device
+    buffers for the next group (the cache is emptied at the end).  Returns
     ``(results: {i: value}, groups: [GroupStats])``.
     """
     if count < 0:
@@ -89,19 +98,18 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
         on_result = lambda i, est: est.download(["depth"])  # noqa: E731
     results: Dict[int, Any] = {}
     groups: List[GroupStats] = []
-    pending = None            # an estimator built but not admitted into the previous group
-    i = 0
-    while i < count or pending is not None:
-        t_group = time.perf_counter()
-        ests, refs, used = [], [], 0
+
+    def next_group(pending, i):
+        """Estimators from index i on (after ``pending``, built earlier but
+        not admitted) while they fit the budget."""
         t0 = time.perf_counter()
+        ests, refs, used = [], [], 0
         while True:
             if pending is not None:
                 idx, est, need = pending
                 pending = None
             elif i < count:
-                idx = i
-                est = build(i)
+                idx, est = i, build(i)
                 need = int(est.device_bytes())
                 i += 1
             else:
@@ -112,30 +120,50 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
             ests.append(est)
             refs.append(idx)
             used += need
-        st = GroupStats(refs=refs, device_bytes=used, build_s=time.perf_counter() - t0)
-        batch = pydense.DepthmapBatch(device)
-        for e in ests:
-            batch.add(e)
-        if batch_config:
-            batch.set_config(**batch_config)
-        t0 = time.perf_counter()
-        batch.prepare()
-        st.prepare_s = time.perf_counter() - t0
-        st.run_ms = batch.run()
-        for e in ests:
-            if refine_steps > 0:
-                e.gauss_newton(refine_steps, damping)
-                st.refine_ms += e.last_refine_ms()
-        t0 = time.perf_counter()
-        for idx, e in zip(refs, ests):
-            results[idx] = on_result(idx, e)
-        del batch
-        for e in ests:
-            e.release_device()
-        ests.clear()
-        st.results_s = time.perf_counter() - t0
-        st.wall_s = time.perf_counter() - t_group
-        groups.append(st)
-        if on_group is not None:
-            on_group(st)
+        return ests, refs, used, pending, i, time.perf_counter() - t0
+
+    est_cls = pydense.DepthmapEstimator
+    if reuse_buffers:
+        est_cls.set_buffer_cache(device, budget)
+    try:
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            fut = pool.submit(next_group, None, 0)
+            while fut is not None:
+                t_group = time.perf_counter()
+                ests, refs, used, pending, i, build_s = fut.result()
+                wait_s = time.perf_counter() - t_group
+                fut = pool.submit(next_group, pending, i) if (pending is not None or i < count) else None
+                if not ests:
+                    break
+                st = GroupStats(refs=refs, device_bytes=used, build_s=build_s, wait_s=wait_s)
+                batch = pydense.DepthmapBatch(device)
+                for e in ests:
+                    batch.add(e)
+                if batch_config:
+                    batch.set_config(**batch_config)
+                t0 = time.perf_counter()
+                batch.prepare()
+                st.prepare_s = time.perf_counter() - t0
+                st.run_ms = batch.run()
+                for e in ests:
+                    if refine_steps > 0:
+                        e.gauss_newton(refine_steps, damping)
+                        st.refine_ms += e.last_refine_ms()
+                t0 = time.perf_counter()
+                for idx, e in zip(refs, ests):
+                    results[idx] = on_result(idx, e)
+                st.results_s = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                del batch
+                for e in ests:
+                    e.release_device()
+                ests.clear()
+                st.release_s = time.perf_counter() - t0
+                st.wall_s = time.perf_counter() - t_group
+                groups.append(st)
+                if on_group is not None:
+                    on_group(st)
+    finally:
+        if reuse_buffers:
+            est_cls.set_buffer_cache(device, 0)
     return results, groups

@@ -96,6 +96,73 @@ struct HostView {
   std::vector<uint8_t> raw;  // the u8 input, kept for the image pyramid
 };
 
+// Device blocks released by idle handles, kept for reuse by size
+// (pm_set_buffer_cache), per device.  Runs of many references of one shape
+// then stop freeing and re-allocating hundreds of GB between groups.
+// Blocks enter the cache only from pm_release_device / pm_destroy, after
+// the handle's stream has drained.
+struct BlockCache {
+  std::mutex mu;
+  uint64_t limit = 0, held = 0;
+  std::vector<std::pair<size_t, void*>> blocks;
+};
+constexpr int kMaxCacheDevices = 64;
+BlockCache& block_cache(int device) {
+  static BlockCache caches[kMaxCacheDevices];
+  return caches[device >= 0 && device < kMaxCacheDevices ? device : 0];
+}
+int current_device() {
+  int d = 0;
+  return hipGetDevice(&d) == hipSuccess ? d : 0;
+}
+// Frees cached blocks of the current device until at most keep bytes stay.
+void trim_cache(BlockCache& c, uint64_t keep) {
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    while (c.held > keep && !c.blocks.empty()) {
+      c.held -= c.blocks.back().first;
+      drop.push_back(c.blocks.back().second);
+      c.blocks.pop_back();
+    }
+  }
+  for (void* q : drop) (void)hipFree(q);
+}
+void* device_alloc(size_t bytes) {
+  BlockCache& c = block_cache(current_device());
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (size_t i = 0; i < c.blocks.size(); ++i) {
+      if (c.blocks[i].first == bytes) {
+        void* q = c.blocks[i].second;
+        c.blocks[i] = c.blocks.back();
+        c.blocks.pop_back();
+        c.held -= bytes;
+        return q;
+      }
+    }
+  }
+  void* q = nullptr;
+  if (hipMalloc(&q, bytes) == hipSuccess) return q;
+  (void)hipGetLastError();
+  trim_cache(c, 0);  // blocks of other sizes may be what is missing
+  if (hipMalloc(&q, bytes) == hipSuccess) return q;
+  (void)hipGetLastError();
+  return nullptr;
+}
+void device_park(void* q, size_t bytes) {
+  BlockCache& c = block_cache(current_device());
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (c.held + bytes <= c.limit) {
+      c.blocks.emplace_back(bytes, q);
+      c.held += bytes;
+      return;
+    }
+  }
+  (void)hipFree(q);
+}
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -104,8 +171,8 @@ struct DevBuf {
     if (count == n && p) return PM_OK;
     release();
     if (count == 0) return PM_OK;
-    if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
-      p = nullptr;
+    p = static_cast<T*>(device_alloc(count * sizeof(T)));
+    if (!p) {
       n = 0;
       return fail(PM_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
     }
@@ -114,6 +181,12 @@ struct DevBuf {
   }
   void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Into the block cache (the owner's stream has drained).
+  void park() {
+    if (p) device_park(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }
@@ -190,15 +263,25 @@ struct pm_handle {
   // batch knows when its argument tables are stale.
   uint64_t gen = 0;
 
+  // Device buffers into the block cache (or freed); the stream has drained.
+  // The coarse level's result stays unless keep_coarse is false: the next
+  // pyramid level starts from it after the handle released its buffers.
+  void park_buffers(bool keep_coarse) {
+    images.park(); quads.park(); gviews.park(); gcams.park(); ref_taps.park(); ref_sums.park();
+    vgeom.park();
+    ca.park(); vwb.park(); wnb.park(); cpos.park(); nref.park(); refp.park();
+    fb_masks.park();
+    planes.park(); prior.park();
+    snapshot.park(); costs.park(); prevd.park(); rng.park(); sel.park(); masks.park();
+    out_depth.park(); out_normal.park(); out_cost.park(); out_conf.park();
+    if (!keep_coarse) {
+      coarse_depth.park();
+      coarse_normal.park();
+    }
+  }
+
   ~pm_handle() {
-    images.release(); quads.release(); gviews.release(); gcams.release(); ref_taps.release(); ref_sums.release();
-    vgeom.release();
-    ca.release(); vwb.release(); wnb.release(); cpos.release(); nref.release(); refp.release();
-    fb_masks.release();
-    planes.release(); prior.release();
-    snapshot.release(); costs.release(); prevd.release(); rng.release(); sel.release(); masks.release();
-    out_depth.release(); out_normal.release(); out_cost.release(); out_conf.release();
-    coarse_depth.release(); coarse_normal.release();
+    park_buffers(false);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : kev) if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1230,13 +1313,7 @@ int pm_release_device(pm_handle* h) {
   ++h->gen;
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   PM_HIP(hipStreamSynchronize(h->stream), "pm_release_device");
-  h->images.release(); h->quads.release(); h->gviews.release(); h->gcams.release(); h->vgeom.release();
-  h->ref_taps.release(); h->ref_sums.release();
-  h->ca.release(); h->vwb.release(); h->wnb.release(); h->cpos.release(); h->nref.release(); h->refp.release();
-  h->fb_masks.release();
-  h->planes.release(); h->prior.release(); h->snapshot.release(); h->costs.release(); h->prevd.release();
-  h->rng.release(); h->sel.release(); h->masks.release();
-  h->out_depth.release(); h->out_normal.release(); h->out_cost.release(); h->out_conf.release();
+  h->park_buffers(true);
   h->uploaded = false;
   h->state_ready = false;
   h->ref_ready = false;
@@ -1981,6 +2058,32 @@ int pm_batch_stage_timing(pm_batch* b, int stage, float* ms, int* launches) {
 }  // extern "C"
 
 extern "C" {
+
+int pm_set_buffer_cache(int device, uint64_t max_bytes) {
+  int n = 0;
+  PM_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (device < 0 || device >= n || device >= kMaxCacheDevices) return fail(PM_E_INVALID, "no such device");
+  int prev = 0;
+  PM_HIP(hipGetDevice(&prev), "hipGetDevice");
+  PM_HIP(hipSetDevice(device), "hipSetDevice");
+  BlockCache& c = block_cache(device);
+  {
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.limit = max_bytes;
+  }
+  trim_cache(c, max_bytes);
+  PM_HIP(hipSetDevice(prev), "hipSetDevice");
+  return PM_OK;
+}
+
+int pm_buffer_cache_bytes(int device, uint64_t* held) {
+  if (!held) return fail(PM_E_INVALID, "null output");
+  if (device < 0 || device >= kMaxCacheDevices) return fail(PM_E_INVALID, "no such device");
+  BlockCache& c = block_cache(device);
+  std::lock_guard<std::mutex> lock(c.mu);
+  *held = c.held;
+  return PM_OK;
+}
 
 int pm_device_bytes(pm_handle* h, uint64_t* bytes) {
   if (int r = check_handle(h)) return r;

@@ -331,7 +331,10 @@ __device__ __forceinline__ float census_inner(const Ref& ref, const Sampler& sm,
                                               int cx, int cy, const Warp& w) {
   uint32_t rc = 0u, sc = 0u;
   int bit = 0;
+  // rows one at a time: a rare path that must not set the kernel's registers
+#pragma unroll 1
   for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
       const float ref_val = ref(cx + dx, cy + dy);
@@ -1427,12 +1430,19 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 #define PM_BLOCK static_cast<int>(blockIdx.x)
 
 // Occupancy targets (waves per SIMD) of the cost kernels and the fallback;
-// 0 = the compiler's choice.
+// 0 = the compiler's choice.  The fallback is compiled for at most
+// PM_FALLBACK_WAVES_MAX waves per SIMD: its strided gathers need many loads
+// in flight per wave, and more resident waves only compete for L1/L2 (at
+// 1600x1200, N=8: 33 ms per step at 3 waves, 46 ms at the 6 waves the
+// compiler picks on its own).
 #ifndef PM_COST_WAVES_PER_EU
 #define PM_COST_WAVES_PER_EU 0
 #endif
 #ifndef PM_FALLBACK_WAVES_PER_EU
 #define PM_FALLBACK_WAVES_PER_EU 0
+#endif
+#ifndef PM_FALLBACK_WAVES_MAX
+#define PM_FALLBACK_WAVES_MAX 3
 #endif
 #if PM_COST_WAVES_PER_EU
 #define PM_COST_ATTR __attribute__((amdgpu_waves_per_eu(PM_COST_WAVES_PER_EU)))
@@ -1442,7 +1452,7 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 #if PM_FALLBACK_WAVES_PER_EU
 #define PM_FALLBACK_ATTR __attribute__((amdgpu_waves_per_eu(PM_FALLBACK_WAVES_PER_EU)))
 #else
-#define PM_FALLBACK_ATTR
+#define PM_FALLBACK_ATTR __attribute__((amdgpu_waves_per_eu(1, PM_FALLBACK_WAVES_MAX)))
 #endif
 
 template <int HP, typename T>

@@ -60,8 +60,10 @@ void add_view_to(pm_handle* h, py::handle K, py::handle R, py::handle t, py::arr
     auto ma = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>::ensure(mask);
     m.assign(ma.data(), ma.data() + ma.size());
   }
-  check(pm_add_view(h, k.data(), r.data(), tt.data(), img.data(), fmt, static_cast<int>(img.shape(1)),
-                    static_cast<int>(img.shape(0)), 0, m.empty() ? nullptr : m.data()));
+  const void* data = img.data();
+  const int w = static_cast<int>(img.shape(1)), hgt = static_cast<int>(img.shape(0));
+  py::gil_scoped_release nogil;  // img keeps the buffer alive
+  check(pm_add_view(h, k.data(), r.data(), tt.data(), data, fmt, w, hgt, 0, m.empty() ? nullptr : m.data()));
 }
 
 // (N, 3) float64 world points -> pm_set_sfm_points.
@@ -761,6 +763,14 @@ PYBIND11_MODULE(pydense, m) {
         int n = 0;
         return pm_device_count(&n) == PM_OK && n > 0;
       })
+      .def_static("set_buffer_cache", [](int i, uint64_t max_bytes) { check(pm_set_buffer_cache(i, max_bytes)); },
+                  py::arg("device"), py::arg("max_bytes"),
+                  "Keep up to max_bytes of released device buffers for reuse by later estimators (0: off)")
+      .def_static("buffer_cache_bytes", [](int i) {
+        uint64_t held = 0;
+        check(pm_buffer_cache_bytes(i, &held));
+        return held;
+      }, py::arg("device") = 0)
       .def_static("device_free_memory", [](int i) {
         uint64_t total = 0, free_b = 0;
         check(pm_device_memory(i, &total, &free_b));

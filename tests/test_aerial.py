@@ -136,6 +136,7 @@ def test_admitted_groups_equal_single_runs(pydense, survey):
         got, groups = run_admitted(pydense, len(refs), build, budget=budget, refine_steps=steps,
                                    on_result=lambda i, e: e.download(["depth", "normal", "cost"]))
         assert [g.refs for g in groups] == [[0, 1], [2, 3], [4, 5]]
+        assert pydense.DepthmapEstimator.buffer_cache_bytes(0) == 0       # emptied at the end
         assert all(g.device_bytes <= budget and g.run_ms > 0 for g in groups)
         assert all((g.refine_ms > 0) == (steps > 0) for g in groups)
         print(f"aerial groups (refine {steps}): " + ", ".join(f"{g.refs} {g.run_ms:.2f}+{g.refine_ms:.2f} ms"
@@ -160,3 +161,27 @@ def test_aerial_reference_matches_oracle(pydense, survey):
     assert np.array_equal(d, d_o)
     gt = sc.gt_depth
     print(f"aerial reference 4: GT within 1% {np.mean(np.abs(d - gt) < 0.01 * gt):.4f}", flush=True)
+
+
+@pytest.mark.gpu
+def test_buffer_cache_reuses_released_blocks(pydense, survey):
+    E = pydense.DepthmapEstimator
+    refs = [survey.reference(i) for i in (1, 2)]
+    fresh = [gpu_estimator(pydense, r, params_for(r), storage="f16").compute()["depth"] for r in refs]
+    try:
+        E.set_buffer_cache(0, 1 << 40)
+        a = gpu_estimator(pydense, refs[0], params_for(refs[0]), storage="f16")
+        assert np.array_equal(a.compute()["depth"], fresh[0])
+        assert E.buffer_cache_bytes(0) == 0
+        a.release_device()
+        held = E.buffer_cache_bytes(0)
+        assert held >= 0.9 * a.device_bytes()
+        b = gpu_estimator(pydense, refs[1], params_for(refs[1]), storage="f16")
+        assert np.array_equal(b.compute()["depth"], fresh[1])            # on the recycled blocks
+        assert E.buffer_cache_bytes(0) < 0.1 * held
+        assert np.array_equal(a.compute()["depth"], fresh[0])            # uploads again
+        E.set_buffer_cache(0, held // 2)
+        assert E.buffer_cache_bytes(0) <= held // 2
+    finally:
+        E.set_buffer_cache(0, 0)
+    assert E.buffer_cache_bytes(0) == 0
