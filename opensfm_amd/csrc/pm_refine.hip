@@ -102,11 +102,33 @@ __device__ __forceinline__ ViewTerm ncc_view(const PmArgs& a, const Sampler& sm,
   const float scen = sample_grad<T>(sm, u[0], v[0], gcx, gcy);
   float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
   float ds_s[3] = {0, 0, 0}, ds_ss[3] = {0, 0, 0}, ds_rs[3] = {0, 0, 0};
+  // Tap (i, j) sits at uv0 + i (uv1 - uv0) + j (uv2 - uv0): its tangent is
+  // du0 + i (du1 - du0) + j (du2 - du0), one FMA per tap from a per-row base;
+  // the centre sample's term is the same for every tap.
+  float au[3], av[3], bu[3], bv[3], cterm[3];
+  if constexpr (kGrad) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      au[j] = du[1][j] - du[0][j];
+      av[j] = dv[1][j] - dv[0][j];
+      bu[j] = du[2][j] - du[0][j];
+      bv[j] = dv[2][j] - dv[0][j];
+      cterm[j] = fmaf(gcx, du[0][j], gcy * dv[0][j]);
+    }
+  }
   constexpr int D = 2 * HP + 1;
   const PM_GLOBAL float2* taps = gp(a.ref_taps);
 #pragma unroll 1
   for (int iy = -HP; iy <= HP; ++iy) {
     const float fiy = static_cast<float>(iy);
+    float rowu[3], rowv[3];
+    if constexpr (kGrad) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        rowu[j] = fmaf(fiy, bu[j], du[0][j]);
+        rowv[j] = fmaf(fiy, bv[j], dv[0][j]);
+      }
+    }
 #pragma unroll
     for (int ix = -HP; ix <= HP; ++ix) {
       const float fix = static_cast<float>(ix);
@@ -120,12 +142,11 @@ __device__ __forceinline__ ViewTerm ncc_view(const PmArgs& a, const Sampler& sm,
       s_ss = fmaf(wsd, sd, s_ss);
       s_rs = fmaf(wv.y, sd, s_rs);
       if constexpr (kGrad) {
-        const float c0w = 1.0f - fix - fiy;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          const float dsx = c0w * du[0][j] + fix * du[1][j] + fiy * du[2][j];
-          const float dsy = c0w * dv[0][j] + fix * dv[1][j] + fiy * dv[2][j];
-          const float dsd = (gx * dsx + gy * dsy) - (gcx * du[0][j] + gcy * dv[0][j]);
+          const float dsx = fmaf(fix, au[j], rowu[j]);
+          const float dsy = fmaf(fix, av[j], rowv[j]);
+          const float dsd = fmaf(gx, dsx, gy * dsy) - cterm[j];
           ds_s[j] = fmaf(wv.x, dsd, ds_s[j]);
           ds_ss[j] = fmaf(2.0f * wsd, dsd, ds_ss[j]);
           ds_rs[j] = fmaf(wv.y, dsd, ds_rs[j]);
