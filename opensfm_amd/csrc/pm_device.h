@@ -129,8 +129,10 @@ __device__ __forceinline__ float pm_logf(float x) {
 __device__ __forceinline__ float rand_float(uint32_t& s0, uint32_t& s1, uint32_t key) {
 #pragma unroll
   for (uint32_t i = 0; i < 4; ++i) {
-    const uint32_t lo = s0 * 0xD256D193u;
-    const uint32_t hi = __umulhi(s0, 0xD256D193u);
+    // one 32 x 32 -> 64 product gives both halves (a single v_mad_u64_u32)
+    const uint64_t prod = static_cast<uint64_t>(s0) * 0xD256D193u;
+    const uint32_t lo = static_cast<uint32_t>(prod);
+    const uint32_t hi = static_cast<uint32_t>(prod >> 32);
     s0 = s1 ^ (key + i * 0x9E3779B9u) ^ hi;
     s1 = lo;
   }
