@@ -379,13 +379,15 @@ __device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const Ref& ref, 
 #pragma unroll 1
   for (int iy = -HP; iy <= HP; ++iy) {
     const float fdy = stride * static_cast<float>(iy);
+    const float bx = fmaf(w.jyx, fdy, w.scx);
+    const float by = fmaf(w.jyy, fdy, w.scy);
     const float* rrow = nullptr;
     if constexpr (kLds) rrow = ref.t + (y + SS * iy - ref.y0) * ref.pitch + (x - ref.x0);
 #pragma unroll
     for (int ix = -HP; ix <= HP; ++ix) {
       const float fdx = stride * static_cast<float>(ix);
-      const float sx = w.scx + w.jxx * fdx + w.jyx * fdy;
-      const float sy = w.scy + w.jxy * fdx + w.jyy * fdy;
+      const float sx = fmaf(w.jxx, fdx, bx);
+      const float sy = fmaf(w.jxy, fdx, by);
       const float sd = sample<T>(sm, sx, sy) - src_center;
       float rv;
       if constexpr (kLds) {
@@ -398,9 +400,10 @@ __device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const Ref& ref, 
       const float r = rv - rs.x;
       const float wt = pm_expf_nonpos(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
       const float wr = wt * r;
-      sum_s += wt * sd;
-      sum_ss += wt * sd * sd;
-      sum_rs += wr * sd;
+      const float wsd = wt * sd;
+      sum_s += wsd;
+      sum_ss = fmaf(wsd, sd, sum_ss);
+      sum_rs = fmaf(wr, sd, sum_rs);
     }
   }
   if (rs.y < 1e-6f) return -1.0f;

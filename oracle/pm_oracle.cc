@@ -323,24 +323,29 @@ float ncc_at_stride(const Ctx& c, int s, float ref_center, float src_center, flo
   const int hp = c.pr->half_patch;
   float sum_w = 0.0f, sum_r = 0.0f, sum_s = 0.0f, sum_rr = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
   for (int iy = -hp; iy <= hp; ++iy) {
+    const float fdy = stride * static_cast<float>(iy);
+    // source coordinates and sums contracted as the reference's
+    // -cl-fast-relaxed-math build allows (opencl_utils.h:84-85)
+    const float bx = std::fmaf(jyx, fdy, scx);
+    const float by = std::fmaf(jyy, fdy, scy);
     for (int ix = -hp; ix <= hp; ++ix) {
       const float fdx = stride * static_cast<float>(ix);
-      const float fdy = stride * static_cast<float>(iy);
       const float rx = static_cast<float>(cx) + fdx;
       const float ry = static_cast<float>(cy) + fdy;
       const float ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));
-      const float sx = scx + jxx * fdx + jyx * fdy;
-      const float sy = scy + jxy * fdx + jyy * fdy;
+      const float sx = std::fmaf(jxx, fdx, bx);
+      const float sy = std::fmaf(jxy, fdx, by);
       const float src_val = c.src[s].bilinear(sx, sy);
       const float r = ref_val - ref_center;
       const float w = pm_expf(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +
                               r * r * c.color_factor);
       const float sv = src_val - src_center;
+      const float wsv = w * sv;
       sum_r += w * r;
-      sum_s += w * sv;
+      sum_s += wsv;
       sum_rr += w * r * r;
-      sum_ss += w * sv * sv;
-      sum_rs += w * r * sv;
+      sum_ss = std::fmaf(wsv, sv, sum_ss);
+      sum_rs = std::fmaf(w * r, sv, sum_rs);
       sum_w += w;
     }
   }
