@@ -77,14 +77,18 @@ def configure(est, scene, iters: int, storage: str = "f32"):
     est.set_image_storage(storage)
 
 
-def cpu_baseline(scene, iters: int, rows: int, out: dict = None):
+def cpu_baseline(scene, iters: int, rows: int, out: dict = None, half: bool = False):
     """CPU oracle on a band of reference rows (all sources), init + iterations.
-    With ``out``, the band scene and the oracle's depth go into it."""
+    ``half``: the images as the f16 storage holds them.  With ``out``, the
+    band scene and the oracle's depth go into it."""
     from oracle import pm_oracle as po
     y0 = scene.gt_depth.shape[0] // 2 - rows // 2
     band = scene.crop_rows(y0, y0 + rows)
     p = po.Params(patch_size=PATCH, depth_min=scene.depth_min, depth_max=scene.depth_max, max_iterations=iters)
-    sc = po.OracleScene([np.asarray(im, np.float32) for im in band.images], band.Ks, band.Rs, band.ts)
+    imgs = [np.asarray(im, np.float32) for im in band.images]
+    if half:
+        imgs = [im.astype(np.float16).astype(np.float32) for im in imgs]
+    sc = po.OracleScene(imgs, band.Ks, band.Rs, band.ts)
     st = po.OracleState(sc.width, sc.height)
     t0 = time.perf_counter()
     po.run(sc, p, st)
@@ -465,11 +469,21 @@ def main() -> int:
                   "gt_within_1pct_after": round(acc_a[-1], 4)}
         log(f"Gauss-Newton {args.refine_steps} steps: {ms:.2f} ms")
 
-    cpu = None
+    cpu, parity = None, None
     if args.cpu_rows > 0:
         log(f"CPU oracle on {args.cpu_rows} rows ...")
-        cpu = cpu_baseline(scene, args.iters, args.cpu_rows)
+        keep = {}
+        cpu = cpu_baseline(scene, args.iters, args.cpu_rows, keep, half=args.storage == "f16")
         cpu["value"] = round(cpu["value"], 5)
+        # the same rows on the GPU: parity of the measured path with the oracle
+        band_est = pydense.DepthmapEstimator()
+        configure(band_est, keep["band"], args.iters, args.storage)
+        d_band = band_est.compute()["depth"]
+        d_o = keep["depth"]
+        m = d_o > 0
+        parity = {"sample": "the CPU-baseline rows on the GPU, same inputs",
+                  "bit_equal": round(float(np.mean(d_band == d_o)), 6),
+                  "within_1pct_of_oracle": round(float(np.mean(np.abs(d_band[m] - d_o[m]) < 0.01 * d_o[m])), 6)}
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -483,6 +497,7 @@ def main() -> int:
         "roofline": roofline,
         "roofline_path": roofline_path,
         "cpu_baseline": cpu,
+        "parity": parity,
         "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
         "stage_ms_per_step": stage_ms,
         "pcie_inclusive_value": round(pcie_value, 3),
