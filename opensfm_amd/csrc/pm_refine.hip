@@ -213,9 +213,18 @@ __device__ __forceinline__ float pixel_energy(const PmArgs& a, const VGeom* vg, 
   return e * inv;
 }
 
+// Both kernels are compiled for at most PM_REFINE_WAVES_MAX waves per SIMD:
+// with the registers that frees, a thread keeps the gathers of a whole tap
+// row and its tangent sums in flight (at 1024x768, N=6: 10 Gauss-Newton
+// steps take 7.5 ms at 2 waves, 10.5 ms at the 3 the compiler picks).
+#ifndef PM_REFINE_WAVES_MAX
+#define PM_REFINE_WAVES_MAX 2
+#endif
+#define PM_REFINE_ATTR __attribute__((amdgpu_waves_per_eu(1, PM_REFINE_WAVES_MAX)))
+
 // cost[p] and grad[p] = d cost / d(d, a, b) for every reference pixel.
 template <int HP, typename T>
-__global__ __launch_bounds__(256) void ncc_energy_kernel(PmArgs a, const VGeom* __restrict__ vg,
+__global__ __launch_bounds__(256) PM_REFINE_ATTR void ncc_energy_kernel(PmArgs a, const VGeom* __restrict__ vg,
                                                         const float* __restrict__ depth, const float* __restrict__ ab,
                                                         const uint32_t* __restrict__ views, float* __restrict__ cost,
                                                         float* __restrict__ grad) {
@@ -262,7 +271,7 @@ __device__ __forceinline__ bool solve3(const float H[6], const float g[3], float
 // One damped Gauss-Newton step on the state planes of every pixel with a
 // view selection; a step is kept only where it lowers the pixel's cost.
 template <int HP, typename T>
-__global__ __launch_bounds__(256) void gauss_newton_kernel(PmArgs a, const VGeom* __restrict__ vg, float lambda) {
+__global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs a, const VGeom* __restrict__ vg, float lambda) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= static_cast<int64_t>(a.width) * a.height) return;
   const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
