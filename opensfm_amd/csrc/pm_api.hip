@@ -229,6 +229,7 @@ struct pm_handle {
   DevBuf<float> vgeom;         // per source: reference camera frame -> source pixels (12 floats)
   float last_gn_ms = 0.0f;
   DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
+  DevBuf<float> gn_state;      // Gauss-Newton state when ca is too small (one source)
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
   DevBuf<uint64_t> fb_masks;   // pending textureless fallbacks per (tile, slot, view)
@@ -274,6 +275,7 @@ struct pm_handle {
     planes.park(); prior.park();
     snapshot.park(); costs.park(); prevd.park(); rng.park(); sel.park(); masks.park();
     out_depth.park(); out_normal.park(); out_cost.park(); out_conf.park();
+    gn_state.park();
     if (!keep_coarse) {
       coarse_depth.park();
       coarse_normal.park();
@@ -1364,8 +1366,18 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   const PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
   PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
+  // The per-pixel state between steps lives in the colour passes' candidate
+  // cost buffer, idle after a run (the next run rewrites it), when it is big
+  // enough (two or more sources); else in a buffer of its own.
+  const size_t need = static_cast<size_t>(gauss_newton_state_floats()) * h->W * h->H;
+  float* state = h->ca.p;
+  if (h->ca.n < need) {
+    if (int r = h->gn_state.alloc(need)) return r;
+    state = h->gn_state.p;
+  }
   for (int i = 0; i < steps; ++i)
-    PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, h->stream), "Gauss-Newton kernel launch");
+    PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, state, i == 0, h->stream),
+           "Gauss-Newton kernel launch");
   PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
   if (int r = pm_decode(h)) return r;
   if (int r = postprocess(h)) return r;

@@ -149,9 +149,12 @@ hipError_t launch_storage_f16(int kernel, const PmArgs* a, const PmBatch* bt, un
 // H*W view masks; cost: H*W mean residual, grad: 3*H*W (may be null).
 hipError_t launch_ncc_energy(const PmArgs& a, int half_patch, bool f16, const float* vgeom, const float* depth,
                              const float* ab, const uint32_t* views, float* cost, float* grad, hipStream_t stream);
-// One damped Gauss-Newton step on a.planes over the views of a.sel.
+// One damped Gauss-Newton step on a.planes over the views of a.sel.  state:
+// gauss_newton_state_floats() x W*H floats carried from step to step; first:
+// the first step of a refinement (starts from a.planes).
+int gauss_newton_state_floats();
 hipError_t launch_gauss_newton(const PmArgs& a, int half_patch, bool f16, const float* vgeom, float lambda,
-                               hipStream_t stream);
+                               float* state, bool first, hipStream_t stream);
 // conf = logistic(cost) where depth > 0, else 0 (n pixels).
 hipError_t launch_confidence(int64_t n, const float* depth, const float* cost, float* conf, hipStream_t stream);
 hipError_t launch_score(const PmArgs& a, int half_patch, bool f16, int n, const int32_t* xy,

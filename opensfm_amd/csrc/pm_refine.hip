@@ -268,34 +268,76 @@ __device__ __forceinline__ bool solve3(const float H[6], const float g[3], float
   return true;
 }
 
-// One damped Gauss-Newton step on the state planes of every pixel with a
-// view selection; a step is kept only where it lowers the pixel's cost.
+// One damped Gauss-Newton step per pixel with a view selection; a step is
+// kept only where it lowers the pixel's cost.  The trial point is scored
+// with its gradient: when the step is kept, that point, its cost and its
+// normal equations are the next step's start (state st, kGnState planes of
+// W*H floats: theta, e, used, H, g), so a step costs one gradient
+// evaluation.  A rejected step would be proposed again unchanged (fixed
+// damping), so the pixel is done (used = -1).  The first step starts from
+// the PatchMatch planes.
+constexpr int kGnState = 14;
+
 template <int HP, typename T>
-__global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs a, const VGeom* __restrict__ vg, float lambda) {
+__global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs a, const VGeom* __restrict__ vg,
+                                                                          float lambda, float* __restrict__ st,
+                                                                          int first) {
+  const int64_t n = static_cast<int64_t>(a.width) * a.height;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= static_cast<int64_t>(a.width) * a.height) return;
+  if (i >= n) return;
   const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
   const uint32_t views = a.sel[i];
   if (!views) return;
+  float* const s_used = st + 4 * n + i;
   const PmCam c0 = ((const PM_CONST PmCam*)(a.cams))[0];
-  const float4 p = a.planes[i];
-  if (!(p.z < -1e-6f)) return;
-  const float d = depth_from_plane(static_cast<float>(x), static_cast<float>(y), Plane{p.x, p.y, p.z, p.w}, c0);
-  if (!(d > 0.0f)) return;
-  const Theta th{d, p.x / -p.z, p.y / -p.z};
-  float gr[3], H[6], gv[3], step[3];
+  Theta th;
+  float e0, H[6], gv[3];
   int used;
-  const float e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used);
-  if (used == 0 || !solve3(H, gv, lambda, step)) return;
+  if (first) {
+    *s_used = __int_as_float(-1);
+    const float4 p = a.planes[i];
+    if (!(p.z < -1e-6f)) return;
+    const float d = depth_from_plane(static_cast<float>(x), static_cast<float>(y), Plane{p.x, p.y, p.z, p.w}, c0);
+    if (!(d > 0.0f)) return;
+    th = Theta{d, p.x / -p.z, p.y / -p.z};
+    float gr[3];
+    e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used);
+  } else {
+    used = __float_as_int(*s_used);
+    if (used <= 0) return;
+    th = Theta{st[i], st[n + i], st[2 * n + i]};
+    e0 = st[3 * n + i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) H[j] = st[(5 + j) * n + i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gv[j] = st[(11 + j) * n + i];
+  }
+  float step[3];
+  if (used == 0 || !solve3(H, gv, lambda, step)) {
+    *s_used = __int_as_float(-1);
+    return;
+  }
   Theta t1{th.d + step[0], th.a + step[1], th.b + step[2]};
   t1.d = fminf(fmaxf(t1.d, a.depth_min), a.depth_max);
-  float u0[3], u1[6], u2[3];
+  float gr1[3], H1[6], gv1[3];
   int used1;
-  const float e1 = pixel_energy<HP, T, false>(a, vg, x, y, views, t1, u0, u1, u2, used1);
-  if (used1 != used || !(e1 < e0)) return;
-  const V3 n = normalize3(V3{t1.a, t1.b, -1.0f});
-  const Plane np = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), t1.d, n, c0);
+  const float e1 = pixel_energy<HP, T, true>(a, vg, x, y, views, t1, gr1, H1, gv1, used1);
+  if (used1 != used || !(e1 < e0)) {
+    *s_used = __int_as_float(-1);
+    return;
+  }
+  const V3 nv = normalize3(V3{t1.a, t1.b, -1.0f});
+  const Plane np = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), t1.d, nv, c0);
   a.planes[i] = make_float4(np.x, np.y, np.z, np.w);
+  st[i] = t1.d;
+  st[n + i] = t1.a;
+  st[2 * n + i] = t1.b;
+  st[3 * n + i] = e1;
+  *s_used = __int_as_float(used1);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) st[(5 + j) * n + i] = H1[j];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) st[(11 + j) * n + i] = gv1[j];
 }
 
 }  // namespace
@@ -322,12 +364,15 @@ hipError_t launch_ncc_energy(const PmArgs& a, int hp, bool f16, const float* vge
   PM_REFINE_DISPATCH(ncc_energy_kernel, hp, f16, grid, a, vg, depth, ab, views, cost, grad);
 }
 
-hipError_t launch_gauss_newton(const PmArgs& a, int hp, bool f16, const float* vgeom, float lambda,
-                               hipStream_t stream) {
+int gauss_newton_state_floats() { return kGnState; }
+
+hipError_t launch_gauss_newton(const PmArgs& a, int hp, bool f16, const float* vgeom, float lambda, float* state,
+                               bool first, hipStream_t stream) {
   const int64_t n = static_cast<int64_t>(a.width) * a.height;
   const dim3 grid(static_cast<unsigned>((n + 255) / 256));
   const VGeom* vg = reinterpret_cast<const VGeom*>(vgeom);
-  PM_REFINE_DISPATCH(gauss_newton_kernel, hp, f16, grid, a, vg, lambda);
+  const int fst = first ? 1 : 0;
+  PM_REFINE_DISPATCH(gauss_newton_kernel, hp, f16, grid, a, vg, lambda, state, fst);
 }
 
 }  // namespace pmhip
