@@ -136,3 +136,30 @@ def test_gauss_newton_lowers_the_cost(pydense, bumps):
     assert np.mean(c1[m] <= c0[m] + 1e-4) >= 0.999
     assert c1[m].mean() < c0[m].mean()
     assert acc1 >= acc0 - 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("num_src", [1, 2])
+def test_gauss_newton_state_buffers(pydense, num_src):
+    # one source: the step-to-step state needs a buffer of its own (the idle
+    # candidate-cost buffer holds 9 floats per pixel per source, 14 are
+    # needed); two sources: the state lives in that buffer.  Both must lower
+    # the cost, and a second refinement restarts from the refined planes.
+    from opensfm_amd import synthetic
+    s = synthetic.three_planes(7, width=160, height=120, num_src=num_src, base_cell=0.3)
+    p = params_for(s, patch_size=5)
+    est = gpu_estimator(pydense, s, p)
+    before = est.compute()
+    d0, ab0 = _planes_to_theta(before["plane"], np.asarray(s.Ks[0]))
+    c0, _ = est.ncc_cost_grad(d0, ab0, before["views"])
+    est.gauss_newton(4, 1e-3)
+    mid = est.download()
+    est.gauss_newton(4, 1e-3)
+    after = est.download()
+    d1, ab1 = _planes_to_theta(after["plane"], np.asarray(s.Ks[0]))
+    c1, _ = est.ncc_cost_grad(d1, ab1, before["views"])
+    m = before["views"] > 0
+    print(f"{num_src} source(s): mean cost {c0[m].mean():.5f} -> {c1[m].mean():.5f}", flush=True)
+    assert np.all(np.isfinite(after["depth"]))
+    assert np.mean(c1[m] <= c0[m] + 1e-4) >= 0.999 and c1[m].mean() < c0[m].mean()
+    assert not np.array_equal(mid["plane"], before["plane"])
