@@ -243,14 +243,15 @@ def run_c5(args, pydense) -> int:
     log(f"device budget {budget / 1e9:.1f} GB; one reference needs {build(0).device_bytes() / 1e9:.2f} GB")
     for i in range(args.warmup):        # first use: autotuner, code objects
         run_admitted(pydense, min(2, views), build, budget=budget, refine_steps=args.refine_steps,
-                     on_result=lambda i, e: None)
+                     on_result=lambda i, e: None, overlap_staging=not args.no_overlap)
         log(f"warmup {i + 1}/{args.warmup} done")
     depths = None
     groups = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         depths, groups = run_admitted(pydense, views, build, budget=budget, refine_steps=args.refine_steps,
-                                      on_result=keep_depth, on_group=group_log)
+                                      on_result=keep_depth, on_group=group_log,
+                                      overlap_staging=not args.no_overlap)
         log(f"step {k + 1}/{args.steps}: {len(groups)} groups")
     elapsed = time.perf_counter() - t0
     value = units * args.steps / elapsed
@@ -294,6 +295,7 @@ def run_c5(args, pydense) -> int:
                                                          refine=args.refine_steps),
                    "references": views, "megapixels": round(views * w * h / 1e6, 3),
                    "iterations": args.iters, "patch": PATCH, "units_per_step": round(units, 3),
+                   "staging": "serial" if args.no_overlap else "overlapped with the previous group",
                    "value_scope": "whole job: building, staging, batched PatchMatch, Gauss-Newton and depth "
                                   "download of every group"},
         "roofline_path": {"scope": "batched PatchMatch device time of all groups (init, colour passes, decode)",
@@ -360,6 +362,8 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=4000, help="c5 image width")
     ap.add_argument("--height", type=int, default=3000, help="c5 image height")
     ap.add_argument("--neighbours", type=int, default=10, help="c5 sources per reference")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="c5: stage each group after the previous one instead of while it runs")
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
                     help="c5 device budget per group in GB (default: 90%% of the free device memory)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,

@@ -15,11 +15,14 @@ references, captured in a HIP graph), is refined by Gauss-Newton when asked,
 and hands each estimator to ``on_result`` while its results are on the
 device; then the group's device memory is released and its estimators are
 dropped.  While a group runs on the device, a worker thread builds the next
-group's estimators (host work: views and parameters), so the host holds at
-most two groups' images.  During the run the library keeps released device
-buffers for reuse (``DepthmapEstimator.set_buffer_cache``, up to the
-budget): groups of same-shaped references then reuse the blocks of the
-previous group instead of freeing and re-allocating them.
+group's estimators (host work: views and parameters) and, with
+``overlap_staging``, stages it (host conversion, uploads and graph capture
+on the batch's own stream): each group then gets half the budget, so that
+two groups fit at once.  The host holds at most two groups' images.  During
+the run the library keeps released device buffers for reuse
+(``DepthmapEstimator.set_buffer_cache``, up to the budget): groups of
+same-shaped references then reuse the blocks of an earlier group instead of
+freeing and re-allocating them.
 """
 from __future__ import annotations
 
@@ -34,8 +37,8 @@ class GroupStats:
     refs: List[int]            # reference indices of the group
     device_bytes: int          # sum of the references' device_bytes()
     build_s: float = 0.0       # host: building the estimators (views, parameters), on the worker
-    wait_s: float = 0.0        # time the run waited for that build (the part not overlapped)
-    prepare_s: float = 0.0     # staging, first-use tuning and graph capture
+    wait_s: float = 0.0        # time the run waited for the worker (the part not overlapped)
+    prepare_s: float = 0.0     # staging, first-use tuning and graph capture (on the worker when overlapped)
     run_ms: float = 0.0        # device time of the batched PatchMatch run
     refine_ms: float = 0.0     # device time of the Gauss-Newton steps (sum over the group)
     results_s: float = 0.0     # on_result calls (downloads)
@@ -75,7 +78,8 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                  refine_steps: int = 0, damping: float = 1e-3,
                  on_result: Optional[Callable[[int, Any], Any]] = None, device: int = 0,
                  batch_config: Optional[Dict[str, Any]] = None,
-                 on_group: Optional[Callable[[GroupStats], None]] = None, reuse_buffers: bool = True):
+                 on_group: Optional[Callable[[GroupStats], None]] = None, reuse_buffers: bool = True,
+                 overlap_staging: bool = False):
     """Run references 0..count-1 in memory-admitted groups.
 
     ``build(i)`` returns a configured ``DepthmapEstimator`` of reference i
@@ -85,7 +89,9 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
     collected (default: ``est.download(["depth"])``).  ``batch_config`` goes
     to ``DepthmapBatch.set_config``.  ``on_group`` is called with each
     finished group's timing.  ``reuse_buffers``: keep released device
-    buffers for the next group (the cache is emptied at the end).  Returns
+    buffers for the next group (the cache is emptied at the end).
+    ``overlap_staging``: stage the next group while the current one runs
+    (groups of half the budget).  Returns
     ``(results: {i: value}, groups: [GroupStats])``.
     """
     if count < 0:
@@ -98,10 +104,20 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
         on_result = lambda i, est: est.download(["depth"])  # noqa: E731
     results: Dict[int, Any] = {}
     groups: List[GroupStats] = []
+    group_budget = budget // 2 if overlap_staging else budget
+
+    def make_batch(ests):
+        batch = pydense.DepthmapBatch(device)
+        for e in ests:
+            batch.add(e)
+        if batch_config:
+            batch.set_config(**batch_config)
+        return batch
 
     def next_group(pending, i):
         """Estimators from index i on (after ``pending``, built earlier but
-        not admitted) while they fit the budget."""
+        not admitted) while they fit the group budget; staged when
+        overlapping."""
         t0 = time.perf_counter()
         ests, refs, used = [], [], 0
         while True:
@@ -114,13 +130,20 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                 i += 1
             else:
                 break
-            if refs and used + need > budget:
+            if refs and used + need > group_budget:
                 pending = (idx, est, need)
                 break
             ests.append(est)
             refs.append(idx)
             used += need
-        return ests, refs, used, pending, i, time.perf_counter() - t0
+        build_s = time.perf_counter() - t0
+        batch, prep_s = None, 0.0
+        if overlap_staging and ests:
+            batch = make_batch(ests)
+            t0 = time.perf_counter()
+            batch.prepare()
+            prep_s = time.perf_counter() - t0
+        return ests, refs, used, pending, i, build_s, batch, prep_s
 
     est_cls = pydense.DepthmapEstimator
     if reuse_buffers:
@@ -130,20 +153,17 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
             fut = pool.submit(next_group, None, 0)
             while fut is not None:
                 t_group = time.perf_counter()
-                ests, refs, used, pending, i, build_s = fut.result()
+                ests, refs, used, pending, i, build_s, batch, prep_s = fut.result()
                 wait_s = time.perf_counter() - t_group
                 fut = pool.submit(next_group, pending, i) if (pending is not None or i < count) else None
                 if not ests:
                     break
-                st = GroupStats(refs=refs, device_bytes=used, build_s=build_s, wait_s=wait_s)
-                batch = pydense.DepthmapBatch(device)
-                for e in ests:
-                    batch.add(e)
-                if batch_config:
-                    batch.set_config(**batch_config)
-                t0 = time.perf_counter()
-                batch.prepare()
-                st.prepare_s = time.perf_counter() - t0
+                st = GroupStats(refs=refs, device_bytes=used, build_s=build_s, wait_s=wait_s, prepare_s=prep_s)
+                if batch is None:
+                    batch = make_batch(ests)
+                    t0 = time.perf_counter()
+                    batch.prepare()
+                    st.prepare_s = time.perf_counter() - t0
                 st.run_ms = batch.run()
                 for e in ests:
                     if refine_steps > 0:

@@ -131,15 +131,17 @@ def test_admitted_groups_equal_single_runs(pydense, survey):
         return gpu_estimator(pydense, refs[i], params_for(refs[i]), storage="f16")
 
     sizes = [build(i).device_bytes() for i in range(len(refs))]
-    budget = int(2.5 * max(sizes))
-    for steps in (0, 3):
+    for steps, overlap in ((0, False), (3, False), (3, True)):
+        # overlapping groups get half the budget each: the same pairs either way
+        budget = int(2.5 * max(sizes)) * (2 if overlap else 1)
         got, groups = run_admitted(pydense, len(refs), build, budget=budget, refine_steps=steps,
-                                   on_result=lambda i, e: e.download(["depth", "normal", "cost"]))
+                                   on_result=lambda i, e: e.download(["depth", "normal", "cost"]),
+                                   overlap_staging=overlap)
         assert [g.refs for g in groups] == [[0, 1], [2, 3], [4, 5]]
         assert pydense.DepthmapEstimator.buffer_cache_bytes(0) == 0       # emptied at the end
         assert all(g.device_bytes <= budget and g.run_ms > 0 for g in groups)
         assert all((g.refine_ms > 0) == (steps > 0) for g in groups)
-        print(f"aerial groups (refine {steps}): " + ", ".join(f"{g.refs} {g.run_ms:.2f}+{g.refine_ms:.2f} ms"
+        print(f"aerial groups (refine {steps}, overlap {overlap}): " + ", ".join(f"{g.refs} {g.run_ms:.2f}+{g.refine_ms:.2f} ms"
                                                              for g in groups), flush=True)
         for i in range(len(refs)):
             est = build(i)
