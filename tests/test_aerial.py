@@ -187,3 +187,106 @@ def test_buffer_cache_reuses_released_blocks(pydense, survey):
     finally:
         E.set_buffer_cache(0, 0)
     assert E.buffer_cache_bytes(0) == 0
+
+
+class _StubEstimator:
+    def __init__(self, log, i, nbytes):
+        self.log, self.i, self.nbytes = log, i, nbytes
+
+    def device_bytes(self):
+        return self.nbytes
+
+    def gauss_newton(self, steps, damping):
+        self.log.append(("gn", self.i, steps))
+
+    def last_refine_ms(self):
+        return 1.0
+
+    def download(self, maps=None):
+        return {"depth": np.full((2, 2), float(self.i), np.float32)}
+
+    def release_device(self):
+        self.log.append(("release", self.i))
+
+
+class _StubLibrary:
+    """The parts of pydense that run_admitted uses, recording the calls."""
+
+    def __init__(self):
+        log = self.log = []
+
+        class Est:
+            @staticmethod
+            def set_buffer_cache(device, max_bytes):
+                log.append(("cache", max_bytes))
+
+            @staticmethod
+            def device_free_memory(device):
+                return 1000
+
+        class Batch:
+            def __init__(self, device):
+                self.refs = []
+
+            def add(self, e):
+                self.refs.append(e.i)
+
+            def set_config(self, **kw):
+                log.append(("config", kw))
+
+            def prepare(self):
+                log.append(("prepare", tuple(self.refs)))
+
+            def run(self):
+                log.append(("run", tuple(self.refs)))
+                return 5.0
+
+        self.DepthmapEstimator = Est
+        self.DepthmapBatch = Batch
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_run_admitted_control_flow(overlap):
+    from opensfm_amd.admission import run_admitted
+    lib = _StubLibrary()
+    sizes = [30, 30, 30, 50, 10, 90, 20]
+    built = []
+
+    def build(i):
+        built.append(i)
+        return _StubEstimator(lib.log, i, sizes[i])
+
+    budget = 200 if overlap else 100          # groups of at most 100 either way
+    res, groups = run_admitted(lib, len(sizes), build, budget=budget, refine_steps=2, overlap_staging=overlap,
+                               batch_config={"use_graph": True})
+    assert built == list(range(len(sizes)))
+    assert [g.refs for g in groups] == [[0, 1, 2], [3, 4], [5], [6]]
+    assert all(g.device_bytes <= 100 for g in groups)
+    assert sorted(res) == list(range(len(sizes))) and all(res[i]["depth"][0, 0] == i for i in res)
+    runs = [e[1] for e in lib.log if e[0] == "run"]
+    assert runs == [(0, 1, 2), (3, 4), (5,), (6,)]
+    assert [e[1] for e in lib.log if e[0] == "prepare"] == runs
+    for g in runs:       # every reference refined and released after its group ran
+        for i in g:
+            k = lib.log.index(("run", g))
+            assert lib.log.index(("gn", i, 2)) > k and lib.log.index(("release", i)) > k
+    assert lib.log[0] == ("cache", budget) and lib.log[-1] == ("cache", 0)
+    assert all(g.run_ms == 5.0 and g.refine_ms == len(g.refs) for g in groups)
+
+
+def test_run_admitted_errors_reset_the_cache():
+    from opensfm_amd.admission import run_admitted
+    lib = _StubLibrary()
+
+    def build(i):
+        if i == 2:
+            raise RuntimeError("no such image")
+        return _StubEstimator(lib.log, i, 10)
+
+    with pytest.raises(RuntimeError, match="no such image"):
+        run_admitted(lib, 4, build, budget=100)
+    assert lib.log[-1] == ("cache", 0)
+    with pytest.raises(ValueError):
+        run_admitted(lib, 1, build, budget=0)
+    res, groups = run_admitted(lib, 0, build, budget=100)
+    assert res == {} and groups == []
