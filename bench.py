@@ -77,6 +77,18 @@ def configure(est, scene, iters: int, storage: str = "f32"):
     est.set_image_storage(storage)
 
 
+def storage_effect(pydense, scene, iters: int, depth_f16) -> dict:
+    """Depth of the f16-storage run against the same scene run with f32
+    storage (full-precision texels, the oracle's arithmetic otherwise)."""
+    e = pydense.DepthmapEstimator()
+    configure(e, scene, iters, "f32")
+    d32 = e.compute()["depth"]
+    m = d32 > 0
+    return {"reference": "the same run with f32 image storage",
+            "within_1pct": round(float(np.mean(np.abs(depth_f16[m] - d32[m]) < 0.01 * d32[m])), 5),
+            "bit_equal": round(float(np.mean(depth_f16 == d32)), 5)}
+
+
 def cpu_baseline(scene, iters: int, rows: int, out: dict = None, half: bool = False):
     """CPU oracle on a band of reference rows (all sources), init + iterations.
     ``half``: the images as the f16 storage holds them.  With ``out``, the
@@ -180,6 +192,10 @@ def run_c3(args, pydense) -> int:
             np.abs(outs[small][m] - d_o[m]) < 0.01 * d_o[m])), 5),
             "bit_equal": round(float(np.mean(outs[small] == d_o)), 5)}
 
+    f16_effect = storage_effect(pydense, sc, args.iters, outs[small]) if storage == "f16" else None
+    if f16_effect:
+        f16_effect["reference"] += f" (reference {small})"
+
     sweep_tflops = units * 1e6 * FLOP_PER_UNIT / (float(np.mean(dev_ms)) / 1e3) / 1e12
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -200,6 +216,7 @@ def run_c3(args, pydense) -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "accuracy": {"gt_within_1pct": round(hit / max(tot, 1), 4), "gt_within_5pct": round(hit5 / max(tot, 1), 4)},
+        "f16_storage_vs_f32": f16_effect,
     }
     if cpu and cpu["value"] > 0:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
@@ -431,6 +448,7 @@ def main() -> int:
     gt = scene.gt_depth
     acc1 = float(np.mean(np.abs(d - gt) < 0.01 * gt))
     acc5 = float(np.mean(np.abs(d - gt) < 0.05 * gt))
+    f16_effect = storage_effect(pydense, scene, args.iters, d) if args.storage == "f16" else None
 
     # Roofline of the candidates kernel (fixed work per launch): one launch scores the
     # 8 propagated candidates + the current plane of every pixel of one
@@ -505,6 +523,7 @@ def main() -> int:
         "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
         "stage_ms_per_step": stage_ms,
         "pcie_inclusive_value": round(pcie_value, 3),
+        "f16_storage_vs_f32": f16_effect,
     }
     if refine is not None:
         line["refine"] = refine
