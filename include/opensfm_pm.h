@@ -253,6 +253,13 @@ int pm_batch_size(pm_batch* b, int* n);
 /* cost_tile_x / pixel_tile_x: 0 = chosen by the autotuner; use_graph: 0 =
  * enqueue the launches on every run (per-stage timing available). */
 int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int use_graph);
+/* Pyramid level the batch runs (default 0): every reference is staged at
+ * that level of its hierarchy_levels pyramid; level 0 starts from random
+ * planes, finer levels from each reference's previous, coarser result, which
+ * a run keeps for the next level.  The batched form of running RunLevel
+ * level by level for every reference (estimator.cc:172-255,
+ * cluster_estimator.cc:117-151); levels run in order. */
+int pm_batch_set_level(pm_batch* b, int level);
 /* Configuration in use after pm_batch_prepare / pm_batch_run, and the time
  * the autotuner took (0 when its result came from the per-process cache). */
 int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, float* tune_ms);

@@ -1562,6 +1562,7 @@ struct pm_batch {
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<pm_handle*> refs;
   bool use_graph = true;
+  int level = 0;  // pyramid level every reference runs (pm_batch_set_level)
   // Requested tile widths: 0 = autotune.
   int want_cost_x = 0, want_pixel_x = 0;
   // Configuration in use.
@@ -1702,7 +1703,30 @@ int batch_priors(pm_batch* b) {
 // The whole schedule of one run (Run, estimator.cc:134-255, one level).
 int batch_schedule(pm_batch* b) {
   const int n = static_cast<int>(b->refs.size());
-  if (int r = batch_launch(b, PM_K_INIT, 0, n, 0)) return r;
+  if (b->level == 0) {
+    if (int r = batch_launch(b, PM_K_INIT, 0, n, 0)) return r;
+  } else {
+    // finer levels start from each reference's coarser result
+    // (UpsampleDepthNormal, estimator.cc:761-821): one small launch each
+    for (pm_handle* h : b->refs) {
+      const bool mark = b->timing;
+      if (mark) {
+        while (b->kev.size() < b->kev_used + 2) {
+          hipEvent_t e;
+          PM_HIP(hipEventCreate(&e), "hipEventCreate");
+          b->kev.push_back(e);
+        }
+        PM_HIP(hipEventRecord(b->kev[b->kev_used], b->stream), "event");
+      }
+      PM_HIP(launch_upsample(make_args(h), h->coarse_depth.p, h->coarse_normal.p, h->coarse_w, h->coarse_h,
+                             b->stream), "upsample kernel launch");
+      if (mark) {
+        PM_HIP(hipEventRecord(b->kev[b->kev_used + 1], b->stream), "event");
+        b->kev_marks.emplace_back(PM_STAGE_INIT, static_cast<int>(b->kev_used));
+        b->kev_used += 2;
+      }
+    }
+  }
   if (int r = batch_priors(b)) return r;
   for (int it = 0; it < b->iters; ++it)
     for (int c = 0; c < 2; ++c)
@@ -1787,11 +1811,13 @@ int batch_tune(pm_batch* b) {
 int batch_validate(pm_batch* b) {
   if (b->refs.empty()) return fail(PM_E_INVALID, "batch: no references added");
   const pm_params& p0 = b->refs[0]->params;
+  if (b->level >= level_count(p0)) return fail(PM_E_INVALID, "batch: level out of range for the references' pyramid");
   for (pm_handle* h : b->refs) {
     if (h->device != b->device) return fail(PM_E_INVALID, "batch: every reference must be on the batch's device");
     if (int r = validate(h)) return r;
     const pm_params& p = h->params;
-    if (p.hierarchy_levels != 1) return fail(PM_E_INVALID, "batch: references run at one pyramid level");
+    if (level_count(p) != level_count(p0))
+      return fail(PM_E_INVALID, "batch: every reference must have the same number of pyramid levels");
     if (p.patch_size != p0.patch_size || p.image_storage != p0.image_storage ||
         p.max_iterations != p0.max_iterations || p.checkerboard_filter != p0.checkerboard_filter)
       return fail(PM_E_INVALID,
@@ -1820,7 +1846,14 @@ int batch_prepare(pm_batch* b) {
       for (int i = next++; i < n; i = next++) {
         pm_handle* h = b->refs[static_cast<size_t>(i)];
         int r = PM_OK;
-        if (!h->uploaded || h->W != h->views[0].w || h->H != h->views[0].h) r = upload_level(h, 0, 1);
+        const int nl = level_count(h->params);
+        if (nl > 1) {
+          r = upload_level(h, b->level, nl);
+          if (r == PM_OK && b->level > 0 && (h->coarse_w <= 0 || !h->coarse_depth.p))
+            r = fail(PM_E_STATE, "batch: no coarser level to start from (run the levels in order)");
+        } else if (!h->uploaded || h->W != h->views[0].w || h->H != h->views[0].h) {
+          r = upload_level(h, 0, 1);
+        }
         if (r == PM_OK) r = ensure_refpatch(h);
         if (r == PM_OK && hipStreamSynchronize(h->stream) != hipSuccess) r = fail(PM_E_DEVICE, "batch staging");
         rcs[static_cast<size_t>(i)] = r;
@@ -1993,6 +2026,14 @@ int pm_batch_set_config(pm_batch* b, int cost_tile_x, int pixel_tile_x, int use_
   return PM_OK;
 }
 
+int pm_batch_set_level(pm_batch* b, int level) {
+  if (int r = check_batch(b)) return r;
+  if (level < 0) return fail(PM_E_INVALID, "batch: negative level");
+  b->level = level;
+  b->prepared = false;
+  return PM_OK;
+}
+
 int pm_batch_get_config(pm_batch* b, int* cost_tile_x, int* pixel_tile_x, float* tune_ms) {
   if (int r = check_batch(b)) return r;
   if (cost_tile_x) *cost_tile_x = b->cost_x;
@@ -2049,6 +2090,19 @@ int pm_batch_run(pm_batch* b) {
   for (pm_handle* h : b->refs) {
     h->state_ready = true;
     if (int r = postprocess(h)) return r;
+    h->next_level = b->level + 1;
+    if (b->level < level_count(h->params) - 1) {  // keep the decoded level for the next one
+      const size_t npix = static_cast<size_t>(h->W) * h->H;
+      if (int r = h->coarse_depth.alloc(npix)) return r;
+      if (int r = h->coarse_normal.alloc(3 * npix)) return r;
+      PM_HIP(hipMemcpyAsync(h->coarse_depth.p, h->out_depth.p, npix * sizeof(float), hipMemcpyDeviceToDevice,
+                            h->stream), "keep coarse depth");
+      PM_HIP(hipMemcpyAsync(h->coarse_normal.p, h->out_normal.p, 3 * npix * sizeof(float),
+                            hipMemcpyDeviceToDevice, h->stream), "keep coarse normal");
+      h->coarse_w = h->W;
+      h->coarse_h = h->H;
+      PM_HIP(hipStreamSynchronize(h->stream), "keep coarse level");
+    }
   }
   return PM_OK;
 }

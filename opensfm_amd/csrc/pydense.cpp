@@ -374,6 +374,7 @@ class Batch {
   void set_config(int cost_x, int pixel_x, bool use_graph) {
     check(pm_batch_set_config(b_, cost_x, pixel_x, use_graph ? 1 : 0));
   }
+  void set_level(int level) { check(pm_batch_set_level(b_, level)); }
   py::dict config() {
     int c = 0, p = 0;
     float ms = 0.0f;
@@ -573,47 +574,16 @@ class ClusterEstimator {
     std::vector<Map> prev(static_cast<size_t>(K));
     std::vector<std::vector<float>> out(static_cast<size_t>(K));
     std::vector<std::pair<int, int>> size(static_cast<size_t>(K));
-    if (levels == 1) {
-      // One level: every reference runs in one batch (in groups that fit the
-      // free device memory), instead of one after the other.
+    {
+      // Every level runs all references as one batch (one launch per kernel,
+      // one HIP graph), in groups that fit the free device memory, instead
+      // of one reference after the other; finer levels start from each
+      // reference's coarser result and see the linked references' previous
+      // level through the geometric-consistency term.
       py::gil_scoped_release nogil;
       uint64_t total_mem = 0, free_mem = 0;
       check(pm_device_memory(device_, &total_mem, &free_mem));
       const uint64_t budget = free_mem / 10 * 9;
-      int i = 0;
-      while (i < K) {
-        pm_batch* b = nullptr;
-        check(pm_batch_create(device_, &b));
-        std::vector<int> group;
-        uint64_t used = 0;
-        try {
-          for (; i < K; ++i) {
-            uint64_t need = 0;
-            check(pm_device_bytes(refs_[i].h, &need));
-            if (!group.empty() && used + need > budget) break;
-            check(pm_batch_add(b, refs_[i].h));
-            group.push_back(i);
-            used += need;
-          }
-          check(pm_batch_run(b));
-          for (int j : group) {
-            int w = 0, h = 0;
-            check(pm_level_size(refs_[j].h, &w, &h));
-            const size_t n = static_cast<size_t>(w) * h;
-            std::vector<float>& o = out[j];
-            o.resize(6 * n);
-            check(pm_download(refs_[j].h, o.data(), nullptr, o.data() + n, o.data() + 4 * n, nullptr, o.data() + 5 * n));
-            size[j] = {w, h};
-            check(pm_release_device(refs_[j].h));
-          }
-        } catch (...) {
-          pm_batch_destroy(b);
-          throw;
-        }
-        pm_batch_destroy(b);
-      }
-    } else {
-      py::gil_scoped_release nogil;
       for (int level = 0; level < levels; ++level) {
         const bool last = level == levels - 1;
         std::vector<Map> cur(static_cast<size_t>(K));
@@ -627,22 +597,47 @@ class ClusterEstimator {
                 check(pm_set_previous_depth(r.h, link.first, prev[from].d.data(), prev[from].w, prev[from].h));
             }
           }
-          check(pm_run_level(r.h, level));
-          int w = 0, h = 0;
-          check(pm_level_size(r.h, &w, &h));
-          const size_t n = static_cast<size_t>(w) * h;
-          if (last) {
-            std::vector<float>& o = out[i];
-            o.resize(6 * n);  // depth, normal (3), cost, confidence
-            check(pm_download(r.h, o.data(), nullptr, o.data() + n, o.data() + 4 * n, nullptr, o.data() + 5 * n));
-            size[i] = {w, h};
-          } else {
-            cur[i].w = w;
-            cur[i].h = h;
-            cur[i].d.resize(n);
-            check(pm_download(r.h, cur[i].d.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+        }
+        int i = 0;
+        while (i < K) {
+          pm_batch* b = nullptr;
+          check(pm_batch_create(device_, &b));
+          std::vector<int> group;
+          uint64_t used = 0;
+          try {
+            for (; i < K; ++i) {
+              uint64_t need = 0;  // at the finest level: an upper bound for the coarser ones
+              check(pm_device_bytes(refs_[i].h, &need));
+              if (!group.empty() && used + need > budget) break;
+              check(pm_batch_add(b, refs_[i].h));
+              group.push_back(i);
+              used += need;
+            }
+            check(pm_batch_set_level(b, level));
+            check(pm_batch_run(b));
+            for (int j : group) {
+              int w = 0, h = 0;
+              check(pm_level_size(refs_[j].h, &w, &h));
+              const size_t n = static_cast<size_t>(w) * h;
+              if (last) {
+                std::vector<float>& o = out[j];
+                o.resize(6 * n);  // depth, normal (3), cost, confidence
+                check(pm_download(refs_[j].h, o.data(), nullptr, o.data() + n, o.data() + 4 * n, nullptr,
+                                  o.data() + 5 * n));
+                size[j] = {w, h};
+              } else {
+                cur[j].w = w;
+                cur[j].h = h;
+                cur[j].d.resize(n);
+                check(pm_download(refs_[j].h, cur[j].d.data(), nullptr, nullptr, nullptr, nullptr, nullptr));
+              }
+              check(pm_release_device(refs_[j].h));
+            }
+          } catch (...) {
+            pm_batch_destroy(b);
+            throw;
           }
-          check(pm_release_device(r.h));
+          pm_batch_destroy(b);
         }
         prev = std::move(cur);
       }
@@ -932,6 +927,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("add", &Batch::add, py::arg("estimator"), py::keep_alive<1, 2>())
       .def("clear", &Batch::clear)
       .def("size", &Batch::size)
+      .def("set_level", &Batch::set_level, py::arg("level"),
+           "Pyramid level to run (0 = coarsest); finer levels start from each reference's previous result")
       .def("set_config", &Batch::set_config, py::arg("cost_tile_x") = 0, py::arg("pixel_tile_x") = 0,
            py::arg("use_graph") = true, "tile widths: 0 = chosen by the autotuner")
       .def("config", &Batch::config)

@@ -198,3 +198,29 @@ def test_camera_walk_batch_equals_single_runs(pydense, walk):
     print(f"camera walk: {len(refs)} refs batched in {ms:.2f} ms", flush=True)
     for a, g in zip(alone, got):
         _same(a, g)
+
+
+@pytest.mark.gpu
+def test_batch_levels_equal_single_multilevel_runs(pydense, scenes):
+    # every reference with a 3-level pyramid, run level by level through one
+    # batch, equals running each reference's pyramid on its own
+    refs = scenes[:3]
+
+    def make(s):
+        e = gpu_estimator(pydense, s, _params(s), storage="f16")
+        e.set_hierarchy_levels(3)
+        return e
+
+    alone = [make(s).compute() for s in refs]
+    ests = [make(s) for s in refs]
+    b = pydense.DepthmapBatch()
+    for e in ests:
+        b.add(e)
+    for level in range(3):
+        b.set_level(level)
+        b.run()
+    for a, e in zip(alone, ests):
+        _same(a, e.download())
+    with pytest.raises(RuntimeError, match="level"):
+        b.set_level(3)
+        b.run()
