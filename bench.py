@@ -54,6 +54,10 @@ WORKLOADS = {
     "c3": ("camera_walk", "batched PatchMatch, {views} references of 640x480..2000x1500 with 4-12 neighbours each "
                           "(camera walk around a 20x20 m box scene, seed {seed}), {storage} images, patch 7x7, "
                           "{iters} iterations, one HIP-graph launch per step"),
+    "cluster": ("camera_walk", "cluster surface, {views} references of 640x480..2000x1500 with 4-12 neighbours each "
+                               "(camera walk, seed {seed}), {levels} pyramid levels with geometric links "
+                               "(weight 0.05), f32 images, patch 7x7, {iters} iterations per level, every level "
+                               "one batch"),
     "c5": ("aerial_survey", "aerial batch, {views} nadir {width}x{height} {storage} views over a fractal terrain "
                             "(2x2 km, 0-80 m relief, 1 m grid, seed {seed}), 80%/70% overlap at 100 m, "
                             "{neighbours} neighbours each, depth [60,140] m, patch 7x7, {iters} iterations, "
@@ -224,6 +228,92 @@ def run_c3(args, pydense) -> int:
     return 0
 
 
+def run_cluster(args, pydense) -> int:
+    """The pipeline's caller surface: DepthmapClusterEstimator over the
+    camera walk with hierarchy levels and geometric links between the
+    references (depthmaps.py:154-286 drives it so)."""
+    import torch
+    from opensfm_amd import synthetic
+
+    seed = 3 if args.seed == 2 else args.seed
+    views = args.views or 64
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    log(f"rendering camera walk ({views} views) on {dev} ...")
+    ws = synthetic.camera_walk(seed=seed, num_views=views, device=dev)
+    torch.cuda.empty_cache()
+    refs = ws.refs
+    index_of = {v: i for i, (v, _, _, _) in enumerate(refs)}
+
+    def cluster():
+        c = pydense.DepthmapClusterEstimator()
+        c.set_hierarchy_levels(args.levels)
+        c.set_geom_consistency_weight(0.05 if args.levels > 1 else 0.0)
+        c.set_max_iterations(args.iters)
+        c.set_patch_size(PATCH)
+        for v, srcs, dmin, dmax in refs:
+            c.begin_ref_view(ws.Ks[v], ws.Rs[v], ws.ts[v], ws.images[v], dmin, dmax)
+            for j in srcs:
+                c.add_source_view(ws.Ks[j], ws.Rs[j], ws.ts[j], ws.images[j])
+        for i, (v, srcs, _, _) in enumerate(refs):
+            for pos, j in enumerate(srcs):
+                if j in index_of:
+                    c.add_geom_link(i, pos, index_of[j])
+        return c
+
+    c = cluster()
+    for i in range(args.warmup):
+        c.run()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res = c.run()
+        log(f"step {k + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    # work: every level's references, Mpix x sources x iterations
+    units = 0.0
+    for v, srcs, _, _ in refs:
+        h, w = ws.images[v].shape
+        for lv in range(args.levels):
+            f = 2 ** (args.levels - 1 - lv)
+            units += (w // f) * (h // f) / 1e6 * len(srcs) * args.iters
+    value = units * args.steps / elapsed
+    hit = tot = 0
+    for i, (v, _, _, _) in enumerate(refs):
+        d = np.asarray(res[i][0])
+        gt = ws.gt_depth[v]
+        m = gt > 0
+        hit += int(np.sum(np.abs(d[m] - gt[m]) < 0.01 * gt[m]))
+        tot += int(m.sum())
+    # the same references' pyramids one after the other, each its own estimator (no links)
+    ests = []
+    for v, srcs, dmin, dmax in refs:
+        e = pydense.DepthmapEstimator()
+        configure(e, ws.reference(len(ests)), args.iters, "f32")
+        e.set_hierarchy_levels(args.levels)
+        ests.append(e)
+    t1 = time.perf_counter()
+    for e in ests:
+        e.compute()
+        e.release_device()
+    seq_s = time.perf_counter() - t1
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["cluster"][1].format(views=len(refs), seed=seed, levels=args.levels,
+                                                              iters=args.iters),
+                   "references": len(refs), "levels": args.levels, "units_per_step": round(units, 3),
+                   "value_scope": "DepthmapClusterEstimator.run(): staging, every level, downloads"},
+        "sequential": {"scope": "each reference's pyramid through its own DepthmapEstimator, one after the "
+                                "other, without links", "s": round(seq_s, 3),
+                       "value": round(units / seq_s, 3)},
+        "accuracy": {"gt_within_1pct": round(hit / max(tot, 1), 4)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_c5(args, pydense) -> int:
     """Nadir survey: every view a reference, in groups admitted by device
     memory, each group one batched run followed by Gauss-Newton refinement."""
@@ -373,9 +463,11 @@ def main() -> int:
                     help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
                     help="c2: the headline single-view workload; c3: many references in one batched launch; "
-                         "c4: PatchMatch + Gauss-Newton refinement; c5: aerial survey in memory-admitted groups")
+                         "c4: PatchMatch + Gauss-Newton refinement; c5: aerial survey in memory-admitted groups; "
+                         "cluster: the cluster surface with pyramid levels and geometric links")
     ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 and c5 workloads")
     ap.add_argument("--views", type=int, default=None, help="references of the c3 (64) and c5 (200) workloads")
+    ap.add_argument("--levels", type=int, default=3, help="pyramid levels of the cluster workload")
     ap.add_argument("--width", type=int, default=4000, help="c5 image width")
     ap.add_argument("--height", type=int, default=3000, help="c5 image height")
     ap.add_argument("--neighbours", type=int, default=10, help="c5 sources per reference")
@@ -388,7 +480,7 @@ def main() -> int:
     args = ap.parse_args()
     args.storage_set = args.storage is not None
     if args.steps is None:
-        args.steps = 1 if args.workload == "c5" else 5
+        args.steps = 1 if args.workload in ("c5", "cluster") else 5
     if args.storage is None:
         args.storage = "f32"
     if args.gpus != 1:
@@ -408,6 +500,8 @@ def main() -> int:
         return run_c3(args, pydense)
     if args.workload == "c5":
         return run_c5(args, pydense)
+    if args.workload == "cluster":
+        return run_cluster(args, pydense)
     if args.workload == "c4" and args.seed == 2:
         args.seed = 4
     log("rendering scene ...")
