@@ -169,6 +169,16 @@ int pm_compute(pm_handle* h, float* depth, float* plane, float* normal, float* c
 
 /* Staged execution (after pm_upload). */
 int pm_init_state(pm_handle* h);
+/* Fronto-parallel plane sweep, then decode and post-processing, as pm_run
+ * ends (after pm_upload).  For compute_brute_force() of the pydense stub
+ * (pydense.pyi:53, whose num_planes comes from set_depth_range,
+ * pydense.pyi:57); the reference tree has no implementation, so the method
+ * is defined here: num_planes >= 1 depths log-spaced in
+ * [depth_min, depth_max] (the range's geometric mean for one plane), normal
+ * facing the camera, each scored like a random-init plane (stride-1 NCC,
+ * fallbacks, top-k mean); per pixel the lowest cost wins, the first plane on
+ * ties. */
+int pm_plane_sweep(pm_handle* h, int num_planes);
 int pm_iterate(pm_handle* h, int iteration, int color /* 0, 1 or -1 for both */);
 int pm_filter(pm_handle* h);
 int pm_decode(pm_handle* h);

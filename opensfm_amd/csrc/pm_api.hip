@@ -1203,6 +1203,20 @@ int pm_decode(pm_handle* h) {
   return PM_OK;
 }
 
+int pm_plane_sweep(pm_handle* h, int num_planes) {
+  if (int r = check_handle(h)) return r;
+  if (num_planes < 1) return fail(PM_E_INVALID, "pm_plane_sweep: num_planes must be at least 1");
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_plane_sweep: views are not uploaded (call pm_upload)");
+  if (int r = validate(h)) return r;
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  if (int r = ensure_refpatch(h)) return r;
+  PmArgs a = make_args(h);
+  PM_HIP(launch_plane_sweep(a, h->params.patch_size / 2, h->f16, num_planes, h->stream), "plane sweep kernel launch");
+  h->state_ready = true;
+  if (int r = pm_decode(h)) return r;
+  return postprocess(h);
+}
+
 }  // extern "C"
 
 namespace {

@@ -814,6 +814,50 @@ __device__ __forceinline__ void init_body(const PmArgs& a, int b) {
 }
 
 // ---------------------------------------------------------------------------
+// Fronto-parallel plane sweep (compute_brute_force of the pydense stub,
+// pydense.pyi:53, which has no implementation in the reference tree), one
+// launch per plane: plane a.sweep_index of a.sweep_count has depth
+// exp(log dmin + i * (log dmax - log dmin) / (n - 1)) (the geometric mean of
+// the range for one plane) and normal (0, 0, -1), and is scored like a
+// random-init plane.  The first launch writes every pixel; later ones keep
+// the lower cost, so the first plane wins ties.  The generator state is the
+// random init's seed, before any draw.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sweep_depth(const PmArgs& a) {
+  const float ldmin = pm_logf(fmaxf(a.depth_min, 1e-6f));
+  const float ldmax = pm_logf(fmaxf(a.depth_max, 1e-5f));
+  if (a.sweep_count <= 1) return pm_expf(0.5f * (ldmin + ldmax));
+  const float step = (ldmax - ldmin) / static_cast<float>(a.sweep_count - 1);
+  return pm_expf(ldmin + static_cast<float>(a.sweep_index) * step);
+}
+
+template <int HP, typename T>
+__device__ __forceinline__ void plane_sweep_body(const PmArgs& a, int b) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  int x, y;
+  if (!full_tile_pixel(a.width, a.height, b, x, y)) return;
+  const int idx = y * a.width + x;
+  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), sweep_depth(a),
+                                              V3{0.0f, 0.0f, -1.0f}, kcam(a, 0));
+  float* cv = smem + lane;
+  float* tmp = smem + a.n_src * kWave + lane;
+  full_cost_vector<HP>(a, images_of<T>(a), x, y, colour_index(a, x, y), plane, cv);
+  uint32_t sel = 0u;
+  const float cost = initial_cost_and_views(a, cv, tmp, &sel);
+  const bool first = a.sweep_index == 0;
+  if (first) {
+    a.rng[idx] = make_uint2(static_cast<uint32_t>(idx) * 1099087573u + 2654435769u,
+                            static_cast<uint32_t>(idx) * 2246822519u + 13u);
+  }
+  if (first || cost < a.costs[idx]) {
+    a.planes[idx] = to_f4(plane);
+    a.costs[idx] = cost;
+    a.sel[idx] = sel;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // acmmp_prior_reinit (opencl_kernels.h:1056-1182); neighbour planes of the
 // smoothness term come from the pre-launch snapshot, as in the reference.
 // ---------------------------------------------------------------------------
@@ -1471,6 +1515,9 @@ template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void init_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { init_body<HP, T>(a, b); });
 }
+
+template <int HP, typename T>
+__global__ __launch_bounds__(kWave) void plane_sweep_kernel(PmArgs a) { plane_sweep_body<HP, T>(a, PM_BLOCK); }
 
 template <int HP, typename T>
 __global__ __launch_bounds__(kWave) void prior_kernel(PmArgs a) { prior_body<HP, T>(a, PM_BLOCK); }

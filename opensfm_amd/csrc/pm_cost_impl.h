@@ -1,5 +1,6 @@
 // Launchers of the kernels templated on the image storage type (reference
-// patch, init, prior re-init, candidate and refinement costs, fallback).
+// patch, init, plane sweep, prior re-init, candidate and refinement costs,
+// fallback).
 // Included once per storage type, by pm_cost_f32.hip and pm_cost_f16.hip,
 // with PM_STORAGE_T (the texel type) and PM_STORAGE_FN (the launcher name),
 // so the two sets of instantiations compile in parallel.
@@ -37,6 +38,10 @@ hipError_t launch_hp(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, 
     case PM_K_INIT:
       if (a) hipLaunchKernelGGL((init_kernel<HP, T>), grid, dim3(kWave), lds, s, *a);
       else hipLaunchKernelGGL((init_batch<HP, T>), grid, dim3(kWave), lds, s, *bt);
+      break;
+    case PM_K_SWEEP:  // single-reference launches only
+      if (!a) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((plane_sweep_kernel<HP, T>), grid, dim3(kWave), lds, s, *a);
       break;
     case PM_K_PRIOR:
       if (a) hipLaunchKernelGGL((prior_kernel<HP, T>), grid, dim3(kWave), lds, s, *a);

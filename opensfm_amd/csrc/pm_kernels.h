@@ -61,6 +61,7 @@ struct PmArgs {
   int anchor_views;
   int color;
   int iteration;
+  int sweep_index, sweep_count;  // plane sweep: this launch's plane, planes in all
   PmDecodeOut out;             // decode targets
 };
 
@@ -88,6 +89,7 @@ enum PmKernel {
   PM_K_REFINE,
   PM_K_FILTER,
   PM_K_DECODE,
+  PM_K_SWEEP,
   PM_K_COUNT
 };
 
@@ -116,6 +118,9 @@ hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t strea
 int64_t fallback_mask_count(int width, int height, int n_src);
 hipError_t launch_init(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_prior(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
+// Fronto-parallel plane sweep over num_planes log-spaced depths: one launch
+// per plane, each keeping the lower cost per pixel (the first writes all).
+hipError_t launch_plane_sweep(const PmArgs& a, int half_patch, bool f16, int num_planes, hipStream_t stream);
 // Coarse-to-fine transfer from a decoded src_w x src_h level (depth, normal
 // 3 per pixel) into the planes/costs/rng/sel of the current level.
 hipError_t launch_upsample(const PmArgs& a, const float* cdepth, const float* cnormal, int src_w, int src_h,

@@ -200,6 +200,7 @@ int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x) 
     case PM_K_REFPATCH:
     case PM_K_INIT:
     case PM_K_PRIOR:
+    case PM_K_SWEEP:
       return xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
                        ((height + kFullTileY - 1) / kFullTileY));
     case PM_K_FALLBACK: {
@@ -235,6 +236,18 @@ hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream
 hipError_t launch_init(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
   return launch_storage(f16, PM_K_INIT, &a, nullptr, blocks_of(PM_K_INIT, a, 0), cost_column_lds(a.n_src), hp, 0,
                         stream);
+}
+
+hipError_t launch_plane_sweep(const PmArgs& a, int hp, bool f16, int num_planes, hipStream_t stream) {
+  PmArgs b = a;
+  b.sweep_count = num_planes;
+  for (int i = 0; i < num_planes; ++i) {
+    b.sweep_index = i;
+    const hipError_t e = launch_storage(f16, PM_K_SWEEP, &b, nullptr, blocks_of(PM_K_SWEEP, b, 0),
+                                        cost_column_lds(b.n_src), hp, 0, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_prior(const PmArgs& a, int hp, bool f16, hipStream_t stream) {

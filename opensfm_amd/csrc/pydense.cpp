@@ -85,7 +85,11 @@ class Estimator {
     add_view_to(h_, K, R, t, image, mask);
   }
 
-  void set_depth_range(float dmin, float dmax, int /*num_planes*/) { check(pm_set_depth_range(h_, dmin, dmax)); }
+  // num_planes: the depths of compute_brute_force (PatchMatch does not use it).
+  void set_depth_range(float dmin, float dmax, int num_planes) {
+    check(pm_set_depth_range(h_, dmin, dmax));
+    num_planes_ = num_planes;
+  }
 
   template <typename F>
   void update(F f) {
@@ -119,8 +123,25 @@ class Estimator {
     return d;
   }
 
-  py::list compute_patch_match() {
-    py::dict r = compute();
+  py::list compute_patch_match() { return as_list(compute()); }
+
+  // PatchMatch already samples each pixel's views (the Monte-Carlo view
+  // draws of acmmp_patchmatch, opencl_kernels.h:910-969), so the sampled
+  // variant of the stub (pydense.pyi:55) is the same computation.
+  py::list compute_patch_match_sample() { return as_list(compute()); }
+
+  // Fronto-parallel plane sweep over the set_depth_range num_planes depths
+  // (pydense.pyi:53).
+  py::list compute_brute_force() {
+    {
+      py::gil_scoped_release nogil;
+      check(pm_upload(h_));
+      check(pm_plane_sweep(h_, num_planes_));
+    }
+    return as_list(download());
+  }
+
+  static py::list as_list(py::dict r) {
     py::list out;
     out.append(r["depth"]);
     out.append(r["normal"]);
@@ -353,6 +374,7 @@ class Estimator {
  private:
   pm_handle* h_ = nullptr;
   py::ssize_t ref_w_ = -1, ref_h_ = -1;
+  int num_planes_ = 0;
 };
 
 // Many reference estimators run through one set of launches (pm_batch_*).
@@ -725,6 +747,11 @@ PYBIND11_MODULE(pydense, m) {
       .def("params", &Estimator::params)
       .def("compute_patch_match", &Estimator::compute_patch_match,
            "Run PatchMatch; returns [depth (H,W), plane normal (H,W,3), score (H,W), selected-view bitmask (H,W)]")
+      .def("compute_patch_match_sample", &Estimator::compute_patch_match_sample,
+           "compute_patch_match (PatchMatch samples each pixel's views already)")
+      .def("compute_brute_force", &Estimator::compute_brute_force,
+           "Fronto-parallel plane sweep over set_depth_range's num_planes log-spaced depths; returns the same "
+           "list as compute_patch_match")
       .def("compute", &Estimator::compute, "Run PatchMatch; returns a dict of every decoded map")
       .def("upload", &Estimator::upload)
       .def("run", &Estimator::run)

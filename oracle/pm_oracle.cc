@@ -38,6 +38,9 @@
 // checkerboard median, prior re-init bounds, decode, pyramid resampling,
 // coarse-to-fine transfer, speckle removal and gap filling.  The whole
 // algorithm is pinned by the plane-sweep argmin and ground-truth tests.
+// The fronto-parallel plane sweep (pmo_plane_sweep) is pinned by recovering
+// the exact sweep depth of a rendered fronto-parallel plane and by the
+// one-plane case (the range's geometric mean).
 
 #include <cmath>
 #include <cstdint>
@@ -675,6 +678,47 @@ void random_init_px(const Ctx& c, pmo_state* st, int x, int y) {
   uint32_t sel = 0u;
   st->costs[idx] = initial_cost_and_views(c, x, y, plane, rp, &sel);
   st->sel[idx] = sel;
+}
+
+// Fronto-parallel plane sweep (compute_brute_force of the pydense stub,
+// pydense.pyi:53, which has no implementation in the reference tree).
+// Plane i of n has depth exp(log dmin + i * (log dmax - log dmin) / (n - 1))
+// (the geometric mean of the range when n == 1) and normal (0, 0, -1); it is
+// scored like a random-init plane (stride-1 NCC, fallbacks, top-k mean of
+// opencl_kernels.h:565-621), the lowest cost wins and the first plane wins
+// ties.  The generator state is the random init's seed, before any draw.
+float sweep_depth(const pmo_params& P, int i, int n) {
+  const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
+  const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
+  if (n <= 1) return pm_expf(0.5f * (ldmin + ldmax));
+  const float step = (ldmax - ldmin) / static_cast<float>(n - 1);
+  return pm_expf(ldmin + static_cast<float>(i) * step);
+}
+
+void plane_sweep_px(const Ctx& c, pmo_state* st, int x, int y, int n) {
+  const int idx = y * c.sc->width + x;
+  st->rng[2 * idx] = static_cast<uint32_t>(idx) * 1099087573u + 2654435769u;
+  st->rng[2 * idx + 1] = static_cast<uint32_t>(idx) * 2246822519u + 13u;
+  RefPatch rp;
+  compute_ref_patch(c, x, y, &rp);
+  const V3 nrm{0.0f, 0.0f, -1.0f};
+  float best = 0.0f;
+  uint32_t best_sel = 0u;
+  Plane best_plane{};
+  for (int i = 0; i < n; ++i) {
+    const float depth = sweep_depth(*c.pr, i, n);
+    const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, nrm, c.cams[0]);
+    uint32_t sel = 0u;
+    const float cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
+    if (i == 0 || cost < best) {
+      best = cost;
+      best_sel = sel;
+      best_plane = plane;
+    }
+  }
+  store_plane(st->planes, idx, best_plane);
+  st->costs[idx] = best;
+  st->sel[idx] = best_sel;
 }
 
 // acmmp_prior_reinit: opencl_kernels.h:1056-1182.
@@ -1516,6 +1560,17 @@ void pmo_random_init(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
   for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) random_init_px(c, st, x, y);
 }
+
+void pmo_plane_sweep(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, int num_planes) {
+  Ctx c(sc, pr);
+  const int W = sc->width, H = sc->height;
+  if (num_planes < 1) return;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) plane_sweep_px(c, st, x, y, num_planes);
+}
+
+float pmo_sweep_depth(const pmo_params* pr, int i, int n) { return sweep_depth(*pr, i, n); }
 
 void pmo_prior_reinit(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
   Ctx c(sc, pr);

@@ -79,6 +79,8 @@ def lib():
         for name in ("pmo_random_init", "pmo_prior_reinit", "pmo_run"):
             getattr(L, name).argtypes = [vp, vp, vp]
         L.pmo_sweep.argtypes = [vp, vp, vp, i, i, f]
+        L.pmo_plane_sweep.argtypes = [vp, vp, vp, i]
+        L.pmo_sweep_depth.restype = f; L.pmo_sweep_depth.argtypes = [vp, i, i]
         L.pmo_checkerboard.argtypes = [vp, vp, vp, i]
         L.pmo_smooth_weight.restype = f; L.pmo_smooth_weight.argtypes = [vp, i]
         L.pmo_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -278,6 +280,28 @@ def ref_patch(scene: OracleScene, params: Params, x: int, y: int) -> np.ndarray:
 def random_init(scene: OracleScene, params: Params, state: OracleState) -> None:
     p = params.c()
     lib().pmo_random_init(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c))
+
+
+def plane_sweep(scene: OracleScene, params: Params, state: OracleState, num_planes: int) -> None:
+    """Fronto-parallel plane sweep over ``num_planes`` log-spaced depths
+    (the state a brute-force run decodes)."""
+    p = params.c()
+    lib().pmo_plane_sweep(ctypes.byref(scene.c), ctypes.byref(p), ctypes.byref(state.c), int(num_planes))
+
+
+def sweep_depth(params: Params, i: int, n: int) -> float:
+    """Depth of plane i of an n-plane sweep (float32 value)."""
+    p = params.c()
+    return lib().pmo_sweep_depth(ctypes.byref(p), int(i), int(n))
+
+
+def estimate_plane_sweep(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params, num_planes: int):
+    """Plane sweep + decode; returns (depth, normal, cost, confidence, state)."""
+    sc = OracleScene(images, Ks, Rs, ts)
+    st = OracleState(sc.width, sc.height)
+    plane_sweep(sc, params, st, num_planes)
+    d, n, c, conf = decode(sc, params, st)
+    return d, n, c, conf, st
 
 
 def prior_reinit(scene: OracleScene, params: Params, state: OracleState) -> None:

@@ -91,7 +91,8 @@ def test_pydense_surface_names():
     pd = native()
     cls = pd.DepthmapEstimator
     for m in ("add_view", "set_depth_range", "set_patchmatch_iterations", "set_patch_size",
-              "set_min_patch_sd", "compute_patch_match", "score_planes", "is_available", "num_devices"):
+              "set_min_patch_sd", "compute_patch_match", "compute_patch_match_sample", "compute_brute_force",
+              "score_planes", "is_available", "num_devices"):
         assert hasattr(cls, m), m
     for m in ("add", "set_config", "config", "tune_log", "prepare", "run", "reset_tuning"):
         assert hasattr(pd.DepthmapBatch, m), m
