@@ -78,25 +78,30 @@ def test_pydense_has_the_stub_methods():
         assert hasattr(cls, name), name
 
 
+# (image storage, planes, patch size, sources, top-k)
+SWEEP_CASES = [("f32", 24, 7, 3, 4), ("f16", 24, 7, 3, 4), ("f32", 1, 3, 2, 1), ("f16", 5, 5, 5, 2)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_gpu_sweep_matches_oracle(pydense, storage):
-    s = synthetic.three_planes(11, width=72, height=56, num_src=3, base_cell=0.3, supersample=1)
-    p = params_for(s, use_census=True)
+@pytest.mark.parametrize("storage,planes,patch,nsrc,top_k", SWEEP_CASES,
+                         ids=[f"{c[0]}-n{c[1]}-p{c[2]}-s{c[3]}" for c in SWEEP_CASES])
+def test_gpu_sweep_matches_oracle(pydense, storage, planes, patch, nsrc, top_k):
+    s = synthetic.three_planes(11, width=72, height=56, num_src=nsrc, base_cell=0.3, supersample=1)
+    p = params_for(s, use_census=True, patch_size=patch, top_k=top_k)
     est = gpu_estimator(pydense, s, p, storage=storage)
-    est.set_depth_range(p.depth_min, p.depth_max, 24)
-    print(f"plane sweep 72x56, 3 sources, 24 planes, {storage}", flush=True)
+    est.set_depth_range(p.depth_min, p.depth_max, planes)
+    print(f"plane sweep 72x56, {nsrc} sources, {planes} planes, patch {patch}, {storage}", flush=True)
     depth, normal, cost, views = est.compute_brute_force()
     imgs = half_round(s.images) if storage == "f16" else s.images
-    d_o, n_o, c_o, _, st = po.estimate_plane_sweep(imgs, s.Ks, s.Rs, s.ts, p, 24)
+    d_o, n_o, c_o, _, st = po.estimate_plane_sweep(imgs, s.Ks, s.Rs, s.ts, p, planes)
     assert np.array_equal(depth, d_o)
     assert np.array_equal(normal, n_o)
     assert np.array_equal(cost, c_o)
     assert np.array_equal(views, st.sel)
-    # the sweep's best planes beat a random init and land near the scene
-    gt = s.gt_depth
-    m = (gt > 0) & (depth > 0)
-    assert np.mean(np.abs(depth[m] - gt[m]) < 0.1 * gt[m]) > 0.5
+    if planes >= 24:  # a fine sweep lands near the scene
+        gt = s.gt_depth
+        m = (gt > 0) & (depth > 0)
+        assert np.mean(np.abs(depth[m] - gt[m]) < 0.1 * gt[m]) > 0.5
 
 
 @pytest.mark.gpu
