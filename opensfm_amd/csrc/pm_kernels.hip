@@ -233,9 +233,26 @@ hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream
   return launch_storage(f16, PM_K_REFPATCH, &a, nullptr, blocks_of(PM_K_REFPATCH, a, 0), 0, hp, 0, stream);
 }
 
+// Waves per SIMD of the random init and the plane sweep (one-wave
+// workgroups; 0 = no cap).  Their random planes gather with no locality, and
+// fewer waves with more loads in flight each thrash L1/L2 less: at
+// 1600x1200, N=8 the init takes 7.8 ms uncapped (4 waves, 128 VGPRs),
+// 6.9 ms at 3, 6.3 ms at 2 and 6.9 ms at 1 (f32; f16 5.4 / 4.8 / 4.6 / 6.8).
+#ifndef PM_INIT_WAVES
+#define PM_INIT_WAVES 2
+#endif
+// Dynamic LDS of an init or sweep launch: its cost columns, or more to cap
+// its one-wave workgroups at PM_INIT_WAVES per SIMD.
+size_t init_lds(int n_src) {
+  constexpr size_t kLdsPerCu = 160 * 1024;  // MI355X
+  constexpr int kSimdsPerCu = 4;
+  const size_t cols = cost_column_lds(n_src);
+  const size_t cap = PM_INIT_WAVES > 0 ? kLdsPerCu / (static_cast<size_t>(kSimdsPerCu) * PM_INIT_WAVES + 1) + 1 : 0;
+  return cols > cap ? cols : cap;
+}
+
 hipError_t launch_init(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
-  return launch_storage(f16, PM_K_INIT, &a, nullptr, blocks_of(PM_K_INIT, a, 0), cost_column_lds(a.n_src), hp, 0,
-                        stream);
+  return launch_storage(f16, PM_K_INIT, &a, nullptr, blocks_of(PM_K_INIT, a, 0), init_lds(a.n_src), hp, 0, stream);
 }
 
 hipError_t launch_plane_sweep(const PmArgs& a, int hp, bool f16, int num_planes, hipStream_t stream) {
@@ -244,7 +261,7 @@ hipError_t launch_plane_sweep(const PmArgs& a, int hp, bool f16, int num_planes,
   for (int i = 0; i < num_planes; ++i) {
     b.sweep_index = i;
     const hipError_t e = launch_storage(f16, PM_K_SWEEP, &b, nullptr, blocks_of(PM_K_SWEEP, b, 0),
-                                        cost_column_lds(b.n_src), hp, 0, stream);
+                                        init_lds(b.n_src), hp, 0, stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -327,6 +344,7 @@ hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int hp, bo
     case PM_K_REFPATCH:
       return launch_storage(f16, kernel, nullptr, &bt, grid, 0, hp, 0, stream);
     case PM_K_INIT:
+      return launch_storage(f16, kernel, nullptr, &bt, grid, init_lds(max_src), hp, 0, stream);
     case PM_K_PRIOR:
       return launch_storage(f16, kernel, nullptr, &bt, grid, cost_column_lds(max_src), hp, 0, stream);
     case PM_K_CANDIDATES:
