@@ -73,12 +73,15 @@ def make_scene(seed: int, workload: str = "c2"):
 
 
 def configure(est, scene, iters: int, storage: str = "f32"):
+    """The workload's estimator: the dense pipeline's parameters at one
+    pyramid level (the workloads are single-level), patch 7x7, the scene's
+    depth range, no post-processing."""
+    from opensfm_amd import native
     for im, K, R, t in zip(scene.images, scene.Ks, scene.Rs, scene.ts):
         est.add_view(K, R, t, im)
     est.set_depth_range(scene.depth_min, scene.depth_max, 0)
-    est.set_patchmatch_iterations(iters)
-    est.set_patch_size(PATCH)
-    est.set_image_storage(storage)
+    est.set_params({**native().pipeline_params(), "hierarchy_levels": 1, "max_iterations": iters,
+                    "patch_size": PATCH, "image_storage": storage, "speckle_min_size": 0, "gap_max_size": 0})
 
 
 def storage_effect(pydense, scene, iters: int, depth_f16) -> dict:
@@ -246,6 +249,7 @@ def run_cluster(args, pydense) -> int:
 
     def cluster():
         c = pydense.DepthmapClusterEstimator()
+        c.set_params(pydense.pipeline_params())   # what depthmaps.py:154-175 sets
         c.set_hierarchy_levels(args.levels)
         c.set_geom_consistency_weight(0.05 if args.levels > 1 else 0.0)
         c.set_max_iterations(args.iters)

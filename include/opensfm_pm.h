@@ -16,6 +16,9 @@
  *                               add_view(K, R, t, image, mask) (pydense.pyi:45-52)
  *   pm_set_depth_range          SetDepthRange (estimator.cc:44-47),
  *                               set_depth_range(min, max, num_planes) (pydense.pyi:56)
+ *   pm_default_params           DepthmapParams{} (estimator.h:16-36)
+ *   pm_pipeline_params          the parameters _setup_cluster_params sets
+ *                               (opensfm/dense/depthmaps.py:154-175, config.py:407-466)
  *   pm_set_params/pm_get_params SetParams(DepthmapParams) (estimator.cc:49-51,
  *                               estimator.h:16-36); set_patchmatch_iterations,
  *                               set_patch_size, set_min_patch_sd (pydense.pyi:57-59)
@@ -93,7 +96,21 @@ typedef struct pm_params {
                                (GapInterpolation, estimator.cc:1319-1418); 0 = off */
 } pm_params;
 
+/* The member initialisers of DepthmapParams (estimator.h:16-36): iterations
+ * 3, patch 5, sigma_spatial 3, sigma_color 3/255, top_k 4, census on,
+ * depth [0, 1], 1 level, smooth 0, anchors 0, filter off, speckle 100,
+ * gap 7; plus var_threshold 1.5e-6 (the kernel's gate), f32 storage, no
+ * half rounding.  pm_create starts every handle from these, as the
+ * reference's DepthmapEstimator starts from a default DepthmapParams. */
 void pm_default_params(pm_params* out);
+/* The dense pipeline's profile: what _setup_cluster_params
+ * (opensfm/dense/depthmaps.py:154-175) sets from the config defaults
+ * (opensfm/config.py:407-466): iterations 4, patch 5, sigma_spatial 1.5,
+ * sigma_color 25/255, top_k 8 (depthmap_num_matching_views), census on,
+ * 3 levels, smooth 0.2, anchors 2, filter off, speckle 0, gap 0; the other
+ * fields as pm_default_params.  (The pipeline's geometric-consistency
+ * weight, 0.05, goes through pm_set_geom_consistency.) */
+void pm_pipeline_params(pm_params* out);
 
 /* Last error message of the calling thread ("" if none). */
 const char* pm_last_error(void);

@@ -11,7 +11,8 @@ while their sum fits the budget (by default 90 % of the device memory free
 when the run starts).  A reference larger than the budget runs alone.
 
 A group runs as one ``DepthmapBatch`` (one launch per kernel for all its
-references, captured in a HIP graph), is refined by Gauss-Newton when asked,
+references, captured in a HIP graph), level by level when the references
+have several pyramid levels, is refined by Gauss-Newton when asked,
 and hands each estimator to ``on_result`` while its results are on the
 device; then the group's device memory is released and its estimators are
 dropped.  While a group runs on the device, a worker thread builds the next
@@ -164,7 +165,13 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                     t0 = time.perf_counter()
                     batch.prepare()
                     st.prepare_s = time.perf_counter() - t0
+                # coarse to fine: a batch starts at level 0 of every reference's
+                # pyramid; each finer level starts from the level below it
+                # (the references of a batch share their number of levels)
                 st.run_ms = batch.run()
+                for level in range(1, int(ests[0].params()["hierarchy_levels"])):
+                    batch.set_level(level)
+                    st.run_ms += batch.run()
                 for e in ests:
                     if refine_steps > 0:
                         e.gauss_newton(refine_steps, damping)

@@ -426,24 +426,46 @@ extern "C" {
 
 void pm_default_params(pm_params* o) {
   if (!o) return;
-  // Reference pipeline profile (opensfm/config.py:407-466) at one level.
+  // DepthmapParams' member initialisers (estimator.h:16-36), field by field.
   o->max_iterations = 3;
-  o->patch_size = 7;
-  o->sigma_spatial = 1.5f;
-  o->sigma_color = 25.0f / 255.0f;
-  o->top_k = 8;
+  o->patch_size = 5;
+  o->sigma_spatial = 3.0f;
+  o->sigma_color = 3.0f / 255.0f;
+  o->top_k = 4;
   o->use_census = 1;
   o->depth_min = 0.0f;
   o->depth_max = 1.0f;
   o->hierarchy_levels = 1;
-  o->smooth_weight = 0.2f;
-  o->anchor_views = 2;
+  o->smooth_weight = 0.0f;
+  o->anchor_views = 0;
   o->checkerboard_filter = 0;
+  o->speckle_min_size = 100;
+  o->gap_max_size = 7;
+  // Not DepthmapParams fields: the kernel's hard-coded variance gate
+  // (opencl_kernels.h:254, 516, 519) and this library's storage options.
   o->var_threshold = 1.5e-6f;
   o->image_storage = PM_STORE_F32;
   o->quantize_half = 0;
-  o->speckle_min_size = 0;  // config.py: depthmap_speckle_min_size
-  o->gap_max_size = 0;      // config.py: depthmap_gap_max_size
+}
+
+void pm_pipeline_params(pm_params* o) {
+  if (!o) return;
+  // What the dense pipeline sets through _setup_cluster_params
+  // (opensfm/dense/depthmaps.py:154-175) from the config defaults
+  // (opensfm/config.py:407-466); the rest as pm_default_params.
+  pm_default_params(o);
+  o->max_iterations = 4;                // depthmap_max_iterations
+  o->patch_size = 5;                    // depthmap_patch_size
+  o->sigma_spatial = 1.5f;              // depthmap_sigma_spatial
+  o->sigma_color = 25.0f / 255.0f;      // depthmap_sigma_color
+  o->top_k = 8;                         // depthmap_num_matching_views
+  o->use_census = 1;                    // depthmap_use_census
+  o->hierarchy_levels = 3;              // depthmap_hierarchy_levels
+  o->smooth_weight = 0.2f;              // depthmap_smooth_weight
+  o->anchor_views = 2;                  // depthmap_anchor_views
+  o->checkerboard_filter = 0;           // depthmap_checkerboard_filter
+  o->speckle_min_size = 0;              // depthmap_speckle_min_size
+  o->gap_max_size = 0;                  // depthmap_gap_max_size
 }
 
 const char* pm_last_error(void) { return g_last_error.c_str(); }
@@ -1252,6 +1274,7 @@ int run_level(pm_handle* h, int level, int nlevels) {
   if (int r = postprocess(h)) return r;
   if (!last) {  // keep the decoded level for the next one (estimator.cc:238-246)
     const size_t npix = static_cast<size_t>(h->W) * h->H;
+    ++h->gen;  // a batch's captured upsample launches may point at the buffers replaced here
     if (int r = h->coarse_depth.alloc(npix)) return r;
     if (int r = h->coarse_normal.alloc(3 * npix)) return r;
     PM_HIP(hipMemcpyAsync(h->coarse_depth.p, h->out_depth.p, npix * sizeof(float), hipMemcpyDeviceToDevice,
@@ -1829,6 +1852,11 @@ int batch_validate(pm_batch* b) {
   for (pm_handle* h : b->refs) {
     if (h->device != b->device) return fail(PM_E_INVALID, "batch: every reference must be on the batch's device");
     if (int r = validate(h)) return r;
+    // A finer level starts from the level just below it, once: the coarse
+    // buffers then hold this level's result, not the one it started from.
+    if (b->level > 0 && h->next_level != b->level)
+      return fail(PM_E_STATE, "batch: levels run in order, each once (level " + std::to_string(b->level) +
+                                  " needs level " + std::to_string(b->level - 1) + " just before it)");
     const pm_params& p = h->params;
     if (level_count(p) != level_count(p0))
       return fail(PM_E_INVALID, "batch: every reference must have the same number of pyramid levels");
@@ -2076,6 +2104,11 @@ int pm_batch_prepare(pm_batch* b) {
 
 int pm_batch_run(pm_batch* b) {
   if (int r = check_batch(b)) return r;
+  if (b->level > 0)  // before a graph replay too: its upsample launches read the coarse buffers
+    for (pm_handle* h : b->refs)
+      if (h->next_level != b->level)
+        return fail(PM_E_STATE, "batch: levels run in order, each once (level " + std::to_string(b->level) +
+                                    " needs level " + std::to_string(b->level - 1) + " just before it)");
   if (batch_stale(b))
     if (int r = batch_prepare(b)) return r;
   PM_HIP(hipSetDevice(b->device), "hipSetDevice");
@@ -2107,6 +2140,7 @@ int pm_batch_run(pm_batch* b) {
     h->next_level = b->level + 1;
     if (b->level < level_count(h->params) - 1) {  // keep the decoded level for the next one
       const size_t npix = static_cast<size_t>(h->W) * h->H;
+      ++h->gen;  // captured upsample launches may point at the buffers replaced here
       if (int r = h->coarse_depth.alloc(npix)) return r;
       if (int r = h->coarse_normal.alloc(3 * npix)) return r;
       PM_HIP(hipMemcpyAsync(h->coarse_depth.p, h->out_depth.p, npix * sizeof(float), hipMemcpyDeviceToDevice,

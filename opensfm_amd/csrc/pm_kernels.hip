@@ -2,6 +2,8 @@
 // the set-up kernels: quad copy of the views, coarse-to-fine transfer,
 // confidence, scoring of given planes.  The kernels templated on the image
 // storage type are instantiated in pm_cost_f32.hip / pm_cost_f16.hip.
+#include <atomic>
+
 #include "pm_body.h"
 
 namespace pmhip {
@@ -241,13 +243,35 @@ hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream
 #ifndef PM_INIT_WAVES
 #define PM_INIT_WAVES 2
 #endif
+// LDS bytes per CU of the current device (160 KiB on MI355X), queried once
+// per device; the occupancy caps below reserve a share of it per workgroup.
+constexpr int kSimdsPerCu = 4;
+size_t lds_per_cu() {
+  constexpr int kDevices = 64;
+  static std::atomic<size_t> cache[kDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) dev = -1;
+  if (dev >= 0) {
+    const size_t known = cache[dev].load(std::memory_order_relaxed);
+    if (known) return known;
+  }
+  int bytes = 0;
+  if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev < 0 ? 0 : dev) !=
+          hipSuccess ||
+      bytes <= 0) {
+    (void)hipGetLastError();
+    return 0;  // unknown: no cap
+  }
+  if (dev >= 0) cache[dev].store(static_cast<size_t>(bytes), std::memory_order_relaxed);
+  return static_cast<size_t>(bytes);
+}
+
 // Dynamic LDS of an init or sweep launch: its cost columns, or more to cap
 // its one-wave workgroups at PM_INIT_WAVES per SIMD.
 size_t init_lds(int n_src) {
-  constexpr size_t kLdsPerCu = 160 * 1024;  // MI355X
-  constexpr int kSimdsPerCu = 4;
   const size_t cols = cost_column_lds(n_src);
-  const size_t cap = PM_INIT_WAVES > 0 ? kLdsPerCu / (static_cast<size_t>(kSimdsPerCu) * PM_INIT_WAVES + 1) + 1 : 0;
+  const size_t lds = lds_per_cu();
+  const size_t cap = PM_INIT_WAVES > 0 && lds > 0 ? lds / (static_cast<size_t>(kSimdsPerCu) * PM_INIT_WAVES + 1) + 1 : 0;
   return cols > cap ? cols : cap;
 }
 
@@ -294,12 +318,12 @@ hipError_t launch_refine_cost(const PmArgs& a, int hp, bool f16, int cost_x, hip
 
 size_t fallback_cap_lds(int hp, int tile_x, int fallback_waves) {
   if (fallback_waves <= 0 || tile_x <= 0) return 0;
-  constexpr size_t kLdsPerCu = 160 * 1024;  // MI355X
-  constexpr int kSimdsPerCu = 4;
+  const size_t lds = lds_per_cu();
+  if (lds == 0) return 0;
   const int r = 3 * hp > 2 ? 3 * hp : 2;
   const size_t static_bytes = sizeof(float) * (2 * tile_x + 2 * r) * (kWave / tile_x + 2 * r);
   // more than 1/(n+1) of the LDS per one-wave workgroup: at most n per CU
-  const size_t per_wg = kLdsPerCu / (static_cast<size_t>(kSimdsPerCu) * fallback_waves + 1) + 1;
+  const size_t per_wg = lds / (static_cast<size_t>(kSimdsPerCu) * fallback_waves + 1) + 1;
   return per_wg > static_bytes ? per_wg - static_bytes : 0;
 }
 

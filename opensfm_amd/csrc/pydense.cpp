@@ -74,6 +74,87 @@ void set_points(pm_handle* h, py::handle points) {
   check(pm_set_sfm_points(h, a.size() ? a.data() : nullptr, static_cast<int>(a.size() / 3)));
 }
 
+// pm_params <-> {field: value}, the field names of pm_params.
+py::dict params_dict(const pm_params& p) {
+  py::dict d;
+  d["max_iterations"] = p.max_iterations;
+  d["patch_size"] = p.patch_size;
+  d["sigma_spatial"] = p.sigma_spatial;
+  d["sigma_color"] = p.sigma_color;
+  d["top_k"] = p.top_k;
+  d["use_census"] = static_cast<bool>(p.use_census);
+  d["depth_min"] = p.depth_min;
+  d["depth_max"] = p.depth_max;
+  d["hierarchy_levels"] = p.hierarchy_levels;
+  d["smooth_weight"] = p.smooth_weight;
+  d["anchor_views"] = p.anchor_views;
+  d["checkerboard_filter"] = static_cast<bool>(p.checkerboard_filter);
+  d["var_threshold"] = p.var_threshold;
+  d["image_storage"] = p.image_storage == PM_STORE_F16 ? "f16" : "f32";
+  d["quantize_half"] = static_cast<bool>(p.quantize_half);
+  d["speckle_min_size"] = p.speckle_min_size;
+  d["gap_max_size"] = p.gap_max_size;
+  return d;
+}
+
+int storage_code(const std::string& s) {
+  if (s == "f32") return PM_STORE_F32;
+  if (s == "f16") return PM_STORE_F16;
+  throw std::invalid_argument("image storage must be 'f32' or 'f16'");
+}
+
+void params_from_dict(pm_params& p, py::dict d) {
+  for (auto item : d) {
+    const std::string k = py::str(item.first);
+    py::handle v = item.second;
+    if (k == "max_iterations") p.max_iterations = v.cast<int>();
+    else if (k == "patch_size") p.patch_size = v.cast<int>();
+    else if (k == "sigma_spatial") p.sigma_spatial = v.cast<float>();
+    else if (k == "sigma_color") p.sigma_color = v.cast<float>();
+    else if (k == "top_k") p.top_k = v.cast<int>();
+    else if (k == "use_census") p.use_census = v.cast<bool>() ? 1 : 0;
+    else if (k == "depth_min") p.depth_min = v.cast<float>();
+    else if (k == "depth_max") p.depth_max = v.cast<float>();
+    else if (k == "hierarchy_levels") p.hierarchy_levels = v.cast<int>();
+    else if (k == "smooth_weight") p.smooth_weight = v.cast<float>();
+    else if (k == "anchor_views") p.anchor_views = v.cast<int>();
+    else if (k == "checkerboard_filter") p.checkerboard_filter = v.cast<bool>() ? 1 : 0;
+    else if (k == "var_threshold") p.var_threshold = v.cast<float>();
+    else if (k == "image_storage") p.image_storage = storage_code(v.cast<std::string>());
+    else if (k == "quantize_half") p.quantize_half = v.cast<bool>() ? 1 : 0;
+    else if (k == "speckle_min_size") p.speckle_min_size = v.cast<int>();
+    else if (k == "gap_max_size") p.gap_max_size = v.cast<int>();
+    else throw std::invalid_argument("unknown parameter '" + k + "'");
+  }
+}
+
+using F32Array = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using U32Array = py::array_t<uint32_t, py::array::c_style | py::array::forcecast>;
+
+// A previous depth map: None, or a non-empty (H, W) array (returned as
+// float32, C order); anything else raises ValueError.
+F32Array depth_map_array(py::handle item) {
+  if (item.is_none()) return F32Array();
+  auto a = F32Array::ensure(item);
+  if (!a || a.ndim() != 2 || a.shape(0) <= 0 || a.shape(1) <= 0)
+    throw std::invalid_argument("previous depth map must be a non-empty (H, W) array");
+  return a;
+}
+
+// Planar prior of a w x h reference: planes (h, w, 4) and masks (h, w);
+// anything else raises ValueError.
+std::pair<F32Array, U32Array> prior_arrays(py::handle planes, py::handle masks, py::ssize_t w, py::ssize_t h) {
+  auto p = F32Array::ensure(planes);
+  auto m = U32Array::ensure(masks);
+  if (!p || p.ndim() != 3 || p.shape(0) != h || p.shape(1) != w || p.shape(2) != 4)
+    throw std::invalid_argument("prior planes must be an (H, W, 4) array of the reference view's size (" +
+                                std::to_string(h) + ", " + std::to_string(w) + ", 4)");
+  if (!m || m.ndim() != 2 || m.shape(0) != h || m.shape(1) != w)
+    throw std::invalid_argument("prior masks must be an (H, W) array of the reference view's size (" +
+                                std::to_string(h) + ", " + std::to_string(w) + ")");
+  return {p, m};
+}
+
 class Estimator {
  public:
   explicit Estimator(int device) { check(pm_create(device, &h_)); }
@@ -102,25 +183,12 @@ class Estimator {
   py::dict params() {
     pm_params p;
     check(pm_get_params(h_, &p));
-    py::dict d;
-    d["max_iterations"] = p.max_iterations;
-    d["patch_size"] = p.patch_size;
-    d["sigma_spatial"] = p.sigma_spatial;
-    d["sigma_color"] = p.sigma_color;
-    d["top_k"] = p.top_k;
-    d["use_census"] = static_cast<bool>(p.use_census);
-    d["depth_min"] = p.depth_min;
-    d["depth_max"] = p.depth_max;
-    d["hierarchy_levels"] = p.hierarchy_levels;
-    d["smooth_weight"] = p.smooth_weight;
-    d["anchor_views"] = p.anchor_views;
-    d["checkerboard_filter"] = static_cast<bool>(p.checkerboard_filter);
-    d["var_threshold"] = p.var_threshold;
-    d["image_storage"] = p.image_storage == PM_STORE_F16 ? "f16" : "f32";
-    d["quantize_half"] = static_cast<bool>(p.quantize_half);
-    d["speckle_min_size"] = p.speckle_min_size;
-    d["gap_max_size"] = p.gap_max_size;
-    return d;
+    return params_dict(p);
+  }
+
+  // Any subset of the params() fields, by name.
+  void set_params(py::dict d) {
+    update([&](pm_params& p) { params_from_dict(p, d); });
   }
 
   py::list compute_patch_match() { return as_list(compute()); }
@@ -194,7 +262,7 @@ class Estimator {
         want[k] = true;
       }
     }
-    const auto [w, h] = ref_size();
+    const auto [w, h] = staged_size();
     py::array_t<float> depth, plane, normal, cost, conf;
     py::array_t<uint32_t> views;
     if (want[0]) depth = py::array_t<float>(shape({h, w}));
@@ -221,7 +289,7 @@ class Estimator {
   void release_device() { check(pm_release_device(h_)); }
 
   py::tuple read_state() {
-    const auto [w, h] = ref_size();
+    const auto [w, h] = staged_size();
     py::array_t<float> planes(shape({h, w, 4})), costs(shape({h, w}));
     py::array_t<uint32_t> rng(shape({h, w, 2})), sel(shape({h, w}));
     check(pm_read_state(h_, planes.mutable_data(), costs.mutable_data(), rng.mutable_data(), sel.mutable_data()));
@@ -232,7 +300,7 @@ class Estimator {
                    py::array_t<float, py::array::c_style | py::array::forcecast> costs,
                    py::array_t<uint32_t, py::array::c_style | py::array::forcecast> rng,
                    py::array_t<uint32_t, py::array::c_style | py::array::forcecast> sel) {
-    const auto [w, h] = ref_size();
+    const auto [w, h] = staged_size();
     const py::ssize_t n = static_cast<py::ssize_t>(w) * h;
     if (planes.size() != 4 * n || costs.size() != n || rng.size() != 2 * n || sel.size() != n)
       throw std::invalid_argument("write_state: arrays do not match the reference view size");
@@ -262,7 +330,7 @@ class Estimator {
   py::tuple ncc_cost_grad(py::array_t<float, py::array::c_style | py::array::forcecast> depth,
                           py::array_t<float, py::array::c_style | py::array::forcecast> ab,
                           py::array_t<uint32_t, py::array::c_style | py::array::forcecast> views) {
-    const auto [w, h] = ref_size();
+    const auto [w, h] = staged_size();
     const py::ssize_t n = w * h;
     if (depth.size() != n || ab.size() != 2 * n || views.size() != n)
       throw std::invalid_argument("ncc_cost_grad: arrays do not match the reference view size");
@@ -316,34 +384,47 @@ class Estimator {
     return d;
   }
 
+  // Each previous depth map is an (H, W) array of its own size; the library
+  // resizes it to every pyramid level (SetPreviousDepths takes maps of any
+  // size, estimator.cc:59-116).
   void set_geom_consistency(float weight, py::list prev) {
-    std::vector<py::array_t<float, py::array::c_style | py::array::forcecast>> keep;
-    std::vector<const float*> ptrs;
-    for (auto item : prev) {
-      if (item.is_none()) {
-        ptrs.push_back(nullptr);
-      } else {
-        keep.push_back(py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(item));
-        ptrs.push_back(keep.back().data());
-      }
+    if (py::len(prev) > PM_MAX_SOURCES)
+      throw std::invalid_argument("set_geom_consistency: at most " + std::to_string(PM_MAX_SOURCES) +
+                                  " previous depth maps");
+    std::vector<py::array_t<float, py::array::c_style | py::array::forcecast>> maps;
+    for (auto item : prev) maps.push_back(depth_map_array(item));  // validate every map before any change
+    check(pm_set_geom_consistency(h_, weight, nullptr, 0));
+    check(pm_clear_previous_depths(h_));
+    for (size_t i = 0; i < maps.size(); ++i) {
+      if (!maps[i]) continue;
+      check(pm_set_previous_depth(h_, static_cast<int>(i), maps[i].data(), static_cast<int>(maps[i].shape(1)),
+                                  static_cast<int>(maps[i].shape(0))));
     }
-    check(pm_set_geom_consistency(h_, weight, ptrs.data(), static_cast<int>(ptrs.size())));
   }
 
+  // planes (H, W, 4) and masks (H, W) at the reference view's size.
   void set_prior(py::object planes, py::object masks) {
     if (planes.is_none() || masks.is_none()) {
       check(pm_set_prior(h_, nullptr, nullptr));
       return;
     }
-    auto p = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(planes);
-    auto m = py::array_t<uint32_t, py::array::c_style | py::array::forcecast>::ensure(masks);
-    check(pm_set_prior(h_, p.data(), m.data()));
+    const auto [w, h] = ref_size();
+    auto pm = prior_arrays(planes, masks, w, h);
+    check(pm_set_prior(h_, pm.first.data(), pm.second.data()));
   }
 
  private:
   std::pair<py::ssize_t, py::ssize_t> ref_size() {
     if (ref_w_ < 0) throw std::runtime_error("DepthmapEstimator: no views added");
     return {ref_w_, ref_h_};
+  }
+  // Size of the pyramid level staged on the device (the decoded maps and the
+  // per-pixel state have it; it is the reference size after a full run).
+  std::pair<py::ssize_t, py::ssize_t> staged_size() {
+    (void)ref_size();
+    int w = 0, h = 0;
+    check(pm_level_size(h_, &w, &h));
+    return {w, h};
   }
 
  public:
@@ -704,6 +785,22 @@ class ClusterEstimator {
 
 PYBIND11_MODULE(pydense, m) {
   m.doc() = "MI355X PatchMatch multi-view depth estimator (pydense.DepthmapEstimator surface)";
+  // The argument checks of set_prior / set_geom_consistency, callable without
+  // a device (they raise ValueError as those methods do).
+  m.def("_check_prior_arrays", [](py::handle planes, py::handle masks, py::ssize_t width, py::ssize_t height) {
+    prior_arrays(planes, masks, width, height);
+  }, py::arg("planes"), py::arg("masks"), py::arg("width"), py::arg("height"));
+  m.def("_check_depth_map", [](py::handle item) { depth_map_array(item); }, py::arg("depth"));
+  m.def("default_params", []() {
+    pm_params p;
+    pm_default_params(&p);
+    return params_dict(p);
+  }, "DepthmapParams' defaults (estimator.h:16-36): what a new estimator starts from");
+  m.def("pipeline_params", []() {
+    pm_params p;
+    pm_pipeline_params(&p);
+    return params_dict(p);
+  }, "The dense pipeline's parameters (depthmaps.py:154-175 from config.py:407-466)");
   py::class_<Estimator>(m, "DepthmapEstimator")
       .def(py::init<int>(), py::arg("device") = 0)
       .def("add_view",
@@ -734,8 +831,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("set_var_threshold", [](Estimator& e, float v) { e.update([v](pm_params& p) { p.var_threshold = v; }); })
       .def("set_image_storage",
            [](Estimator& e, const std::string& s) {
-             if (s != "f32" && s != "f16") throw std::invalid_argument("image storage must be 'f32' or 'f16'");
-             e.update([s](pm_params& p) { p.image_storage = s == "f16" ? PM_STORE_F16 : PM_STORE_F32; });
+             const int code = storage_code(s);
+             e.update([code](pm_params& p) { p.image_storage = code; });
            })
       .def("set_quantize_half", [](Estimator& e, bool v) { e.update([v](pm_params& p) { p.quantize_half = v; }); })
       .def("set_speckle_min_size", [](Estimator& e, int v) { e.update([v](pm_params& p) { p.speckle_min_size = v; }); })
@@ -745,6 +842,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("set_sfm_points", [](Estimator& e, py::handle pts) { set_points(e.handle(), pts); }, py::arg("points"),
            "SfM points (N, 3) of the reference: planar prior per level (ComputeSfMPlanarPrior)")
       .def("params", &Estimator::params)
+      .def("set_params", &Estimator::set_params, py::arg("params"),
+           "Set any subset of the params() fields, e.g. set_params(pydense.pipeline_params())")
       .def("compute_patch_match", &Estimator::compute_patch_match,
            "Run PatchMatch; returns [depth (H,W), plane normal (H,W,3), score (H,W), selected-view bitmask (H,W)]")
       .def("compute_patch_match_sample", &Estimator::compute_patch_match_sample,
@@ -983,6 +1082,8 @@ PYBIND11_MODULE(pydense, m) {
       .def("set_speckle_min_size", set(&pm_params::speckle_min_size))
       .def("set_gap_max_size", set(&pm_params::gap_max_size))
       .def("set_geom_consistency_weight", &C::set_geom_consistency_weight)
+      .def("params", [](C& c) { return params_dict(c.params()); })
+      .def("set_params", [](C& c, py::dict d) { params_from_dict(c.params(), d); }, py::arg("params"))
       .def("set_device", &C::set_device)
       .def("begin_ref_view", &C::begin_ref_view, py::arg("K"), py::arg("R"), py::arg("t"), py::arg("image"),
            py::arg("depth_min"), py::arg("depth_max"))

@@ -385,6 +385,43 @@ float census_inner(const Ctx& c, int s, float ref_center, float src_center, int 
   return static_cast<float>(hamming) / 24.0f;
 }
 
+// The plane's points at (x, y), (x+1, y) and (x, y+1) in the world
+// (opencl_kernels.h:401-416); false when a depth is <= 0 (cost 2.0).
+struct PlanePoints { V3 cw, wdx, wdy; };
+bool plane_points(const pmo_camera& c0, int x, int y, Plane plane, PlanePoints* pp) {
+  const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane, c0);
+  if (depth <= 0.0f) return false;
+  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), plane, c0);
+  const float depth_dy = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), plane, c0);
+  if (depth_dx <= 0.0f || depth_dy <= 0.0f) return false;
+  pp->cw = backproject(static_cast<float>(x), static_cast<float>(y), depth, c0);
+  pp->wdx = backproject(static_cast<float>(x + 1), static_cast<float>(y), depth_dx, c0);
+  pp->wdy = backproject(static_cast<float>(x), static_cast<float>(y + 1), depth_dy, c0);
+  return true;
+}
+
+// Local affine warp into source camera cs (opencl_kernels.h:432-456): the
+// projected centre and the forward-difference Jacobian, source pixel
+// (sx, sy) = (scx + jxx*dx + jyx*dy, scy + jxy*dx + jyy*dy) for a reference
+// offset (dx, dy).  0: valid; 1: a point is behind the source camera;
+// 2: det J outside [0.02, 50].  (Both invalid cases cost 2.0.)
+struct Warp { float scx, scy, jxx, jxy, jyx, jyy; };
+int source_warp(const PlanePoints& pp, const pmo_camera& cs, Warp* w) {
+  const V3 pc = project(pp.cw, cs);
+  if (pc.z < 1e-6f) return 1;
+  w->scx = pc.x / pc.z;
+  w->scy = pc.y / pc.z;
+  const V3 pdx = project(pp.wdx, cs);
+  const V3 pdy = project(pp.wdy, cs);
+  if (pdx.z < 1e-6f || pdy.z < 1e-6f) return 1;
+  w->jxx = pdx.x / pdx.z - w->scx;
+  w->jxy = pdx.y / pdx.z - w->scy;
+  w->jyx = pdy.x / pdy.z - w->scx;
+  w->jyy = pdy.y / pdy.z - w->scy;
+  const float det = w->jxx * w->jyy - w->jxy * w->jyx;
+  return (det < 0.02f || det > 50.0f) ? 2 : 0;
+}
+
 // Per-view cost vector ("ComputePlaneScore"): opencl_kernels.h:381-557.
 // The reference loops taps outer / views inner; each view's sums are
 // accumulated over the taps in the same order either way, so the oracle
@@ -392,21 +429,11 @@ float census_inner(const Ctx& c, int s, float ref_center, float src_center, int 
 int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, float* out) {
   const int n_src = c.n_src;
   const int hp = c.pr->half_patch;
-  const pmo_camera& c0 = c.cams[0];
-  const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane, c0);
-  if (depth <= 0.0f) {
+  PlanePoints pp;
+  if (!plane_points(c.cams[0], x, y, plane, &pp)) {
     for (int i = 0; i < n_src; ++i) out[i] = 2.0f;
     return n_src;
   }
-  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), plane, c0);
-  const float depth_dy = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), plane, c0);
-  if (depth_dx <= 0.0f || depth_dy <= 0.0f) {
-    for (int i = 0; i < n_src; ++i) out[i] = 2.0f;
-    return n_src;
-  }
-  const V3 cw = backproject(static_cast<float>(x), static_cast<float>(y), depth, c0);
-  const V3 wdx = backproject(static_cast<float>(x + 1), static_cast<float>(y), depth_dx, c0);
-  const V3 wdy = backproject(static_cast<float>(x), static_cast<float>(y + 1), depth_dy, c0);
 
   const bool ref_ok = rp.sum_w >= 1e-6f;
   float inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
@@ -417,20 +444,9 @@ int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, flo
   }
 
   for (int s = 0; s < n_src; ++s) {
-    const pmo_camera& cs = c.cams[s + 1];
-    const V3 pc = project(cw, cs);
-    if (pc.z < 1e-6f) { out[s] = 2.0f; continue; }
-    const float scx = pc.x / pc.z;
-    const float scy = pc.y / pc.z;
-    const V3 pdx = project(wdx, cs);
-    const V3 pdy = project(wdy, cs);
-    if (pdx.z < 1e-6f || pdy.z < 1e-6f) { out[s] = 2.0f; continue; }
-    const float jxx = pdx.x / pdx.z - scx;
-    const float jxy = pdx.y / pdx.z - scy;
-    const float jyx = pdy.x / pdy.z - scx;
-    const float jyy = pdy.y / pdy.z - scy;
-    const float det = jxx * jyy - jxy * jyx;
-    if (det < 0.02f || det > 50.0f) { out[s] = 2.0f; continue; }
+    Warp wp;
+    if (source_warp(pp, c.cams[s + 1], &wp) != 0) { out[s] = 2.0f; continue; }
+    const float scx = wp.scx, scy = wp.scy, jxx = wp.jxx, jxy = wp.jxy, jyx = wp.jyx, jyy = wp.jyy;
     const float scen = c.src[s].bilinear(scx, scy);
 
     // Fused stride-1 ZNCC over the cached reference weights (:471-499).
@@ -1533,6 +1549,30 @@ int pmo_refinement_hypotheses(const pmo_scene* sc, const pmo_params* pr, const f
 
 // Score n (pixel, plane) pairs: per-view costs (n x num_src), top-k initial
 // cost and view bitmask.  xy: 2n ints; planes: 4n floats.
+// The local affine warp of cost_vector for one source: out = (scx, scy,
+// jxx, jxy, jyx, jyy); returns -1 when a plane depth is <= 0, else
+// source_warp's code (0 valid, 1 behind the source camera, 2 det gate).
+int pmo_warp(const pmo_scene* sc, const pmo_params* pr, int x, int y, const float* plane, int src, float* out) {
+  Ctx c(sc, pr);
+  PlanePoints pp;
+  if (!plane_points(c.cams[0], x, y, load_plane(plane, 0), &pp)) return -1;
+  Warp w{};
+  const int code = source_warp(pp, c.cams[src + 1], &w);
+  const float v[6] = {w.scx, w.scy, w.jxx, w.jxy, w.jyx, w.jyy};
+  std::memcpy(out, v, sizeof(v));
+  return code;
+}
+
+// The strided NCC of the textureless fallback on a given warp w = (scx, scy,
+// jxx, jxy, jyx, jyy) of source src at reference pixel (x, y); the source
+// centre value is the bilinear sample at (scx, scy), as in cost_vector.
+float pmo_ncc_at_stride(const pmo_scene* sc, const pmo_params* pr, int x, int y, int src, float stride,
+                        const float* w) {
+  Ctx c(sc, pr);
+  const float scen = c.src[src].bilinear(w[0], w[1]);
+  return ncc_at_stride(c, src, c.ref.texel(x, y), scen, w[0], w[1], x, y, stride, w[2], w[3], w[4], w[5]);
+}
+
 void pmo_score_planes(const pmo_scene* sc, const pmo_params* pr, int n, const int* xy,
                       const float* planes, float* costs, float* init_cost, uint32_t* sel) {
   Ctx c(sc, pr);

@@ -21,7 +21,9 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libpm_oracle.so")
+# PM_ORACLE_LIB: load another build of pm_oracle.cc instead (the mutation
+# tests point it at copies with planted mistakes).
+_LIB_PATH = os.environ.get("PM_ORACLE_LIB") or os.path.join(_HERE, "libpm_oracle.so")
 _lib = None
 
 
@@ -76,6 +78,8 @@ def lib():
         L.pmo_plane_from_depth_normal.argtypes = [vp, f, f, f, vp, vp]
         L.pmo_ref_patch.argtypes = [vp, vp, i, i, vp]
         L.pmo_score_planes.argtypes = [vp, vp, i, vp, vp, vp, vp, vp]
+        L.pmo_warp.restype = i; L.pmo_warp.argtypes = [vp, vp, i, i, vp, i, vp]
+        L.pmo_ncc_at_stride.restype = f; L.pmo_ncc_at_stride.argtypes = [vp, vp, i, i, i, f, vp]
         for name in ("pmo_random_init", "pmo_prior_reinit", "pmo_run"):
             getattr(L, name).argtypes = [vp, vp, vp]
         L.pmo_sweep.argtypes = [vp, vp, vp, i, i, f]
@@ -267,6 +271,26 @@ def score_planes(scene: OracleScene, params: Params, xy: np.ndarray, planes: np.
     lib().pmo_score_planes(ctypes.byref(scene.c), ctypes.byref(p), n, _ptr(xy), _ptr(planes),
                            _ptr(costs), _ptr(init), _ptr(sel))
     return costs, init, sel
+
+
+def warp(scene: OracleScene, params: Params, x: int, y: int, plane, src: int):
+    """The cost vector's local affine warp into source ``src`` (0-based):
+    (code, (scx, scy, jxx, jxy, jyx, jyy)); code -1 = a plane depth <= 0,
+    0 = valid, 1 = behind the source camera, 2 = determinant gate."""
+    pl = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros(6, np.float32)
+    p = params.c()
+    code = lib().pmo_warp(ctypes.byref(scene.c), ctypes.byref(p), int(x), int(y), _ptr(pl), int(src), _ptr(out))
+    return code, out
+
+
+def ncc_at_stride(scene: OracleScene, params: Params, x: int, y: int, src: int, stride: float, w) -> float:
+    """The textureless fallback's strided NCC at reference pixel (x, y) on the
+    warp w = (scx, scy, jxx, jxy, jyx, jyy) of source ``src``."""
+    wa = np.ascontiguousarray(w, np.float32)
+    p = params.c()
+    return float(lib().pmo_ncc_at_stride(ctypes.byref(scene.c), ctypes.byref(p), int(x), int(y), int(src),
+                                         float(stride), _ptr(wa)))
 
 
 def ref_patch(scene: OracleScene, params: Params, x: int, y: int) -> np.ndarray:

@@ -190,11 +190,14 @@ def test_buffer_cache_reuses_released_blocks(pydense, survey):
 
 
 class _StubEstimator:
-    def __init__(self, log, i, nbytes):
-        self.log, self.i, self.nbytes = log, i, nbytes
+    def __init__(self, log, i, nbytes, levels=1):
+        self.log, self.i, self.nbytes, self.levels = log, i, nbytes, levels
 
     def device_bytes(self):
         return self.nbytes
+
+    def params(self):
+        return {"hierarchy_levels": self.levels}
 
     def gauss_newton(self, steps, damping):
         self.log.append(("gn", self.i, steps))
@@ -227,6 +230,11 @@ class _StubLibrary:
         class Batch:
             def __init__(self, device):
                 self.refs = []
+                self.level = 0
+
+            def set_level(self, level):
+                self.level = level
+                log.append(("level", tuple(self.refs), level))
 
             def add(self, e):
                 self.refs.append(e.i)
@@ -238,7 +246,7 @@ class _StubLibrary:
                 log.append(("prepare", tuple(self.refs)))
 
             def run(self):
-                log.append(("run", tuple(self.refs)))
+                log.append(("run", tuple(self.refs)) if self.level == 0 else ("run", tuple(self.refs), self.level))
                 return 5.0
 
         self.DepthmapEstimator = Est
@@ -272,6 +280,22 @@ def test_run_admitted_control_flow(overlap):
             assert lib.log.index(("gn", i, 2)) > k and lib.log.index(("release", i)) > k
     assert lib.log[0] == ("cache", budget) and lib.log[-1] == ("cache", 0)
     assert all(g.run_ms == 5.0 and g.refine_ms == len(g.refs) for g in groups)
+
+
+def test_run_admitted_runs_every_pyramid_level_in_order():
+    # a batch starts at level 0 of each reference's pyramid; the finer levels
+    # run after it, in order, before the refinement and the results
+    from opensfm_amd.admission import run_admitted
+    lib = _StubLibrary()
+    res, groups = run_admitted(lib, 3, lambda i: _StubEstimator(lib.log, i, 40, levels=3), budget=100,
+                               refine_steps=1)
+    runs = [e for e in lib.log if e[0] in ("run", "level")]
+    assert runs == [("run", (0, 1)), ("level", (0, 1), 1), ("run", (0, 1), 1), ("level", (0, 1), 2),
+                    ("run", (0, 1), 2), ("run", (2,)), ("level", (2,), 1), ("run", (2,), 1),
+                    ("level", (2,), 2), ("run", (2,), 2)]
+    last = lib.log.index(("run", (0, 1), 2))
+    assert lib.log.index(("gn", 0, 1)) > last and lib.log.index(("release", 1)) > last
+    assert [g.run_ms for g in groups] == [15.0, 15.0]
 
 
 def test_run_admitted_errors_reset_the_cache():

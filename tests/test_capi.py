@@ -45,13 +45,82 @@ def test_library_exports_every_declared_symbol(lib):
     assert not missing, missing
 
 
-def test_default_params_are_the_pipeline_profile(lib):
-    buf = (ctypes.c_char * 256)()
-    lib.pm_default_params(buf)
-    ints = np.frombuffer(bytes(buf)[:8], np.int32)
-    floats = np.frombuffer(bytes(buf)[8:16], np.float32)
-    assert ints.tolist() == [3, 7]                      # iterations, patch size
-    assert np.allclose(floats, [1.5, 25 / 255])         # config.py:447-449
+class PmParams(ctypes.Structure):
+    """pm_params of include/opensfm_pm.h, field by field."""
+    _fields_ = [("max_iterations", ctypes.c_int), ("patch_size", ctypes.c_int), ("sigma_spatial", ctypes.c_float),
+                ("sigma_color", ctypes.c_float), ("top_k", ctypes.c_int), ("use_census", ctypes.c_int),
+                ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float), ("hierarchy_levels", ctypes.c_int),
+                ("smooth_weight", ctypes.c_float), ("anchor_views", ctypes.c_int),
+                ("checkerboard_filter", ctypes.c_int), ("var_threshold", ctypes.c_float),
+                ("image_storage", ctypes.c_int), ("quantize_half", ctypes.c_int),
+                ("speckle_min_size", ctypes.c_int), ("gap_max_size", ctypes.c_int)]
+
+
+def _fields(p):
+    return {name: getattr(p, name) for name, _ in PmParams._fields_}
+
+
+# DepthmapParams' member initialisers (opensfm/src/lib/dense/estimator.h:16-36)
+DEPTHMAP_PARAMS = dict(max_iterations=3, patch_size=5, sigma_spatial=3.0, sigma_color=np.float32(3.0 / 255.0),
+                       top_k=4, use_census=1, depth_min=0.0, depth_max=1.0, hierarchy_levels=1, smooth_weight=0.0,
+                       anchor_views=0, checkerboard_filter=0, speckle_min_size=100, gap_max_size=7,
+                       # not DepthmapParams fields: the kernel's variance gate
+                       # (opencl_kernels.h:254), f32 storage, no half rounding
+                       var_threshold=np.float32(1.5e-6), image_storage=0, quantize_half=0)
+# What _setup_cluster_params sets (opensfm/dense/depthmaps.py:154-175) from
+# the config defaults (opensfm/config.py:407-466)
+PIPELINE = dict(DEPTHMAP_PARAMS, max_iterations=4, patch_size=5, sigma_spatial=1.5,
+                sigma_color=np.float32(25.0 / 255.0), top_k=8, use_census=1, hierarchy_levels=3, smooth_weight=0.2,
+                anchor_views=2, checkerboard_filter=0, speckle_min_size=0, gap_max_size=0)
+
+
+def _check_profile(got, want):
+    assert set(got) == set(want)
+    for k, v in want.items():
+        assert np.float32(got[k]) == np.float32(v), (k, got[k], v)
+
+
+def test_default_params_are_depthmap_params(lib):
+    p = PmParams()
+    lib.pm_default_params(ctypes.byref(p))
+    _check_profile(_fields(p), DEPTHMAP_PARAMS)
+
+
+def test_pipeline_params_are_the_pipeline_profile(lib):
+    p = PmParams()
+    lib.pm_pipeline_params(ctypes.byref(p))
+    _check_profile(_fields(p), PIPELINE)
+
+
+def test_pydense_profiles_match_the_abi():
+    from opensfm_amd import native
+    pd = native()
+    for got, want in ((pd.default_params(), DEPTHMAP_PARAMS), (pd.pipeline_params(), PIPELINE)):
+        got = dict(got)
+        got["use_census"] = int(got["use_census"])
+        got["checkerboard_filter"] = int(got["checkerboard_filter"])
+        got["quantize_half"] = int(got["quantize_half"])
+        got["image_storage"] = {"f32": 0, "f16": 1}[got["image_storage"]]
+        _check_profile(got, want)
+
+
+def test_prior_and_previous_depth_shapes_are_checked():
+    # set_prior / set_geom_consistency validate shapes before copying the
+    # arrays (a smaller array would be read past its end)
+    from opensfm_amd import native
+    pd = native()
+    w, h = 12, 8
+    pd._check_prior_arrays(np.zeros((h, w, 4), np.float32), np.zeros((h, w), np.uint32), w, h)
+    for planes, masks in ((np.zeros((h, w, 3)), np.zeros((h, w))), (np.zeros((h - 1, w, 4)), np.zeros((h, w))),
+                          (np.zeros((h, w, 4)), np.zeros((h, w - 1))), (np.zeros(h * w * 4), np.zeros((h, w))),
+                          (np.zeros((h, w, 4)), np.zeros(h * w))):
+        with pytest.raises(ValueError):
+            pd._check_prior_arrays(planes, masks, w, h)
+    pd._check_depth_map(np.ones((5, 7), np.float32))
+    pd._check_depth_map(None)
+    for bad in (np.ones(35), np.ones((0, 7)), np.ones((5, 7, 1))):
+        with pytest.raises(ValueError):
+            pd._check_depth_map(bad)
 
 
 def test_null_handle_is_an_error(lib):

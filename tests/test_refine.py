@@ -19,6 +19,14 @@ def bumps():
     return synthetic.sinusoidal_bumps(4, 256, 192, 4)
 
 
+@pytest.fixture(scope="module")
+def bumps_full():
+    """The refinement workload's scene: 1024x768, 6 sources (seed 4)."""
+    from opensfm_amd import synthetic
+    print("rendering the 1024x768 sinusoidal-bump scene (6 sources) ...", flush=True)
+    return synthetic.sinusoidal_bumps(4, 1024, 768, 6)
+
+
 def test_f64_cost_vanishes_for_an_identical_view():
     # a source with the reference's camera and image warps every tap onto
     # itself (identity warp, det 1): NCC = 1 for any plane, cost 0 up to rounding
@@ -43,9 +51,11 @@ def _planes_to_theta(planes, K):
 
 
 @pytest.mark.gpu
-def test_adjoint_matches_fp64_finite_differences(pydense, bumps):
-    s = bumps
-    p = params_for(s, patch_size=5)
+@pytest.mark.parametrize("scene,patch", [("bumps", 5), ("bumps_full", 7)])
+def test_adjoint_matches_fp64_finite_differences(pydense, request, scene, patch):
+    s = request.getfixturevalue(scene)
+    print(f"adjoint check on {s.name}, patch {patch}", flush=True)
+    p = params_for(s, patch_size=patch)
     est = gpu_estimator(pydense, s, p)
     out = est.compute()
     depth, ab = _planes_to_theta(out["plane"], np.asarray(s.Ks[0]))
