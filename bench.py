@@ -79,9 +79,9 @@ def configure(est, scene, iters: int, storage: str = "f32"):
     from opensfm_amd import native
     for im, K, R, t in zip(scene.images, scene.Ks, scene.Rs, scene.ts):
         est.add_view(K, R, t, im)
-    est.set_depth_range(scene.depth_min, scene.depth_max, 0)
     est.set_params({**native().pipeline_params(), "hierarchy_levels": 1, "max_iterations": iters,
                     "patch_size": PATCH, "image_storage": storage, "speckle_min_size": 0, "gap_max_size": 0})
+    est.set_depth_range(scene.depth_min, scene.depth_max, 0)
 
 
 def storage_effect(pydense, scene, iters: int, depth_f16) -> dict:

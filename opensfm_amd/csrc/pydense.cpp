@@ -391,12 +391,16 @@ class Estimator {
     if (py::len(prev) > PM_MAX_SOURCES)
       throw std::invalid_argument("set_geom_consistency: at most " + std::to_string(PM_MAX_SOURCES) +
                                   " previous depth maps");
-    std::vector<py::array_t<float, py::array::c_style | py::array::forcecast>> maps;
-    for (auto item : prev) maps.push_back(depth_map_array(item));  // validate every map before any change
+    std::vector<F32Array> maps;
+    std::vector<bool> given;
+    for (auto item : prev) {  // validate every map before any change
+      given.push_back(!item.is_none());
+      maps.push_back(depth_map_array(item));
+    }
     check(pm_set_geom_consistency(h_, weight, nullptr, 0));
     check(pm_clear_previous_depths(h_));
     for (size_t i = 0; i < maps.size(); ++i) {
-      if (!maps[i]) continue;
+      if (!given[i]) continue;
       check(pm_set_previous_depth(h_, static_cast<int>(i), maps[i].data(), static_cast<int>(maps[i].shape(1)),
                                   static_cast<int>(maps[i].shape(0))));
     }

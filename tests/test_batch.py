@@ -224,3 +224,45 @@ def test_batch_levels_equal_single_multilevel_runs(pydense, scenes):
     with pytest.raises(RuntimeError, match="level"):
         b.set_level(3)
         b.run()
+
+
+@pytest.mark.gpu
+def test_batch_levels_run_in_order_once(pydense, scenes):
+    # a finer level starts from the level just below it: running level 1
+    # twice (the graph would replay an upsample of buffers replaced by the
+    # first run) or skipping a level is refused before any launch
+    refs = scenes[:2]
+    ests = []
+    for s in refs:
+        e = gpu_estimator(pydense, s, _params(s), storage="f32")
+        e.set_hierarchy_levels(3)
+        ests.append(e)
+    b = pydense.DepthmapBatch()
+    for e in ests:
+        b.add(e)
+    b.set_level(1)
+    with pytest.raises(RuntimeError, match="levels run in order"):
+        b.run()                      # level 0 has not run
+    b.set_level(0)
+    b.run()
+    b.set_level(1)
+    b.run()
+    with pytest.raises(RuntimeError, match="levels run in order"):
+        b.run()                      # level 1 again
+    b.set_level(2)
+    b.run()
+    assert ests[0].download(["depth"])["depth"].shape == refs[0].gt_depth.shape
+
+
+@pytest.mark.gpu
+def test_download_has_the_staged_level_size(pydense, scenes):
+    # after a coarse level only, the decoded maps have that level's size
+    s = scenes[0]
+    e = gpu_estimator(pydense, s, _params(s), storage="f32")
+    e.set_hierarchy_levels(2)
+    b = pydense.DepthmapBatch()
+    b.add(e)
+    b.run()
+    h, w = s.gt_depth.shape
+    d = e.download(["depth", "normal"])
+    assert d["depth"].shape == (h // 2, w // 2) and d["normal"].shape == (h // 2, w // 2, 3)

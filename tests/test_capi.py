@@ -205,6 +205,28 @@ def test_call_order_and_argument_errors(pydense, scene_tiny):
 
 
 @pytest.mark.gpu
+def test_prior_and_previous_depth_shapes_are_checked_on_the_estimator(pydense, scene_tiny):
+    s = scene_tiny
+    est = pydense.DepthmapEstimator()
+    for im, K, R, t in zip(s.images, s.Ks, s.Rs, s.ts):
+        est.add_view(K, R, t, im)
+    h, w = s.gt_depth.shape
+    with pytest.raises(ValueError, match="prior planes"):
+        est.set_prior(np.zeros((h - 1, w, 4), np.float32), np.zeros((h, w), np.uint32))
+    with pytest.raises(ValueError, match="prior masks"):
+        est.set_prior(np.zeros((h, w, 4), np.float32), np.zeros(h * w - 1, np.uint32))
+    est.set_prior(np.zeros((h, w, 4), np.float32), np.zeros((h, w), np.uint32))
+    with pytest.raises(ValueError, match="previous depth"):
+        est.set_geom_consistency(0.1, [np.ones(w * h, np.float32)])
+    # maps of any 2-D size are resized to the level (SetPreviousDepths)
+    est.set_geom_consistency(0.1, [np.ones((h // 2, w // 2), np.float32), None])
+    est.set_depth_range(s.depth_min, s.depth_max, 0)
+    est.set_patch_size(5)
+    depth, *_ = est.compute_patch_match()
+    assert depth.shape == (h, w)
+
+
+@pytest.mark.gpu
 def test_too_many_sources(pydense, scene_tiny):
     est = pydense.DepthmapEstimator()
     s = scene_tiny

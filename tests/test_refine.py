@@ -71,11 +71,11 @@ def test_adjoint_matches_fp64_finite_differences(pydense, request, scene, patch)
     cand = cand[(cand[:, 0] > 3) & (cand[:, 0] < h - 4) & (cand[:, 1] > 3) & (cand[:, 1] < w - 4)]
     pick = cand[rng.choice(len(cand), 4096, replace=False)]
     hstep = 1e-4
-    ok_cost, ok_grad, n = 0, 0, 0
+    ok_cost, ok_grad, ok_central, n = 0, 0, 0, 0
     for y, x in pick:
         th = [float(depth[y, x]), float(ab[y, x, 0]), float(ab[y, x, 1])]
         e0, u0 = po.ncc_energy_f64(sc, p, x, y, *th, int(views[y, x]))
-        fd = []
+        fd, fd_side = [], []
         stable = u0 > 0
         for j in range(3):
             tp, tm = list(th), list(th)
@@ -85,18 +85,32 @@ def test_adjoint_matches_fp64_finite_differences(pydense, request, scene, patch)
             em, um = po.ncc_energy_f64(sc, p, x, y, *tm, int(views[y, x]))
             stable &= up == u0 and um == u0
             fd.append((ep - em) / (2 * hstep))
+            fd_side.append(((ep - e0) / hstep, (e0 - em) / hstep))
         if not stable:
             continue
         n += 1
         ok_cost += abs(cost[y, x] - e0) <= 1e-4
         g = grad[y, x].astype(np.float64)
         fd = np.asarray(fd)
-        ok_grad += np.all(np.abs(g - fd) <= 2e-3 + 2e-2 * np.abs(fd))
-    print(f"adjoint check: {n} pixels, cost within 1e-4 {ok_cost / n:.4f}, gradient within tolerance "
-          f"{ok_grad / n:.4f}", flush=True)
+
+        def close(ref):
+            return np.abs(g - ref) <= 2e-3 + 2e-2 * np.abs(ref)
+
+        central = close(fd)
+        ok_central += np.all(central)
+        # Bilinear sampling is piecewise linear: where a tap crosses a texel
+        # edge within +-h the cost has a kink, the central difference averages
+        # the two one-sided slopes, and the derivative at theta is the slope
+        # of the side theta lies on.  So each component may match the central
+        # difference or one of the one-sided differences.
+        ok_grad += all(central[j] or close(fd_side[j][0])[j] or close(fd_side[j][1])[j] for j in range(3))
+    print(f"adjoint check: {n} pixels, cost within 1e-4 {ok_cost / n:.4f}, gradient within tolerance of the "
+          f"central difference {ok_central / n:.4f}, of the central or a one-sided difference {ok_grad / n:.4f}",
+          flush=True)
     assert n > 3000
     assert ok_cost / n >= 0.99
-    assert ok_grad / n >= 0.98
+    assert ok_central / n >= 0.95
+    assert ok_grad / n >= 0.99
 
 
 @pytest.mark.gpu
