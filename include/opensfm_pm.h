@@ -209,6 +209,18 @@ int pm_write_state(pm_handle* h, const float* planes, const float* costs, const 
 int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs,
                     float* init_cost, uint32_t* sel);
 
+/* pm_score_planes without any device allocation: the pairs and results are
+ * staged in a fixed device array of the library (6 MiB per GPU, held until
+ * the process ends) instead of buffers allocated per call.  No counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  n <= PM_NOALLOC_MAX_PAIRS (else
+ * PM_E_INVALID).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (from several threads or handles) race.  Same arguments and
+ * results as pm_score_planes. */
+#define PM_NOALLOC_MAX_PAIRS 65536
+int pm_score_planes_noalloc(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs,
+                            float* init_cost, uint32_t* sel);
+
 /* Differentiable photometric cost of per-pixel planes (no counterpart in the
  * reference; the refinement after PatchMatch).  Per reference pixel: depth
  * d, normal slopes (a, b) of the camera-facing normal (a, b, -1), and a

@@ -1551,6 +1551,94 @@ int pm_score_planes(pm_handle* h, int n, const int32_t* xy, const float* planes,
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// Allocation-free scoring.  pm_score_planes stages its (pixel, plane) pairs
+// and results in device buffers it allocates per call; callers that cannot
+// allocate device memory (code under HIP graph capture, tight serving loops)
+// use pm_score_planes_noalloc instead, which stages them in this file-scope
+// array.  Each device holds its own copy at its own address, taken once per
+// device with hipGetSymbolAddress (the host-side name is not a device
+// address).  Every call on a device shares it: calls that overlap on one
+// device race.  Nothing in it needs clearing: the score kernel writes every
+// output of the pairs it is given.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocPlanes = 0;                                                    // float4 per pair
+constexpr size_t kNoAllocXy = kNoAllocPlanes + 16 * static_cast<size_t>(PM_NOALLOC_MAX_PAIRS);  // 2 int32
+constexpr size_t kNoAllocCosts = kNoAllocXy + 8 * static_cast<size_t>(PM_NOALLOC_MAX_PAIRS);    // n_src floats
+constexpr size_t kNoAllocInit = kNoAllocCosts + 4 * static_cast<size_t>(PM_NOALLOC_MAX_PAIRS) * PM_MAX_SOURCES;
+constexpr size_t kNoAllocSel = kNoAllocInit + 4 * static_cast<size_t>(PM_NOALLOC_MAX_PAIRS);
+constexpr size_t kNoAllocBytes = kNoAllocSel + 4 * static_cast<size_t>(PM_NOALLOC_MAX_PAIRS);  // 6 MiB
+static_assert(kNoAllocBytes == 96u * PM_NOALLOC_MAX_PAIRS, "scratch layout");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device, taken on the first
+// allocation-free call there (with that device current).
+std::mutex g_scratch_mutex;
+std::vector<unsigned char*> g_scratch_ptr;
+
+int scratch_of(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0) return fail(PM_E_INVALID, "scratch: bad device");
+  if (g_scratch_ptr.size() <= static_cast<size_t>(device)) g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+  unsigned char*& ptr = g_scratch_ptr[static_cast<size_t>(device)];
+  if (!ptr) {
+    void* p = nullptr;
+    PM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(PM_E_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+    ptr = static_cast<unsigned char*>(p);
+  }
+  *out = ptr;
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_score_planes_noalloc(pm_handle* h, int n, const int32_t* xy, const float* planes, float* costs,
+                            float* init_cost, uint32_t* sel) {
+  if (int r = check_handle(h)) return r;
+  if (!h->uploaded) return fail(PM_E_STATE, "pm_score_planes_noalloc: views are not uploaded (call pm_upload)");
+  if (n < 0 || (n > 0 && (!xy || !planes || !costs)))
+    return fail(PM_E_INVALID, "pm_score_planes_noalloc: bad arguments");
+  if (n > PM_NOALLOC_MAX_PAIRS)
+    return fail(PM_E_INVALID, "pm_score_planes_noalloc: at most " + std::to_string(PM_NOALLOC_MAX_PAIRS) +
+                                  " (pixel, plane) pairs per call (asked for " + std::to_string(n) + ")");
+  if (n == 0) return PM_OK;
+  for (int i = 0; i < n; ++i)
+    if (xy[2 * i] < 0 || xy[2 * i] >= h->W || xy[2 * i + 1] < 0 || xy[2 * i + 1] >= h->H)
+      return fail(PM_E_INVALID, "pm_score_planes_noalloc: pixel outside the reference view");
+  PM_HIP(hipSetDevice(h->device), "hipSetDevice");
+  unsigned char* base = nullptr;
+  if (int r = scratch_of(h->device, &base)) return r;
+  if (int r = ensure_refpatch(h)) return r;  // a launch into the handle's buffers, no allocation
+  auto* dpl = reinterpret_cast<float4*>(base + kNoAllocPlanes);
+  auto* dxy = reinterpret_cast<int32_t*>(base + kNoAllocXy);
+  auto* dcost = reinterpret_cast<float*>(base + kNoAllocCosts);
+  auto* dinit = reinterpret_cast<float*>(base + kNoAllocInit);
+  auto* dsel = reinterpret_cast<uint32_t*>(base + kNoAllocSel);
+  const size_t un = static_cast<size_t>(n);
+  PM_HIP(hipMemcpyAsync(dxy, xy, 8 * un, hipMemcpyHostToDevice, h->stream), "pm_score_planes_noalloc: upload");
+  PM_HIP(hipMemcpyAsync(dpl, planes, 16 * un, hipMemcpyHostToDevice, h->stream), "pm_score_planes_noalloc: upload");
+  PmArgs a = make_args(h);
+  PM_HIP(launch_score(a, h->params.patch_size / 2, h->f16, n, dxy, dpl, dcost, dinit, dsel, h->stream),
+         "pm_score_planes_noalloc: score kernel");
+  PM_HIP(hipMemcpyAsync(costs, dcost, 4 * un * h->n_src, hipMemcpyDeviceToHost, h->stream),
+         "pm_score_planes_noalloc: download");
+  if (init_cost)
+    PM_HIP(hipMemcpyAsync(init_cost, dinit, 4 * un, hipMemcpyDeviceToHost, h->stream),
+           "pm_score_planes_noalloc: download");
+  if (sel)
+    PM_HIP(hipMemcpyAsync(sel, dsel, 4 * un, hipMemcpyDeviceToHost, h->stream), "pm_score_planes_noalloc: download");
+  PM_HIP(hipStreamSynchronize(h->stream), "pm_score_planes_noalloc");
+  return PM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // Colour-tile widths of a single estimator.
 // ---------------------------------------------------------------------------
 extern "C" {
