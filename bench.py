@@ -1,19 +1,28 @@
 """Benchmark of the PatchMatch depth path on one MI355X.
 
-Workload (one "step"): plain single-view PatchMatch on a 1600x1200 float32
-reference with 8 neighbours (seeded 64-facet piecewise-planar heightfield,
-3x3 camera grid, depth range [4, 40] m, patch 7x7, 3 iterations):
-random init + 3 x (red pass + black pass) + decode, all on the device with
-the views already resident in HBM.  Metric: Mpix*views*iters/s =
-(W*H/1e6) * N_src * iterations / seconds.
+Default workload: the aerial survey -- 200 nadir views of 4000x3000 over a
+seeded fractal terrain, every view a reference with its 10 nearest views as
+sources, float16 image storage, patch 7x7, 3 iterations, then 10
+Gauss-Newton steps of the differentiable cost per reference.  The views are
+rendered on the GPU and stay in HBM.  References run in groups admitted by
+device memory; one "step" is 10 references, taken in order (--steps 20 is
+the whole survey): building their estimators, staging their views on the
+device (the reference's UploadData: padding, conversion, quads, reference
+taps), the batched PatchMatch (init, iterations, decode), Gauss-Newton and
+the depth download, in groups of as many references as device memory holds.
+Metric: Mpix*views*iters/s = sum over references of (W*H/1e6) * N_src *
+iterations / seconds.
 
-Also reported: the roofline of the candidate-scoring kernel
-(candidates_kernel, fixed algorithmic work per launch, timed with HIP events
-on the estimator's stream inside the timed steps) and of the whole colour
-pass against the FP32 vector peak, per-stage device time, accuracy against
-the synthetic ground truth, and the CPU oracle timed on the host cores on a
-bounded sample of the same scene.  Each est.run() returns only after its HIP stream is idle
-(hipStreamSynchronize), so the wall clock brackets complete GPU work.
+Also reported in the same line: the roofline of the candidate-scoring kernel
+(fixed algorithmic work per launch; HIP events around every launch on the
+batch stream, over the timed steps), the whole PatchMatch against the FP32
+vector peak, per-stage device time, the rate from host buffers (PCIe
+included, never the headline), accuracy against the synthetic ground truth,
+the CPU oracle on a bounded band of reference 0 (host cores) and the GPU's
+parity with it, and the plain single-view float32 path (1600x1200, 8
+neighbours; upload timed per step) with its own CPU baseline.  --workload
+c2/c3/c4/cluster run the other workloads alone.  Every run call returns
+after its HIP stream is idle, so the wall clock brackets complete GPU work.
 """
 from __future__ import annotations
 
@@ -72,12 +81,14 @@ def make_scene(seed: int, workload: str = "c2"):
     return synthetic.facet_heightfield(seed=seed, width=1600, height=1200, num_src=8)
 
 
-def configure(est, scene, iters: int, storage: str = "f32"):
+def configure(est, scene, iters: int, storage: str = "f32", images=None):
     """The workload's estimator: the dense pipeline's parameters at one
     pyramid level (the workloads are single-level), patch 7x7, the scene's
-    depth range, no post-processing."""
+    depth range, no post-processing; views from host memory (``images``
+    overrides the scene's; [] adds none)."""
     from opensfm_amd import native
-    for im, K, R, t in zip(scene.images, scene.Ks, scene.Rs, scene.ts):
+    imgs = scene.images if images is None else images
+    for im, K, R, t in zip(imgs, scene.Ks, scene.Rs, scene.ts):
         est.add_view(K, R, t, im)
     est.set_params({**native().pipeline_params(), "hierarchy_levels": 1, "max_iterations": iters,
                     "patch_size": PATCH, "image_storage": storage, "speckle_min_size": 0, "gap_max_size": 0})
@@ -318,9 +329,78 @@ def run_cluster(args, pydense) -> int:
     return 0
 
 
+def _device_views(est, images, Ks, Rs, ts, storage: str):
+    """Adds device-resident views (torch tensors, (H, W) contiguous) to ``est``."""
+    for im, K, R, t in zip(images, Ks, Rs, ts):
+        dt = {2: "f16", 4: "f32", 1: "u8"}[im.element_size()]
+        est.add_view_device(K, R, t, im.data_ptr(), dt, im.shape[1], im.shape[0], 0)
+
+
+def plain_leg(pydense, steps: int, warmup: int, cpu_rows: int, seed: int = 2, iters: int = 3) -> dict:
+    """The plain single-view float32 path on the 1600x1200 facet heightfield
+    with 8 neighbours: views resident in HBM, each step = upload (padding,
+    quads, reference taps) + init + iterations + decode; CPU oracle on a band."""
+    import torch
+    log("plain single-view leg: rendering the 1600x1200 facet heightfield ...")
+    scene = make_scene(seed, "c2")
+    dev = [torch.from_numpy(np.asarray(im, np.float32)).cuda() for im in scene.images]
+    torch.cuda.synchronize()
+    h, w = scene.gt_depth.shape
+    units = w * h / 1e6 * scene.num_src * iters
+    est = pydense.DepthmapEstimator()
+    configure(est, scene, iters, "f32", images=[])
+    _device_views(est, dev, scene.Ks, scene.Rs, scene.ts, "f32")
+    for _ in range(max(warmup, 1)):
+        est.upload()
+        est.run()
+    t0 = time.perf_counter()
+    cand = [0.0, 0]
+    for i in range(steps):
+        est.upload()
+        est.run()
+        ms, n = est.stage_timing()["candidates"]
+        cand[0] += ms
+        cand[1] += n
+    elapsed = time.perf_counter() - t0
+    value = units * steps / elapsed
+    d = est.download(["depth"])["depth"]
+    gt = scene.gt_depth
+    out = {"workload": "plain PatchMatch, one 1600x1200 f32 reference + 8 neighbours, 64-facet heightfield "
+                       f"(seed {seed}), patch 7x7, {iters} iterations; per step upload (views resident in HBM) "
+                       "+ init + iterations + decode",
+           "value": round(value, 3), "ms_per_step": round(elapsed / steps * 1e3, 3), "steps": steps,
+           "units_per_step": units,
+           "candidates_avg_launch_ms": round(cand[0] / max(cand[1], 1), 4),
+           "accuracy": {"gt_within_1pct": round(float(np.mean(np.abs(d - gt) < 0.01 * gt)), 4)}}
+    if cpu_rows > 0:
+        log(f"plain leg: CPU oracle on {cpu_rows} rows ...")
+        keep = {}
+        cpu = cpu_baseline(scene, iters, cpu_rows, keep)
+        cpu["value"] = round(cpu["value"], 5)
+        band = pydense.DepthmapEstimator()
+        configure(band, keep["band"], iters, "f32")
+        d_band = band.compute()["depth"]
+        m = keep["depth"] > 0
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        out["parity"] = {"sample": "the CPU-baseline rows on the GPU", "bit_equal": round(float(np.mean(
+            d_band == keep["depth"])), 6), "within_1pct_of_oracle": round(float(np.mean(
+                np.abs(d_band[m] - keep["depth"][m]) < 0.01 * keep["depth"][m])), 6)}
+    del dev
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_c5(args, pydense) -> int:
     """Nadir survey: every view a reference, in groups admitted by device
-    memory, each group one batched run followed by Gauss-Newton refinement."""
+    memory, each group one batched run followed by Gauss-Newton refinement.
+
+    The 200 views are rendered on the GPU and stay there (float16 tensors);
+    a step is a fixed number of references (10), in order, so 20 steps run
+    the whole 200-view survey; they run in groups admitted by device memory
+    (a group's references are built, their views staged on the device --
+    padding, quads, reference taps: the reference's UploadData --, run as
+    one batch, refined by Gauss-Newton, and their depths downloaded)."""
     import torch
     from opensfm_amd import synthetic
     from opensfm_amd.admission import default_budget, run_admitted
@@ -328,109 +408,186 @@ def run_c5(args, pydense) -> int:
     seed = 5 if args.seed == 2 else args.seed
     storage = args.storage if args.storage_set else "f16"
     views = args.views or 200
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
-    log(f"rendering aerial survey ({views} views of {args.width}x{args.height}) on {dev} ...")
+    w, h = args.width, args.height
+    log(f"rendering aerial survey ({views} views of {w}x{h}) on the GPU, kept in HBM ...")
     t0 = time.perf_counter()
-    ws = synthetic.aerial_survey(seed=seed, num_views=views, width=args.width, height=args.height,
-                                 neighbours=args.neighbours, device=dev)
+    ws = synthetic.aerial_survey(seed=seed, num_views=views, width=w, height=h, neighbours=args.neighbours,
+                                 device="cuda", images_on_device=True)
+    if storage == "f32":
+        ws.images = [im.float().contiguous() for im in ws.images]
+    torch.cuda.synchronize()
     torch.cuda.empty_cache()
     log(f"rendered in {time.perf_counter() - t0:.1f} s")
-    units = ws.units(args.iters)
-    budget = int(args.hbm_budget_gb * 1e9) if args.hbm_budget_gb > 0 else default_budget(pydense)
 
-    def build(i):
+    def build_at(i):
+        v, srcs, dmin, dmax = ws.refs[i % views]
+        order = [v] + list(srcs)
         est = pydense.DepthmapEstimator()
-        configure(est, ws.reference(i), args.iters, storage)
+        est.set_params({**pydense.pipeline_params(), "hierarchy_levels": 1, "max_iterations": args.iters,
+                        "patch_size": PATCH, "image_storage": storage, "speckle_min_size": 0, "gap_max_size": 0})
+        est.set_depth_range(dmin, dmax, 0)
+        _device_views(est, [ws.images[j] for j in order], [ws.Ks[j] for j in order], [ws.Rs[j] for j in order],
+                      [ws.ts[j] for j in order], storage)
         return est
+
+    budget = int(args.hbm_budget_gb * 1e9) if args.hbm_budget_gb > 0 else default_budget(pydense)
+    ref_bytes = build_at(0).device_bytes()
+    group = args.refs_per_step
+    per_ref_units = w * h / 1e6 * args.neighbours * args.iters
+    log(f"device budget {budget / 1e9:.1f} GB; one reference needs {ref_bytes / 1e9:.2f} GB; a step is "
+        f"{group} references")
 
     def keep_depth(i, est):
         return est.download(["depth"])["depth"]
 
     def group_log(g):
-        log(f"group of {len(g.refs)} (refs {g.refs[0]}..{g.refs[-1]}, {g.device_bytes / 1e9:.1f} GB): build "
-            f"{g.build_s:.1f} s (waited {g.wait_s:.1f} s), prepare {g.prepare_s:.1f} s, PatchMatch {g.run_ms:.0f} ms, Gauss-Newton "
+        log(f"group of {len(g.refs)} (refs {g.refs[0]}..{g.refs[-1]}): build {g.build_s:.1f} s (waited "
+            f"{g.wait_s:.1f} s), staging {g.prepare_s:.1f} s, PatchMatch {g.run_ms:.0f} ms, Gauss-Newton "
             f"{g.refine_ms:.0f} ms, results {g.results_s:.1f} s, release {g.release_s:.1f} s, wall {g.wall_s:.1f} s")
 
-    log(f"device budget {budget / 1e9:.1f} GB; one reference needs {build(0).device_bytes() / 1e9:.2f} GB")
-    for i in range(args.warmup):        # first use: autotuner, code objects
-        run_admitted(pydense, min(2, views), build, budget=budget, refine_steps=args.refine_steps,
-                     on_result=lambda i, e: None, overlap_staging=not args.no_overlap)
+    cfg = {"use_graph": False}     # per-launch HIP events on the batch stream (the roofline below)
+    for i in range(args.warmup):  # first use: autotuner, code objects
+        run_admitted(pydense, group, build_at, budget=budget, refine_steps=args.refine_steps,
+                     on_result=lambda i, e: None, overlap_staging=not args.no_overlap, batch_config=cfg)
         log(f"warmup {i + 1}/{args.warmup} done")
-    depths = None
-    groups = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        depths, groups = run_admitted(pydense, views, build, budget=budget, refine_steps=args.refine_steps,
-                                      on_result=keep_depth, on_group=group_log,
-                                      overlap_staging=not args.no_overlap)
-        log(f"step {k + 1}/{args.steps}: {len(groups)} groups")
+    depths, groups = run_admitted(pydense, group * args.steps, build_at, budget=budget,
+                                  refine_steps=args.refine_steps, on_result=keep_depth, on_group=group_log,
+                                  overlap_staging=not args.no_overlap, batch_config=cfg)
+    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    value = units * args.steps / elapsed
+    nrefs = group * args.steps
+    units = per_ref_units * nrefs
+    value = units / elapsed
+    if len(groups) != args.steps:
+        log(f"{len(groups)} admitted groups ran the {args.steps} steps")
     pm_ms = sum(g.run_ms for g in groups)
     gn_ms = sum(g.refine_ms for g in groups)
+    stages = {}
+    for g in groups:
+        for k, (ms, n) in g.stage_ms.items():
+            acc = stages.setdefault(k, [0.0, 0])
+            acc[0] += ms
+            acc[1] += n
 
     hit = hit5 = tot = 0
     for i, d in depths.items():
-        gt = ws.gt_depth[ws.refs[i][0]]
+        gt = ws.gt_depth[ws.refs[i % views][0]]
         m = gt > 0
         hit += int(np.sum(np.abs(d[m] - gt[m]) < 0.01 * gt[m]))
         hit5 += int(np.sum(np.abs(d[m] - gt[m]) < 0.05 * gt[m]))
         tot += int(m.sum())
     del depths
 
+    # Roofline of the candidate-scoring kernel (fixed algorithmic work per
+    # launch): one batched launch scores 9 hypotheses of every pixel of one
+    # colour of every reference of the group, 21 FLOP per tap and 75 per
+    # (hypothesis, view).  Launch times: HIP events on the batch stream
+    # around every launch, over the timed steps.
+    # A launch covers one colour of every reference of an admitted group, so
+    # the work per launch is the timed steps' total over the launch count.
+    cand_ms, cand_n = stages.get("candidates", (0.0, 0))
+    total_flop = nrefs * (w * h / 2.0) * 9 * args.neighbours * (21 * PATCH * PATCH + 75) * 2 * args.iters
+    flop_per_launch = total_flop / max(cand_n, 1)
+    avg_launch_s = cand_ms / max(cand_n, 1) / 1e3
+    achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    sweep_ms = sum(v[0] for k, v in stages.items() if k != "init")
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel": f"candidates_batch<3,{'_Float16' if storage == 'f16' else 'float'},*>",
+                "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
+    path_tflops = units * 1e6 * FLOP_PER_UNIT / (pm_ms / 1e3) / 1e12 if pm_ms > 0 else 0.0
+
+    # the same job from host buffers (PCIe and host staging included): one group
+    log("one group from host buffers ...")
+    hviews = {}
+
+    def build_host(i):
+        v, srcs, dmin, dmax = ws.refs[i % views]
+        est = pydense.DepthmapEstimator()
+        est.set_params({**pydense.pipeline_params(), "hierarchy_levels": 1, "max_iterations": args.iters,
+                        "patch_size": PATCH, "image_storage": storage, "speckle_min_size": 0, "gap_max_size": 0})
+        est.set_depth_range(dmin, dmax, 0)
+        for j in [v] + list(srcs):
+            if j not in hviews:
+                hviews[j] = ws.images[j].cpu().numpy()
+            est.add_view(ws.Ks[j], ws.Rs[j], ws.ts[j], hviews[j])
+        return est
+
+    t1 = time.perf_counter()
+    run_admitted(pydense, group, build_host, budget=budget, refine_steps=args.refine_steps, on_result=keep_depth,
+                 overlap_staging=False)
+    pcie_value = per_ref_units * group / (time.perf_counter() - t1)
+    hviews.clear()
+
     cpu, parity = None, None
     if args.cpu_rows > 0:
         log(f"CPU oracle on {args.cpu_rows} rows of reference 0 ...")
+        v, srcs, dmin, dmax = ws.refs[0]
+        order = [v] + list(srcs)
+        sc0 = synthetic.Scene([ws.images[j].float().cpu().numpy() for j in order], [ws.Ks[j] for j in order],
+                              [ws.Rs[j] for j in order], [ws.ts[j] for j in order], ws.gt_depth[v], dmin, dmax,
+                              "aerial reference 0")
         keep = {}
-        sc0 = ws.reference(0)
         cpu = cpu_baseline(sc0, args.iters, args.cpu_rows, keep)
         cpu["value"] = round(cpu["value"], 5)
         est = pydense.DepthmapEstimator()
         configure(est, keep["band"], args.iters, storage)
         d = est.compute()["depth"]
         m = keep["depth"] > 0
-        parity = {"sample": "the same rows of reference 0 on the GPU (no refinement)",
+        parity = {"sample": "the CPU-baseline rows of reference 0 on the GPU (no refinement)",
                   "bit_equal": round(float(np.mean(d == keep["depth"])), 5),
                   "within_1pct_of_oracle": round(float(np.mean(np.abs(d[m] - keep["depth"][m])
                                                                < 0.01 * keep["depth"][m])), 5)}
+    del ws
+    torch.cuda.empty_cache()
 
-    pm_tflops = units * 1e6 * FLOP_PER_UNIT / (pm_ms / 1e3) / 1e12
-    w, h = args.width, args.height
+    plain = None
+    if not args.no_plain_leg:
+        plain = plain_leg(pydense, steps=5, warmup=1, cpu_rows=192 if args.cpu_rows > 0 else 0)
+
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32" if storage == "f32" else "f16-storage/f32",
-        "data": "synthetic",
+        "data": "synthetic (rendered on the GPU, resident in HBM before the timed region)",
         "config": {"workload": WORKLOADS["c5"][1].format(views=views, width=w, height=h, storage=storage, seed=seed,
                                                          neighbours=args.neighbours, iters=args.iters,
                                                          refine=args.refine_steps),
-                   "references": views, "megapixels": round(views * w * h / 1e6, 3),
-                   "iterations": args.iters, "patch": PATCH, "units_per_step": round(units, 3),
+                   "references": nrefs, "references_per_step": group, "admitted_groups": len(groups),
+                   "megapixels": round(nrefs * w * h / 1e6, 3),
+                   "iterations": args.iters, "patch": PATCH, "units_per_step": round(per_ref_units * group, 3),
                    "staging": "serial" if args.no_overlap else "overlapped with the previous group",
-                   "value_scope": "whole job: building, staging, batched PatchMatch, Gauss-Newton and depth "
-                                  "download of every group"},
+                   "value_scope": "per step: building the group's estimators, staging its device-resident views "
+                                  "(padding, f16 conversion, quads, reference taps), batched PatchMatch (init, "
+                                  "iterations, decode), Gauss-Newton, depth download"},
+        "roofline": roofline,
         "roofline_path": {"scope": "batched PatchMatch device time of all groups (init, colour passes, decode)",
-                          "achieved": round(pm_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(pm_tflops / PEAK_FP32_TFLOPS, 4), "device_ms": round(pm_ms, 3)},
-        "patchmatch_device_value": round(units / (pm_ms / 1e3), 3),
-        "refine": {"steps": args.refine_steps, "ms": round(gn_ms, 3),
-                   "value": round(views * w * h / 1e6 * args.neighbours * args.refine_steps / (gn_ms / 1e3), 3)
-                   if gn_ms > 0 else None, "unit": "Mpx·views·steps/s"},
-        "admission": {"budget_bytes": budget, "groups": len(groups),
-                      "per_group": [{"refs": len(g.refs), "device_gb": round(g.device_bytes / 1e9, 2),
-                                     "build_s": round(g.build_s, 2), "wait_s": round(g.wait_s, 2),
-                                     "prepare_s": round(g.prepare_s, 2),
-                                     "run_ms": round(g.run_ms, 1), "refine_ms": round(g.refine_ms, 1),
-                                     "results_s": round(g.results_s, 2), "release_s": round(g.release_s, 2),
-                                     "wall_s": round(g.wall_s, 2)}
-                                    for g in groups]},
+                          "achieved": round(path_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(path_tflops / PEAK_FP32_TFLOPS, 4), "device_ms": round(pm_ms, 3),
+                          "colour_pass_ms": round(sweep_ms, 3)},
         "cpu_baseline": cpu,
         "parity": parity,
+        "patchmatch_device_value": round(units / (pm_ms / 1e3), 3) if pm_ms > 0 else None,
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
+        "refine": {"steps": args.refine_steps, "ms": round(gn_ms, 3),
+                   "value": round(nrefs * w * h / 1e6 * args.neighbours * args.refine_steps / (gn_ms / 1e3), 3)
+                   if gn_ms > 0 else None, "unit": "Mpx·views·steps/s"},
+        "pcie_inclusive_value": round(pcie_value, 3),
+        "admission": {"budget_bytes": budget, "groups": len(groups), "reference_bytes": ref_bytes,
+                      "per_group_wall_s": [round(g.wall_s, 2) for g in groups]},
         "accuracy": {"scope": "after refinement, all references", "gt_within_1pct": round(hit / max(tot, 1), 4),
                      "gt_within_5pct": round(hit5 / max(tot, 1), 4)},
     }
     if cpu and cpu["value"] > 0:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if plain is not None:
+        line["plain_single_view"] = plain
+        if cpu and plain.get("gpu_over_cpu"):
+            line["speedup_ratio_vs_plain"] = {
+                "scope": "(this workload's GPU/CPU) / (plain single-view f32 GPU/CPU)",
+                "value": round(line["gpu_over_cpu"] / plain["gpu_over_cpu"], 3)}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -457,18 +614,23 @@ def copy_probe(mib: int) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; 1 for c5)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: c5 6 groups, 20 = the whole survey; cluster 1; others 5)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--storage", choices=["f32", "f16"], default=None,
                     help="image storage (default f32; f16 for the c3 workload)")
-    ap.add_argument("--cpu-rows", type=int, default=384,
-                    help="reference rows in the CPU-oracle sample, about 10 s on 16 host threads (0 = skip)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
-                    help="c2: the headline single-view workload; c3: many references in one batched launch; "
-                         "c4: PatchMatch + Gauss-Newton refinement; c5: aerial survey in memory-admitted groups; "
+    ap.add_argument("--cpu-rows", type=int, default=None,
+                    help="reference rows in the CPU-oracle sample (default 128 for c5, else 384; about 10 s on "
+                         "16 host threads; 0 = skip)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5",
+                    help="c5 (default): the aerial survey, batched f16 PatchMatch + Gauss-Newton in memory-admitted "
+                         "groups, with the plain single-view leg; c2: the plain single-view workload alone; c3: "
+                         "many references in one batched launch; c4: PatchMatch + Gauss-Newton refinement; "
                          "cluster: the cluster surface with pyramid levels and geometric links")
+    ap.add_argument("--no-plain-leg", action="store_true", help="c5: skip the plain single-view leg")
+    ap.add_argument("--refs-per-step", type=int, default=10, help="c5: references per timed step")
     ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 and c5 workloads")
     ap.add_argument("--views", type=int, default=None, help="references of the c3 (64) and c5 (200) workloads")
     ap.add_argument("--levels", type=int, default=3, help="pyramid levels of the cluster workload")
@@ -484,7 +646,9 @@ def main() -> int:
     args = ap.parse_args()
     args.storage_set = args.storage is not None
     if args.steps is None:
-        args.steps = 1 if args.workload in ("c5", "cluster") else 5
+        args.steps = {"c5": 6, "cluster": 1}.get(args.workload, 5)
+    if args.cpu_rows is None:
+        args.cpu_rows = 128 if args.workload == "c5" else 384
     if args.storage is None:
         args.storage = "f32"
     if args.gpus != 1:
@@ -514,16 +678,23 @@ def main() -> int:
     n_src = scene.num_src
     units_per_step = w * h / 1e6 * n_src * args.iters
 
+    # views resident in HBM; each step stages them (padding, conversion,
+    # quads, reference taps) and runs init + iterations + decode
+    import torch
+    dev_views = [torch.from_numpy(np.asarray(im, np.float32)).cuda() for im in scene.images]
+    torch.cuda.synchronize()
     est = pydense.DepthmapEstimator()
-    configure(est, scene, args.iters, args.storage)
-    est.upload()
+    configure(est, scene, args.iters, args.storage, images=[])
+    _device_views(est, dev_views, scene.Ks, scene.Rs, scene.ts, args.storage)
     for i in range(args.warmup):
+        est.upload()
         est.run()
         log(f"warmup {i + 1}/{args.warmup}: {est.last_timing()['run_ms']:.1f} ms")
 
     stages = {}
     t0 = time.perf_counter()
     for i in range(args.steps):
+        est.upload()
         est.run()
         tm = est.last_timing()
         for k, (ms, n) in est.stage_timing().items():
@@ -538,10 +709,14 @@ def main() -> int:
         copy_probe(args.copy_probe_mb)
 
     out = est.download()
-    # Host-buffer rate: padding/conversion + upload + run + download.
+    # Host-buffer rate: host padding/conversion + PCIe upload + run + download.
+    host_est = pydense.DepthmapEstimator()
+    configure(host_est, scene, args.iters, args.storage)
+    host_est.compute_patch_match()
     t1 = time.perf_counter()
-    est.compute_patch_match()
+    host_est.compute_patch_match()
     pcie_value = units_per_step / (time.perf_counter() - t1)
+    del host_est
     d = out["depth"]
     gt = scene.gt_depth
     acc1 = float(np.mean(np.abs(d - gt) < 0.01 * gt))

@@ -131,6 +131,14 @@ int pm_destroy(pm_handle* h);
 int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t,
                 const void* image, int format, int width, int height, int64_t row_stride,
                 const uint8_t* mask);
+/* pm_add_view for an image already in device memory of the handle's device
+ * (row_stride in bytes, 0 = packed): AddView (estimator.cc:33-42) for
+ * callers whose views are resident in HBM.  The image is not copied: it is
+ * read at the next pm_upload (or batch staging), which pads and converts it
+ * on the device exactly as the host path does, so it must stay valid and
+ * unchanged until then.  Such views run at one pyramid level. */
+int pm_add_view_device(pm_handle* h, const double* K, const double* R, const double* t,
+                       const void* device_image, int format, int width, int height, int64_t row_stride);
 int pm_clear_views(pm_handle* h);
 int pm_num_views(pm_handle* h, int* n);
 

@@ -45,6 +45,7 @@ class GroupStats:
     results_s: float = 0.0     # on_result calls (downloads)
     release_s: float = 0.0     # destroying the batch, releasing the device memory
     wall_s: float = 0.0        # the whole group, end to end
+    stage_ms: Dict[str, List[float]] = dataclasses.field(default_factory=dict)  # {stage: [ms, launches]}, eager runs
 
 
 def admit(sizes: Sequence[int], budget: int) -> List[List[int]]:
@@ -168,10 +169,18 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                 # coarse to fine: a batch starts at level 0 of every reference's
                 # pyramid; each finer level starts from the level below it
                 # (the references of a batch share their number of levels)
-                st.run_ms = batch.run()
+                def run_level():
+                    ms = batch.run()
+                    for k, (sm, n) in batch.stage_timing().items():   # zero after a graph run
+                        acc = st.stage_ms.setdefault(k, [0.0, 0])
+                        acc[0] += sm
+                        acc[1] += n
+                    return ms
+
+                st.run_ms = run_level()
                 for level in range(1, int(ests[0].params()["hierarchy_levels"])):
                     batch.set_level(level)
-                    st.run_ms += batch.run()
+                    st.run_ms += run_level()
                 for e in ests:
                     if refine_steps > 0:
                         e.gauss_newton(refine_steps, damping)

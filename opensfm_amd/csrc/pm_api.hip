@@ -94,6 +94,11 @@ struct HostView {
   int w, h;
   std::vector<float> pix;    // [0,1] values as given (u8 already /255)
   std::vector<uint8_t> raw;  // the u8 input, kept for the image pyramid
+  // A view added from device memory (pm_add_view_device): read from there
+  // at upload, pix and raw stay empty.
+  const void* dev = nullptr;
+  int dev_format = 0;
+  int64_t dev_stride = 0;
 };
 
 // Device blocks released by idle handles, kept for reuse by size
@@ -577,6 +582,39 @@ int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t,
   return PM_OK;
 }
 
+int pm_add_view_device(pm_handle* h, const double* K, const double* R, const double* t, const void* image,
+                       int format, int width, int height, int64_t row_stride) {
+  if (int r = check_handle(h)) return r;
+  ++h->gen;
+  if (!K || !R || !t || !image) return fail(PM_E_INVALID, "add_view_device: null argument");
+  if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "add_view_device: empty image");
+  const int esize = format == PM_U8 ? 1 : format == PM_F32 ? 4 : format == PM_F16 ? 2 : 0;
+  if (!esize) return fail(PM_E_INVALID, "add_view_device: unknown pixel format");
+  if (row_stride == 0) row_stride = static_cast<int64_t>(width) * esize;
+  if (row_stride < static_cast<int64_t>(width) * esize || row_stride % esize != 0)
+    return fail(PM_E_INVALID, "add_view_device: bad row stride");
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, image) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PM_E_INVALID, "add_view_device: the image is not device memory");
+  }
+  if ((attr.type != hipMemoryTypeDevice && !attr.isManaged) || attr.device != h->device)
+    return fail(PM_E_INVALID, "add_view_device: the image is not device memory of the estimator's device");
+  HostView v;
+  std::memcpy(v.K, K, sizeof(v.K));
+  std::memcpy(v.R, R, sizeof(v.R));
+  std::memcpy(v.t, t, sizeof(v.t));
+  v.w = width;
+  v.h = height;
+  v.dev = image;
+  v.dev_format = format;
+  v.dev_stride = row_stride;
+  h->views.push_back(std::move(v));
+  h->uploaded = false;
+  h->state_ready = false;
+  return PM_OK;
+}
+
 int pm_clear_views(pm_handle* h) {
   if (int r = check_handle(h)) return r;
   ++h->gen;
@@ -916,7 +954,10 @@ int upload_level(pm_handle* h, int level, int nlevels) {
   // Level views: level l of N is scaled by 1 / 2^(N-1-l) (estimator.cc:182-201).
   std::vector<HostView> scaled;
   const double scale = 1.0 / std::pow(2.0, nlevels - 1 - level);
+  bool any_dev = false, any_host = false;
+  for (const HostView& v : h->views) (v.dev ? any_dev : any_host) = true;
   if (scale < 0.999) {
+    if (any_dev) return fail(PM_E_INVALID, "views added from device memory run at one pyramid level only");
     for (const HostView& v : h->views) scaled.push_back(scaled_view(v, static_cast<float>(scale)));
   }
   const std::vector<HostView>& views = scale < 0.999 ? scaled : h->views;
@@ -942,9 +983,12 @@ int upload_level(pm_handle* h, int level, int nlevels) {
     total += static_cast<int64_t>(v.w + 2) * (v.h + 2);
     total = (total + 63) & ~int64_t(63);
   }
-  std::vector<unsigned char> host(static_cast<size_t>(total) * esz, 0);
+  // Host views are padded and converted here; views in device memory by a
+  // kernel after the upload (the same conversion, see pad_view_kernel).
+  std::vector<unsigned char> host(any_host ? static_cast<size_t>(total) * esz : 0, 0);
   for (int i = 0; i < nv; ++i) {
     const HostView& v = views[static_cast<size_t>(i)];
+    if (v.dev) continue;
     for (int y = -1; y <= v.h; ++y) {
       const int cy = std::min(std::max(y, 0), v.h - 1);
       for (int x = -1; x <= v.w; ++x) {
@@ -1005,7 +1049,7 @@ int upload_level(pm_handle* h, int level, int nlevels) {
     }
   }
   const size_t npix = static_cast<size_t>(h->W) * h->H;
-  if (int r = h->images.alloc(host.size())) return r;
+  if (int r = h->images.alloc(static_cast<size_t>(total) * esz)) return r;
   if (int r = h->quads.alloc(static_cast<size_t>(qtotal) * 4 * esz)) return r;
   if (int r = h->gviews.alloc(vt.size())) return r;
   if (int r = h->gcams.alloc(cams.size())) return r;
@@ -1030,7 +1074,14 @@ int upload_level(pm_handle* h, int level, int nlevels) {
   if (int r = h->out_normal.alloc(3 * npix)) return r;
   if (int r = h->out_cost.alloc(npix)) return r;
   if (int r = h->out_conf.alloc(npix)) return r;
-  PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
+  if (any_host)
+    PM_HIP(hipMemcpyAsync(h->images.p, host.data(), host.size(), hipMemcpyHostToDevice, h->stream), "upload images");
+  for (int i = 0; i < nv; ++i) {
+    const HostView& v = views[static_cast<size_t>(i)];
+    if (!v.dev) continue;
+    PM_HIP(launch_pad_view(v.dev, v.dev_format, v.dev_stride, v.w, v.h, h->images.p, vt[i].base, vt[i].pitch, h->f16,
+                           p.quantize_half != 0, h->stream), "pad kernel launch");
+  }
   PM_HIP(hipMemcpyAsync(h->gviews.p, vt.data(), vt.size() * sizeof(PmView), hipMemcpyHostToDevice, h->stream),
          "upload view table");
   if (int r = h->vgeom.alloc(vgeom.size())) return r;

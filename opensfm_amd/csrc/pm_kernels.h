@@ -113,6 +113,10 @@ hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int half_p
 // All launchers enqueue on `stream` and return the launch status.
 hipError_t launch_refpatch(const PmArgs& a, int half_patch, bool f16, hipStream_t stream);
 hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t stream);
+// Padded layout of one view from a device image (format PM_U8/PM_F32/PM_F16,
+// row stride in bytes) at element offset `base` of `images`.
+hipError_t launch_pad_view(const void* src, int format, int64_t stride, int w, int h, void* images, int64_t base,
+                           int pitch, bool f16, bool quantize_half, hipStream_t stream);
 // Length of the fallback mask array for a w x h reference (enough for
 // every cost-tile width).
 int64_t fallback_mask_count(int width, int height, int n_src);

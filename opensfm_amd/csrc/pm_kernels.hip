@@ -2,6 +2,7 @@
 // the set-up kernels: quad copy of the views, coarse-to-fine transfer,
 // confidence, scoring of given planes.  The kernels templated on the image
 // storage type are instantiated in pm_cost_f32.hip / pm_cost_f16.hip.
+#include <algorithm>
 #include <atomic>
 
 #include "pm_body.h"
@@ -228,6 +229,71 @@ hipError_t launch_quads(const PmArgs& a, int nviews, bool f16, hipStream_t strea
   } else {
     hipLaunchKernelGGL(quad_kernel<float>, dim3(2048), dim3(256), 0, stream, a, nviews);
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident views (pm_add_view_device): the padded layout of one view
+// written from the caller's device image, as upload_level writes it from
+// host memory (estimator.cc:313-361): u8 -> value * (1/255), clamp-to-edge
+// border, then float16 storage (round to nearest even) or float32 storage
+// (optionally rounded through float16 first).  The conversion is the
+// integer restatement the host uses, so both paths store the same bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f32_to_f16_rne(float f) {
+  const uint32_t x = __float_as_uint(f);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t absx = x & 0x7fffffffu;
+  if (absx >= 0x7f800000u) return sign | (absx > 0x7f800000u ? 0x7e00u : 0x7c00u);
+  if (absx >= 0x477ff000u) return sign | 0x7c00u;
+  if (absx < 0x38800000u) {
+    if (absx < 0x33000000u) return sign;
+    const uint32_t mant = (absx & 0x7fffffu) | 0x800000u;
+    const int shift = 126 - static_cast<int>(absx >> 23);
+    const uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1);
+    return sign | (q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u));
+  }
+  uint32_t r = (((absx >> 23) - 112u) << 10) | ((absx & 0x7fffffu) >> 13);
+  const uint32_t rem = absx & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+  return sign | r;
+}
+
+__device__ __forceinline__ float pad_src_value(const unsigned char* src, int format, int64_t stride, int x, int y) {
+  const unsigned char* row = src + stride * y;
+  if (format == 0) return static_cast<float>(row[x]) * static_cast<float>(1.0 / 255.0);   // PM_U8
+  if (format == 1) return reinterpret_cast<const float*>(row)[x];                          // PM_F32
+  return static_cast<float>(reinterpret_cast<const _Float16*>(row)[x]);                    // PM_F16
+}
+
+__global__ __launch_bounds__(256) void pad_view_kernel(const unsigned char* __restrict__ src, int format,
+                                                       int64_t stride, int w, int h, void* __restrict__ dst,
+                                                       int64_t base, int pitch, int f16, int quantize_half) {
+  const int64_t n = static_cast<int64_t>(pitch) * (h + 2);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int px = static_cast<int>(i % pitch), py = static_cast<int>(i / pitch);
+    const int cx = min(max(px - 1, 0), w - 1), cy = min(max(py - 1, 0), h - 1);
+    const float v = pad_src_value(src, format, stride, cx, cy);
+    if (f16) {
+      static_cast<uint16_t*>(dst)[base + i] = static_cast<uint16_t>(f32_to_f16_rne(v));
+    } else {
+      float out = v;
+      if (quantize_half) out = static_cast<float>(__builtin_bit_cast(_Float16,
+                                                  static_cast<uint16_t>(f32_to_f16_rne(v))));
+      static_cast<float*>(dst)[base + i] = out;
+    }
+  }
+}
+
+hipError_t launch_pad_view(const void* src, int format, int64_t stride, int w, int h, void* images, int64_t base,
+                           int pitch, bool f16, bool quantize_half, hipStream_t stream) {
+  const int64_t n = static_cast<int64_t>(pitch) * (h + 2);
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(pad_view_kernel, dim3(grid), dim3(256), 0, stream, static_cast<const unsigned char*>(src),
+                     format, stride, w, h, images, base, pitch, f16 ? 1 : 0, quantize_half ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -166,6 +166,21 @@ class Estimator {
     add_view_to(h_, K, R, t, image, mask);
   }
 
+  void add_view_device(py::handle K, py::handle R, py::handle t, uintptr_t data, const std::string& dtype, int width,
+                       int height, int64_t row_stride) {
+    auto k = as_f64(K, 9, "K");
+    auto r = as_f64(R, 9, "R");
+    auto tt = as_f64(t, 3, "t");
+    const int fmt = dtype == "u8" ? PM_U8 : dtype == "f32" ? PM_F32 : dtype == "f16" ? PM_F16 : -1;
+    if (fmt < 0) throw std::invalid_argument("dtype must be 'u8', 'f16' or 'f32'");
+    check(pm_add_view_device(h_, k.data(), r.data(), tt.data(), reinterpret_cast<const void*>(data), fmt, width,
+                             height, row_stride));
+    if (ref_w_ < 0) {
+      ref_w_ = width;
+      ref_h_ = height;
+    }
+  }
+
   // num_planes: the depths of compute_brute_force (PatchMatch does not use it).
   void set_depth_range(float dmin, float dmax, int num_planes) {
     check(pm_set_depth_range(h_, dmin, dmax));
@@ -813,6 +828,10 @@ PYBIND11_MODULE(pydense, m) {
              e.note_size(image);
            },
            py::arg("K"), py::arg("R"), py::arg("t"), py::arg("image"), py::arg("mask") = py::none())
+      .def("add_view_device", &Estimator::add_view_device, py::arg("K"), py::arg("R"), py::arg("t"),
+           py::arg("data_ptr"), py::arg("dtype"), py::arg("width"), py::arg("height"), py::arg("row_stride") = 0,
+           "add_view for an image in device memory (e.g. a torch tensor's data_ptr(); dtype 'u8', 'f16' or "
+           "'f32'); it is read at the next upload and must stay valid until then")
       .def("clear_views", [](Estimator& e) { e.clear(); })
       .def("set_depth_range", &Estimator::set_depth_range, py::arg("min_depth"), py::arg("max_depth"),
            py::arg("num_planes") = 0)

@@ -593,12 +593,13 @@ def _rot_xyz(rx: float, ry: float, rz: float) -> np.ndarray:
 
 
 def _render_dem_torch(dem, grid: float, noise: ValueNoise3D, base_cell: float, K, R, t, width: int, height: int,
-                      sigma: float, gen, device, iters: int = 48):
+                      sigma: float, gen, device, iters: int = 48, keep_image: bool = False):
     """Ray-cast the bilinear surface of ``dem`` (heights at x = j * grid,
     y = i * grid for dem[i, j]) by bracketed Newton along each pixel ray.
     The caller guarantees that the height along every ray is monotone (one
     crossing), so the root is the first hit.  Returns (image float16, depth
-    float32) numpy arrays; rays leaving the grid get depth 0 and grey 0.5."""
+    float32) numpy arrays (``keep_image``: the image as a torch tensor on
+    ``device``); rays leaving the grid get depth 0 and grey 0.5."""
     import torch
 
     dt = torch.float64
@@ -655,7 +656,8 @@ def _render_dem_torch(dem, grid: float, noise: ValueNoise3D, base_cell: float, K
     img = torch.where(ok, tex, torch.full_like(tex, 0.5))
     img = img + sigma * torch.randn(img.shape, generator=gen, dtype=dt, device=device)
     depth = torch.where(ok, lam, torch.zeros_like(lam))
-    return (img.reshape(height, width).to(torch.float16).cpu().numpy(),
+    img16 = img.reshape(height, width).to(torch.float16).contiguous()
+    return (img16 if keep_image else img16.cpu().numpy(),
             depth.reshape(height, width).to(torch.float32).cpu().numpy())
 
 
@@ -663,7 +665,7 @@ def aerial_survey(seed: int = 5, num_views: int = 200, width: int = 4000, height
                   neighbours: int = 10, extent: float = 2000.0, grid: float = 1.0, relief: float = 80.0,
                   flight_height: float = 100.0, forward_overlap: float = 0.8, side_overlap: float = 0.7,
                   focal_frac: float = 0.9, depth_range: Tuple[float, float] = (60.0, 140.0),
-                  sigma: float = 0.01, device: str = "cpu") -> WalkScene:
+                  sigma: float = 0.01, device: str = "cpu", images_on_device: bool = False) -> WalkScene:
     """Nadir views of a seeded fractal terrain, flown in a lawn-mower pattern.
 
     Terrain: ``fractal_dem`` over ``extent`` x ``extent`` m at ``grid`` m,
@@ -678,7 +680,9 @@ def aerial_survey(seed: int = 5, num_views: int = 200, width: int = 4000, height
     180 degrees).  Strips: about sqrt(num_views / 2), the survey centred on
     the terrain.  Every view is a reference; its sources are the
     ``neighbours`` views with the nearest camera centres (ties by index),
-    and its depth range is ``depth_range``.  Images are float16.
+    and its depth range is ``depth_range``.  Images are float16: numpy
+    arrays, or with ``images_on_device`` the rendering device's torch
+    tensors (contiguous (H, W)), which stay there.
     """
     import torch
 
@@ -728,7 +732,8 @@ def aerial_survey(seed: int = 5, num_views: int = 200, width: int = 4000, height
     gen.manual_seed(seed + 1000)
     images, gts = [], []
     for K_, R, t in zip(Ks, Rs, ts):
-        img, depth = _render_dem_torch(dem, grid, noise, 0.8, K_, R, t, width, height, sigma, gen, device)
+        img, depth = _render_dem_torch(dem, grid, noise, 0.8, K_, R, t, width, height, sigma, gen, device,
+                                       keep_image=images_on_device)
         images.append(img)
         gts.append(depth)
     return WalkScene(images, Ks, Rs, ts, gts, refs,

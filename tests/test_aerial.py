@@ -249,6 +249,9 @@ class _StubLibrary:
                 log.append(("run", tuple(self.refs)) if self.level == 0 else ("run", tuple(self.refs), self.level))
                 return 5.0
 
+            def stage_timing(self):
+                return {"candidates": (2.0, 3), "fallback": (1.0, 6)}
+
         self.DepthmapEstimator = Est
         self.DepthmapBatch = Batch
 
@@ -296,6 +299,7 @@ def test_run_admitted_runs_every_pyramid_level_in_order():
     last = lib.log.index(("run", (0, 1), 2))
     assert lib.log.index(("gn", 0, 1)) > last and lib.log.index(("release", 1)) > last
     assert [g.run_ms for g in groups] == [15.0, 15.0]
+    assert groups[0].stage_ms == {"candidates": [6.0, 9], "fallback": [3.0, 18]}
 
 
 def test_run_admitted_errors_reset_the_cache():
