@@ -457,6 +457,8 @@ def run_c5(args, pydense) -> int:
                                   overlap_staging=not args.no_overlap, batch_config=cfg)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
     nrefs = group * args.steps
     units = per_ref_units * nrefs
     value = units / elapsed

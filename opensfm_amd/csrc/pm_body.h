@@ -358,20 +358,16 @@ __device__ __forceinline__ float census_inner(const Ref& ref, const Sampler& sm,
 // descriptors, one source image per gather instruction, neighbouring
 // pixels), which keeps the gathers coherent.
 // Strided NCC of the textureless fallback (opencl_kernels.h:188-258) at
-// stride S.  The reference-side (w, w*r) of each tap are recomputed from the
-// reference texels with the arithmetic of refpatch_kernel (bit-identical to
-// the stored strides 2 and 3), which is cheaper than reading them back for
-// scattered jobs; the moments come from ref_sums.  The tap at (ix, iy) reads
-// the reference texel (x + S*ix, y + S*iy) -- the int conversion of
-// x + float(S*ix) is exact -- so an LDS copy of the reference is read at a
-// per-row base plus a constant offset.  The weight's argument is <= 0.
-// S = 0: the stride is the runtime value rs (one loop for both strides:
-// smaller code where the fallback runs inline, as in the init kernel).
-template <int HP, typename T, int S, typename Ref>
+// stride SS, where the fallback runs inline (random init, plane sweep,
+// caller-given planes).  The reference-side (w, w*r) of each tap are
+// recomputed from the reference texels with the arithmetic of
+// refpatch_kernel (bit-identical to the stored strides 2 and 3); the moments
+// come from ref_sums.  The tap at (ix, iy) reads the reference texel
+// (x + SS*ix, y + SS*iy) -- the int conversion of x + float(SS*ix) is exact.
+// The weight's argument is <= 0.
+template <int HP, typename T, typename Ref>
 __device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const Ref& ref, const Sampler& sm, int x, int y,
-                                               int cidx, float src_center, const Warp& w, int rs_ = S) {
-  constexpr bool kLds = std::is_same<Ref, LdsRef>::value && S > 0;
-  const int SS = S > 0 ? S : rs_;
+                                               int cidx, float src_center, const Warp& w, int SS) {
   const float stride = static_cast<float>(SS);
   const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(SS - 1) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
   const float sf = a.spatial_factor, cf = a.color_factor;
@@ -381,23 +377,15 @@ __device__ __forceinline__ float ncc_at_stride(const PmArgs& a, const Ref& ref, 
     const float fdy = stride * static_cast<float>(iy);
     const float bx = fmaf(w.jyx, fdy, w.scx);
     const float by = fmaf(w.jyy, fdy, w.scy);
-    const float* rrow = nullptr;
-    if constexpr (kLds) rrow = ref.t + (y + SS * iy - ref.y0) * ref.pitch + (x - ref.x0);
 #pragma unroll
     for (int ix = -HP; ix <= HP; ++ix) {
       const float fdx = stride * static_cast<float>(ix);
       const float sx = fmaf(w.jxx, fdx, bx);
       const float sy = fmaf(w.jxy, fdx, by);
       const float sd = sample<T>(sm, sx, sy) - src_center;
-      float rv;
-      if constexpr (kLds) {
-        rv = rrow[SS * ix];
-      } else {
-        const float rx = static_cast<float>(x) + fdx;
-        const float ry = static_cast<float>(y) + fdy;
-        rv = ref(static_cast<int>(rx), static_cast<int>(ry));
-      }
-      const float r = rv - rs.x;
+      const float rx = static_cast<float>(x) + fdx;
+      const float ry = static_cast<float>(y) + fdy;
+      const float r = ref(static_cast<int>(rx), static_cast<int>(ry)) - rs.x;
       const float wt = pm_expf_nonpos(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
       const float wr = wt * r;
       const float wsd = wt * sd;
@@ -425,20 +413,11 @@ __device__ __forceinline__ float fallback_cost(const PmArgs& a, const Ref& ref, 
                                                int cidx, float ref_center, const Warp& w, float ncc1) {
   const float scen = sample<T>(sm, w.scx, w.scy);
   float best = ncc1;
-  if constexpr (std::is_same<Ref, LdsRef>::value) {  // strides 2 then 3 (:533-542)
-    const float ncc2 = ncc_at_stride<HP, T, 2>(a, ref, sm, x, y, cidx, scen, w);
-    if (ncc2 > best) best = ncc2;
-    if (!(best > 0.1f)) {
-      const float ncc3 = ncc_at_stride<HP, T, 3>(a, ref, sm, x, y, cidx, scen, w);
-      if (ncc3 > best) best = ncc3;
-    }
-  } else {
 #pragma unroll 1
-    for (int s = 2; s <= 3; ++s) {
-      const float ncc = ncc_at_stride<HP, T, 0>(a, ref, sm, x, y, cidx, scen, w, s);
-      if (ncc > best) best = ncc;
-      if (best > 0.1f) break;
-    }
+  for (int s = 2; s <= 3; ++s) {  // strides 2 then 3 (:533-542)
+    const float ncc = ncc_at_stride<HP, T>(a, ref, sm, x, y, cidx, scen, w, s);
+    if (ncc > best) best = ncc;
+    if (best > 0.1f) break;
   }
   if (best <= -0.5f) return (a.census_weight > 1e-6f) ? census_inner<T>(ref, sm, ref_center, scen, x, y, w) : 2.0f;
   return 1.0f - best;
@@ -507,54 +486,208 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
   return pos;
 }
 
-// One wave per (source view, tile).  Each XCD takes the tiles of its band
-// (the tiles its colour-pass workgroups had, see tile_of) view by view, so
-// the waves in flight on an XCD gather from one source image and one band of
-// the reference, a footprint its L2 holds.
+// ---------------------------------------------------------------------------
+// Tile-workgroup fallback: one workgroup of kFbWaves waves per colour tile.
+// The bilateral weights of the strided taps depend only on the reference
+// pixel, the stride and the tap, not on the plane or the view, so the
+// workgroup computes them once for the tile's pixels that have pending jobs
+// (strides 2 and 3, into LDS; bit-identical to computing them per job) and
+// every (slot, view) job of those pixels reads them back: about half of a
+// strided tap's instructions are the weight's argument and exp.  The waves
+// then take (view, 64-job chunk) items from an LDS counter in view order,
+// so the lanes of a wave share a source view (scalar camera and sampler,
+// neighbouring pixels) and the workgroup's waves stay busy when views have
+// uneven job counts.
+// ---------------------------------------------------------------------------
+#ifndef PM_FB_WAVES
+#define PM_FB_WAVES 4
+#endif
+constexpr int kFbWaves = PM_FB_WAVES;
+
+// Strided NCC of one job with the tile's LDS weights wt[tap * kWave] of
+// the job's pixel (opencl_kernels.h:188-258): the same sums, in the same
+// order, as ncc_at_stride.  When the reference moments already fail the
+// variance gate the result is -1 whatever the source, so the taps are skipped.
+template <typename T>
+struct GlobalSampler {
+  Sampler sm;
+  __device__ __forceinline__ float operator()(float sx, float sy) const { return sample<T>(sm, sx, sy); }
+};
+
+// Strided NCC of one job with the tile's LDS weights wt[tap * kWave] of
+// the job's pixel (opencl_kernels.h:188-258): the same sums, in the same
+// order, as ncc_at_stride.  When the reference moments already fail the
+// variance gate the result is -1 whatever the source, so the taps are skipped.
+template <int HP, int S, typename Smp>
+__device__ __forceinline__ float ncc_at_stride_w(const PmArgs& a, const LdsRef& ref, const float* wt,
+                                                 const Smp& smp, int x, int y, int cidx, float src_center,
+                                                 const Warp& w) {
+  constexpr float stride = static_cast<float>(S);
+  const float4 rs = gp(a.ref_sums)[static_cast<int64_t>(S - 1) * a.ref_plane + cidx];  // (c, sum_w, sum_r, sum_rr)
+  if (rs.y < 1e-6f) return -1.0f;
+  const float inv_w = 1.0f / rs.y;
+  const float mean_r = rs.z * inv_w;
+  const float var_ref = rs.w * inv_w - mean_r * mean_r;
+  if (var_ref < a.var_threshold) return -1.0f;
+  float sum_s = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+#pragma unroll 1
+  for (int iy = -HP; iy <= HP; ++iy) {
+    const float fdy = stride * static_cast<float>(iy);
+    const float bx = fmaf(w.jyx, fdy, w.scx);
+    const float by = fmaf(w.jyy, fdy, w.scy);
+    const float* rrow = ref.t + (y + S * iy - ref.y0) * ref.pitch + (x - ref.x0);
+    const float* wrow = wt + (iy + HP) * (2 * HP + 1) * kWave;
+#pragma unroll
+    for (int ix = -HP; ix <= HP; ++ix) {
+      const float fdx = stride * static_cast<float>(ix);
+      const float sx = fmaf(w.jxx, fdx, bx);
+      const float sy = fmaf(w.jxy, fdx, by);
+      const float sd = smp(sx, sy) - src_center;
+      const float r = rrow[S * ix] - rs.x;
+      const float wgt = wrow[(ix + HP) * kWave];
+      const float wr = wgt * r;
+      const float wsd = wgt * sd;
+      sum_s += wsd;
+      sum_ss = fmaf(wsd, sd, sum_ss);
+      sum_rs = fmaf(wr, sd, sum_rs);
+    }
+  }
+  const float mean_s = sum_s * inv_w;
+  const float var_src = sum_ss * inv_w - mean_s * mean_s;
+  if (var_src < a.var_threshold) return -1.0f;
+  const float covar = sum_rs * inv_w - mean_r * mean_s;
+  return covar / sqrtf(var_ref * var_src);
+}
+
+// Census on the job's sampler (opencl_kernels.h:270-315), as census_inner.
+template <typename Smp>
+__device__ __forceinline__ float census_w(const LdsRef& ref, const Smp& smp, float ref_center, float src_center,
+                                          int cx, int cy, const Warp& w) {
+  uint32_t rc = 0u, sc = 0u;
+  int bit = 0;
+#pragma unroll 1
+  for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const float ref_val = ref(cx + dx, cy + dy);
+      const float sx = w.scx + w.jxx * static_cast<float>(dx) + w.jyx * static_cast<float>(dy);
+      const float sy = w.scy + w.jxy * static_cast<float>(dx) + w.jyy * static_cast<float>(dy);
+      const float src_val = smp(sx, sy);
+      if (ref_val > ref_center) rc |= (1u << bit);
+      if (src_val > src_center) sc |= (1u << bit);
+      ++bit;
+    }
+  }
+  const int hamming = __popc(rc ^ sc);
+  if (__popc(rc) < 3 && __popc(sc) < 3) return 1.0f;
+  return static_cast<float>(hamming) / 24.0f;
+}
+
+// Strides 2 then 3 (stop once NCC > 0.1), then census or 2.0
+// (opencl_kernels.h:531-553) on one sampler.
+template <int HP, typename Smp>
+__device__ __forceinline__ float fallback_w(const PmArgs& a, const LdsRef& ref, const float* wt, const Smp& smp,
+                                            int x, int y, int cidx, float scen, const Warp& w, float ncc1) {
+  constexpr int P = (2 * HP + 1) * (2 * HP + 1);
+  float best = ncc1;
+  const float ncc2 = ncc_at_stride_w<HP, 2>(a, ref, wt, smp, x, y, cidx, scen, w);
+  if (ncc2 > best) best = ncc2;
+  if (!(best > 0.1f)) {
+    const float ncc3 = ncc_at_stride_w<HP, 3>(a, ref, wt + P * kWave, smp, x, y, cidx, scen, w);
+    if (ncc3 > best) best = ncc3;
+  }
+  if (best <= -0.5f) return (a.census_weight > 1e-6f) ? census_w(ref, smp, ref(x, y), scen, x, y, w) : 2.0f;
+  return 1.0f - best;
+}
+
 template <int HP, typename T, int TX>
-__device__ __forceinline__ void fallback_body(const PmArgs& a, int b) {
+__device__ __forceinline__ void fallback_tile_body(const PmArgs& a, int b) {
+  constexpr int D = 2 * HP + 1;
+  constexpr int P = D * D;
   constexpr int TY = kWave / TX;
   constexpr int R = (3 * HP > 2) ? 3 * HP : 2;       // reach of stride 3 and of the 5 x 5 census
   constexpr int RW = 2 * TX + 2 * R, RH = TY + 2 * R;
   __shared__ float ref_tile[RW * RH];
-  const int tiles_x = colour_tiles_x<TX>(a);
-  const int ntiles = tiles_x * colour_tiles_y<TX>(a);
-  const int per = (ntiles + kXcds - 1) / kXcds;
-  const int j = b / kXcds;
-  const int s = j / per;
-  const int tile = (b % kXcds) * per + j % per;
+  __shared__ float wts[2 * P * kWave];               // [stride - 2][tap][tile lane]
+  __shared__ int next_item;
+  int tx, ty;
+  if (!colour_tile<TX>(a, b, tx, ty)) return;        // uniform over the workgroup
+  const int tile = ty * colour_tiles_x<TX>(a) + tx;
   const int n_src = a.n_src;
-  if (s >= n_src || tile >= ntiles) return;
   const int nslots = a.phase == 0 ? 9 : 6;
-  // written by the cost kernel before this launch: read-only here
-  const PM_CONST uint64_t* mk = (const PM_CONST uint64_t*)(a.fb_masks) + static_cast<int64_t>(tile) * 9 * n_src + s;
-  uint64_t m[9];
-  int cum[9];
-  int total = 0;
+  const int lane = static_cast<int>(threadIdx.x);
+  const int t = static_cast<int>(threadIdx.y) * kWave + lane;
+  // written by the cost kernel before this launch: read-only here (scalar loads)
+  const PM_CONST uint64_t* mk = (const PM_CONST uint64_t*)(a.fb_masks) + static_cast<int64_t>(tile) * 9 * n_src;
+  uint64_t any = 0ull;
+  int chunks = 0;
+#pragma unroll 1
+  for (int s = 0; s < n_src; ++s) {
+    int tot = 0;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    m[k] = k < nslots ? mk[k * n_src] : 0ull;
-    total += __popcll(m[k]);
-    cum[k] = total;
+    for (int k = 0; k < 9; ++k) {
+      const uint64_t m = k < nslots ? mk[k * n_src + s] : 0ull;
+      any |= m;
+      tot += __popcll(m);
+    }
+    chunks += (tot + kWave - 1) / kWave;
   }
-  if (total == 0) return;  // uniform over the wave
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  if (chunks == 0) return;                           // uniform over the workgroup
   const int x0 = tx * 2 * TX - R, y0 = ty * TY - R;
   {
     const T* img = images_of<T>(a);
     const PmView rv = kview(a, 0);
-    for (int i = static_cast<int>(threadIdx.x); i < RW * RH; i += kWave)
-      ref_tile[i] = texel(img, rv, x0 + i % RW, y0 + i / RW);
+    for (int i = t; i < RW * RH; i += kWave * kFbWaves) ref_tile[i] = texel(img, rv, x0 + i % RW, y0 + i / RW);
+  }
+  if (t == 0) next_item = 0;
+  __syncthreads();
+  // weights of the strided taps of the tile's pixels with pending jobs
+  {
+    const float sf = a.spatial_factor, cf = a.color_factor;
+    for (int i = t; i < 2 * P * kWave; i += kWave * kFbWaves) {
+      const int l = i % kWave;
+      if (!((any >> l) & 1ull)) continue;
+      const int tap = (i / kWave) % P;
+      const int SS = 2 + i / (P * kWave);
+      const int iy = tap / D - HP, ix = tap % D - HP;
+      const int yy = ty * TY + l / TX;
+      const int xx = (tx * TX + l % TX) * 2 + ((yy + a.color) & 1);
+      const float fdx = static_cast<float>(SS) * static_cast<float>(ix);
+      const float fdy = static_cast<float>(SS) * static_cast<float>(iy);
+      const float r = ref_tile[(yy + SS * iy - y0) * RW + (xx + SS * ix - x0)] - ref_tile[(yy - y0) * RW + (xx - x0)];
+      wts[i] = pm_expf_nonpos(fdx * fdx * sf + fdy * fdy * sf + r * r * cf);
+    }
   }
   __syncthreads();
   const LdsRef ref{ref_tile, x0, y0, RW};
   const int64_t pl = a.ref_plane;
   const PmCam c0 = kcam(a, 0);
-  const PmCam cs = kcam(a, s + 1);
-  const Sampler sm = make_sampler<T>(a, kview(a, s + 1));
 #pragma unroll 1
-  for (int q0 = 0; q0 < total; q0 += kWave) {
-    const int q = q0 + static_cast<int>(threadIdx.x);
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(&next_item, 1);
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    if (item >= chunks) break;
+    // view s and chunk c of this item (views in order)
+    int s = 0, c = item;
+    uint64_t m[9];
+    int cum[9];
+    int total = 0;
+#pragma unroll 1
+    for (;; ++s) {
+      total = 0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        m[k] = k < nslots ? mk[k * n_src + s] : 0ull;
+        total += __popcll(m[k]);
+        cum[k] = total;
+      }
+      const int nc = (total + kWave - 1) / kWave;
+      if (c < nc) break;
+      c -= nc;
+    }
+    const int q = c * kWave + lane;
     if (q >= total) continue;
     int slot = 0, prev = 0;
     uint64_t mm = m[0];
@@ -575,11 +708,13 @@ __device__ __forceinline__ void fallback_body(const PmArgs& a, int b) {
       pl4 = to_plane(gp(a.refp)[slot * pl + cidx]);
     }
     float* out = a.ca + (static_cast<int64_t>(slot) * n_src + s) * pl + cidx;
+    const GlobalSampler<T> smp{make_sampler<T>(a, kview(a, s + 1))};
     PlanePts pp;
     plane_points(c0, x, y, pl4, pp);
     Warp w;
-    warp_of(pp.cw, pp.wdx, pp.wdy, cs, w);  // valid: it was valid at stride 1
-    *out = fallback_cost<HP, T>(a, ref, sm, x, y, cidx, ref(x, y), w, *out);
+    warp_of(pp.cw, pp.wdx, pp.wdy, kcam(a, s + 1), w);  // valid: it was valid at stride 1
+    const float scen = smp(w.scx, w.scy);
+    *out = fallback_w<HP>(a, ref, wts + l, smp, x, y, cidx, scen, w, *out);
   }
 }
 
@@ -1479,9 +1614,7 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 // Occupancy targets (waves per SIMD) of the cost kernels and the fallback;
 // 0 = the compiler's choice.  The fallback is compiled for at most
 // PM_FALLBACK_WAVES_MAX waves per SIMD: its strided gathers need many loads
-// in flight per wave, and more resident waves only compete for L1/L2 (at
-// 1600x1200, N=8: 33 ms per step at 3 waves, 46 ms at the 6 waves the
-// compiler picks on its own).
+// in flight per wave, and more resident waves only compete for L1/L2.
 #ifndef PM_COST_WAVES_PER_EU
 #define PM_COST_WAVES_PER_EU 0
 #endif
@@ -1545,12 +1678,12 @@ __global__ __launch_bounds__(kWave * 6) PM_COST_ATTR void refine_cost_batch(PmBa
 }
 
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave) PM_FALLBACK_ATTR void fallback_kernel(PmArgs a) {
-  fallback_body<HP, T, TX>(a, PM_BLOCK);
+__global__ __launch_bounds__(kWave * kFbWaves) PM_FALLBACK_ATTR void fallback_kernel(PmArgs a) {
+  fallback_tile_body<HP, T, TX>(a, PM_BLOCK);
 }
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave) PM_FALLBACK_ATTR void fallback_batch(PmBatch bt) {
-  batch_item(bt, [](const PmArgs& a, int b) { fallback_body<HP, T, TX>(a, b); });
+__global__ __launch_bounds__(kWave * kFbWaves) PM_FALLBACK_ATTR void fallback_batch(PmBatch bt) {
+  batch_item(bt, [](const PmArgs& a, int b) { fallback_tile_body<HP, T, TX>(a, b); });
 }
 
 template <int NV, int TX>

@@ -22,8 +22,8 @@ void launch_tiled(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, siz
       else hipLaunchKernelGGL((refine_cost_batch<HP, T, TX>), grid, dim3(kWave, 6), 0, s, *bt);
       break;
     default:  // PM_K_FALLBACK; lds: dynamic LDS that only caps its occupancy
-      if (a) hipLaunchKernelGGL((fallback_kernel<HP, T, TX>), grid, dim3(kWave), lds, s, *a);
-      else hipLaunchKernelGGL((fallback_batch<HP, T, TX>), grid, dim3(kWave), lds, s, *bt);
+      if (a) hipLaunchKernelGGL((fallback_kernel<HP, T, TX>), grid, dim3(kWave, kFbWaves), lds, s, *a);
+      else hipLaunchKernelGGL((fallback_batch<HP, T, TX>), grid, dim3(kWave, kFbWaves), lds, s, *bt);
       break;
   }
 }

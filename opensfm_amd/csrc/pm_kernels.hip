@@ -207,8 +207,7 @@ int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x) 
       return xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
                        ((height + kFullTileY - 1) / kFullTileY));
     case PM_K_FALLBACK: {
-      const int64_t per = (colour_tiles(width, height, tile_x) + kXcds - 1) / kXcds;
-      return kXcds * per * n_src;
+      return xcd_round(colour_tiles(width, height, tile_x));  // one workgroup per colour tile
     }
     case PM_K_DECODE:
       return xcd_round((static_cast<int64_t>(width) * height + kDecodeThreads - 1) / kDecodeThreads);
@@ -387,9 +386,13 @@ size_t fallback_cap_lds(int hp, int tile_x, int fallback_waves) {
   const size_t lds = lds_per_cu();
   if (lds == 0) return 0;
   const int r = 3 * hp > 2 ? 3 * hp : 2;
-  const size_t static_bytes = sizeof(float) * (2 * tile_x + 2 * r) * (kWave / tile_x + 2 * r);
-  // more than 1/(n+1) of the LDS per one-wave workgroup: at most n per CU
-  const size_t per_wg = lds / (static_cast<size_t>(kSimdsPerCu) * fallback_waves + 1) + 1;
+  const int taps = (2 * hp + 1) * (2 * hp + 1);
+  const size_t static_bytes = sizeof(float) * ((2 * tile_x + 2 * r) * (kWave / tile_x + 2 * r) +
+                                               2 * taps * kWave);
+  // more than 1/(n+1) of the LDS per workgroup: at most n workgroups per CU,
+  // n = waves per SIMD x SIMDs / waves per workgroup
+  const size_t wgs = std::max<size_t>(1, static_cast<size_t>(kSimdsPerCu) * fallback_waves / kFbWaves);
+  const size_t per_wg = lds / (wgs + 1) + 1;
   return per_wg > static_bytes ? per_wg - static_bytes : 0;
 }
 
