@@ -235,6 +235,7 @@ struct pm_handle {
   float last_gn_ms = 0.0f;
   DevBuf<float> ca, vwb, wnb;  // colour-pass intermediate state
   DevBuf<float> gn_state;      // Gauss-Newton state when ca is too small (one source)
+  DevBuf<float4> gn_orig;      // the PatchMatch planes, for the refinement guard
   DevBuf<int> cpos, nref;
   DevBuf<float4> refp;
   DevBuf<uint64_t> fb_masks;   // pending textureless fallbacks per (tile, slot, view)
@@ -281,6 +282,7 @@ struct pm_handle {
     snapshot.park(); costs.park(); prevd.park(); rng.park(); sel.park(); masks.park();
     out_depth.park(); out_normal.park(); out_cost.park(); out_conf.park();
     gn_state.park();
+    gn_orig.park();
     if (!keep_coarse) {
       coarse_depth.park();
       coarse_normal.park();
@@ -1463,9 +1465,16 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
     if (int r = h->gn_state.alloc(need)) return r;
     state = h->gn_state.p;
   }
+  const size_t npix = static_cast<size_t>(h->W) * h->H;
+  if (steps > 0) {
+    if (int r = h->gn_orig.alloc(npix)) return r;
+    PM_HIP(hipMemcpyAsync(h->gn_orig.p, h->planes.p, npix * sizeof(float4), hipMemcpyDeviceToDevice, h->stream),
+           "keep the PatchMatch planes");
+  }
   for (int i = 0; i < steps; ++i)
     PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, state, i == 0, h->stream),
            "Gauss-Newton kernel launch");
+  if (steps > 0) PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, h->stream), "refinement guard launch");
   PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
   if (int r = pm_decode(h)) return r;
   if (int r = postprocess(h)) return r;

@@ -108,6 +108,51 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
 }
 
 
+// ---------------------------------------------------------------------------
+// Guard of the Gauss-Newton refinement.  The steps lower the stride-1
+// photometric cost of one pixel's views whose NCC is above 0.1, pixel by
+// pixel; the PatchMatch objective also has the strided and census fallbacks
+// of every selected view (opencl_kernels.h:381-557) and the smoothness term
+// against the neighbours' planes (:649-707).  Per pixel whose plane the
+// steps moved, the refined plane is kept only if that objective -- the mean
+// full cost over the selected views plus smooth_w times the smoothness cost
+// against the neighbours' PatchMatch planes -- does not exceed the one of
+// the PatchMatch plane `orig`.  (Without the smoothness term, a step that
+// fits the images' noise moves a pixel away from its neighbours' surface.)
+// ---------------------------------------------------------------------------
+template <int HP, typename T>
+__global__ __launch_bounds__(kWave) void refine_guard_kernel(PmArgs a, const float4* __restrict__ orig) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kWave + lane;
+  if (i >= static_cast<int64_t>(a.width) * a.height) return;
+  const float4 pn = a.planes[i], po = orig[i];
+  if (pn.x == po.x && pn.y == po.y && pn.z == po.z && pn.w == po.w) return;
+  const uint32_t sel = a.sel[i];
+  const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
+  const T* img = images_of<T>(a);
+  float* cv = smem + lane;
+  Nbrs nb;
+  load_nbrs(a, orig, x, y, nb);
+  float obj[2];
+  const float4 pl[2] = {po, pn};
+#pragma unroll 1
+  for (int k = 0; k < 2; ++k) {
+    const Plane p = to_plane(pl[k]);
+    full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), p, cv);
+    float sum = 0.0f;
+    int n = 0;
+    for (int s = 0; s < a.n_src; ++s) {
+      if (!((sel >> s) & 1u)) continue;
+      sum += cv[s * kWave];
+      ++n;
+    }
+    obj[k] = n > 0 ? sum / static_cast<float>(n) : 0.0f;
+    if (a.smooth_w > 1e-6f) obj[k] += a.smooth_w * smoothness_cost(a, nb, p, x, y);
+  }
+  if (!(obj[1] <= obj[0])) a.planes[i] = po;
+}
+
 int64_t xcd_round(int64_t blocks) { return kXcds * ((blocks + kXcds - 1) / kXcds); }
 
 int64_t colour_tiles(int width, int height, int tile_x) {
@@ -465,6 +510,13 @@ hipError_t launch_score(const PmArgs& a, int hp, bool f16, int n, const int32_t*
   const size_t lds = sizeof(float) * 2 * a.n_src * kWave;
   const dim3 grid((n + kWave - 1) / kWave);
   PM_DISPATCH(score_kernel, hp, f16, grid, lds, stream, a, n, xy, planes, costs, init_cost, sel);
+}
+
+hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, hipStream_t stream) {
+  const int64_t n = static_cast<int64_t>(a.width) * a.height;
+  const dim3 grid(static_cast<unsigned>((n + kWave - 1) / kWave));
+  const size_t lds = cost_column_lds(a.n_src);
+  PM_DISPATCH(refine_guard_kernel, hp, f16, grid, lds, stream, a, orig);
 }
 
 }  // namespace pmhip

@@ -175,13 +175,17 @@ __device__ __forceinline__ ViewTerm ncc_view(const PmArgs& a, const Sampler& sm,
 
 // Mean residual over the valid views of `views` and, with gradients, its
 // derivative and the Gauss-Newton sums H = sum J J^T, g = sum J r (J = -dNCC).
+// `good`: the views of `views` whose NCC is above 0.1 (those whose
+// PatchMatch cost is 1 - NCC, opencl_kernels.h:526-529).
 template <int HP, typename T, bool kGrad>
 __device__ __forceinline__ float pixel_energy(const PmArgs& a, const VGeom* vg, int x, int y, uint32_t views,
-                                              const Theta& th, float grad[3], float H[6], float gv[3], int& used) {
+                                              const Theta& th, float grad[3], float H[6], float gv[3], int& used,
+                                              uint32_t& good) {
   const int cidx = colour_index_r(a, x, y);
   const float4 rs = gp(a.ref_sums)[cidx];
   float e = 0.0f;
   used = 0;
+  good = 0u;
   if constexpr (kGrad) {
     for (int j = 0; j < 3; ++j) { grad[j] = 0.0f; gv[j] = 0.0f; }
     for (int j = 0; j < 6; ++j) H[j] = 0.0f;
@@ -197,6 +201,7 @@ __device__ __forceinline__ float pixel_energy(const PmArgs& a, const VGeom* vg, 
     const ViewTerm t = ncc_view<HP, T, kGrad>(a, sm, vg[s], x, y, cidx, rs, th);
     if (!t.ok) continue;
     ++used;
+    if (t.ncc > 0.1f) good |= 1u << s;
     const float r = 1.0f - t.ncc;
     e += r;
     if constexpr (kGrad) {
@@ -234,7 +239,8 @@ __global__ __launch_bounds__(256) PM_REFINE_ATTR void ncc_energy_kernel(PmArgs a
   const Theta th{depth[i], ab[2 * i], ab[2 * i + 1]};
   float gr[3], H[6], gv[3];
   int used;
-  const float e = pixel_energy<HP, T, true>(a, vg, x, y, views[i], th, gr, H, gv, used);
+  uint32_t good;
+  const float e = pixel_energy<HP, T, true>(a, vg, x, y, views[i], th, gr, H, gv, used, good);
   cost[i] = e;
   if (grad) {
     grad[3 * i] = gr[0];
@@ -269,14 +275,24 @@ __device__ __forceinline__ bool solve3(const float H[6], const float g[3], float
 }
 
 // One damped Gauss-Newton step per pixel with a view selection; a step is
-// kept only where it lowers the pixel's cost.  The trial point is scored
-// with its gradient: when the step is kept, that point, its cost and its
-// normal equations are the next step's start (state st, kGnState planes of
-// W*H floats: theta, e, used, H, g), so a step costs one gradient
-// evaluation.  A rejected step would be proposed again unchanged (fixed
-// damping), so the pixel is done (used = -1).  The first step starts from
-// the PatchMatch planes.
-constexpr int kGnState = 14;
+// kept only where it lowers the pixel's cost.  The views are the pixel's
+// selected views whose stride-1 NCC is above 0.1 at the PatchMatch plane:
+// for those the PatchMatch cost is 1 - NCC (opencl_kernels.h:526-529), so
+// the steps lower the PatchMatch cost itself; views whose cost comes from the
+// textureless fallback (not differentiable) are left out, and a step that
+// would send a kept view's NCC to 0.1 or below is refused.  The trial point
+// is scored with its gradient: when the step is kept, that point, its cost
+// and its normal equations are the next step's start (state st, kGnState
+// planes of W*H floats: theta, e, used, H, g, views, PatchMatch depth), so a
+// step costs one gradient evaluation.  A rejected step would be proposed
+// again unchanged (fixed damping), so the pixel is done (used = -1).  The
+// first step starts from the PatchMatch planes.  A step whose depth leaves
+// kGnDepthRange of the PatchMatch depth ends the pixel's refinement: the
+// refinement polishes within the range of PatchMatch's own depth perturbation
+// (opencl_kernels.h:1671); a pixel whose photometric optimum lies further away
+// is fitting its images' noise, which PatchMatch's smoothness held back.
+constexpr int kGnState = 16;
+constexpr float kGnDepthRange = 0.005f;
 
 template <int HP, typename T>
 __global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs a, const VGeom* __restrict__ vg,
@@ -286,9 +302,11 @@ __global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
-  const uint32_t views = a.sel[i];
+  uint32_t views = a.sel[i];
   if (!views) return;
   float* const s_used = st + 4 * n + i;
+  float* const s_views = st + 14 * n + i;
+  float* const s_dpm = st + 15 * n + i;
   const PmCam c0 = ((const PM_CONST PmCam*)(a.cams))[0];
   Theta th;
   float e0, H[6], gv[3];
@@ -301,10 +319,19 @@ __global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs
     if (!(d > 0.0f)) return;
     th = Theta{d, p.x / -p.z, p.y / -p.z};
     float gr[3];
-    e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used);
+    uint32_t good;
+    e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used, good);
+    if (good != views) {  // keep the views whose PatchMatch cost is 1 - NCC
+      views = good;
+      if (!views) return;
+      e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used, good);
+    }
+    *s_views = __uint_as_float(views);
+    *s_dpm = d;
   } else {
     used = __float_as_int(*s_used);
     if (used <= 0) return;
+    views = __float_as_uint(*s_views);
     th = Theta{st[i], st[n + i], st[2 * n + i]};
     e0 = st[3 * n + i];
 #pragma unroll
@@ -318,11 +345,17 @@ __global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs
     return;
   }
   Theta t1{th.d + step[0], th.a + step[1], th.b + step[2]};
+  const float dpm = *s_dpm;
   t1.d = fminf(fmaxf(t1.d, a.depth_min), a.depth_max);
+  if (!(fabsf(t1.d - dpm) <= kGnDepthRange * dpm)) {  // not a local polish: the pixel keeps its plane
+    *s_used = __int_as_float(-1);
+    return;
+  }
   float gr1[3], H1[6], gv1[3];
   int used1;
-  const float e1 = pixel_energy<HP, T, true>(a, vg, x, y, views, t1, gr1, H1, gv1, used1);
-  if (used1 != used || !(e1 < e0)) {
+  uint32_t good1;
+  const float e1 = pixel_energy<HP, T, true>(a, vg, x, y, views, t1, gr1, H1, gv1, used1, good1);
+  if (used1 != used || good1 != views || !(e1 < e0)) {
     *s_used = __int_as_float(-1);
     return;
   }

@@ -139,27 +139,50 @@ def test_torch_autograd_returns_the_kernel_gradient(pydense, bumps):
 
 
 @pytest.mark.gpu
-def test_gauss_newton_lowers_the_cost(pydense, bumps):
-    s = bumps
-    p = params_for(s, patch_size=5)
+@pytest.mark.parametrize("scene,patch", [("bumps", 5), ("bumps_full", 7)])
+def test_gauss_newton_lowers_the_cost(pydense, request, scene, patch):
+    s = request.getfixturevalue(scene)
+    p = params_for(s, patch_size=patch)
     est = gpu_estimator(pydense, s, p)
     before = est.compute()
     d0, ab0 = _planes_to_theta(before["plane"], np.asarray(s.Ks[0]))
-    c0, _ = est.ncc_cost_grad(d0, ab0, before["views"])
+    views = before["views"]
+    # the views Gauss-Newton refines: selected, valid, stride-1 NCC > 0.1
+    # (cost 1 - NCC < 0.9) at the PatchMatch plane
+    gated = np.zeros_like(views)
+    for v in range(s.num_src):
+        one = np.where(views & (1 << v), np.uint32(1 << v), np.uint32(0))
+        cv, _ = est.ncc_cost_grad(d0, ab0, one)
+        gated |= np.where((one > 0) & (cv > 0.0) & (cv < 0.9), one, np.uint32(0))
+    c0, _ = est.ncc_cost_grad(d0, ab0, gated)
     est.gauss_newton(10, 1e-3)
     after = est.download()
     d1, ab1 = _planes_to_theta(after["plane"], np.asarray(s.Ks[0]))
-    c1, _ = est.ncc_cost_grad(d1, ab1, before["views"])
+    c1, _ = est.ncc_cost_grad(d1, ab1, gated)
     gt = s.gt_depth
-    acc0 = np.mean(np.abs(before["depth"] - gt) < 0.01 * gt)
-    acc1 = np.mean(np.abs(after["depth"] - gt) < 0.01 * gt)
-    m = before["views"] > 0
-    print(f"Gauss-Newton: mean cost {c0[m].mean():.5f} -> {c1[m].mean():.5f}, GT within 1% {acc0:.4f} -> "
-          f"{acc1:.4f}, {est.last_refine_ms():.2f} ms", flush=True)
-    # a step is kept only where it lowers the cost (up to the plane <-> (d, a, b) round trip)
+    e0 = np.abs(before["depth"] - gt) / gt
+    e1 = np.abs(after["depth"] - gt) / gt
+    in0, in1 = e0 < 0.01, e1 < 0.01
+    m = gated > 0
+    mv = m & (after["depth"] != before["depth"])
+    moved = np.mean(mv[m])
+    print(f"Gauss-Newton ({s.name}, patch {patch}): {np.mean(m):.3f} of pixels refined, {moved:.4f} of them moved; "
+          f"mean cost {c0[m].mean():.6f} -> {c1[m].mean():.6f}, GT within 1% {in0.mean():.6f} -> {in1.mean():.6f} "
+          f"(into the 1% band {np.sum(in1 & ~in0)}, out of it {np.sum(in0 & ~in1)}), moved pixels' mean GT error "
+          f"{e0[mv].mean():.5f} -> {e1[mv].mean():.5f}, {est.last_refine_ms():.2f} ms", flush=True)
+    # a step is kept only where it lowers the cost of the refined views (up to
+    # the plane <-> (d, a, b) round trip)
     assert np.mean(c1[m] <= c0[m] + 1e-4) >= 0.999
     assert c1[m].mean() < c0[m].mean()
-    assert acc1 >= acc0 - 0.01
+    # and the depth does not get worse at the refinement workload's size
+    # (1024x768, N=6, patch 7; a few hundred pixels move): the moved pixels'
+    # mean error against the ground truth does not rise and no fewer pixels
+    # are within 1 % of it.  At 256x192 about fifteen pixels move, too few for
+    # either statistic; the cost bar above holds there.
+    if s.gt_depth.size >= 1024 * 768:
+        assert mv.sum() >= 100
+        assert e1[mv].mean() <= e0[mv].mean()
+        assert np.sum(in1) >= np.sum(in0)
 
 
 @pytest.mark.gpu
