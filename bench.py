@@ -107,6 +107,20 @@ def storage_effect(pydense, scene, iters: int, depth_f16) -> dict:
             "bit_equal": round(float(np.mean(depth_f16 == d32)), 5)}
 
 
+def host_cpu() -> str:
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def cpu_baseline(scene, iters: int, rows: int, out: dict = None, half: bool = False):
     """CPU oracle on a band of reference rows (all sources), init + iterations.
     ``half``: the images as the f16 storage holds them.  With ``out``, the
@@ -127,7 +141,7 @@ def cpu_baseline(scene, iters: int, rows: int, out: dict = None, half: bool = Fa
     if out is not None:
         out["band"], out["depth"] = band, depth
     units = sc.width * sc.height / 1e6 * band.num_src * iters
-    return {"value": units / dt, "unit": UNIT, "cores": po.max_threads(), "kind": "port",
+    return {"value": units / dt, "unit": UNIT, "cores": po.max_threads(), "kind": "port", "cpu": host_cpu(),
             "sample": f"reference rows {y0}..{y0 + rows} of the same {sc.width}x{scene.gt_depth.shape[0]} scene "
                       f"({sc.width}x{rows} px, "
                       f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
@@ -202,7 +216,7 @@ def run_c3(args, pydense) -> int:
         dt = time.perf_counter() - t1
         hh, ww = sc.gt_depth.shape
         u = ww * hh / 1e6 * sc.num_src * args.iters
-        cpu = {"value": round(u / dt, 5), "unit": UNIT, "cores": po.max_threads(), "kind": "port",
+        cpu = {"value": round(u / dt, 5), "unit": UNIT, "cores": po.max_threads(), "kind": "port", "cpu": host_cpu(),
                "sample": f"reference {small} of the walk ({ww}x{hh}, {sc.num_src} sources, {args.iters} iterations, "
                          f"init+iterations+decode), {dt:.1f} s"}
         m = d_o > 0
@@ -624,8 +638,8 @@ def main() -> int:
     ap.add_argument("--storage", choices=["f32", "f16"], default=None,
                     help="image storage (default f32; f16 for the c3 workload)")
     ap.add_argument("--cpu-rows", type=int, default=None,
-                    help="reference rows in the CPU-oracle sample (default 128 for c5, else 384; about 10 s on "
-                         "16 host threads; 0 = skip)")
+                    help="reference rows in the CPU-oracle sample (default 128 for c5, the whole frame for c4, "
+                         "else 384; about 10 s on 16 host threads; 0 = skip)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5",
                     help="c5 (default): the aerial survey, batched f16 PatchMatch + Gauss-Newton in memory-admitted "
                          "groups, with the plain single-view leg; c2: the plain single-view workload alone; c3: "
@@ -649,8 +663,8 @@ def main() -> int:
     args.storage_set = args.storage is not None
     if args.steps is None:
         args.steps = {"c5": 6, "cluster": 1}.get(args.workload, 5)
-    if args.cpu_rows is None:
-        args.cpu_rows = 128 if args.workload == "c5" else 384
+    if args.cpu_rows is None:  # c4: the whole 1024x768 frame
+        args.cpu_rows = {"c5": 128, "c4": 768}.get(args.workload, 384)
     if args.storage is None:
         args.storage = "f32"
     if args.gpus != 1:
