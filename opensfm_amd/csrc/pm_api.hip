@@ -1796,7 +1796,7 @@ namespace {
 // Stage of a kernel for the per-stage timing (-1: not attributed).
 int stage_of(int kernel) {
   switch (kernel) {
-    case PM_K_INIT: return PM_STAGE_INIT;
+    case PM_K_INIT_VIEWS: case PM_K_INIT: return PM_STAGE_INIT;
     case PM_K_CANDIDATES: return PM_STAGE_CANDIDATES;
     case PM_K_FALLBACK: return PM_STAGE_FALLBACK;
     case PM_K_SELECT: return PM_STAGE_SELECT;
@@ -1823,7 +1823,7 @@ int batch_slice(int it, int color, int phase) { return 1 + ((it * 2 + color) * 2
 // Launch view of the first `nrefs` references for `kernel` with tile width tile_x.
 PmBatch batch_view(const pm_batch* b, int kernel, int tile_x, int slice, int nrefs) {
   const int kind = grid_kind(kernel);
-  const int tile = (kind == PM_K_INIT || kind == PM_K_DECODE) ? 0 : tile_x;
+  const int tile = (kind == PM_K_INIT || kind == PM_K_INIT_VIEWS || kind == PM_K_DECODE) ? 0 : tile_x;
   const int n = static_cast<int>(b->refs.size());
   for (size_t i = 0; i < b->base_keys.size(); ++i) {
     if (b->base_keys[i].first == kind && b->base_keys[i].second == tile) {
@@ -1858,6 +1858,12 @@ int batch_launch(pm_batch* b, int kernel, int slice, int nrefs, int tile_x) {
   return PM_OK;
 }
 
+// Random init of the first nrefs references: per-view scoring, then top-k.
+int batch_init(pm_batch* b, int nrefs) {
+  if (int r = batch_launch(b, PM_K_INIT_VIEWS, 0, nrefs, 0)) return r;
+  return batch_launch(b, PM_K_INIT, 0, nrefs, 0);
+}
+
 // One colour pass of iteration `it` over the first nrefs references;
 // stage_ev (optional): 7 events bracketing the 6 kernels.
 int batch_pass(pm_batch* b, int it, int c, int nrefs, int cx, int px, hipEvent_t* stage_ev) {
@@ -1889,7 +1895,7 @@ int batch_priors(pm_batch* b) {
 int batch_schedule(pm_batch* b) {
   const int n = static_cast<int>(b->refs.size());
   if (b->level == 0) {
-    if (int r = batch_launch(b, PM_K_INIT, 0, n, 0)) return r;
+    if (int r = batch_init(b, n)) return r;
   } else {
     // finer levels start from each reference's coarser result
     // (UpsampleDepthNormal, estimator.cc:761-821): one small launch each
@@ -1953,7 +1959,7 @@ int batch_tune(pm_batch* b) {
     hipEvent_t ev[9];
     for (auto& e : ev) PM_HIP(hipEventCreate(&e), "hipEventCreate");
     auto trial = [&](int cx, int pxw, float* st) -> int {
-      if (int r = batch_launch(b, PM_K_INIT, 0, k, 0)) return r;
+      if (int r = batch_init(b, k)) return r;
       if (int r = batch_pass(b, 0, 0, k, cx, pxw, ev)) return r;
       PM_HIP(hipEventSynchronize(ev[6]), "tune sync");
       for (int j = 0; j < 6; ++j) PM_HIP(hipEventElapsedTime(&st[j], ev[j], ev[j + 1]), "event time");
@@ -2091,6 +2097,7 @@ int batch_prepare(pm_batch* b) {
   // Block offsets per grid kind and tile width.
   b->base_keys.clear();
   b->base_keys.emplace_back(PM_K_INIT, 0);
+  b->base_keys.emplace_back(PM_K_INIT_VIEWS, 0);
   b->base_keys.emplace_back(PM_K_DECODE, 0);
   for (int w : kCostTileWidths) {
     b->base_keys.emplace_back(PM_K_CANDIDATES, w);

@@ -209,14 +209,16 @@ constexpr int kViewChunk = PM_VIEW_CHUNK;
 
 // kLdsTaps: the pixel's stride-1 (w, w*r) are read from the workgroup's LDS
 // copy (lt[tap * kWave], see stage_tile_taps) instead of ref_taps.
+// Only the views [v_lo, v_hi) are scored (all by default).
 template <int HP, typename T, bool kLdsTaps = false>
 __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __restrict__ img, int x, int y, int cidx,
-                                                Plane pl, float* out, int64_t os, const float2* lt = nullptr) {
+                                                Plane pl, float* out, int64_t os, const float2* lt = nullptr,
+                                                int v_lo = 0, int v_hi = kMaxSrc) {
   constexpr int D = 2 * HP + 1;
-  const int n_src = a.n_src;
+  const int n_src = min(a.n_src, v_hi);
   PlanePts pp;
   if (!plane_points(kcam(a, 0), x, y, pl, pp)) {
-    for (int s = 0; s < n_src; ++s) out[s * os] = 2.0f;
+    for (int s = v_lo; s < n_src; ++s) out[s * os] = 2.0f;
     return 0u;
   }
   const float4 rs = gp(a.ref_sums)[cidx];
@@ -231,7 +233,7 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
   const uint32_t ucidx = static_cast<uint32_t>(cidx);
   uint32_t pending = 0u;
 #pragma unroll 1
-  for (int c0v = 0; c0v < n_src; c0v += kViewChunk) {
+  for (int c0v = v_lo; c0v < n_src; c0v += kViewChunk) {
     // Views of the chunk past n_src, or with an invalid warp, are sampled
     // anyway (coordinates are clamped, so every access stays inside the
     // view) and their sums discarded: the tap loop stays branch-free and
@@ -927,22 +929,53 @@ __device__ __forceinline__ Plane random_plane(const PmArgs& a, int x, int y, uin
   return plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, kcam(a, 0));
 }
 
+// The init runs in two kernels.  init_views: one wave per (source view,
+// 16x4 tile) scores the tile's random planes in that view (stride-1 NCC,
+// then the textureless fallback inline) into ca[view][cidx].  Each XCD takes
+// its band of tiles view by view, so the gathers in flight on one L2 are
+// those of a single source view: random depths scatter a band's patches
+// along that view's epipolar band, which then stays resident, instead of
+// along all views' bands at once.  init_select then reduces each pixel's
+// column of costs (top-k, selected views).  Every pixel regenerates its
+// plane from its seeded stream in every view's wave; the view-0 wave stores
+// it and the generator state.
 template <int HP, typename T>
-__device__ __forceinline__ void init_body(const PmArgs& a, int b) {
-  extern __shared__ float smem[];
-  const int lane = threadIdx.x;
-  int x, y;
-  if (!full_tile_pixel(a.width, a.height, b, x, y)) return;
+__device__ __forceinline__ void init_views_body(const PmArgs& a, int b) {
+  const int tiles_x = (a.width + kFullTileX - 1) / kFullTileX;
+  const int ntiles = tiles_x * ((a.height + kFullTileY - 1) / kFullTileY);
+  const int per = (ntiles + kXcds - 1) / kXcds;
+  const int q = b / kXcds;  // position in this XCD's sequence
+  const int v = q / per;
+  const int t = (b % kXcds) * per + (q - v * per);
+  if (v >= a.n_src || t >= ntiles) return;
+  const int x = (t % tiles_x) * kFullTileX + static_cast<int>(threadIdx.x % kFullTileX);
+  const int y = (t / tiles_x) * kFullTileY + static_cast<int>(threadIdx.x / kFullTileX);
+  if (x >= a.width || y >= a.height) return;
   const T* img = images_of<T>(a);
   const int idx = y * a.width + x;
   uint32_t s0 = static_cast<uint32_t>(idx) * 1099087573u + 2654435769u;
   uint32_t s1 = static_cast<uint32_t>(idx) * 2246822519u + 13u;
   const Plane plane = random_plane(a, x, y, s0, s1, static_cast<uint32_t>(idx + 7919));
-  a.planes[idx] = to_f4(plane);
-  a.rng[idx] = make_uint2(s0, s1);
+  if (v == 0) {
+    a.planes[idx] = to_f4(plane);
+    a.rng[idx] = make_uint2(s0, s1);
+  }
+  const int cidx = colour_index(a, x, y);
+  float* out = a.ca + cidx;
+  const uint32_t pending = cost_vector<HP>(a, img, x, y, cidx, plane, out, a.ref_plane, nullptr, v, v + 1);
+  resolve_fallbacks<HP>(a, img, x, y, cidx, plane, out, a.ref_plane, pending);
+}
+
+__device__ __forceinline__ void init_select_body(const PmArgs& a, int b) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  int x, y;
+  if (!full_tile_pixel(a.width, a.height, b, x, y)) return;
+  const int idx = y * a.width + x;
+  const int cidx = colour_index(a, x, y);
   float* cv = smem + lane;
   float* tmp = smem + a.n_src * kWave + lane;
-  full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), plane, cv);
+  for (int s = 0; s < a.n_src; ++s) cv[s * kWave] = a.ca[s * a.ref_plane + cidx];
   uint32_t sel = 0u;
   a.costs[idx] = initial_cost_and_views(a, cv, tmp, &sel);
   a.sel[idx] = sel;
@@ -1643,10 +1676,16 @@ __global__ __launch_bounds__(kWave) void refpatch_batch(PmBatch bt) {
 }
 
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) void init_kernel(PmArgs a) { init_body<HP, T>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave) void init_views_kernel(PmArgs a) { init_views_body<HP, T>(a, PM_BLOCK); }
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) void init_batch(PmBatch bt) {
-  batch_item(bt, [](const PmArgs& a, int b) { init_body<HP, T>(a, b); });
+__global__ __launch_bounds__(kWave) void init_views_batch(PmBatch bt) {
+  batch_item(bt, [](const PmArgs& a, int b) { init_views_body<HP, T>(a, b); });
+}
+template <int D = 0>
+__global__ __launch_bounds__(kWave) void init_select_kernel(PmArgs a) { init_select_body(a, PM_BLOCK); }
+template <int D = 0>
+__global__ __launch_bounds__(kWave) void init_select_batch(PmBatch bt) {
+  batch_item(bt, [](const PmArgs& a, int b) { init_select_body(a, b); });
 }
 
 template <int HP, typename T>

@@ -1,5 +1,5 @@
 // Launchers of the kernels templated on the image storage type (reference
-// patch, init, plane sweep, prior re-init, candidate and refinement costs,
+// patch, init scoring, plane sweep, prior re-init, candidate and refinement costs,
 // fallback).
 // Included once per storage type, by pm_cost_f32.hip and pm_cost_f16.hip,
 // with PM_STORAGE_T (the texel type) and PM_STORAGE_FN (the launcher name),
@@ -35,9 +35,9 @@ hipError_t launch_hp(int kernel, const PmArgs* a, const PmBatch* bt, dim3 grid, 
       if (a) hipLaunchKernelGGL((refpatch_kernel<HP, T>), grid, dim3(kWave), 0, s, *a);
       else hipLaunchKernelGGL((refpatch_batch<HP, T>), grid, dim3(kWave), 0, s, *bt);
       break;
-    case PM_K_INIT:
-      if (a) hipLaunchKernelGGL((init_kernel<HP, T>), grid, dim3(kWave), lds, s, *a);
-      else hipLaunchKernelGGL((init_batch<HP, T>), grid, dim3(kWave), lds, s, *bt);
+    case PM_K_INIT_VIEWS:
+      if (a) hipLaunchKernelGGL((init_views_kernel<HP, T>), grid, dim3(kWave), lds, s, *a);
+      else hipLaunchKernelGGL((init_views_batch<HP, T>), grid, dim3(kWave), lds, s, *bt);
       break;
     case PM_K_SWEEP:  // single-reference launches only
       if (!a) return hipErrorInvalidValue;

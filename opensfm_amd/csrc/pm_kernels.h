@@ -90,6 +90,7 @@ enum PmKernel {
   PM_K_FILTER,
   PM_K_DECODE,
   PM_K_SWEEP,
+  PM_K_INIT_VIEWS,  // the init's per-view scoring; PM_K_INIT is its per-pixel top-k
   PM_K_COUNT
 };
 

@@ -251,6 +251,9 @@ int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x) 
     case PM_K_SWEEP:
       return xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
                        ((height + kFullTileY - 1) / kFullTileY));
+    case PM_K_INIT_VIEWS:  // one workgroup per (source view, full tile)
+      return static_cast<int64_t>(n_src) * xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
+                                                     ((height + kFullTileY - 1) / kFullTileY));
     case PM_K_FALLBACK: {
       return xcd_round(colour_tiles(width, height, tile_x));  // one workgroup per colour tile
     }
@@ -345,13 +348,22 @@ hipError_t launch_refpatch(const PmArgs& a, int hp, bool f16, hipStream_t stream
   return launch_storage(f16, PM_K_REFPATCH, &a, nullptr, blocks_of(PM_K_REFPATCH, a, 0), 0, hp, 0, stream);
 }
 
-// Waves per SIMD of the random init and the plane sweep (one-wave
-// workgroups; 0 = no cap).  Their random planes gather with no locality, and
-// fewer waves with more loads in flight each thrash L1/L2 less: at
-// 1600x1200, N=8 the init takes 7.8 ms uncapped (4 waves, 128 VGPRs),
-// 6.9 ms at 3, 6.3 ms at 2 and 6.9 ms at 1 (f32; f16 5.4 / 4.8 / 4.6 / 6.8).
+// Waves per SIMD of the plane sweep and of the init's per-view scoring
+// (one-wave workgroups; 0 = no cap).  Their planes gather along whole
+// epipolar segments, and fewer waves with more loads in flight each thrash
+// L1/L2 less.  The sweep keeps the cap measured on the fused init (at
+// 1600x1200, N=8: 7.8 ms uncapped, 6.9 ms at 3, 6.3 ms at 2, f32).  The
+// view-major init scoring takes 5.35 / 5.20 / 5.35 / 5.35 ms per step at
+// 2 / 3 / 4 / no cap on float32 images and 5.24 / 4.48 / 4.36 / 4.36 ms on
+// float16 images.
 #ifndef PM_INIT_WAVES
 #define PM_INIT_WAVES 2
+#endif
+#ifndef PM_INIT_VIEWS_WAVES_F32
+#define PM_INIT_VIEWS_WAVES_F32 3
+#endif
+#ifndef PM_INIT_VIEWS_WAVES_F16
+#define PM_INIT_VIEWS_WAVES_F16 4
 #endif
 // LDS bytes per CU of the current device (160 KiB on MI355X), queried once
 // per device; the occupancy caps below reserve a share of it per workgroup.
@@ -376,17 +388,23 @@ size_t lds_per_cu() {
   return static_cast<size_t>(bytes);
 }
 
-// Dynamic LDS of an init or sweep launch: its cost columns, or more to cap
-// its one-wave workgroups at PM_INIT_WAVES per SIMD.
-size_t init_lds(int n_src) {
-  const size_t cols = cost_column_lds(n_src);
+// Dynamic LDS of a launch of one-wave workgroups: `cols` bytes of cost
+// columns, or more to cap the workgroups at `waves` per SIMD (0: no cap).
+size_t capped_lds(size_t cols, int waves) {
   const size_t lds = lds_per_cu();
-  const size_t cap = PM_INIT_WAVES > 0 && lds > 0 ? lds / (static_cast<size_t>(kSimdsPerCu) * PM_INIT_WAVES + 1) + 1 : 0;
+  const size_t cap = waves > 0 && lds > 0 ? lds / (static_cast<size_t>(kSimdsPerCu) * waves + 1) + 1 : 0;
   return cols > cap ? cols : cap;
 }
+size_t init_lds(int n_src) { return capped_lds(cost_column_lds(n_src), PM_INIT_WAVES); }
+size_t init_views_lds(bool f16) { return capped_lds(0, f16 ? PM_INIT_VIEWS_WAVES_F16 : PM_INIT_VIEWS_WAVES_F32); }
 
 hipError_t launch_init(const PmArgs& a, int hp, bool f16, hipStream_t stream) {
-  return launch_storage(f16, PM_K_INIT, &a, nullptr, blocks_of(PM_K_INIT, a, 0), init_lds(a.n_src), hp, 0, stream);
+  const hipError_t e = launch_storage(f16, PM_K_INIT_VIEWS, &a, nullptr, blocks_of(PM_K_INIT_VIEWS, a, 0),
+                                      init_views_lds(f16), hp, 0, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(init_select_kernel<>, dim3(blocks_of(PM_K_INIT, a, 0)), dim3(kWave), cost_column_lds(a.n_src),
+                     stream, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_plane_sweep(const PmArgs& a, int hp, bool f16, int num_planes, hipStream_t stream) {
@@ -481,8 +499,11 @@ hipError_t launch_batch(int kernel, const PmBatch& bt, unsigned grid, int hp, bo
   switch (kernel) {
     case PM_K_REFPATCH:
       return launch_storage(f16, kernel, nullptr, &bt, grid, 0, hp, 0, stream);
+    case PM_K_INIT_VIEWS:
+      return launch_storage(f16, kernel, nullptr, &bt, grid, init_views_lds(f16), hp, 0, stream);
     case PM_K_INIT:
-      return launch_storage(f16, kernel, nullptr, &bt, grid, init_lds(max_src), hp, 0, stream);
+      hipLaunchKernelGGL(init_select_batch<>, dim3(grid), dim3(kWave), cost_column_lds(max_src), stream, bt);
+      return hipGetLastError();
     case PM_K_PRIOR:
       return launch_storage(f16, kernel, nullptr, &bt, grid, cost_column_lds(max_src), hp, 0, stream);
     case PM_K_CANDIDATES:

@@ -19,8 +19,47 @@ import sqlite3
 import sys
 
 
+_TYPES = {"f": "float", "d": "double", "i": "int", "j": "unsigned int", "DF16_": "_Float16"}
+
+
+def _demangle(name: str) -> str:
+    """Readable name of a mangled kernel symbol (rocprofv3 leaves kernels with
+    _Float16 template arguments mangled): the last component of the nested
+    name with its template arguments (int literals and the scalar types
+    above), or the symbol unchanged when it has another form."""
+    if not name.startswith("_ZN"):
+        return name
+    pos, last = 3, None
+    while pos < len(name) and name[pos].isdigit():
+        n = re.match(r"\d+", name[pos:]).group(0)
+        pos += len(n)
+        last = name[pos:pos + int(n)]
+        pos += int(n)
+    if last is None:
+        return name
+    args = []
+    if name.startswith("I", pos):
+        pos += 1
+        while pos < len(name) and name[pos] != "E":
+            lit = re.match(r"Li(-?\d+)E", name[pos:])
+            if lit:
+                args.append(lit.group(1))
+                pos += len(lit.group(0))
+                continue
+            for code, tname in _TYPES.items():
+                if name.startswith(code, pos):
+                    args.append(tname)
+                    pos += len(code)
+                    break
+            else:
+                return name
+    targs = f"<{', '.join(args)}>" if args else ""
+    return f"pmhip::{last}{targs}("
+
+
 def kernel_name(full: str) -> str:
     """Template-qualified name without the argument list."""
+    full = _demangle(full)
     m = re.search(r"::((?:\w+)(?:<[^()]*>)?)\(", full)
     name = m.group(1) if m else full
     return name[:80]
