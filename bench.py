@@ -147,6 +147,31 @@ def cpu_baseline(scene, iters: int, rows: int, out: dict = None, half: bool = Fa
                       f"{band.num_src} sources, {iters} iterations, init+iterations+decode), {dt:.1f} s"}
 
 
+def fp64_error(pydense, band, iters: int) -> dict:
+    """Depth of ``band`` run on the GPU with float32 and with float16 image
+    storage, against the FP64 oracle on the same band (the float32 texels
+    widened exactly, float64 cameras, double arithmetic throughout)."""
+    from oracle import pm_oracle as po
+    p = po.Params(patch_size=PATCH, depth_min=band.depth_min, depth_max=band.depth_max, max_iterations=iters)
+    imgs = [np.asarray(im, np.float32) for im in band.images]
+    t0 = time.perf_counter()
+    d64, _, _, _ = po.estimate_f64(imgs, band.Ks, band.Rs, band.ts, p)
+    dt = time.perf_counter() - t0
+    h, w = d64.shape
+    m = d64 > 0
+    out = {"reference": f"FP64 oracle on the CPU-baseline rows ({w}x{h}, {band.num_src} sources, {iters} "
+                        f"iterations), {dt:.1f} s", "valid_pixels": int(m.sum())}
+    for storage in ("f32", "f16"):
+        e = pydense.DepthmapEstimator()
+        configure(e, band, iters, storage)
+        d = e.compute()["depth"]
+        rel = np.abs(d[m] - d64[m]) / d64[m]
+        out[storage] = {"within_1pct": round(float(np.mean(rel < 0.01)), 5),
+                        "median_rel": float(f"{np.median(rel):.3g}"), "mean_rel": float(f"{np.mean(rel):.3g}")}
+    log(f"FP64 depth error: f32 {out['f32']}, f16 {out['f16']}")
+    return out
+
+
 def run_c3(args, pydense) -> int:
     """Many references of mixed resolution in one batched, graph-captured run."""
     import torch
@@ -206,7 +231,7 @@ def run_c3(args, pydense) -> int:
     # CPU oracle on the smallest reference, and parity of the batched result there
     small = min(range(len(ws.refs)), key=lambda i: ws.images[ws.refs[i][0]].size * len(ws.refs[i][1]))
     sc = ws.reference(small)
-    cpu, parity = None, None
+    cpu, parity, fp64 = None, None, None
     if args.cpu_rows > 0:
         log(f"CPU oracle on reference {small} ...")
         p = po.Params(patch_size=PATCH, depth_min=sc.depth_min, depth_max=sc.depth_max, max_iterations=args.iters)
@@ -223,6 +248,8 @@ def run_c3(args, pydense) -> int:
         parity = {"reference": small, "within_1pct_of_oracle": round(float(np.mean(
             np.abs(outs[small][m] - d_o[m]) < 0.01 * d_o[m])), 5),
             "bit_equal": round(float(np.mean(outs[small] == d_o)), 5)}
+        fp64 = fp64_error(pydense, sc, args.iters)
+        fp64["reference"] += f" (reference {small})"
 
     f16_effect = storage_effect(pydense, sc, args.iters, outs[small]) if storage == "f16" else None
     if f16_effect:
@@ -247,6 +274,7 @@ def run_c3(args, pydense) -> int:
                   "tune_log": batch.tune_log().round(3).tolist()},
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp64_depth_error": fp64,
         "accuracy": {"gt_within_1pct": round(hit / max(tot, 1), 4), "gt_within_5pct": round(hit5 / max(tot, 1), 4)},
         "f16_storage_vs_f32": f16_effect,
     }
@@ -400,6 +428,7 @@ def plain_leg(pydense, steps: int, warmup: int, cpu_rows: int, seed: int = 2, it
         out["parity"] = {"sample": "the CPU-baseline rows on the GPU", "bit_equal": round(float(np.mean(
             d_band == keep["depth"])), 6), "within_1pct_of_oracle": round(float(np.mean(
                 np.abs(d_band[m] - keep["depth"][m]) < 0.01 * keep["depth"][m])), 6)}
+        out["fp64_depth_error"] = fp64_error(pydense, keep["band"], iters)
     del dev
     torch.cuda.empty_cache()
     return out
@@ -537,7 +566,7 @@ def run_c5(args, pydense) -> int:
     pcie_value = per_ref_units * group / (time.perf_counter() - t1)
     hviews.clear()
 
-    cpu, parity = None, None
+    cpu, parity, fp64 = None, None, None
     if args.cpu_rows > 0:
         log(f"CPU oracle on {args.cpu_rows} rows of reference 0 ...")
         v, srcs, dmin, dmax = ws.refs[0]
@@ -556,6 +585,7 @@ def run_c5(args, pydense) -> int:
                   "bit_equal": round(float(np.mean(d == keep["depth"])), 5),
                   "within_1pct_of_oracle": round(float(np.mean(np.abs(d[m] - keep["depth"][m])
                                                                < 0.01 * keep["depth"][m])), 5)}
+        fp64 = fp64_error(pydense, keep["band"], args.iters)
     del ws
     torch.cuda.empty_cache()
 
@@ -585,6 +615,7 @@ def run_c5(args, pydense) -> int:
                           "colour_pass_ms": round(sweep_ms, 3)},
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp64_depth_error": fp64,
         "patchmatch_device_value": round(units / (pm_ms / 1e3), 3) if pm_ms > 0 else None,
         "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
         "refine": {"steps": args.refine_steps, "ms": round(gn_ms, 3),
@@ -780,7 +811,7 @@ def main() -> int:
                   "gt_within_1pct_after": round(acc_a[-1], 4)}
         log(f"Gauss-Newton {args.refine_steps} steps: {ms:.2f} ms")
 
-    cpu, parity = None, None
+    cpu, parity, fp64 = None, None, None
     if args.cpu_rows > 0:
         log(f"CPU oracle on {args.cpu_rows} rows ...")
         keep = {}
@@ -795,6 +826,7 @@ def main() -> int:
         parity = {"sample": "the CPU-baseline rows on the GPU, same inputs",
                   "bit_equal": round(float(np.mean(d_band == d_o)), 6),
                   "within_1pct_of_oracle": round(float(np.mean(np.abs(d_band[m] - d_o[m]) < 0.01 * d_o[m])), 6)}
+        fp64 = fp64_error(pydense, keep["band"], args.iters)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -809,6 +841,7 @@ def main() -> int:
         "roofline_path": roofline_path,
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp64_depth_error": fp64,
         "accuracy": {"gt_within_1pct": round(acc1, 4), "gt_within_5pct": round(acc5, 4)},
         "stage_ms_per_step": stage_ms,
         "pcie_inclusive_value": round(pcie_value, 3),

@@ -49,11 +49,23 @@
 #include <vector>
 #include <omp.h>
 
+// Arithmetic type.  The default build computes in float under the numerics
+// contract above.  The FP64 build (-DPMO_F64, libpm_oracle64.so) runs the
+// same algorithm with every value, array and interface field in double and
+// libm's exp and log: the float64 evaluation that float32 and
+// float16-storage results are measured against.  The random generator and
+// the contract's bit-level helpers stay float in both.
+#ifdef PMO_F64
+typedef double real;
+#else
+typedef float real;
+#endif
+
 namespace {
 
 constexpr int kMaxSrc = 16;   // reference: MAX_SOURCES 8 (opencl_kernels.h:26), lifted
 constexpr int kMcSamples = 32;  // MC_NUM_SAMPLES (opencl_kernels.h:910)
-constexpr float kPiF = 3.14159274101257324219f;  // M_PI_F
+constexpr real kPiF = 3.14159274101257324219f;  // M_PI_F
 
 }  // namespace
 
@@ -62,9 +74,9 @@ extern "C" {
 // Camera layout of the reference's CLCamera (opencl_utils.h:258-280; kernel
 // struct opencl_kernels.h:62-68): K and R row-major, t, size.  92 bytes.
 typedef struct {
-  float K[9];
-  float R[9];
-  float t[3];
+  real K[9];
+  real R[9];
+  real t[3];
   int width;
   int height;
 } pmo_camera;
@@ -72,37 +84,37 @@ typedef struct {
 typedef struct {
   int width, height;          // reference view size
   int num_src;                // number of source views (<= 16)
-  const float* images;        // all views, row-major, concatenated (ref first)
+  const real* images;        // all views, row-major, concatenated (ref first)
   const int64_t* offsets;     // num_src+1 element offsets into images
   const int* widths;          // num_src+1
   const int* heights;         // num_src+1
   const pmo_camera* cams;     // num_src+1 (index 0 = reference)
-  const float* angle_weights; // num_src (estimator.cc:380-407)
+  const real* angle_weights; // num_src (estimator.cc:380-407)
 } pmo_scene;
 
 typedef struct {
   int half_patch;
-  float sigma_spatial;
-  float sigma_color;
+  real sigma_spatial;
+  real sigma_color;
   int top_k;
-  float census_weight;   // use_census ? 1 : 0 (estimator.cc:536)
-  float depth_min, depth_max;
+  real census_weight;   // use_census ? 1 : 0 (estimator.cc:536)
+  real depth_min, depth_max;
   int max_iterations;
-  float smooth_weight;
+  real smooth_weight;
   int anchor_views;
-  float geom_weight;
-  float var_threshold;   // 1.5e-6 in the reference (opencl_kernels.h:254,516,519)
+  real geom_weight;
+  real var_threshold;   // 1.5e-6 in the reference (opencl_kernels.h:254,516,519)
   int checkerboard_filter;
 } pmo_params;
 
 typedef struct {
-  float* planes;               // 4 * npix (nx, ny, nz, d)
-  float* costs;                // npix
+  real* planes;               // 4 * npix (nx, ny, nz, d)
+  real* costs;                // npix
   uint32_t* rng;               // 2 * npix
   uint32_t* sel;               // npix
-  const float* prior_planes;   // 4 * npix or null
+  const real* prior_planes;   // 4 * npix or null
   const uint32_t* plane_masks; // npix or null
-  const float* prev_depths;    // num_src * npix (ref resolution) or null
+  const real* prev_depths;    // num_src * npix (ref resolution) or null
   uint32_t prev_depth_mask;
 } pmo_state;
 
@@ -167,18 +179,27 @@ inline float pm_logf(float x) {
   return std::fmaf(fe, 0.693145751953125f, std::fmaf(fe, 1.42860682030941723212e-6f, lm));
 }
 
-struct V3 { float x, y, z; };
-inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline float len3(V3 a) { return std::sqrt(dot3(a, a)); }
+inline real rfma(real a, real b, real c) { return std::fma(a, b, c); }
+#ifdef PMO_F64
+inline real rexp(real x) { return std::exp(x); }
+inline real rlog(real x) { return std::log(x); }
+#else
+inline real rexp(real x) { return pm_expf(x); }
+inline real rlog(real x) { return pm_logf(x); }
+#endif
+
+struct V3 { real x, y, z; };
+inline real dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline real len3(V3 a) { return std::sqrt(dot3(a, a)); }
 // OpenCL normalize(): all-zero input is returned unchanged.
 inline V3 normalize3(V3 a) {
-  const float l = len3(a);
+  const real l = len3(a);
   if (l == 0.0f) return a;
   return V3{a.x / l, a.y / l, a.z / l};
 }
-inline float clampf(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), hi); }
+inline real clampf(real v, real lo, real hi) { return std::fmin(std::fmax(v, lo), hi); }
 
-struct Plane { float x, y, z, w; };
+struct Plane { real x, y, z, w; };
 
 // ---------------------------------------------------------------------------
 // RNG: opencl_kernels.h:78-97.
@@ -212,8 +233,8 @@ inline V3 project(V3 X, const pmo_camera& c) {
   return p;
 }
 
-inline V3 backproject(float x, float y, float depth, const pmo_camera& c) {
-  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+inline V3 backproject(real x, real y, real depth, const pmo_camera& c) {
+  const real fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
   V3 pc{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
   V3 d{pc.x - c.t[0], pc.y - c.t[1], pc.z - c.t[2]};
   V3 w;
@@ -223,17 +244,17 @@ inline V3 backproject(float x, float y, float depth, const pmo_camera& c) {
   return w;
 }
 
-inline float depth_from_plane(float x, float y, Plane p, const pmo_camera& c) {
-  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
-  const float denom = p.x * (x - cx) / fx + p.y * (y - cy) / fy + p.z;
+inline real depth_from_plane(real x, real y, Plane p, const pmo_camera& c) {
+  const real fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+  const real denom = p.x * (x - cx) / fx + p.y * (y - cy) / fy + p.z;
   if (std::fabs(denom) < 1e-6f) return 0.0f;
   return -p.w / denom;
 }
 
-inline Plane plane_from_depth_normal(float x, float y, float depth, V3 n, const pmo_camera& c) {
-  const float fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
+inline Plane plane_from_depth_normal(real x, real y, real depth, V3 n, const pmo_camera& c) {
+  const real fx = c.K[0], fy = c.K[4], cx = c.K[2], cy = c.K[5];
   V3 pt{depth * (x - cx) / fx, depth * (y - cy) / fy, depth};
-  const float d = -dot3(n, pt);
+  const real d = -dot3(n, pt);
   return Plane{n.x, n.y, n.z, d};
 }
 
@@ -241,9 +262,9 @@ inline Plane plane_from_depth_normal(float x, float y, float depth, V3 n, const 
 // Image access.
 // ---------------------------------------------------------------------------
 struct Img {
-  const float* data;
+  const real* data;
   int w, h;
-  inline float texel(int ix, int iy) const {
+  inline real texel(int ix, int iy) const {
     ix = std::min(std::max(ix, 0), w - 1);
     iy = std::min(std::max(iy, 0), h - 1);
     return data[static_cast<int64_t>(iy) * w + ix];
@@ -251,18 +272,18 @@ struct Img {
   // Bilinear at a pixel-centre coordinate, clamp-to-edge.  Coordinates are
   // first clamped to [-1, w-1] x [-1, h-1]: outside that range both taps of
   // an axis hit the same edge texel, so the clamp does not change the value
-  // (lerp(a, a, t) == a exactly) and keeps the float->int conversion defined.
-  inline float bilinear(float sx, float sy) const {
-    sx = clampf(sx, -1.0f, static_cast<float>(w - 1));
-    sy = clampf(sy, -1.0f, static_cast<float>(h - 1));
-    const float fx0 = std::floor(sx), fy0 = std::floor(sy);
-    const float ax = sx - fx0, ay = sy - fy0;
+  // (lerp(a, a, t) == a exactly) and keeps the real->int conversion defined.
+  inline real bilinear(real sx, real sy) const {
+    sx = clampf(sx, -1.0f, static_cast<real>(w - 1));
+    sy = clampf(sy, -1.0f, static_cast<real>(h - 1));
+    const real fx0 = std::floor(sx), fy0 = std::floor(sy);
+    const real ax = sx - fx0, ay = sy - fy0;
     const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
-    const float t00 = texel(x0, y0), t01 = texel(x0 + 1, y0);
-    const float t10 = texel(x0, y0 + 1), t11 = texel(x0 + 1, y0 + 1);
-    const float v0 = std::fmaf(ax, t01 - t00, t00);
-    const float v1 = std::fmaf(ax, t11 - t10, t10);
-    return std::fmaf(ay, v1 - v0, v0);
+    const real t00 = texel(x0, y0), t01 = texel(x0 + 1, y0);
+    const real t10 = texel(x0, y0 + 1), t11 = texel(x0 + 1, y0 + 1);
+    const real v0 = rfma(ax, t01 - t00, t00);
+    const real v1 = rfma(ax, t11 - t10, t10);
+    return rfma(ay, v1 - v0, v0);
   }
 };
 
@@ -273,7 +294,7 @@ struct Ctx {
   Img src[kMaxSrc];
   const pmo_camera* cams;
   int n_src;
-  float spatial_factor, color_factor;
+  real spatial_factor, color_factor;
   Ctx(const pmo_scene* s, const pmo_params* p) : sc(s), pr(p) {
     ref = Img{s->images + s->offsets[0], s->widths[0], s->heights[0]};
     n_src = std::min(s->num_src, kMaxSrc);
@@ -289,26 +310,26 @@ struct Ctx {
 // Reference patch: opencl_kernels.h:42-49, 328-366.
 // ---------------------------------------------------------------------------
 struct RefPatch {
-  float ref_center, sum_w, sum_wr, sum_wrr;
-  float w[(2 * 7 + 1) * (2 * 7 + 1)];
-  float wr[(2 * 7 + 1) * (2 * 7 + 1)];
+  real ref_center, sum_w, sum_wr, sum_wrr;
+  real w[(2 * 7 + 1) * (2 * 7 + 1)];
+  real wr[(2 * 7 + 1) * (2 * 7 + 1)];
 };
 
 void compute_ref_patch(const Ctx& c, int x, int y, RefPatch* rp) {
   const int hp = c.pr->half_patch;
-  const float center = c.ref.texel(x, y);
-  float sum_w = 0.0f, sum_wr = 0.0f, sum_wrr = 0.0f;
+  const real center = c.ref.texel(x, y);
+  real sum_w = 0.0f, sum_wr = 0.0f, sum_wrr = 0.0f;
   int p = 0;
   for (int iy = -hp; iy <= hp; ++iy) {
     for (int ix = -hp; ix <= hp; ++ix) {
-      const float rv = c.ref.texel(x + ix, y + iy);
-      const float r = rv - center;
-      const float w = pm_expf(c.spatial_factor * static_cast<float>(ix * ix + iy * iy) +
+      const real rv = c.ref.texel(x + ix, y + iy);
+      const real r = rv - center;
+      const real w = rexp(c.spatial_factor * static_cast<real>(ix * ix + iy * iy) +
                               c.color_factor * r * r);
       sum_w += w;
-      const float wr = w * r;
+      const real wr = w * r;
       sum_wr += wr;
-      sum_wrr = std::fmaf(wr, r, sum_wrr);
+      sum_wrr = rfma(wr, r, sum_wrr);
       rp->w[p] = w;
       rp->wr[p] = wr;
       ++p;
@@ -321,60 +342,60 @@ void compute_ref_patch(const Ctx& c, int x, int y, RefPatch* rp) {
 }
 
 // Strided ZNCC fallback: opencl_kernels.h:188-258.
-float ncc_at_stride(const Ctx& c, int s, float ref_center, float src_center, float scx, float scy,
-                    int cx, int cy, float stride, float jxx, float jxy, float jyx, float jyy) {
+real ncc_at_stride(const Ctx& c, int s, real ref_center, real src_center, real scx, real scy,
+                    int cx, int cy, real stride, real jxx, real jxy, real jyx, real jyy) {
   const int hp = c.pr->half_patch;
-  float sum_w = 0.0f, sum_r = 0.0f, sum_s = 0.0f, sum_rr = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
+  real sum_w = 0.0f, sum_r = 0.0f, sum_s = 0.0f, sum_rr = 0.0f, sum_ss = 0.0f, sum_rs = 0.0f;
   for (int iy = -hp; iy <= hp; ++iy) {
-    const float fdy = stride * static_cast<float>(iy);
+    const real fdy = stride * static_cast<real>(iy);
     // source coordinates and sums contracted as the reference's
     // -cl-fast-relaxed-math build allows (opencl_utils.h:84-85)
-    const float bx = std::fmaf(jyx, fdy, scx);
-    const float by = std::fmaf(jyy, fdy, scy);
+    const real bx = rfma(jyx, fdy, scx);
+    const real by = rfma(jyy, fdy, scy);
     for (int ix = -hp; ix <= hp; ++ix) {
-      const float fdx = stride * static_cast<float>(ix);
-      const float rx = static_cast<float>(cx) + fdx;
-      const float ry = static_cast<float>(cy) + fdy;
-      const float ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));
-      const float sx = std::fmaf(jxx, fdx, bx);
-      const float sy = std::fmaf(jxy, fdx, by);
-      const float src_val = c.src[s].bilinear(sx, sy);
-      const float r = ref_val - ref_center;
-      const float w = pm_expf(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +
+      const real fdx = stride * static_cast<real>(ix);
+      const real rx = static_cast<real>(cx) + fdx;
+      const real ry = static_cast<real>(cy) + fdy;
+      const real ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));
+      const real sx = rfma(jxx, fdx, bx);
+      const real sy = rfma(jxy, fdx, by);
+      const real src_val = c.src[s].bilinear(sx, sy);
+      const real r = ref_val - ref_center;
+      const real w = rexp(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +
                               r * r * c.color_factor);
-      const float sv = src_val - src_center;
-      const float wsv = w * sv;
+      const real sv = src_val - src_center;
+      const real wsv = w * sv;
       sum_r += w * r;
       sum_s += wsv;
       sum_rr += w * r * r;
-      sum_ss = std::fmaf(wsv, sv, sum_ss);
-      sum_rs = std::fmaf(w * r, sv, sum_rs);
+      sum_ss = rfma(wsv, sv, sum_ss);
+      sum_rs = rfma(w * r, sv, sum_rs);
       sum_w += w;
     }
   }
   if (sum_w < 1e-6f) return -1.0f;
-  const float inv_w = 1.0f / sum_w;
-  const float mean_r = sum_r * inv_w;
-  const float mean_s = sum_s * inv_w;
-  const float var_ref = sum_rr * inv_w - mean_r * mean_r;
-  const float var_src = sum_ss * inv_w - mean_s * mean_s;
+  const real inv_w = 1.0f / sum_w;
+  const real mean_r = sum_r * inv_w;
+  const real mean_s = sum_s * inv_w;
+  const real var_ref = sum_rr * inv_w - mean_r * mean_r;
+  const real var_src = sum_ss * inv_w - mean_s * mean_s;
   if (var_ref < c.pr->var_threshold || var_src < c.pr->var_threshold) return -1.0f;
-  const float covar = sum_rs * inv_w - mean_r * mean_s;
+  const real covar = sum_rs * inv_w - mean_r * mean_s;
   return covar / std::sqrt(var_ref * var_src);
 }
 
 // Census fallback: opencl_kernels.h:270-315.
-float census_inner(const Ctx& c, int s, float ref_center, float src_center, int cx, int cy,
-                   float scx, float scy, float jxx, float jxy, float jyx, float jyy) {
+real census_inner(const Ctx& c, int s, real ref_center, real src_center, int cx, int cy,
+                   real scx, real scy, real jxx, real jxy, real jyx, real jyy) {
   uint32_t rc = 0u, sc = 0u;
   int bit = 0;
   for (int dy = -2; dy <= 2; ++dy) {
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
-      const float ref_val = c.ref.texel(cx + dx, cy + dy);
-      const float sx = scx + jxx * static_cast<float>(dx) + jyx * static_cast<float>(dy);
-      const float sy = scy + jxy * static_cast<float>(dx) + jyy * static_cast<float>(dy);
-      const float src_val = c.src[s].bilinear(sx, sy);
+      const real ref_val = c.ref.texel(cx + dx, cy + dy);
+      const real sx = scx + jxx * static_cast<real>(dx) + jyx * static_cast<real>(dy);
+      const real sy = scy + jxy * static_cast<real>(dx) + jyy * static_cast<real>(dy);
+      const real src_val = c.src[s].bilinear(sx, sy);
       if (ref_val > ref_center) rc |= (1u << bit);
       if (src_val > src_center) sc |= (1u << bit);
       ++bit;
@@ -382,21 +403,21 @@ float census_inner(const Ctx& c, int s, float ref_center, float src_center, int 
   }
   const int hamming = __builtin_popcount(rc ^ sc);
   if (__builtin_popcount(rc) < 3 && __builtin_popcount(sc) < 3) return 1.0f;
-  return static_cast<float>(hamming) / 24.0f;
+  return static_cast<real>(hamming) / 24.0f;
 }
 
 // The plane's points at (x, y), (x+1, y) and (x, y+1) in the world
 // (opencl_kernels.h:401-416); false when a depth is <= 0 (cost 2.0).
 struct PlanePoints { V3 cw, wdx, wdy; };
 bool plane_points(const pmo_camera& c0, int x, int y, Plane plane, PlanePoints* pp) {
-  const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane, c0);
+  const real depth = depth_from_plane(static_cast<real>(x), static_cast<real>(y), plane, c0);
   if (depth <= 0.0f) return false;
-  const float depth_dx = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), plane, c0);
-  const float depth_dy = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), plane, c0);
+  const real depth_dx = depth_from_plane(static_cast<real>(x + 1), static_cast<real>(y), plane, c0);
+  const real depth_dy = depth_from_plane(static_cast<real>(x), static_cast<real>(y + 1), plane, c0);
   if (depth_dx <= 0.0f || depth_dy <= 0.0f) return false;
-  pp->cw = backproject(static_cast<float>(x), static_cast<float>(y), depth, c0);
-  pp->wdx = backproject(static_cast<float>(x + 1), static_cast<float>(y), depth_dx, c0);
-  pp->wdy = backproject(static_cast<float>(x), static_cast<float>(y + 1), depth_dy, c0);
+  pp->cw = backproject(static_cast<real>(x), static_cast<real>(y), depth, c0);
+  pp->wdx = backproject(static_cast<real>(x + 1), static_cast<real>(y), depth_dx, c0);
+  pp->wdy = backproject(static_cast<real>(x), static_cast<real>(y + 1), depth_dy, c0);
   return true;
 }
 
@@ -405,7 +426,7 @@ bool plane_points(const pmo_camera& c0, int x, int y, Plane plane, PlanePoints* 
 // (sx, sy) = (scx + jxx*dx + jyx*dy, scy + jxy*dx + jyy*dy) for a reference
 // offset (dx, dy).  0: valid; 1: a point is behind the source camera;
 // 2: det J outside [0.02, 50].  (Both invalid cases cost 2.0.)
-struct Warp { float scx, scy, jxx, jxy, jyx, jyy; };
+struct Warp { real scx, scy, jxx, jxy, jyx, jyy; };
 int source_warp(const PlanePoints& pp, const pmo_camera& cs, Warp* w) {
   const V3 pc = project(pp.cw, cs);
   if (pc.z < 1e-6f) return 1;
@@ -418,7 +439,7 @@ int source_warp(const PlanePoints& pp, const pmo_camera& cs, Warp* w) {
   w->jxy = pdx.y / pdx.z - w->scy;
   w->jyx = pdy.x / pdy.z - w->scx;
   w->jyy = pdy.y / pdy.z - w->scy;
-  const float det = w->jxx * w->jyy - w->jxy * w->jyx;
+  const real det = w->jxx * w->jyy - w->jxy * w->jyx;
   return (det < 0.02f || det > 50.0f) ? 2 : 0;
 }
 
@@ -426,7 +447,7 @@ int source_warp(const PlanePoints& pp, const pmo_camera& cs, Warp* w) {
 // The reference loops taps outer / views inner; each view's sums are
 // accumulated over the taps in the same order either way, so the oracle
 // loops views outer.
-int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, float* out) {
+int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, real* out) {
   const int n_src = c.n_src;
   const int hp = c.pr->half_patch;
   PlanePoints pp;
@@ -436,7 +457,7 @@ int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, flo
   }
 
   const bool ref_ok = rp.sum_w >= 1e-6f;
-  float inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
+  real inv_w = 0.0f, mean_r = 0.0f, var_ref = 0.0f;
   if (ref_ok) {
     inv_w = 1.0f / rp.sum_w;
     mean_r = rp.sum_wr * inv_w;
@@ -446,46 +467,46 @@ int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, flo
   for (int s = 0; s < n_src; ++s) {
     Warp wp;
     if (source_warp(pp, c.cams[s + 1], &wp) != 0) { out[s] = 2.0f; continue; }
-    const float scx = wp.scx, scy = wp.scy, jxx = wp.jxx, jxy = wp.jxy, jyx = wp.jyx, jyy = wp.jyy;
-    const float scen = c.src[s].bilinear(scx, scy);
+    const real scx = wp.scx, scy = wp.scy, jxx = wp.jxx, jxy = wp.jxy, jyx = wp.jyx, jyy = wp.jyy;
+    const real scen = c.src[s].bilinear(scx, scy);
 
     // Fused stride-1 ZNCC over the cached reference weights (:471-499).
-    float sum_ws = 0.0f, sum_wss = 0.0f, sum_wrs = 0.0f;
+    real sum_ws = 0.0f, sum_wss = 0.0f, sum_wrs = 0.0f;
     int p = 0;
     for (int iy = -hp; iy <= hp; ++iy) {
-      const float fiy = static_cast<float>(iy);
-      const float bx = std::fmaf(jyx, fiy, scx);
-      const float by = std::fmaf(jyy, fiy, scy);
+      const real fiy = static_cast<real>(iy);
+      const real bx = rfma(jyx, fiy, scx);
+      const real by = rfma(jyy, fiy, scy);
       for (int ix = -hp; ix <= hp; ++ix) {
-        const float fix = static_cast<float>(ix);
-        const float w = rp.w[p];
-        const float wr = rp.wr[p];
+        const real fix = static_cast<real>(ix);
+        const real w = rp.w[p];
+        const real wr = rp.wr[p];
         ++p;
-        const float sx = std::fmaf(jxx, fix, bx);
-        const float sy = std::fmaf(jxy, fix, by);
-        const float sv = c.src[s].bilinear(sx, sy) - scen;
-        const float wsv = w * sv;
+        const real sx = rfma(jxx, fix, bx);
+        const real sy = rfma(jxy, fix, by);
+        const real sv = c.src[s].bilinear(sx, sy) - scen;
+        const real wsv = w * sv;
         sum_ws += wsv;
-        sum_wss = std::fmaf(wsv, sv, sum_wss);
-        sum_wrs = std::fmaf(wr, sv, sum_wrs);
+        sum_wss = rfma(wsv, sv, sum_wss);
+        sum_wrs = rfma(wr, sv, sum_wrs);
       }
     }
     if (!ref_ok) { out[s] = 2.0f; continue; }  // :502-505
 
-    float ncc1 = -1.0f;
+    real ncc1 = -1.0f;
     if (var_ref >= c.pr->var_threshold) {
-      const float mean_s = sum_ws * inv_w;
-      const float var_src = sum_wss * inv_w - mean_s * mean_s;
+      const real mean_s = sum_ws * inv_w;
+      const real var_src = sum_wss * inv_w - mean_s * mean_s;
       if (var_src >= c.pr->var_threshold) {
-        const float covar = sum_wrs * inv_w - mean_r * mean_s;
+        const real covar = sum_wrs * inv_w - mean_r * mean_s;
         ncc1 = covar / std::sqrt(var_ref * var_src);
       }
     }
     if (ncc1 > 0.1f) { out[s] = 1.0f - ncc1; continue; }
-    float best = ncc1;
-    const float strides[2] = {2.0f, 3.0f};
+    real best = ncc1;
+    const real strides[2] = {2.0f, 3.0f};
     for (int st = 0; st < 2; ++st) {
-      const float ncc = ncc_at_stride(c, s, rp.ref_center, scen, scx, scy, x, y, strides[st],
+      const real ncc = ncc_at_stride(c, s, rp.ref_center, scen, scx, scy, x, y, strides[st],
                                       jxx, jxy, jyx, jyy);
       if (ncc > best) best = ncc;
       if (best > 0.1f) break;
@@ -502,9 +523,9 @@ int cost_vector(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp, flo
 }
 
 // Top-k initial cost + view bitmask: opencl_kernels.h:565-621.
-float initial_cost_and_views(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp,
+real initial_cost_and_views(const Ctx& c, int x, int y, Plane plane, const RefPatch& rp,
                              uint32_t* out_sel) {
-  float cv[kMaxSrc], cc[kMaxSrc];
+  real cv[kMaxSrc], cc[kMaxSrc];
   for (int i = 0; i < kMaxSrc; ++i) { cv[i] = 2.0f; cc[i] = 2.0f; }
   const int n_src = c.n_src;
   cost_vector(c, x, y, plane, rp, cv);
@@ -514,7 +535,7 @@ float initial_cost_and_views(const Ctx& c, int x, int y, Plane plane, const RefP
     if (cv[i] < 2.0f) ++num_valid;
   }
   for (int i = 1; i < n_src; ++i) {
-    const float tmp = cv[i];
+    const real tmp = cv[i];
     int j = i;
     for (; j >= 1 && tmp < cv[j - 1]; --j) cv[j] = cv[j - 1];
     cv[j] = tmp;
@@ -522,45 +543,45 @@ float initial_cost_and_views(const Ctx& c, int x, int y, Plane plane, const RefP
   *out_sel = 0u;
   const int k = std::min(std::min(c.pr->top_k, num_valid), n_src);
   if (k <= 0) return 2.0f;
-  float total = 0.0f;
+  real total = 0.0f;
   for (int i = 0; i < k; ++i) total += cv[i];
-  const float thr = cv[k - 1];
+  const real thr = cv[k - 1];
   for (int i = 0; i < n_src; ++i)
     if (cc[i] <= thr) *out_sel |= (1u << i);
-  return total / static_cast<float>(k);
+  return total / static_cast<real>(k);
 }
 
-inline Plane load_plane(const float* planes, int64_t idx) {
+inline Plane load_plane(const real* planes, int64_t idx) {
   return Plane{planes[4 * idx], planes[4 * idx + 1], planes[4 * idx + 2], planes[4 * idx + 3]};
 }
-inline void store_plane(float* planes, int64_t idx, Plane p) {
+inline void store_plane(real* planes, int64_t idx, Plane p) {
   planes[4 * idx] = p.x; planes[4 * idx + 1] = p.y; planes[4 * idx + 2] = p.z; planes[4 * idx + 3] = p.w;
 }
 
 // Smoothness: opencl_kernels.h:649-707.
-float smoothness_cost(const Ctx& c, const float* planes, Plane cur, int x, int y) {
+real smoothness_cost(const Ctx& c, const real* planes, Plane cur, int x, int y) {
   const pmo_camera& c0 = c.cams[0];
   const int W = c.sc->width, H = c.sc->height;
-  const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
+  const real cur_depth = depth_from_plane(static_cast<real>(x), static_cast<real>(y), cur, c0);
   if (cur_depth <= 0.0f) return 0.0f;
   const V3 cur_n = normalize3(V3{cur.x, cur.y, cur.z});
-  const float sigma_rel = 0.02f;
+  const real sigma_rel = 0.02f;
   const int dxs[4] = {-1, 1, 0, 0};
   const int dys[4] = {0, 0, -1, 1};
-  float tw = 0.0f, tc = 0.0f;
+  real tw = 0.0f, tc = 0.0f;
   for (int k = 0; k < 4; ++k) {
     const int nx = x + dxs[k], ny = y + dys[k];
     if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
     const Plane nb = load_plane(planes, static_cast<int64_t>(ny) * W + nx);
-    const float nb_depth = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), nb, c0);
+    const real nb_depth = depth_from_plane(static_cast<real>(nx), static_cast<real>(ny), nb, c0);
     if (nb_depth <= 0.0f) continue;
-    const float rel = std::fabs(cur_depth - nb_depth) / cur_depth;
-    const float w = pm_expf(-rel * rel / (2.0f * sigma_rel * sigma_rel));
+    const real rel = std::fabs(cur_depth - nb_depth) / cur_depth;
+    const real w = rexp(-rel * rel / (2.0f * sigma_rel * sigma_rel));
     const V3 nb_n = normalize3(V3{nb.x, nb.y, nb.z});
-    const float normal_cost = 1.0f - dot3(cur_n, nb_n);
-    const float predicted = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), cur, c0);
-    const float dev = (predicted > 0.0f) ? std::fabs(nb_depth - predicted) / cur_depth : 0.0f;
-    const float cost = 0.5f * clampf(normal_cost, 0.0f, 1.0f) + 0.5f * clampf(dev * 10.0f, 0.0f, 1.0f);
+    const real normal_cost = 1.0f - dot3(cur_n, nb_n);
+    const real predicted = depth_from_plane(static_cast<real>(nx), static_cast<real>(ny), cur, c0);
+    const real dev = (predicted > 0.0f) ? std::fabs(nb_depth - predicted) / cur_depth : 0.0f;
+    const real cost = 0.5f * clampf(normal_cost, 0.0f, 1.0f) + 0.5f * clampf(dev * 10.0f, 0.0f, 1.0f);
     tc += w * cost;
     tw += w;
   }
@@ -569,25 +590,25 @@ float smoothness_cost(const Ctx& c, const float* planes, Plane cur, int x, int y
 }
 
 // Surface normal from neighbour depths: opencl_kernels.h:725-772.
-V3 surface_normal(const Ctx& c, const float* planes, int x, int y, float center_depth) {
+V3 surface_normal(const Ctx& c, const real* planes, int x, int y, real center_depth) {
   const int W = c.sc->width, H = c.sc->height;
   if (center_depth <= 0.0f) return V3{0, 0, 0};
   if (x < 1 || x >= W - 1 || y < 1 || y >= H - 1) return V3{0, 0, 0};
   const pmo_camera& c0 = c.cams[0];
-  const float fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
+  const real fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
   const Plane pl = load_plane(planes, static_cast<int64_t>(y) * W + (x - 1));
   const Plane pr = load_plane(planes, static_cast<int64_t>(y) * W + (x + 1));
   const Plane pu = load_plane(planes, static_cast<int64_t>(y - 1) * W + x);
   const Plane pd = load_plane(planes, static_cast<int64_t>(y + 1) * W + x);
-  const float dl = depth_from_plane(static_cast<float>(x - 1), static_cast<float>(y), pl, c0);
-  const float dr = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), pr, c0);
-  const float du = depth_from_plane(static_cast<float>(x), static_cast<float>(y - 1), pu, c0);
-  const float dd = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), pd, c0);
+  const real dl = depth_from_plane(static_cast<real>(x - 1), static_cast<real>(y), pl, c0);
+  const real dr = depth_from_plane(static_cast<real>(x + 1), static_cast<real>(y), pr, c0);
+  const real du = depth_from_plane(static_cast<real>(x), static_cast<real>(y - 1), pu, c0);
+  const real dd = depth_from_plane(static_cast<real>(x), static_cast<real>(y + 1), pd, c0);
   if (dl <= 0.0f || dr <= 0.0f || du <= 0.0f || dd <= 0.0f) return V3{0, 0, 0};
-  const float dx = dr - dl;
-  const float dy = dd - du;
-  V3 n{fx * dx, fy * dy, (cx - static_cast<float>(x)) * dx + (cy - static_cast<float>(y)) * dy - center_depth};
-  const float l = len3(n);
+  const real dx = dr - dl;
+  const real dy = dd - du;
+  V3 n{fx * dx, fy * dy, (cx - static_cast<real>(x)) * dx + (cy - static_cast<real>(y)) * dy - center_depth};
+  const real l = len3(n);
   if (l < 1e-8f) return V3{0, 0, 0};
   n = V3{n.x / l, n.y / l, n.z / l};
   if (n.z > 0.0f) n = V3{-n.x, -n.y, -n.z};
@@ -595,48 +616,48 @@ V3 surface_normal(const Ctx& c, const float* planes, int x, int y, float center_
 }
 
 // Geometric consistency: opencl_kernels.h:779-861.
-float geom_cost(const Ctx& c, const pmo_state* st, int sidx, Plane plane, int px, int py) {
+real geom_cost(const Ctx& c, const pmo_state* st, int sidx, Plane plane, int px, int py) {
   const int pdw = c.sc->width, pdh = c.sc->height;
   const pmo_camera& rc = c.cams[0];
   const pmo_camera& scm = c.cams[sidx + 1];
-  const float bad = 1.0f;
-  const float max_dim = static_cast<float>(std::max(pdw, pdh));
+  const real bad = 1.0f;
+  const real max_dim = static_cast<real>(std::max(pdw, pdh));
   if (max_dim < 1.0f) return bad;
-  const float good_thresh = 3.0f / max_dim;
-  const float full_thresh = 6.0f / max_dim;
-  const float depth = depth_from_plane(static_cast<float>(px), static_cast<float>(py), plane, rc);
+  const real good_thresh = 3.0f / max_dim;
+  const real full_thresh = 6.0f / max_dim;
+  const real depth = depth_from_plane(static_cast<real>(px), static_cast<real>(py), plane, rc);
   if (depth <= 0.0f) return bad;
-  const V3 wp = backproject(static_cast<float>(px), static_cast<float>(py), depth, rc);
+  const V3 wp = backproject(static_cast<real>(px), static_cast<real>(py), depth, rc);
   const V3 sp = project(wp, scm);
   if (sp.z < 1e-6f) return bad;
-  const float sx = sp.x / sp.z, sy = sp.y / sp.z;
-  if (sx < 0.0f || sx >= static_cast<float>(pdw) || sy < 0.0f || sy >= static_cast<float>(pdh)) return bad;
+  const real sx = sp.x / sp.z, sy = sp.y / sp.z;
+  if (sx < 0.0f || sx >= static_cast<real>(pdw) || sy < 0.0f || sy >= static_cast<real>(pdh)) return bad;
   const int ix = std::min(std::max(static_cast<int>(sx + 0.5f), 0), pdw - 1);
   const int iy = std::min(std::max(static_cast<int>(sy + 0.5f), 0), pdh - 1);
-  const float sd = st->prev_depths[static_cast<int64_t>(sidx) * pdw * pdh + static_cast<int64_t>(iy) * pdw + ix];
+  const real sd = st->prev_depths[static_cast<int64_t>(sidx) * pdw * pdh + static_cast<int64_t>(iy) * pdw + ix];
   if (sd <= 0.0f) return bad;
   const V3 sw = backproject(sx, sy, sd, scm);
   const V3 rp = project(sw, rc);
   if (rp.z < 1e-6f) return bad;
-  const float bx = rp.x / rp.z, by = rp.y / rp.z;
-  const float dx = static_cast<float>(px) - bx;
-  const float dy = static_cast<float>(py) - by;
-  const float err = std::sqrt(dx * dx + dy * dy) / max_dim;
+  const real bx = rp.x / rp.z, by = rp.y / rp.z;
+  const real dx = static_cast<real>(px) - bx;
+  const real dy = static_cast<real>(py) - by;
+  const real err = std::sqrt(dx * dx + dy * dy) / max_dim;
   if (err <= good_thresh) return 0.0f;
   if (err >= full_thresh) return 1.0f;
   return (err - good_thresh) / (full_thresh - good_thresh);
 }
 
 // View-weighted cost with geometry: opencl_kernels.h:871-900.
-float weighted_cost_geom(const Ctx& c, const pmo_state* st, const float* cv, const float* vw,
-                         float wnorm, Plane plane, int x, int y) {
+real weighted_cost_geom(const Ctx& c, const pmo_state* st, const real* cv, const real* vw,
+                         real wnorm, Plane plane, int x, int y) {
   if (wnorm < 1e-6f) return 2.0f;
-  float total = 0.0f;
-  const float gw = c.pr->geom_weight;
+  real total = 0.0f;
+  const real gw = c.pr->geom_weight;
   for (int i = 0; i < c.n_src; ++i) {
     if (vw[i] > 0.0f) {
-      const float photo = cv[i];
-      float geom = 0.0f;
+      const real photo = cv[i];
+      real geom = 0.0f;
       if (gw > 0.0f && (st->prev_depth_mask & (1u << i))) geom = geom_cost(c, st, i, plane, x, y);
       total += vw[i] * (photo + gw * geom);
     }
@@ -645,29 +666,29 @@ float weighted_cost_geom(const Ctx& c, const pmo_state* st, const float* cv, con
 }
 
 // Monte-Carlo view-sampled cost: opencl_kernels.h:910-969.
-float mc_cost(const Ctx& c, const pmo_state* st, const float* cv, const float* vw, float wnorm,
+real mc_cost(const Ctx& c, const pmo_state* st, const real* cv, const real* vw, real wnorm,
               Plane plane, int x, int y, uint32_t rs[2], uint32_t key) {
   if (wnorm < 1e-6f) return 2.0f;
   const int n_src = c.n_src;
-  float cdf[kMaxSrc];
-  float cum = 0.0f;
+  real cdf[kMaxSrc];
+  real cum = 0.0f;
   for (int i = 0; i < n_src; ++i) { cum += vw[i]; cdf[i] = cum; }
-  const float gw = c.pr->geom_weight;
-  float total = 0.0f;
+  const real gw = c.pr->geom_weight;
+  real total = 0.0f;
   int valid = 0;
   for (int s = 0; s < kMcSamples; ++s) {
-    const float u = rand_float(rs, key + static_cast<uint32_t>(s) * 7919u);
-    const float thr = u * cum;
+    const real u = rand_float(rs, key + static_cast<uint32_t>(s) * 7919u);
+    const real thr = u * cum;
     int v = n_src - 1;
     for (int i = 0; i < n_src; ++i)
       if (cdf[i] > thr) { v = i; break; }
-    const float photo = cv[v];
-    float geom = 0.0f;
+    const real photo = cv[v];
+    real geom = 0.0f;
     if (gw > 0.0f && (st->prev_depth_mask & (1u << v))) geom = geom_cost(c, st, v, plane, x, y);
     total += photo + gw * geom;
     ++valid;
   }
-  return valid > 0 ? total / static_cast<float>(valid) : 2.0f;
+  return valid > 0 ? total / static_cast<real>(valid) : 2.0f;
 }
 
 // acmmp_random_init: opencl_kernels.h:979-1039.
@@ -677,15 +698,15 @@ void random_init_px(const Ctx& c, pmo_state* st, int x, int y) {
   uint32_t rs[2] = {static_cast<uint32_t>(idx) * 1099087573u + 2654435769u,
                     static_cast<uint32_t>(idx) * 2246822519u + 13u};
   const uint32_t key = static_cast<uint32_t>(idx + 7919);
-  const float ldmin = pm_logf(std::fmax(c.pr->depth_min, 1e-6f));
-  const float ldmax = pm_logf(std::fmax(c.pr->depth_max, 1e-5f));
-  const float depth = pm_expf(ldmin + rand_float(rs, key) * (ldmax - ldmin));
+  const real ldmin = rlog(std::fmax(c.pr->depth_min, 1e-6f));
+  const real ldmax = rlog(std::fmax(c.pr->depth_max, 1e-5f));
+  const real depth = rexp(ldmin + rand_float(rs, key) * (ldmax - ldmin));
   V3 n;
   n.x = rand_float(rs, key + 3u) * 2.0f - 1.0f;
   n.y = rand_float(rs, key + 5u) * 2.0f - 1.0f;
   n.z = -1.0f;
   n = normalize3(n);
-  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, n, c.cams[0]);
+  const Plane plane = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), depth, n, c.cams[0]);
   store_plane(st->planes, idx, plane);
   st->rng[2 * idx] = rs[0];
   st->rng[2 * idx + 1] = rs[1];
@@ -703,12 +724,12 @@ void random_init_px(const Ctx& c, pmo_state* st, int x, int y) {
 // scored like a random-init plane (stride-1 NCC, fallbacks, top-k mean of
 // opencl_kernels.h:565-621), the lowest cost wins and the first plane wins
 // ties.  The generator state is the random init's seed, before any draw.
-float sweep_depth(const pmo_params& P, int i, int n) {
-  const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
-  const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
-  if (n <= 1) return pm_expf(0.5f * (ldmin + ldmax));
-  const float step = (ldmax - ldmin) / static_cast<float>(n - 1);
-  return pm_expf(ldmin + static_cast<float>(i) * step);
+real sweep_depth(const pmo_params& P, int i, int n) {
+  const real ldmin = rlog(std::fmax(P.depth_min, 1e-6f));
+  const real ldmax = rlog(std::fmax(P.depth_max, 1e-5f));
+  if (n <= 1) return rexp(0.5f * (ldmin + ldmax));
+  const real step = (ldmax - ldmin) / static_cast<real>(n - 1);
+  return rexp(ldmin + static_cast<real>(i) * step);
 }
 
 void plane_sweep_px(const Ctx& c, pmo_state* st, int x, int y, int n) {
@@ -718,14 +739,14 @@ void plane_sweep_px(const Ctx& c, pmo_state* st, int x, int y, int n) {
   RefPatch rp;
   compute_ref_patch(c, x, y, &rp);
   const V3 nrm{0.0f, 0.0f, -1.0f};
-  float best = 0.0f;
+  real best = 0.0f;
   uint32_t best_sel = 0u;
   Plane best_plane{};
   for (int i = 0; i < n; ++i) {
-    const float depth = sweep_depth(*c.pr, i, n);
-    const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth, nrm, c.cams[0]);
+    const real depth = sweep_depth(*c.pr, i, n);
+    const Plane plane = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), depth, nrm, c.cams[0]);
     uint32_t sel = 0u;
-    const float cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
+    const real cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
     if (i == 0 || cost < best) {
       best = cost;
       best_sel = sel;
@@ -738,34 +759,34 @@ void plane_sweep_px(const Ctx& c, pmo_state* st, int x, int y, int n) {
 }
 
 // acmmp_prior_reinit: opencl_kernels.h:1056-1182.
-void prior_reinit_px(const Ctx& c, pmo_state* st, const float* snap, int x, int y) {
+void prior_reinit_px(const Ctx& c, pmo_state* st, const real* snap, int x, int y) {
   const int W = c.sc->width;
   const int idx = y * W + x;
   if (!st->plane_masks || st->plane_masks[idx] == 0u) return;
   const pmo_camera& c0 = c.cams[0];
   const Plane pp = load_plane(st->prior_planes, idx);
-  const float dprior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
-  const float dmin = c.pr->depth_min, dmax = c.pr->depth_max;
+  const real dprior = depth_from_plane(static_cast<real>(x), static_cast<real>(y), pp, c0);
+  const real dmin = c.pr->depth_min, dmax = c.pr->depth_max;
   if (dprior <= 0.0f || dprior < dmin || dprior > dmax) return;
   uint32_t rs[2] = {st->rng[2 * idx], st->rng[2 * idx + 1]};
   const uint32_t key = static_cast<uint32_t>(idx + 31337u);
-  float dp = dprior * 0.99f + rand_float(rs, key) * dprior * 0.02f;
+  real dp = dprior * 0.99f + rand_float(rs, key) * dprior * 0.02f;
   dp = clampf(dp, dmin, dmax);
   const V3 pn = normalize3(V3{pp.x, pp.y, pp.z});
-  const Plane plane = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), dp, pn, c0);
+  const Plane plane = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), dp, pn, c0);
   const int n_src = c.n_src;
   const uint32_t cur_sel = st->sel[idx];
-  float vw[kMaxSrc];
-  float wnorm = 0.0f;
+  real vw[kMaxSrc];
+  real wnorm = 0.0f;
   for (int i = 0; i < kMaxSrc; ++i) {
     if (i < n_src && (cur_sel & (1u << i))) { vw[i] = 1.0f; wnorm += 1.0f; } else vw[i] = 0.0f;
   }
   RefPatch rp;
   compute_ref_patch(c, x, y, &rp);
-  const float sw = c.pr->smooth_weight;
+  const real sw = c.pr->smooth_weight;
   if (wnorm < 1e-6f) {
     uint32_t sel = 0u;
-    float cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
+    real cost = initial_cost_and_views(c, x, y, plane, rp, &sel);
     if (sw > 1e-6f) cost += sw * smoothness_cost(c, snap, plane, x, y);
     if (cost < st->costs[idx]) {
       store_plane(st->planes, idx, plane);
@@ -776,10 +797,10 @@ void prior_reinit_px(const Ctx& c, pmo_state* st, const float* snap, int x, int 
     st->rng[2 * idx + 1] = rs[1];
     return;
   }
-  float cv[kMaxSrc];
+  real cv[kMaxSrc];
   for (int i = 0; i < kMaxSrc; ++i) cv[i] = 2.0f;
   cost_vector(c, x, y, plane, rp, cv);
-  float cost = weighted_cost_geom(c, st, cv, vw, wnorm, plane, x, y);
+  real cost = weighted_cost_geom(c, st, cv, vw, wnorm, plane, x, y);
   if (sw > 1e-6f) cost += sw * smoothness_cost(c, snap, plane, x, y);
   if (cost < st->costs[idx]) {
     store_plane(st->planes, idx, plane);
@@ -795,46 +816,46 @@ void prior_reinit_px(const Ctx& c, pmo_state* st, const float* snap, int x, int 
 //   4 (perturbed depth, current normal) 5 (current depth, surface normal)
 // With a prior the random draws sit around the prior.  Returns 6, or 5 when
 // no surface normal is available.
-int refinement_hypotheses(const Ctx& c, const float* planes, int x, int y, Plane plane_now, float depth_now,
-                          bool have_prior, float depth_prior, V3 prior_n, float depth_sigma, float angle_sigma,
-                          uint32_t rs[2], uint32_t key, float rd[6], V3 rn[6]) {
+int refinement_hypotheses(const Ctx& c, const real* planes, int x, int y, Plane plane_now, real depth_now,
+                          bool have_prior, real depth_prior, V3 prior_n, real depth_sigma, real angle_sigma,
+                          uint32_t rs[2], uint32_t key, real rd[6], V3 rn[6]) {
   const pmo_params& P = *c.pr;
-  const float depth_perturbation = 0.005f;
-  const float normal_perturbation = 0.01f * kPiF;
+  const real depth_perturbation = 0.005f;
+  const real normal_perturbation = 0.01f * kPiF;
   const V3 cur_normal = normalize3(V3{plane_now.x, plane_now.y, plane_now.z});
-  float depth_rand, depth_pert;
+  real depth_rand, depth_pert;
   V3 nrand, npert;
   if (have_prior && depth_prior > 0.0f) {
     depth_rand = depth_prior - 3.0f * depth_sigma + rand_float(rs, key + 503u) * 6.0f * depth_sigma;
     depth_rand = clampf(depth_rand, P.depth_min, P.depth_max);
-    const float pa1 = (rand_float(rs, key + 504u) - 0.5f) * angle_sigma;
-    const float pa2 = (rand_float(rs, key + 505u) - 0.5f) * angle_sigma;
+    const real pa1 = (rand_float(rs, key + 504u) - 0.5f) * angle_sigma;
+    const real pa2 = (rand_float(rs, key + 505u) - 0.5f) * angle_sigma;
     nrand = normalize3(V3{prior_n.x + pa1, prior_n.y + pa2, prior_n.z});
     if (nrand.z > 0.0f) nrand.z = -nrand.z;
   } else {
-    const float ldmin = pm_logf(std::fmax(P.depth_min, 1e-6f));
-    const float ldmax = pm_logf(std::fmax(P.depth_max, 1e-5f));
-    depth_rand = pm_expf(ldmin + rand_float(rs, key + 503u) * (ldmax - ldmin));
+    const real ldmin = rlog(std::fmax(P.depth_min, 1e-6f));
+    const real ldmax = rlog(std::fmax(P.depth_max, 1e-5f));
+    depth_rand = rexp(ldmin + rand_float(rs, key + 503u) * (ldmax - ldmin));
     nrand.x = rand_float(rs, key + 504u) * 2.0f - 1.0f;
     nrand.y = rand_float(rs, key + 505u) * 2.0f - 1.0f;
     nrand.z = -1.0f;
     nrand = normalize3(nrand);
   }
   {
-    const float dlo = (1.0f - depth_perturbation) * depth_now;
-    const float dhi = (1.0f + depth_perturbation) * depth_now;
+    const real dlo = (1.0f - depth_perturbation) * depth_now;
+    const real dhi = (1.0f + depth_perturbation) * depth_now;
     depth_pert = dlo + rand_float(rs, key + 510u) * (dhi - dlo);
     depth_pert = clampf(depth_pert, P.depth_min, P.depth_max);
   }
   {
-    const float pa1 = (rand_float(rs, key + 520u) - 0.5f) * normal_perturbation;
-    const float pa2 = (rand_float(rs, key + 521u) - 0.5f) * normal_perturbation;
+    const real pa1 = (rand_float(rs, key + 520u) - 0.5f) * normal_perturbation;
+    const real pa2 = (rand_float(rs, key + 521u) - 0.5f) * normal_perturbation;
     npert = normalize3(V3{cur_normal.x + pa1, cur_normal.y + pa2, cur_normal.z});
     if (npert.z > 0.0f) npert.z = -npert.z;
   }
   const V3 sn = surface_normal(c, planes, x, y, depth_now);
   const bool have_sn = len3(sn) > 0.5f;
-  const float d6[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
+  const real d6[6] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert, depth_now};
   const V3 n6[6] = {cur_normal, nrand, nrand, npert, cur_normal, sn};
   for (int h = 0; h < 6; ++h) { rd[h] = d6[h]; rn[h] = n6[h]; }
   return have_sn ? 6 : 5;
@@ -848,17 +869,17 @@ int refinement_hypotheses(const Ctx& c, const float* planes, int x, int y, Plane
 // current plane; if nothing is left, every view with some candidate cost
 // below 2 gets 1.  ca[j * kMaxSrc + i] = cost of candidate j in view i.
 // Returns the weight sum; writes the weights and the selection mask.
-float view_weights(const Ctx& c, const float* ca, const bool* flags, uint32_t prev_sel, Plane cur, int x, int y,
-                   float vw[kMaxSrc], uint32_t* sel_out) {
+real view_weights(const Ctx& c, const real* ca, const bool* flags, uint32_t prev_sel, Plane cur, int x, int y,
+                   real vw[kMaxSrc], uint32_t* sel_out) {
   const pmo_params& P = *c.pr;
   const pmo_camera& c0 = c.cams[0];
   const int n_src = c.n_src;
-  const float vthr = 0.8f;
+  const real vthr = 0.8f;
   for (int i = 0; i < kMaxSrc; ++i) vw[i] = 0.0f;
   for (int i = 0; i < n_src; ++i) {
-    float vc = 0.0f;
+    real vc = 0.0f;
     for (int j = 0; j < 8; ++j) {
-      const float cc = ca[j * kMaxSrc + i];
+      const real cc = ca[j * kMaxSrc + i];
       if (flags[j] && cc < vthr) vc += (1.0f - cc);
     }
     if (vc > 0.0f) vw[i] = vc;
@@ -871,13 +892,13 @@ float view_weights(const Ctx& c, const float* ca, const bool* flags, uint32_t pr
   }
   {
     V3 cur_n{cur.x, cur.y, cur.z};
-    const float nl = len3(cur_n);
+    const real nl = len3(cur_n);
     const bool have_normal = nl > 0.1f;
     if (have_normal) cur_n = V3{cur_n.x / nl, cur_n.y / nl, cur_n.z / nl};
-    const float cur_depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cur, c0);
-    const V3 p3 = backproject(static_cast<float>(x), static_cast<float>(y), cur_depth, c0);
-    const float inc_sigma = 0.6f;
-    const float inv2s2 = -0.5f / (inc_sigma * inc_sigma);
+    const real cur_depth = depth_from_plane(static_cast<real>(x), static_cast<real>(y), cur, c0);
+    const V3 p3 = backproject(static_cast<real>(x), static_cast<real>(y), cur_depth, c0);
+    const real inc_sigma = 0.6f;
+    const real inv2s2 = -0.5f / (inc_sigma * inc_sigma);
     for (int i = 0; i < n_src; ++i) {
       if (vw[i] < 1e-6f) continue;
       vw[i] *= c.sc->angle_weights[i];
@@ -888,14 +909,14 @@ float view_weights(const Ctx& c, const float* ca, const bool* flags, uint32_t pr
         C.y = -(ci.R[1] * ci.t[0] + ci.R[4] * ci.t[1] + ci.R[7] * ci.t[2]);
         C.z = -(ci.R[2] * ci.t[0] + ci.R[5] * ci.t[1] + ci.R[8] * ci.t[2]);
         const V3 ray = normalize3(V3{C.x - p3.x, C.y - p3.y, C.z - p3.z});
-        const float cos_inc = std::fabs(dot3(cur_n, ray));
-        const float xi = 1.0f - cos_inc;
-        vw[i] *= pm_expf(xi * xi * inv2s2);
+        const real cos_inc = std::fabs(dot3(cur_n, ray));
+        const real xi = 1.0f - cos_inc;
+        vw[i] *= rexp(xi * xi * inv2s2);
       }
     }
   }
   uint32_t new_sel = 0u;
-  float wnorm = 0.0f;
+  real wnorm = 0.0f;
   for (int i = 0; i < n_src; ++i)
     if (vw[i] > 0.0f) { new_sel |= (1u << i); wnorm += vw[i]; }
   if (wnorm < 1e-6f) {
@@ -914,15 +935,15 @@ float view_weights(const Ctx& c, const float* ca, const bool* flags, uint32_t pr
 // acmmp_patchmatch, one pixel of one colour: opencl_kernels.h:1197-1778.
 // Updates in place: every neighbour the pixel reads has the other colour
 // (odd Manhattan offsets), so this equals the reference's snapshot reads.
-void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, float smooth_w) {
+void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, real smooth_w) {
   const int W = c.sc->width, H = c.sc->height;
   const int idx = y * W + x;
   const pmo_camera& c0 = c.cams[0];
   const pmo_params& P = *c.pr;
   uint32_t rs[2] = {st->rng[2 * idx], st->rng[2 * idx + 1]};
   const uint32_t key = static_cast<uint32_t>(idx + iteration * 7919 + 42);
-  const float* planes = st->planes;
-  const float* costs = st->costs;
+  const real* planes = st->planes;
+  const real* costs = st->costs;
 
   RefPatch rp;
   compute_ref_patch(c, x, y, &rp);
@@ -930,7 +951,7 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
   // Step 1: candidate positions (:1254-1430).
   int cpx[8], cpy[8];
   bool flags[8];
-  float ca[8 * kMaxSrc];
+  real ca[8 * kMaxSrc];
   for (int i = 0; i < 8; ++i) { cpx[i] = x; cpy[i] = y; flags[i] = false; }
   for (int i = 0; i < 8 * kMaxSrc; ++i) ca[i] = 2.0f;
   auto cost_at = [&](int px, int py) { return costs[static_cast<int64_t>(py) * W + px]; };
@@ -939,81 +960,81 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
     cost_vector(c, x, y, load_plane(planes, static_cast<int64_t>(py) * W + px), rp, &ca[slot * kMaxSrc]);
   };
   if (y > 2) {  // far up (:1262-1278)
-    int by = y - 3; float bc = cost_at(x, by);
+    int by = y - 3; real bc = cost_at(x, by);
     for (int i = 1; i < 11; ++i) { const int yy = y - 3 - 2 * i; if (yy < 0) break;
-      const float cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
+      const real cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
     eval(1, x, by);
   }
   if (y < H - 3) {  // far down (:1281-1296)
-    int by = y + 3; float bc = cost_at(x, by);
+    int by = y + 3; real bc = cost_at(x, by);
     for (int i = 1; i < 11; ++i) { const int yy = y + 3 + 2 * i; if (yy >= H) break;
-      const float cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
+      const real cc = cost_at(x, yy); if (cc < bc) { bc = cc; by = yy; } }
     eval(3, x, by);
   }
   if (x > 2) {  // far left (:1299-1314)
-    int bx = x - 3; float bc = cost_at(bx, y);
+    int bx = x - 3; real bc = cost_at(bx, y);
     for (int i = 1; i < 11; ++i) { const int xx = x - 3 - 2 * i; if (xx < 0) break;
-      const float cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
+      const real cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
     eval(5, bx, y);
   }
   if (x < W - 3) {  // far right (:1317-1332)
-    int bx = x + 3; float bc = cost_at(bx, y);
+    int bx = x + 3; real bc = cost_at(bx, y);
     for (int i = 1; i < 11; ++i) { const int xx = x + 3 + 2 * i; if (xx >= W) break;
-      const float cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
+      const real cc = cost_at(xx, y); if (cc < bc) { bc = cc; bx = xx; } }
     eval(7, bx, y);
   }
   if (y > 0) {  // near up (:1335-1356)
-    int bx = x, by = y - 1; float bc = cost_at(bx, by);
+    int bx = x, by = y - 1; real bc = cost_at(bx, by);
     for (int i = 0; i < 3; ++i) {
-      if (y > 1 + i && x > i) { const int px = x - i - 1, py = y - 2 - i; const float cc = cost_at(px, py);
+      if (y > 1 + i && x > i) { const int px = x - i - 1, py = y - 2 - i; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
-      if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const float cc = cost_at(px, py);
+      if (y > 1 + i && x < W - 1 - i) { const int px = x + i + 1, py = y - 2 - i; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
     }
     eval(0, bx, by);
   }
   if (y < H - 1) {  // near down (:1361-1382)
-    int bx = x, by = y + 1; float bc = cost_at(bx, by);
+    int bx = x, by = y + 1; real bc = cost_at(bx, by);
     for (int i = 0; i < 3; ++i) {
-      if (y < H - 2 - i && x > i) { const int px = x - i - 1, py = y + 2 + i; const float cc = cost_at(px, py);
+      if (y < H - 2 - i && x > i) { const int px = x - i - 1, py = y + 2 + i; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
-      if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const float cc = cost_at(px, py);
+      if (y < H - 2 - i && x < W - 1 - i) { const int px = x + i + 1, py = y + 2 + i; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
     }
     eval(2, bx, by);
   }
   if (x > 0) {  // near left (:1385-1406)
-    int bx = x - 1, by = y; float bc = cost_at(bx, by);
+    int bx = x - 1, by = y; real bc = cost_at(bx, by);
     for (int i = 0; i < 3; ++i) {
-      if (x > 1 + i && y > i) { const int px = x - 2 - i, py = y - i - 1; const float cc = cost_at(px, py);
+      if (x > 1 + i && y > i) { const int px = x - 2 - i, py = y - i - 1; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
-      if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const float cc = cost_at(px, py);
+      if (x > 1 + i && y < H - 1 - i) { const int px = x - 2 - i, py = y + i + 1; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
     }
     eval(4, bx, by);
   }
   if (x < W - 1) {  // near right (:1409-1430)
-    int bx = x + 1, by = y; float bc = cost_at(bx, by);
+    int bx = x + 1, by = y; real bc = cost_at(bx, by);
     for (int i = 0; i < 3; ++i) {
-      if (x < W - 2 - i && y > i) { const int px = x + 2 + i, py = y - i - 1; const float cc = cost_at(px, py);
+      if (x < W - 2 - i && y > i) { const int px = x + 2 + i, py = y - i - 1; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
-      if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const float cc = cost_at(px, py);
+      if (x < W - 2 - i && y < H - 1 - i) { const int px = x + 2 + i, py = y + i + 1; const real cc = cost_at(px, py);
         if (cc < bc) { bc = cc; bx = px; by = py; } }
     }
     eval(6, bx, by);
   }
 
   // Step 2: view weights (:1452-1563).
-  float vw[kMaxSrc];
+  real vw[kMaxSrc];
   uint32_t new_sel = 0u;
-  const float wnorm = view_weights(c, ca, flags, st->sel[idx], load_plane(planes, idx), x, y, vw, &new_sel);
+  const real wnorm = view_weights(c, ca, flags, st->sel[idx], load_plane(planes, idx), x, y, vw, &new_sel);
 
   // Step 3: candidate costs (:1570-1619).
-  float fc[8];
+  real fc[8];
   for (int i = 0; i < 8; ++i) {
     if (flags[i]) {
       const Plane cp = load_plane(planes, static_cast<int64_t>(cpy[i]) * W + cpx[i]);
-      float ph = mc_cost(c, st, &ca[i * kMaxSrc], vw, wnorm, cp, x, y, rs,
+      real ph = mc_cost(c, st, &ca[i * kMaxSrc], vw, wnorm, cp, x, y, rs,
                          key + 600u + static_cast<uint32_t>(i) * 31u);
       if (smooth_w > 1e-6f) ph += smooth_w * smoothness_cost(c, planes, cp, x, y);
       fc[i] = ph;
@@ -1022,33 +1043,33 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
     }
   }
   int best = 0;
-  float best_cost = fc[0];
+  real best_cost = fc[0];
   for (int i = 1; i < 8; ++i) if (fc[i] < best_cost) { best_cost = fc[i]; best = i; }
 
   Plane plane_now = load_plane(planes, idx);
-  float cvn[kMaxSrc];
+  real cvn[kMaxSrc];
   for (int i = 0; i < kMaxSrc; ++i) cvn[i] = 2.0f;
   cost_vector(c, x, y, plane_now, rp, cvn);
-  float cost_now = mc_cost(c, st, cvn, vw, wnorm, plane_now, x, y, rs, key + 700u);
+  real cost_now = mc_cost(c, st, cvn, vw, wnorm, plane_now, x, y, rs, key + 700u);
   if (smooth_w > 1e-6f) cost_now += smooth_w * smoothness_cost(c, planes, plane_now, x, y);
   st->costs[idx] = cost_now;
-  float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), plane_now, c0);
+  real depth_now = depth_from_plane(static_cast<real>(x), static_cast<real>(y), plane_now, c0);
 
   const bool have_prior = st->plane_masks && st->plane_masks[idx] > 0u;
-  float depth_prior = 0.0f;
+  real depth_prior = 0.0f;
   V3 prior_n{0.0f, 0.0f, -1.0f};
-  const float depth_sigma = (P.depth_max - P.depth_min) / 32.0f;
-  const float angle_sigma = kPiF * (5.0f / 180.0f);
+  const real depth_sigma = (P.depth_max - P.depth_min) / 32.0f;
+  const real angle_sigma = kPiF * (5.0f / 180.0f);
   if (have_prior) {
     const Plane pp = load_plane(st->prior_planes, idx);
-    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+    depth_prior = depth_from_plane(static_cast<real>(x), static_cast<real>(y), pp, c0);
     prior_n = normalize3(V3{pp.x, pp.y, pp.z});
   }
 
   // Step 3b (:1643-1656).
   if (flags[best] && best_cost < cost_now) {
     const Plane bp = load_plane(planes, static_cast<int64_t>(cpy[best]) * W + cpx[best]);
-    const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), bp, c0);
+    const real db = depth_from_plane(static_cast<real>(x), static_cast<real>(y), bp, c0);
     if (db >= P.depth_min && db <= P.depth_max) {
       depth_now = db;
       plane_now = bp;
@@ -1059,18 +1080,18 @@ void patchmatch_px(const Ctx& c, pmo_state* st, int x, int y, int iteration, flo
 
   // Step 4: refinement (:1669-1772).
   {
-    float rd[6];
+    real rd[6];
     V3 rn[6];
     const int nc = refinement_hypotheses(c, planes, x, y, plane_now, depth_now, have_prior, depth_prior, prior_n,
                                          depth_sigma, angle_sigma, rs, key, rd, rn);
     for (int h = 0; h < nc; ++h) {
-      const Plane cand = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[h], rn[h], c0);
-      float cv[kMaxSrc];
+      const Plane cand = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), rd[h], rn[h], c0);
+      real cv[kMaxSrc];
       for (int i = 0; i < kMaxSrc; ++i) cv[i] = 2.0f;
       cost_vector(c, x, y, cand, rp, cv);
-      float tc = mc_cost(c, st, cv, vw, wnorm, cand, x, y, rs, key + 800u + static_cast<uint32_t>(h) * 37u);
+      real tc = mc_cost(c, st, cv, vw, wnorm, cand, x, y, rs, key + 800u + static_cast<uint32_t>(h) * 37u);
       if (smooth_w > 1e-6f) tc += smooth_w * smoothness_cost(c, planes, cand, x, y);
-      const float db = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cand, c0);
+      const real db = depth_from_plane(static_cast<real>(x), static_cast<real>(y), cand, c0);
       if (db < P.depth_min || db > P.depth_max) continue;
       if (tc < cost_now) { depth_now = db; plane_now = cand; cost_now = tc; }
     }
@@ -1087,10 +1108,10 @@ void checkerboard_px(const Ctx& c, pmo_state* st, int x, int y) {
   const int center = y * W + x;
   const pmo_camera& c0 = c.cams[0];
   if (st->costs[center] < 0.001f) return;
-  float buf[21];
+  real buf[21];
   int n = 0;
   auto dep = [&](int px, int py) {
-    return depth_from_plane(static_cast<float>(px), static_cast<float>(py),
+    return depth_from_plane(static_cast<real>(px), static_cast<real>(py),
                             load_plane(st->planes, static_cast<int64_t>(py) * W + px), c0);
   };
   buf[n++] = dep(x, y);
@@ -1115,11 +1136,11 @@ void checkerboard_px(const Ctx& c, pmo_state* st, int x, int y) {
   if (x > 0 && y < H - 2) buf[n++] = dep(x - 1, y + 2);
   if (x < W - 1 && y < H - 2) buf[n++] = dep(x + 1, y + 2);
   {
-    const float percent = 0.03f;
-    const float cd = buf[0];
+    const real percent = 0.03f;
+    const real cd = buf[0];
     if (cd > 0.0f) {
-      const float lo = cd * (1 - percent);
-      const float hi = cd * (1 + percent);
+      const real lo = cd * (1 - percent);
+      const real hi = cd * (1 + percent);
       int m = 1;
       for (int i = 1; i < n; ++i)
         if (buf[i] > lo && buf[i] < hi) buf[m++] = buf[i];
@@ -1127,17 +1148,17 @@ void checkerboard_px(const Ctx& c, pmo_state* st, int x, int y) {
     }
   }
   for (int i = 1; i < n; ++i) {
-    const float tmp = buf[i];
+    const real tmp = buf[i];
     int j;
     for (j = i; j >= 1 && tmp < buf[j - 1]; --j) buf[j] = buf[j - 1];
     buf[j] = tmp;
   }
   const int mid = n / 2;
-  const float med = (n % 2 == 0) ? (buf[mid - 1] + buf[mid]) * 0.5f : buf[mid];
+  const real med = (n % 2 == 0) ? (buf[mid - 1] + buf[mid]) * 0.5f : buf[mid];
   if (med > 0.0f) {
     const Plane cp = load_plane(st->planes, center);
     const V3 nn = normalize3(V3{cp.x, cp.y, cp.z});
-    store_plane(st->planes, center, plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), med, nn, c0));
+    store_plane(st->planes, center, plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), med, nn, c0));
   }
 }
 
@@ -1166,30 +1187,30 @@ float pmo_logf(float x) { return pm_logf(x); }
 
 float pmo_rand_float(uint32_t* state, uint32_t key) { return rand_float(state, key); }
 
-float pmo_bilinear(const float* img, int w, int h, float sx, float sy) {
+real pmo_bilinear(const real* img, int w, int h, real sx, real sy) {
   Img im{img, w, h};
   return im.bilinear(sx, sy);
 }
 
-void pmo_project(const pmo_camera* cam, const float* X, float* out) {
+void pmo_project(const pmo_camera* cam, const real* X, real* out) {
   const V3 p = project(V3{X[0], X[1], X[2]}, *cam);
   out[0] = p.x; out[1] = p.y; out[2] = p.z;
 }
-void pmo_backproject(const pmo_camera* cam, float x, float y, float depth, float* out) {
+void pmo_backproject(const pmo_camera* cam, real x, real y, real depth, real* out) {
   const V3 p = backproject(x, y, depth, *cam);
   out[0] = p.x; out[1] = p.y; out[2] = p.z;
 }
-float pmo_depth_from_plane(const pmo_camera* cam, float x, float y, const float* plane) {
+real pmo_depth_from_plane(const pmo_camera* cam, real x, real y, const real* plane) {
   return depth_from_plane(x, y, Plane{plane[0], plane[1], plane[2], plane[3]}, *cam);
 }
-void pmo_plane_from_depth_normal(const pmo_camera* cam, float x, float y, float depth,
-                                 const float* n, float* out) {
+void pmo_plane_from_depth_normal(const pmo_camera* cam, real x, real y, real depth,
+                                 const real* n, real* out) {
   const Plane p = plane_from_depth_normal(x, y, depth, V3{n[0], n[1], n[2]}, *cam);
   out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = p.w;
 }
 
 // Reference patch moments (centre, sum_w, sum_wr, sum_wrr) then w[P], wr[P].
-void pmo_ref_patch(const pmo_scene* sc, const pmo_params* pr, int x, int y, float* out) {
+void pmo_ref_patch(const pmo_scene* sc, const pmo_params* pr, int x, int y, real* out) {
   Ctx c(sc, pr);
   RefPatch rp;
   compute_ref_patch(c, x, y, &rp);
@@ -1202,7 +1223,7 @@ void pmo_ref_patch(const pmo_scene* sc, const pmo_params* pr, int x, int y, floa
 // estimator.cc:727-756): each output pixel is the mean of the input over its
 // footprint, edge pixels weighted by the covered fraction; sums in double,
 // optionally rounded to an integer (u8 images).
-void pmo_resize_area(const float* src, int w, int h, int nw, int nh, int round_int, float* dst) {
+void pmo_resize_area(const real* src, int w, int h, int nw, int nh, int round_int, real* dst) {
   const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
   for (int dy = 0; dy < nh; ++dy) {
     const double y0 = dy * sy, y1 = std::min((dy + 1) * sy, static_cast<double>(h));
@@ -1222,14 +1243,14 @@ void pmo_resize_area(const float* src, int w, int h, int nw, int nh, int round_i
       }
       double v = acc / ((x1 - x0) * (y1 - y0));
       if (round_int) v = std::nearbyint(v);
-      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<float>(v);
+      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<real>(v);
     }
   }
 }
 
 // Bilinear resize with pixel-centre alignment and edge clamping (the
 // cv::resize INTER_LINEAR of UploadPreviousDepths, estimator.cc:92-101).
-void pmo_resize_linear(const float* src, int w, int h, int nw, int nh, float* dst) {
+void pmo_resize_linear(const real* src, int w, int h, int nw, int nh, real* dst) {
   const double sx = static_cast<double>(w) / nw, sy = static_cast<double>(h) / nh;
   for (int dy = 0; dy < nh; ++dy) {
     const double fy = (dy + 0.5) * sy - 0.5;
@@ -1248,7 +1269,7 @@ void pmo_resize_linear(const float* src, int w, int h, int nw, int nh, float* ds
       const double v00 = src[static_cast<int64_t>(y0) * w + x0], v01 = src[static_cast<int64_t>(y0) * w + x1];
       const double v10 = src[static_cast<int64_t>(y1) * w + x0], v11 = src[static_cast<int64_t>(y1) * w + x1];
       const double v = (1.0 - ay) * ((1.0 - ax) * v00 + ax * v01) + ay * ((1.0 - ax) * v10 + ax * v11);
-      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<float>(v);
+      dst[static_cast<int64_t>(dy) * nw + dx] = static_cast<real>(v);
     }
   }
 }
@@ -1258,7 +1279,7 @@ void pmo_resize_linear(const float* src, int w, int h, int nw, int nh, float* ds
 // built from its decoded depth and normal as UpsampleDepthNormal does
 // (estimator.cc:775-791), with the fine camera's intrinsics scaled by the
 // size ratio.  Sets planes, costs (2.0), rng and sel (0) of the fine state.
-void pmo_upsample(const pmo_scene* sc, const pmo_params* pr, const float* cdepth, const float* cnormal, int src_w,
+void pmo_upsample(const pmo_scene* sc, const pmo_params* pr, const real* cdepth, const real* cnormal, int src_w,
                   int src_h, pmo_state* st) {
   Ctx c(sc, pr);
   const int W = sc->width, H = sc->height;
@@ -1269,35 +1290,35 @@ void pmo_upsample(const pmo_scene* sc, const pmo_params* pr, const float* cdepth
       const int idx = y * W + x;
       uint32_t rs[2] = {static_cast<uint32_t>(idx) * 1099087573u + 2654435769u,
                         static_cast<uint32_t>(idx) * 2246822519u + 13u};
-      const float scale_x = static_cast<float>(src_w) / static_cast<float>(W);
-      const float scale_y = static_cast<float>(src_h) / static_cast<float>(H);
-      const int sx = std::min(std::max(static_cast<int>(static_cast<float>(x) * scale_x), 0), src_w - 1);
-      const int sy = std::min(std::max(static_cast<int>(static_cast<float>(y) * scale_y), 0), src_h - 1);
+      const real scale_x = static_cast<real>(src_w) / static_cast<real>(W);
+      const real scale_y = static_cast<real>(src_h) / static_cast<real>(H);
+      const int sx = std::min(std::max(static_cast<int>(static_cast<real>(x) * scale_x), 0), src_w - 1);
+      const int sy = std::min(std::max(static_cast<int>(static_cast<real>(y) * scale_y), 0), src_h - 1);
       const int sidx = sy * src_w + sx;
-      const float d = cdepth[sidx];
+      const real d = cdepth[sidx];
       const V3 cn{cnormal[3 * sidx], cnormal[3 * sidx + 1], cnormal[3 * sidx + 2]};
-      const float fx = c0.K[0] * static_cast<float>(src_w) / static_cast<float>(W);
-      const float fy = c0.K[4] * static_cast<float>(src_h) / static_cast<float>(H);
-      const float cxs = c0.K[2] * static_cast<float>(src_w) / static_cast<float>(W);
-      const float cys = c0.K[5] * static_cast<float>(src_h) / static_cast<float>(H);
-      const V3 pt{d * (static_cast<float>(sx) - cxs) / fx, d * (static_cast<float>(sy) - cys) / fy, d};
+      const real fx = c0.K[0] * static_cast<real>(src_w) / static_cast<real>(W);
+      const real fy = c0.K[4] * static_cast<real>(src_h) / static_cast<real>(H);
+      const real cxs = c0.K[2] * static_cast<real>(src_w) / static_cast<real>(W);
+      const real cys = c0.K[5] * static_cast<real>(src_h) / static_cast<real>(H);
+      const V3 pt{d * (static_cast<real>(sx) - cxs) / fx, d * (static_cast<real>(sy) - cys) / fy, d};
       const Plane cp{cn.x, cn.y, cn.z, -(cn.x * pt.x + cn.y * pt.y + cn.z * pt.z)};
-      const float depth = depth_from_plane(static_cast<float>(x), static_cast<float>(y), cp, c0);
+      const real depth = depth_from_plane(static_cast<real>(x), static_cast<real>(y), cp, c0);
       Plane out;
       if (depth > 0.0f) {
-        out = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), depth,
+        out = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), depth,
                                       normalize3(V3{cp.x, cp.y, cp.z}), c0);
       } else {
         const uint32_t key = static_cast<uint32_t>(idx + 7919);
-        const float ldmin = pm_logf(std::fmax(pr->depth_min, 1e-6f));
-        const float ldmax = pm_logf(std::fmax(pr->depth_max, 1e-5f));
-        const float rd = pm_expf(ldmin + rand_float(rs, key) * (ldmax - ldmin));
+        const real ldmin = rlog(std::fmax(pr->depth_min, 1e-6f));
+        const real ldmax = rlog(std::fmax(pr->depth_max, 1e-5f));
+        const real rd = rexp(ldmin + rand_float(rs, key) * (ldmax - ldmin));
         V3 n;
         n.x = rand_float(rs, key + 3u) * 2.0f - 1.0f;
         n.y = rand_float(rs, key + 5u) * 2.0f - 1.0f;
         n.z = -1.0f;
         n = normalize3(n);
-        out = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd, n, c0);
+        out = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), rd, n, c0);
       }
       store_plane(st->planes, idx, out);
       st->costs[idx] = 2.0f;
@@ -1315,11 +1336,11 @@ void pmo_upsample(const pmo_scene* sc, const pmo_params* pr, const float* cdepth
 // :1319-1418: runs of at most gap_max empty pixels between two depths within
 // 2.5 % are filled linearly, rows first, then still-empty column runs), and
 // the confidence of the result.  Each step runs only when its size is > 0.
-void pmo_postprocess(float* depth, float* normal, const float* cost, float* confidence, int w, int h,
+void pmo_postprocess(real* depth, real* normal, const real* cost, real* confidence, int w, int h,
                      int speckle_min, int gap_max) {
   const int npix = w * h;
   if (speckle_min > 0) {
-    const float thr = 0.01f;
+    const real thr = 0.01f;
     std::vector<char> seen(static_cast<size_t>(npix), 0);
     std::vector<int> comp;
     const int ox[4] = {1, -1, 0, 0}, oy[4] = {0, 0, 1, -1};
@@ -1346,15 +1367,15 @@ void pmo_postprocess(float* depth, float* normal, const float* cost, float* conf
     }
   }
   if (gap_max > 0) {
-    const float thr = 0.025f;
+    const real thr = 0.025f;
     // fill pixel q from valid a to valid b at fraction t
-    auto fill = [&](int a, int b, int q, float t, bool only_empty) {
+    auto fill = [&](int a, int b, int q, real t, bool only_empty) {
       if (only_empty && depth[q] > 0.0f) return;
       depth[q] = depth[a] + t * (depth[b] - depth[a]);
-      float v0 = (1.0f - t) * normal[3 * a] + t * normal[3 * b];
-      float v1 = (1.0f - t) * normal[3 * a + 1] + t * normal[3 * b + 1];
-      float v2 = (1.0f - t) * normal[3 * a + 2] + t * normal[3 * b + 2];
-      const float len = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+      real v0 = (1.0f - t) * normal[3 * a] + t * normal[3 * b];
+      real v1 = (1.0f - t) * normal[3 * a + 1] + t * normal[3 * b + 1];
+      real v2 = (1.0f - t) * normal[3 * a + 2] + t * normal[3 * b + 2];
+      const real len = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
       if (len > 1e-6f) { v0 /= len; v1 /= len; v2 /= len; }
       normal[3 * q] = v0; normal[3 * q + 1] = v1; normal[3 * q + 2] = v2;
     };
@@ -1368,10 +1389,10 @@ void pmo_postprocess(float* depth, float* normal, const float* cost, float* conf
           if (start >= 0 && start - 1 >= 0 && i - start <= gap_max) {
             const int a = idx(l, start - 1);
             // depth[a] is read before the run is filled (the run starts after a)
-            const float da = depth[a], db = depth[b];
+            const real da = depth[a], db = depth[b];
             if (std::fabs(da - db) / std::max(da, db) < thr)
               for (int g = start; g < i; ++g)
-                fill(a, b, idx(l, g), static_cast<float>(g - (start - 1)) / static_cast<float>(i - (start - 1)),
+                fill(a, b, idx(l, g), static_cast<real>(g - (start - 1)) / static_cast<real>(i - (start - 1)),
                      only_empty);
           }
           start = -1;
@@ -1382,7 +1403,7 @@ void pmo_postprocess(float* depth, float* normal, const float* cost, float* conf
     pass(w, h, [w](int l, int i) { return i * w + l; }, true);
   }
   for (int i = 0; i < npix; ++i)
-    confidence[i] = depth[i] > 0.0f ? 1.0f / (1.0f + pm_expf((cost[i] - 0.5f) * 5.0f)) : 0.0f;
+    confidence[i] = depth[i] > 0.0f ? 1.0f / (1.0f + rexp((cost[i] - 0.5f) * 5.0f)) : 0.0f;
 }
 
 // Float64 evaluation of the differentiable photometric cost of one pixel
@@ -1472,29 +1493,29 @@ double pmo_ncc_energy_f64(const pmo_scene* sc, const pmo_params* pr, int x, int 
 }
 
 // Internal pieces, exposed so the tests can pin them one by one.
-float pmo_smoothness_cost(const pmo_scene* sc, const pmo_params* pr, const float* planes, const float* plane,
+real pmo_smoothness_cost(const pmo_scene* sc, const pmo_params* pr, const real* planes, const real* plane,
                           int x, int y) {
   Ctx c(sc, pr);
   return smoothness_cost(c, planes, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
 }
-void pmo_surface_normal(const pmo_scene* sc, const pmo_params* pr, const float* planes, int x, int y,
-                        float center_depth, float* out) {
+void pmo_surface_normal(const pmo_scene* sc, const pmo_params* pr, const real* planes, int x, int y,
+                        real center_depth, real* out) {
   Ctx c(sc, pr);
   const V3 n = surface_normal(c, planes, x, y, center_depth);
   out[0] = n.x; out[1] = n.y; out[2] = n.z;
 }
-float pmo_geom_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, int sidx, const float* plane,
+real pmo_geom_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, int sidx, const real* plane,
                     int x, int y) {
   Ctx c(sc, pr);
   return geom_cost(c, st, sidx, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
 }
-float pmo_weighted_cost_geom(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const float* cv,
-                             const float* vw, float wnorm, const float* plane, int x, int y) {
+real pmo_weighted_cost_geom(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const real* cv,
+                             const real* vw, real wnorm, const real* plane, int x, int y) {
   Ctx c(sc, pr);
   return weighted_cost_geom(c, st, cv, vw, wnorm, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y);
 }
-float pmo_mc_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const float* cv, const float* vw,
-                  float wnorm, const float* plane, int x, int y, uint32_t* rng, uint32_t key) {
+real pmo_mc_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, const real* cv, const real* vw,
+                  real wnorm, const real* plane, int x, int y, uint32_t* rng, uint32_t key) {
   Ctx c(sc, pr);
   return mc_cost(c, st, cv, vw, wnorm, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, rng, key);
 }
@@ -1502,46 +1523,46 @@ float pmo_mc_cost(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st
 // View weights at (x, y): cand_costs is (8, num_src) row-major, flag bit j
 // marks candidate j present.  Writes num_src weights and the selection mask;
 // returns the weight sum.
-float pmo_view_weights(const pmo_scene* sc, const pmo_params* pr, const float* cand_costs, uint32_t flag_bits,
-                       uint32_t prev_sel, const float* plane, int x, int y, float* weights, uint32_t* sel) {
+real pmo_view_weights(const pmo_scene* sc, const pmo_params* pr, const real* cand_costs, uint32_t flag_bits,
+                       uint32_t prev_sel, const real* plane, int x, int y, real* weights, uint32_t* sel) {
   Ctx c(sc, pr);
-  float ca[8 * kMaxSrc];
+  real ca[8 * kMaxSrc];
   bool flags[8];
   for (int j = 0; j < 8; ++j) {
     flags[j] = (flag_bits >> j) & 1u;
     for (int i = 0; i < kMaxSrc; ++i) ca[j * kMaxSrc + i] = i < c.n_src ? cand_costs[j * c.n_src + i] : 2.0f;
   }
-  float vw[kMaxSrc];
-  const float wn = view_weights(c, ca, flags, prev_sel, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, vw, sel);
+  real vw[kMaxSrc];
+  const real wn = view_weights(c, ca, flags, prev_sel, Plane{plane[0], plane[1], plane[2], plane[3]}, x, y, vw, sel);
   for (int i = 0; i < c.n_src; ++i) weights[i] = vw[i];
   return wn;
 }
 
 // Refinement hypotheses at (x, y) for the current plane (and an optional
 // prior plane); writes n planes (n, d) to out and returns n.  rng advances.
-int pmo_refinement_hypotheses(const pmo_scene* sc, const pmo_params* pr, const float* planes, int x, int y,
-                              const float* plane_now, const float* prior_plane, uint32_t* rng, uint32_t key,
-                              float* out) {
+int pmo_refinement_hypotheses(const pmo_scene* sc, const pmo_params* pr, const real* planes, int x, int y,
+                              const real* plane_now, const real* prior_plane, uint32_t* rng, uint32_t key,
+                              real* out) {
   Ctx c(sc, pr);
   const pmo_camera& c0 = c.cams[0];
   const Plane pn{plane_now[0], plane_now[1], plane_now[2], plane_now[3]};
-  const float depth_now = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pn, c0);
+  const real depth_now = depth_from_plane(static_cast<real>(x), static_cast<real>(y), pn, c0);
   bool have_prior = prior_plane != nullptr;
-  float depth_prior = 0.0f;
+  real depth_prior = 0.0f;
   V3 prior_n{0.0f, 0.0f, -1.0f};
   if (have_prior) {
     const Plane pp{prior_plane[0], prior_plane[1], prior_plane[2], prior_plane[3]};
-    depth_prior = depth_from_plane(static_cast<float>(x), static_cast<float>(y), pp, c0);
+    depth_prior = depth_from_plane(static_cast<real>(x), static_cast<real>(y), pp, c0);
     prior_n = normalize3(V3{pp.x, pp.y, pp.z});
   }
-  const float depth_sigma = (pr->depth_max - pr->depth_min) / 32.0f;
-  const float angle_sigma = kPiF * (5.0f / 180.0f);
-  float rd[6];
+  const real depth_sigma = (pr->depth_max - pr->depth_min) / 32.0f;
+  const real angle_sigma = kPiF * (5.0f / 180.0f);
+  real rd[6];
   V3 rn[6];
   const int n = refinement_hypotheses(c, planes, x, y, pn, depth_now, have_prior, depth_prior, prior_n,
                                       depth_sigma, angle_sigma, rng, key, rd, rn);
   for (int h = 0; h < n; ++h) {
-    const Plane q = plane_from_depth_normal(static_cast<float>(x), static_cast<float>(y), rd[h], rn[h], c0);
+    const Plane q = plane_from_depth_normal(static_cast<real>(x), static_cast<real>(y), rd[h], rn[h], c0);
     out[4 * h] = q.x; out[4 * h + 1] = q.y; out[4 * h + 2] = q.z; out[4 * h + 3] = q.w;
   }
   return n;
@@ -1552,13 +1573,13 @@ int pmo_refinement_hypotheses(const pmo_scene* sc, const pmo_params* pr, const f
 // The local affine warp of cost_vector for one source: out = (scx, scy,
 // jxx, jxy, jyx, jyy); returns -1 when a plane depth is <= 0, else
 // source_warp's code (0 valid, 1 behind the source camera, 2 det gate).
-int pmo_warp(const pmo_scene* sc, const pmo_params* pr, int x, int y, const float* plane, int src, float* out) {
+int pmo_warp(const pmo_scene* sc, const pmo_params* pr, int x, int y, const real* plane, int src, real* out) {
   Ctx c(sc, pr);
   PlanePoints pp;
   if (!plane_points(c.cams[0], x, y, load_plane(plane, 0), &pp)) return -1;
   Warp w{};
   const int code = source_warp(pp, c.cams[src + 1], &w);
-  const float v[6] = {w.scx, w.scy, w.jxx, w.jxy, w.jyx, w.jyy};
+  const real v[6] = {w.scx, w.scy, w.jxx, w.jxy, w.jyx, w.jyy};
   std::memcpy(out, v, sizeof(v));
   return code;
 }
@@ -1566,15 +1587,15 @@ int pmo_warp(const pmo_scene* sc, const pmo_params* pr, int x, int y, const floa
 // The strided NCC of the textureless fallback on a given warp w = (scx, scy,
 // jxx, jxy, jyx, jyy) of source src at reference pixel (x, y); the source
 // centre value is the bilinear sample at (scx, scy), as in cost_vector.
-float pmo_ncc_at_stride(const pmo_scene* sc, const pmo_params* pr, int x, int y, int src, float stride,
-                        const float* w) {
+real pmo_ncc_at_stride(const pmo_scene* sc, const pmo_params* pr, int x, int y, int src, real stride,
+                        const real* w) {
   Ctx c(sc, pr);
-  const float scen = c.src[src].bilinear(w[0], w[1]);
+  const real scen = c.src[src].bilinear(w[0], w[1]);
   return ncc_at_stride(c, src, c.ref.texel(x, y), scen, w[0], w[1], x, y, stride, w[2], w[3], w[4], w[5]);
 }
 
 void pmo_score_planes(const pmo_scene* sc, const pmo_params* pr, int n, const int* xy,
-                      const float* planes, float* costs, float* init_cost, uint32_t* sel) {
+                      const real* planes, real* costs, real* init_cost, uint32_t* sel) {
   Ctx c(sc, pr);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < n; ++i) {
@@ -1582,12 +1603,12 @@ void pmo_score_planes(const pmo_scene* sc, const pmo_params* pr, int n, const in
     RefPatch rp;
     compute_ref_patch(c, x, y, &rp);
     const Plane p{planes[4 * i], planes[4 * i + 1], planes[4 * i + 2], planes[4 * i + 3]};
-    float cv[kMaxSrc];
+    real cv[kMaxSrc];
     for (int k = 0; k < kMaxSrc; ++k) cv[k] = 2.0f;
     cost_vector(c, x, y, p, rp, cv);
     for (int k = 0; k < c.n_src; ++k) costs[static_cast<int64_t>(i) * c.n_src + k] = cv[k];
     uint32_t s = 0u;
-    const float ic = initial_cost_and_views(c, x, y, p, rp, &s);
+    const real ic = initial_cost_and_views(c, x, y, p, rp, &s);
     if (init_cost) init_cost[i] = ic;
     if (sel) sel[i] = s;
   }
@@ -1610,7 +1631,7 @@ void pmo_plane_sweep(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, i
     for (int x = 0; x < W; ++x) plane_sweep_px(c, st, x, y, num_planes);
 }
 
-float pmo_sweep_depth(const pmo_params* pr, int i, int n) { return sweep_depth(*pr, i, n); }
+real pmo_sweep_depth(const pmo_params* pr, int i, int n) { return sweep_depth(*pr, i, n); }
 
 void pmo_prior_reinit(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
   Ctx c(sc, pr);
@@ -1619,8 +1640,8 @@ void pmo_prior_reinit(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) 
   // Every pixel runs at once here (no colour split), and the smoothness term
   // reads 4-neighbour planes from the pre-kernel snapshot (planes_img at
   // opencl_kernels.h:1144,1170), so those reads go to a copy.
-  float* snap = new float[static_cast<size_t>(W) * H * 4];
-  std::memcpy(snap, st->planes, sizeof(float) * 4 * static_cast<size_t>(W) * H);
+  real* snap = new real[static_cast<size_t>(W) * H * 4];
+  std::memcpy(snap, st->planes, sizeof(real) * 4 * static_cast<size_t>(W) * H);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) prior_reinit_px(c, st, snap, x, y);
@@ -1628,7 +1649,7 @@ void pmo_prior_reinit(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) 
 }
 
 void pmo_sweep(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, int color, int iteration,
-               float smooth_w) {
+               real smooth_w) {
   Ctx c(sc, pr);
   for_colour(sc->width, sc->height, color, [&](int x, int y) { patchmatch_px(c, st, x, y, iteration, smooth_w); });
 }
@@ -1639,45 +1660,45 @@ void pmo_checkerboard(const pmo_scene* sc, const pmo_params* pr, pmo_state* st, 
 }
 
 // Smooth weight of iteration `iter` (estimator.cc:646-649).
-float pmo_smooth_weight(const pmo_params* pr, int iter) {
-  const float decay = 1.0f - 0.9f * static_cast<float>(iter) /
-                                 static_cast<float>(std::max(pr->max_iterations - 1, 1));
+real pmo_smooth_weight(const pmo_params* pr, int iter) {
+  const real decay = 1.0f - 0.9f * static_cast<real>(iter) /
+                                 static_cast<real>(std::max(pr->max_iterations - 1, 1));
   return pr->smooth_weight * decay;
 }
 
 // Decode planes/costs into depth, normal, cost, confidence
 // (estimator.cc:1112-1224; speckle / gap filling off, median off).
-void pmo_decode(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, float* depth,
-                float* normal, float* cost, float* confidence) {
+void pmo_decode(const pmo_scene* sc, const pmo_params* pr, const pmo_state* st, real* depth,
+                real* normal, real* cost, real* confidence) {
   const int W = sc->width, H = sc->height;
   const pmo_camera& c0 = sc->cams[0];
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < H; ++y) {
     for (int x = 0; x < W; ++x) {
       const int64_t idx = static_cast<int64_t>(y) * W + x;
-      const float* p = &st->planes[4 * idx];
+      const real* p = &st->planes[4 * idx];
       depth[idx] = 0.0f;
       normal[3 * idx] = 0.0f; normal[3 * idx + 1] = 0.0f; normal[3 * idx + 2] = 0.0f;
       cost[idx] = 0.0f;
       confidence[idx] = 0.0f;
       if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || !std::isfinite(p[3]))
         continue;
-      const float rx = (static_cast<float>(x) - c0.K[2]) / c0.K[0];
-      const float ry = (static_cast<float>(y) - c0.K[5]) / c0.K[4];
-      const float denom = p[0] * rx + p[1] * ry + p[2] * 1.0f;
-      float d = (std::fabs(denom) > 1e-6f) ? -p[3] / denom : 0.0f;
+      const real rx = (static_cast<real>(x) - c0.K[2]) / c0.K[0];
+      const real ry = (static_cast<real>(y) - c0.K[5]) / c0.K[4];
+      const real denom = p[0] * rx + p[1] * ry + p[2] * 1.0f;
+      real d = (std::fabs(denom) > 1e-6f) ? -p[3] / denom : 0.0f;
       if (!std::isfinite(d)) d = 0.0f;
       if (d > 0.0f && d >= pr->depth_min && d <= pr->depth_max) {
         depth[idx] = d;
-        const float len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        const real len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
         if (len > 1e-6f) {
           normal[3 * idx] = p[0] / len; normal[3 * idx + 1] = p[1] / len; normal[3 * idx + 2] = p[2] / len;
         }
       }
-      const float cc = st->costs[idx];
+      const real cc = st->costs[idx];
       cost[idx] = std::isfinite(cc) ? cc : 0.0f;
       if (depth[idx] > 0.0f)
-        confidence[idx] = 1.0f / (1.0f + pm_expf((cost[idx] - 0.5f) * 5.0f));
+        confidence[idx] = 1.0f / (1.0f + rexp((cost[idx] - 0.5f) * 5.0f));
     }
   }
 }
@@ -1689,7 +1710,7 @@ void pmo_run(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
   pmo_random_init(sc, pr, st);
   pmo_prior_reinit(sc, pr, st);
   for (int it = 0; it < pr->max_iterations; ++it) {
-    const float sw = pmo_smooth_weight(pr, it);
+    const real sw = pmo_smooth_weight(pr, it);
     pmo_sweep(sc, pr, st, 0, it, sw);
     pmo_sweep(sc, pr, st, 1, it, sw);
   }
@@ -1713,16 +1734,16 @@ void pmo_run(const pmo_scene* sc, const pmo_params* pr, pmo_state* st) {
 // min_consistent_views is the effective minimum Clean passes,
 // min(min_consistent_views, num_views) (cleaner.cc:222).
 // ---------------------------------------------------------------------------
-void pmo_clean_depthmap(const float* ref_depth, const float* normal, int w, int h, const float* const* src_depths,
-                        const pmo_camera* cams, int n_src_in, float same_depth_threshold, int min_consistent_views,
-                        float carving_threshold, int max_carved_views, float grazing_cos_threshold, float* out) {
+void pmo_clean_depthmap(const real* ref_depth, const real* normal, int w, int h, const real* const* src_depths,
+                        const pmo_camera* cams, int n_src_in, real same_depth_threshold, int min_consistent_views,
+                        real carving_threshold, int max_carved_views, real grazing_cos_threshold, real* out) {
   const int n_src = std::min(n_src_in, kMaxSrc);  // MAX_CLEAN_SOURCES 16 (:2057)
   const pmo_camera& rc = cams[0];
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < h; ++y) {
     for (int x = 0; x < w; ++x) {
       const int idx = y * w + x;
-      const float ref_d = ref_depth[idx];
+      const real ref_d = ref_depth[idx];
       if (!(ref_d > 0.0f)) {
         out[idx] = 0.0f;
         continue;
@@ -1730,63 +1751,63 @@ void pmo_clean_depthmap(const float* ref_depth, const float* normal, int w, int 
       // grazing angle from the normal (:2142-2160)
       int is_grazing = 0;
       if (normal) {
-        const float nnx = normal[3 * idx], nny = normal[3 * idx + 1], nnz = normal[3 * idx + 2];
-        const float nlen = std::sqrt(nnx * nnx + nny * nny + nnz * nnz);
+        const real nnx = normal[3 * idx], nny = normal[3 * idx + 1], nnz = normal[3 * idx + 2];
+        const real nlen = std::sqrt(nnx * nnx + nny * nny + nnz * nnz);
         if (nlen > 1e-6f) {
-          const float ray_x = (static_cast<float>(x) - rc.K[2]) / rc.K[0];
-          const float ray_y = (static_cast<float>(y) - rc.K[5]) / rc.K[4];
-          const float ray_z = 1.0f;
-          const float rlen = std::sqrt(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
-          const float cos_angle = std::fabs((nnx * ray_x + nny * ray_y + nnz * ray_z) / (nlen * rlen));
+          const real ray_x = (static_cast<real>(x) - rc.K[2]) / rc.K[0];
+          const real ray_y = (static_cast<real>(y) - rc.K[5]) / rc.K[4];
+          const real ray_z = 1.0f;
+          const real rlen = std::sqrt(ray_x * ray_x + ray_y * ray_y + ray_z * ray_z);
+          const real cos_angle = std::fabs((nnx * ray_x + nny * ray_y + nnz * ray_z) / (nlen * rlen));
           is_grazing = (cos_angle < grazing_cos_threshold) ? 1 : 0;
         }
       }
       const int is_suspicious = is_grazing;
-      const float fx = rc.K[0], cx = rc.K[2], fy = rc.K[4], cy = rc.K[5];
+      const real fx = rc.K[0], cx = rc.K[2], fy = rc.K[4], cy = rc.K[5];
       // back-project into the world (:2164-2177)
-      const float cam_x = (static_cast<float>(x) - cx) / fx * ref_d;
-      const float cam_y = (static_cast<float>(y) - cy) / fy * ref_d;
-      const float cam_z = ref_d;
-      const float pcx = cam_x - rc.t[0], pcy = cam_y - rc.t[1], pcz = cam_z - rc.t[2];
-      const float wx = rc.R[0] * pcx + rc.R[3] * pcy + rc.R[6] * pcz;
-      const float wy = rc.R[1] * pcx + rc.R[4] * pcy + rc.R[7] * pcz;
-      const float wz = rc.R[2] * pcx + rc.R[5] * pcy + rc.R[8] * pcz;
+      const real cam_x = (static_cast<real>(x) - cx) / fx * ref_d;
+      const real cam_y = (static_cast<real>(y) - cy) / fy * ref_d;
+      const real cam_z = ref_d;
+      const real pcx = cam_x - rc.t[0], pcy = cam_y - rc.t[1], pcz = cam_z - rc.t[2];
+      const real wx = rc.R[0] * pcx + rc.R[3] * pcy + rc.R[6] * pcz;
+      const real wy = rc.R[1] * pcx + rc.R[4] * pcy + rc.R[7] * pcz;
+      const real wz = rc.R[2] * pcx + rc.R[5] * pcy + rc.R[8] * pcz;
       int consistent = 1;
       int carved = 0;
       for (int v = 0; v < n_src; ++v) {
         const pmo_camera& sc = cams[v + 1];
-        const float sx = sc.R[0] * wx + sc.R[1] * wy + sc.R[2] * wz + sc.t[0];
-        const float sy = sc.R[3] * wx + sc.R[4] * wy + sc.R[5] * wz + sc.t[1];
-        const float sz = sc.R[6] * wx + sc.R[7] * wy + sc.R[8] * wz + sc.t[2];
+        const real sx = sc.R[0] * wx + sc.R[1] * wy + sc.R[2] * wz + sc.t[0];
+        const real sy = sc.R[3] * wx + sc.R[4] * wy + sc.R[5] * wz + sc.t[1];
+        const real sz = sc.R[6] * wx + sc.R[7] * wy + sc.R[8] * wz + sc.t[2];
         if (sz <= 1e-6f) continue;
-        const float su = sc.K[0] * sx / sz + sc.K[2];
-        const float sv = sc.K[4] * sy / sz + sc.K[5];
-        if (su < 0.0f || sv < 0.0f || su >= static_cast<float>(w - 1) || sv >= static_cast<float>(h - 1)) continue;
-        const float src_d = Img{src_depths[v], w, h}.bilinear(su, sv);
+        const real su = sc.K[0] * sx / sz + sc.K[2];
+        const real sv = sc.K[4] * sy / sz + sc.K[5];
+        if (su < 0.0f || sv < 0.0f || su >= static_cast<real>(w - 1) || sv >= static_cast<real>(h - 1)) continue;
+        const real src_d = Img{src_depths[v], w, h}.bilinear(su, sv);
         if (!(src_d > 0.0f)) continue;
         // the source surface point back in the world (:2212-2226, 2239-2252)
-        const float scx = (su - sc.K[2]) / sc.K[0] * src_d;
-        const float scy = (sv - sc.K[5]) / sc.K[4] * src_d;
-        const float scz = src_d;
-        const float qx = scx - sc.t[0], qy = scy - sc.t[1], qz = scz - sc.t[2];
-        const float wx2 = sc.R[0] * qx + sc.R[3] * qy + sc.R[6] * qz;
-        const float wy2 = sc.R[1] * qx + sc.R[4] * qy + sc.R[7] * qz;
-        const float wz2 = sc.R[2] * qx + sc.R[5] * qy + sc.R[8] * qz;
+        const real scx = (su - sc.K[2]) / sc.K[0] * src_d;
+        const real scy = (sv - sc.K[5]) / sc.K[4] * src_d;
+        const real scz = src_d;
+        const real qx = scx - sc.t[0], qy = scy - sc.t[1], qz = scz - sc.t[2];
+        const real wx2 = sc.R[0] * qx + sc.R[3] * qy + sc.R[6] * qz;
+        const real wy2 = sc.R[1] * qx + sc.R[4] * qy + sc.R[7] * qz;
+        const real wz2 = sc.R[2] * qx + sc.R[5] * qy + sc.R[8] * qz;
         // space carving: the source sees further (:2210-2235)
         if (src_d > sz * (1.0f + carving_threshold)) {
-          const float rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
+          const real rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
           if (rz > 1e-6f && rz > ref_d * (1.0f + same_depth_threshold)) carved++;
           continue;
         }
         // forward depth check, then back into the reference (:2238-2267)
         if (std::fabs(src_d - sz) >= sz * same_depth_threshold) continue;
-        const float rx = rc.R[0] * wx2 + rc.R[1] * wy2 + rc.R[2] * wz2 + rc.t[0];
-        const float ry = rc.R[3] * wx2 + rc.R[4] * wy2 + rc.R[5] * wz2 + rc.t[1];
-        const float rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
+        const real rx = rc.R[0] * wx2 + rc.R[1] * wy2 + rc.R[2] * wz2 + rc.t[0];
+        const real ry = rc.R[3] * wx2 + rc.R[4] * wy2 + rc.R[5] * wz2 + rc.t[1];
+        const real rz = rc.R[6] * wx2 + rc.R[7] * wy2 + rc.R[8] * wz2 + rc.t[2];
         if (rz <= 1e-6f) continue;
-        const float ru = fx * rx / rz + cx;
-        const float rv = fy * ry / rz + cy;
-        if (std::fabs(ru - static_cast<float>(x)) + std::fabs(rv - static_cast<float>(y)) > 2.0f) continue;
+        const real ru = fx * rx / rz + cx;
+        const real rv = fy * ry / rz + cy;
+        if (std::fabs(ru - static_cast<real>(x)) + std::fabs(rv - static_cast<real>(y)) > 2.0f) continue;
         if (std::fabs(rz - ref_d) >= ref_d * same_depth_threshold) continue;
         consistent++;
       }

@@ -133,7 +133,7 @@ def make_camera(K, R, t, w, h) -> PmoCamera:
     return c
 
 
-def angle_weights(Rs: Sequence[np.ndarray], ts: Sequence[np.ndarray]) -> np.ndarray:
+def angle_weights(Rs: Sequence[np.ndarray], ts: Sequence[np.ndarray], dtype=np.float32) -> np.ndarray:
     """max(sin(angle(baseline, reference optical axis)), 0.01) per source (estimator.cc:380-407)."""
     c0 = -np.asarray(Rs[0]).T @ np.asarray(ts[0])
     z0 = np.asarray(Rs[0])[2]
@@ -147,7 +147,7 @@ def angle_weights(Rs: Sequence[np.ndarray], ts: Sequence[np.ndarray]) -> np.ndar
             continue
         ca = abs(z0 @ (b / bl))
         out.append(max(np.sqrt(1.0 - ca * ca), 0.01))
-    return np.asarray(out, np.float64).astype(np.float32)
+    return np.asarray(out, np.float64).astype(dtype)
 
 
 @dataclass
@@ -808,3 +808,123 @@ def select_neighbors(origins, coords, visibility, track_counts, num_neighbors: i
         best.append(b)
         allv.append(b + [o for o in sorted(na) if o not in set(b)])
     return best, allv
+
+
+# ---------------------------------------------------------------------------
+# FP64 evaluation: libpm_oracle64.so is pm_oracle.cc built with -DPMO_F64,
+# the same algorithm (same random draws) with every value, array and
+# interface field in double and libm's exp / log.  Views are passed as their
+# float32 values widened exactly; cameras keep their float64 K, R, t.
+# ---------------------------------------------------------------------------
+_LIB64_PATH = os.path.join(_HERE, "libpm_oracle64.so")
+_lib64 = None
+
+
+class PmoCamera64(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_double * 9), ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3),
+                ("width", ctypes.c_int), ("height", ctypes.c_int)]
+
+
+class PmoParams64(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double if t is ctypes.c_float else t) for n, t in PmoParams._fields_]
+
+
+def lib64():
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(_LIB64_PATH):
+            build()
+        L = ctypes.CDLL(_LIB64_PATH)
+        i, vp = ctypes.c_int, ctypes.c_void_p
+        L.pmo_run.argtypes = [vp, vp, vp]
+        L.pmo_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.pmo_postprocess.argtypes = [vp, vp, vp, vp, i, i, i, i]
+        L.pmo_score_planes.argtypes = [vp, vp, i, vp, vp, vp, vp, vp]
+        L.pmo_set_threads.argtypes = [i]
+        L.pmo_max_threads.restype = i
+        _lib64 = L
+    return _lib64
+
+
+def params64(params: Params) -> PmoParams64:
+    """The pmo_params of ``params`` with its real fields in double."""
+    p = PmoParams64()
+    p.half_patch = params.patch_size // 2
+    p.sigma_spatial = params.sigma_spatial
+    p.sigma_color = params.sigma_color
+    p.top_k = params.top_k
+    p.census_weight = 1.0 if params.use_census else 0.0
+    p.depth_min = params.depth_min
+    p.depth_max = params.depth_max
+    p.max_iterations = params.max_iterations
+    p.smooth_weight = params.smooth_weight
+    p.anchor_views = params.anchor_views
+    p.geom_weight = params.geom_weight
+    p.var_threshold = params.var_threshold
+    p.checkerboard_filter = 1 if params.checkerboard_filter else 0
+    return p
+
+
+class OracleScene64(OracleScene):
+    """Views and cameras packed for the FP64 build."""
+
+    def __init__(self, images: Sequence[np.ndarray], Ks, Rs, ts):
+        super().__init__(images, Ks, Rs, ts)
+        self.images = self.images.astype(np.float64)
+        cams = (PmoCamera64 * self.n_views)()
+        for k in range(self.n_views):
+            c = cams[k]
+            c.K[:] = np.asarray(Ks[k], np.float64).ravel().tolist()
+            c.R[:] = np.asarray(Rs[k], np.float64).ravel().tolist()
+            c.t[:] = np.asarray(ts[k], np.float64).ravel().tolist()
+            c.width, c.height = int(self.widths[k]), int(self.heights[k])
+        self.cams = cams
+        self.aw = angle_weights(Rs, ts, np.float64) if self.n_views > 1 else np.zeros(1, np.float64)
+        self.c.images = self.images.ctypes.data
+        self.c.cams = ctypes.addressof(self.cams)
+        self.c.angle_weights = self.aw.ctypes.data
+
+
+class OracleState64(OracleState):
+    def __init__(self, w: int, h: int):
+        super().__init__(w, h)
+        self.planes = np.zeros((h, w, 4), np.float64)
+        self.costs = np.zeros((h, w), np.float64)
+        self.c.planes = self.planes.ctypes.data
+        self.c.costs = self.costs.ctypes.data
+
+
+def estimate_f64(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params):
+    """Full single-level run + decode in double; returns (depth, normal, cost, confidence), float64."""
+    sc = OracleScene64(images, Ks, Rs, ts)
+    st = OracleState64(sc.width, sc.height)
+    p = params64(params)
+    L = lib64()
+    L.pmo_run(ctypes.byref(sc.c), ctypes.byref(p), ctypes.byref(st.c))
+    h, w = sc.height, sc.width
+    depth = np.zeros((h, w), np.float64)
+    normal = np.zeros((h, w, 3), np.float64)
+    cost = np.zeros((h, w), np.float64)
+    conf = np.zeros((h, w), np.float64)
+    L.pmo_decode(ctypes.byref(sc.c), ctypes.byref(p), ctypes.byref(st.c), _ptr(depth), _ptr(normal), _ptr(cost),
+                 _ptr(conf))
+    if params.speckle_min_size > 0 or params.gap_max_size > 0:
+        L.pmo_postprocess(_ptr(depth), _ptr(normal), _ptr(cost), _ptr(conf), w, h, int(params.speckle_min_size),
+                          int(params.gap_max_size))
+    return depth, normal, cost, conf
+
+
+def score_planes_f64(images: Sequence[np.ndarray], Ks, Rs, ts, params: Params, xy: np.ndarray, planes: np.ndarray):
+    """score_planes of the FP64 build: per-view costs (n, num_src), float64."""
+    sc = OracleScene64(images, Ks, Rs, ts)
+    xy = np.ascontiguousarray(xy, np.int32)
+    planes = np.ascontiguousarray(planes, np.float64)
+    n = xy.shape[0]
+    costs = np.zeros((n, sc.n_views - 1), np.float64)
+    init = np.zeros(n, np.float64)
+    sel = np.zeros(n, np.uint32)
+    p = params64(params)
+    lib64().pmo_score_planes(ctypes.byref(sc.c), ctypes.byref(p), n, _ptr(xy), _ptr(planes), _ptr(costs),
+                             _ptr(init), _ptr(sel))
+    return costs, init, sel
+

@@ -19,20 +19,20 @@ FLAGS = ["-O1", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared
 # (name, text in pm_oracle.cc, planted mistake)
 MUTANTS = [
     ("strided weight without the stride",
-     "const float w = pm_expf(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +",
-     "const float w = pm_expf(static_cast<float>(ix * ix) * c.spatial_factor + fdy * fdy * c.spatial_factor +"),
+     "const real w = rexp(fdx * fdx * c.spatial_factor + fdy * fdy * c.spatial_factor +",
+     "const real w = rexp(static_cast<real>(ix * ix) * c.spatial_factor + fdy * fdy * c.spatial_factor +"),
     ("strided source x/y Jacobian terms swapped",
-     "const float sx = std::fmaf(jxx, fdx, bx);\n      const float sy = std::fmaf(jxy, fdx, by);\n      const float src_val",
-     "const float sx = std::fmaf(jxy, fdx, bx);\n      const float sy = std::fmaf(jxx, fdx, by);\n      const float src_val"),
+     "const real sx = rfma(jxx, fdx, bx);\n      const real sy = rfma(jxy, fdx, by);\n      const real src_val",
+     "const real sx = rfma(jxy, fdx, bx);\n      const real sy = rfma(jxx, fdx, by);\n      const real src_val"),
     ("strided reference row off by one",
-     "const float ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));",
-     "const float ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry) + 1);"),
+     "const real ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry));",
+     "const real ref_val = c.ref.texel(static_cast<int>(rx), static_cast<int>(ry) + 1);"),
     ("strided row offset not scaled by the stride",
-     "const float bx = std::fmaf(jyx, fdy, scx);\n    const float by = std::fmaf(jyy, fdy, scy);",
-     "const float bx = std::fmaf(jyx, static_cast<float>(iy), scx);\n    const float by = std::fmaf(jyy, static_cast<float>(iy), scy);"),
+     "const real bx = rfma(jyx, fdy, scx);\n    const real by = rfma(jyy, fdy, scy);",
+     "const real bx = rfma(jyx, static_cast<real>(iy), scx);\n    const real by = rfma(jyy, static_cast<real>(iy), scy);"),
     ("strided covariance without r",
-     "sum_rs = std::fmaf(w * r, sv, sum_rs);",
-     "sum_rs = std::fmaf(w, sv, sum_rs);"),
+     "sum_rs = rfma(w * r, sv, sum_rs);",
+     "sum_rs = rfma(w, sv, sum_rs);"),
     ("warp Jacobian transposed",
      "  w->jxy = pdx.y / pdx.z - w->scy;\n  w->jyx = pdy.x / pdy.z - w->scx;",
      "  w->jxy = pdy.x / pdy.z - w->scx;\n  w->jyx = pdx.y / pdx.z - w->scy;"),
