@@ -42,12 +42,15 @@ UNIT = "Mpix·views·iters/s"
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PATCH = 7
 FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap x P taps + 75)
-# HBM bytes per candidates_kernel<3, float, 4> launch: rocprofv3 --pmc
-# FETCH_SIZE (x 1.9999) + WRITE_SIZE (x 1.0) on this workload, each factor
-# read off the same counter on a 1 GiB device-to-device copy
-# (profiles/*_hbm_traffic.txt).
+# HBM bytes per candidates launch: rocprofv3 --pmc FETCH_SIZE (x 1.9999) +
+# WRITE_SIZE (x 1.0), each factor read off the same counter on a 1 GiB
+# device-to-device copy (profiles/*_fetch.txt, profiles/*_write.txt).
+# candidates_kernel<3, float, 4>, 1600x1200 with 8 sources:
 TRAFFIC_PER_LAUNCH = 2.135e9
 TRAFFIC_TILE = 4
+# candidates_batch<3, _Float16, 4> over a group of ten 4000x3000 references
+# with 10 sources each:
+TRAFFIC_PER_LAUNCH_C5 = 1.4345e11
 
 
 def log(msg: str) -> None:
@@ -539,7 +542,9 @@ def run_c5(args, pydense) -> int:
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     sweep_ms = sum(v[0] for k, v in stages.items() if k != "init")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": (TRAFFIC_PER_LAUNCH_C5 if storage == "f16" and group == 10 and (w, h) == (4000, 3000)
+                            and args.neighbours == 10 else None),
                 "kernel": f"candidates_batch<3,{'_Float16' if storage == 'f16' else 'float'},*>",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     path_tflops = units * 1e6 * FLOP_PER_UNIT / (pm_ms / 1e3) / 1e12 if pm_ms > 0 else 0.0
