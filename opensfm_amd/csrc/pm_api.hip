@@ -1474,7 +1474,11 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   for (int i = 0; i < steps; ++i)
     PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, state, i == 0, h->stream),
            "Gauss-Newton kernel launch");
-  if (steps > 0) PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, h->stream), "refinement guard launch");
+  // the guard's list of moved pixels takes the state buffer, free after the
+  // last step (it holds gauss_newton_state_floats() >= 2 floats per pixel)
+  if (steps > 0)
+    PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, reinterpret_cast<int*>(state), h->stream),
+           "refinement guard launch");
   PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
   if (int r = pm_decode(h)) return r;
   if (int r = postprocess(h)) return r;

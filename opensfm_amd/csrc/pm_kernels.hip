@@ -120,37 +120,68 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
 // the PatchMatch plane `orig`.  (Without the smoothness term, a step that
 // fits the images' noise moves a pixel away from its neighbours' surface.)
 // ---------------------------------------------------------------------------
+// The guard runs on the moved pixels only, compacted into a list first:
+// moves are sparse, and one moved pixel in a wave would otherwise hold the
+// whole wave for two full cost vectors.  list[] and *count live in the
+// refinement's state buffer, free once its steps are done; the order of the
+// list is arbitrary and changes no result (pixels are independent).
+constexpr unsigned kGuardWaves = 2048;  // one-wave workgroups looping over the list: 8 per CU
+__global__ __launch_bounds__(kWave) void guard_clear_kernel(int* count) { *count = 0; }
+
+__global__ __launch_bounds__(256) void guard_list_kernel(const float4* __restrict__ planes,
+                                                         const float4* __restrict__ orig, int64_t n, int* list,
+                                                         int* count) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool moved = false;
+  if (i < n) {
+    const float4 pn = planes[i], po = orig[i];
+    moved = !(pn.x == po.x && pn.y == po.y && pn.z == po.z && pn.w == po.w);
+  }
+  const uint64_t m = __ballot(moved);
+  if (m == 0ull) return;
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  int base = 0;
+  if (lane == __ffsll(static_cast<long long>(m)) - 1) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, __ffsll(static_cast<long long>(m)) - 1);
+  if (moved) list[base + __popcll(m & ((1ull << lane) - 1ull))] = static_cast<int>(i);
+}
+
 template <int HP, typename T>
-__global__ __launch_bounds__(kWave) void refine_guard_kernel(PmArgs a, const float4* __restrict__ orig) {
+__global__ __launch_bounds__(kWave) void refine_guard_kernel(PmArgs a, const float4* __restrict__ orig,
+                                                             const int* __restrict__ list,
+                                                             const int* __restrict__ count) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kWave + lane;
-  if (i >= static_cast<int64_t>(a.width) * a.height) return;
-  const float4 pn = a.planes[i], po = orig[i];
-  if (pn.x == po.x && pn.y == po.y && pn.z == po.z && pn.w == po.w) return;
-  const uint32_t sel = a.sel[i];
-  const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
+  const int total = __builtin_amdgcn_readfirstlane(*count);
   const T* img = images_of<T>(a);
   float* cv = smem + lane;
-  Nbrs nb;
-  load_nbrs(a, orig, x, y, nb);
-  float obj[2];
-  const float4 pl[2] = {po, pn};
 #pragma unroll 1
-  for (int k = 0; k < 2; ++k) {
-    const Plane p = to_plane(pl[k]);
-    full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), p, cv);
-    float sum = 0.0f;
-    int n = 0;
-    for (int s = 0; s < a.n_src; ++s) {
-      if (!((sel >> s) & 1u)) continue;
-      sum += cv[s * kWave];
-      ++n;
+  for (int base = static_cast<int>(blockIdx.x) * kWave; base < total; base += static_cast<int>(gridDim.x) * kWave) {
+    if (base + lane >= total) continue;
+    const int64_t i = list[base + lane];
+    const float4 pn = a.planes[i], po = orig[i];
+    const uint32_t sel = a.sel[i];
+    const int x = static_cast<int>(i % a.width), y = static_cast<int>(i / a.width);
+    Nbrs nb;
+    load_nbrs(a, orig, x, y, nb);
+    float obj[2];
+    const float4 pl[2] = {po, pn};
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+      const Plane p = to_plane(pl[k]);
+      full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), p, cv);
+      float sum = 0.0f;
+      int n = 0;
+      for (int s = 0; s < a.n_src; ++s) {
+        if (!((sel >> s) & 1u)) continue;
+        sum += cv[s * kWave];
+        ++n;
+      }
+      obj[k] = n > 0 ? sum / static_cast<float>(n) : 0.0f;
+      if (a.smooth_w > 1e-6f) obj[k] += a.smooth_w * smoothness_cost(a, nb, p, x, y);
     }
-    obj[k] = n > 0 ? sum / static_cast<float>(n) : 0.0f;
-    if (a.smooth_w > 1e-6f) obj[k] += a.smooth_w * smoothness_cost(a, nb, p, x, y);
+    if (!(obj[1] <= obj[0])) a.planes[i] = po;
   }
-  if (!(obj[1] <= obj[0])) a.planes[i] = po;
 }
 
 int64_t xcd_round(int64_t blocks) { return kXcds * ((blocks + kXcds - 1) / kXcds); }
@@ -533,11 +564,18 @@ hipError_t launch_score(const PmArgs& a, int hp, bool f16, int n, const int32_t*
   PM_DISPATCH(score_kernel, hp, f16, grid, lds, stream, a, n, xy, planes, costs, init_cost, sel);
 }
 
-hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, hipStream_t stream) {
+hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, int* scratch,
+                               hipStream_t stream) {
   const int64_t n = static_cast<int64_t>(a.width) * a.height;
-  const dim3 grid(static_cast<unsigned>((n + kWave - 1) / kWave));
+  int* count = scratch + n;
+  hipLaunchKernelGGL(guard_clear_kernel, dim3(1), dim3(1), 0, stream, count);
+  hipLaunchKernelGGL(guard_list_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, a.planes,
+                     orig, n, scratch, count);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 grid(kGuardWaves);
   const size_t lds = cost_column_lds(a.n_src);
-  PM_DISPATCH(refine_guard_kernel, hp, f16, grid, lds, stream, a, orig);
+  PM_DISPATCH(refine_guard_kernel, hp, f16, grid, lds, stream, a, orig, scratch, count);
 }
 
 }  // namespace pmhip
