@@ -176,8 +176,10 @@ __device__ __forceinline__ ViewTerm ncc_view(const PmArgs& a, const Sampler& sm,
 // Mean residual over the valid views of `views` and, with gradients, its
 // derivative and the Gauss-Newton sums H = sum J J^T, g = sum J r (J = -dNCC).
 // `good`: the views of `views` whose NCC is above 0.1 (those whose
-// PatchMatch cost is 1 - NCC, opencl_kernels.h:526-529).
-template <int HP, typename T, bool kGrad>
+// PatchMatch cost is 1 - NCC, opencl_kernels.h:526-529).  kGate: the sums
+// take the good views only (the same sums, in the same order, as a call on
+// `good`).
+template <int HP, typename T, bool kGrad, bool kGate = false>
 __device__ __forceinline__ float pixel_energy(const PmArgs& a, const VGeom* vg, int x, int y, uint32_t views,
                                               const Theta& th, float grad[3], float H[6], float gv[3], int& used,
                                               uint32_t& good) {
@@ -200,8 +202,9 @@ __device__ __forceinline__ float pixel_energy(const PmArgs& a, const VGeom* vg, 
                      static_cast<float>(pv.h - 1)};
     const ViewTerm t = ncc_view<HP, T, kGrad>(a, sm, vg[s], x, y, cidx, rs, th);
     if (!t.ok) continue;
-    ++used;
     if (t.ncc > 0.1f) good |= 1u << s;
+    else if (kGate) continue;
+    ++used;
     const float r = 1.0f - t.ncc;
     e += r;
     if constexpr (kGrad) {
@@ -320,12 +323,10 @@ __global__ __launch_bounds__(256) PM_REFINE_ATTR void gauss_newton_kernel(PmArgs
     th = Theta{d, p.x / -p.z, p.y / -p.z};
     float gr[3];
     uint32_t good;
-    e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used, good);
-    if (good != views) {  // keep the views whose PatchMatch cost is 1 - NCC
-      views = good;
-      if (!views) return;
-      e0 = pixel_energy<HP, T, true>(a, vg, x, y, views, th, gr, H, gv, used, good);
-    }
+    // keep the views whose PatchMatch cost is 1 - NCC
+    e0 = pixel_energy<HP, T, true, true>(a, vg, x, y, views, th, gr, H, gv, used, good);
+    views = good;
+    if (!views) return;
     *s_views = __uint_as_float(views);
     *s_dpm = d;
   } else {
