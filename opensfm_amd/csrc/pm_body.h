@@ -1725,17 +1725,30 @@ __global__ __launch_bounds__(kWave * kFbWaves) PM_FALLBACK_ATTR void fallback_ba
   batch_item(bt, [](const PmArgs& a, int b) { fallback_tile_body<HP, T, TX>(a, b); });
 }
 
+// Occupancy target of the select and refine kernels (waves per SIMD; 0 =
+// the compiler's choice).  3 takes select<16> from 173 to 168 VGPRs, so 3
+// waves fit per SIMD instead of 2 (aerial survey, 10 sources: 562 -> 446 ms
+// per step of ten 4000x3000 references); 4 would spill.
+#ifndef PM_SELECT_WAVES_PER_EU
+#define PM_SELECT_WAVES_PER_EU 3
+#endif
+#if PM_SELECT_WAVES_PER_EU > 0
+#define PM_SELECT_ATTR __attribute__((amdgpu_waves_per_eu(PM_SELECT_WAVES_PER_EU)))
+#else
+#define PM_SELECT_ATTR
+#endif
+
 template <int NV, int TX>
-__global__ __launch_bounds__(kWave) void select_kernel(PmArgs a) { select_body<NV, TX>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave) PM_SELECT_ATTR void select_kernel(PmArgs a) { select_body<NV, TX>(a, PM_BLOCK); }
 template <int NV, int TX>
-__global__ __launch_bounds__(kWave) void select_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave) PM_SELECT_ATTR void select_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { select_body<NV, TX>(a, b); });
 }
 
 template <int NV, int TX>
-__global__ __launch_bounds__(kWave) void refine_kernel(PmArgs a) { refine_body<NV, TX>(a, PM_BLOCK); }
+__global__ __launch_bounds__(kWave) PM_SELECT_ATTR void refine_kernel(PmArgs a) { refine_body<NV, TX>(a, PM_BLOCK); }
 template <int NV, int TX>
-__global__ __launch_bounds__(kWave) void refine_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave) PM_SELECT_ATTR void refine_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { refine_body<NV, TX>(a, b); });
 }
 

@@ -215,7 +215,9 @@ __device__ __forceinline__ float texel(const T* __restrict__ img, const PmView& 
 // 8-byte (f16) load per bilinear sample instead of two unaligned pair loads,
 // which halves the work of the texture address unit -- the bottleneck of
 // the gather-bound cost kernels.  4x the image memory, a few hundred MB at
-// most.
+// most.  f32 copies keep the horizontal differences instead of the right-hand
+// texels, (T[y][x], T[y][x+1] - T[y][x], T[y+1][x], T[y+1][x+1] - T[y+1][x]),
+// which are what the lerps use.
 template <typename T> struct QuadOf;
 template <> struct QuadOf<float> { static constexpr uint32_t bytes = 16; };
 template <> struct QuadOf<_Float16> { static constexpr uint32_t bytes = 8; };
@@ -228,18 +230,28 @@ struct Sampler {
   float xmax, ymax;            // w - 1, h - 1
 };
 
-__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& t01, float& t10, float& t11,
+// A bilinear quad as (t00, d0, t10, d1) with d0 = t01 - t00 and d1 = t11 - t10,
+// each difference rounded once to f32.  f32 quads hold this form already.
+__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& d0, float& t10, float& d1,
                                         float) {
   const float4 q = *reinterpret_cast<const PM_GLOBAL float4*>(p);
-  t00 = q.x; t01 = q.y; t10 = q.z; t11 = q.w;
+  t00 = q.x; d0 = q.y; t10 = q.z; d1 = q.w;
 }
-__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& t01, float& t10, float& t11,
+// hi - lo of two packed f16 values as one v_fma_mix_f32 (hi * 1.0 + -lo):
+// one rounding of the exact difference, the same f32 value as converting
+// both halves and subtracting, in one instruction instead of three.
+__device__ __forceinline__ float f16_hi_minus_lo(uint32_t v) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%1 op_sel:[1,0,0] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(v));
+  return r;
+}
+__device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, float& d0, float& t10, float& d1,
                                         _Float16) {
   const uint2 q = *reinterpret_cast<const PM_GLOBAL uint2*>(p);
   t00 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.x & 0xffffu)));
-  t01 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.x >> 16)));
+  d0 = f16_hi_minus_lo(q.x);
   t10 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.y & 0xffffu)));
-  t11 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(q.y >> 16)));
+  d1 = f16_hi_minus_lo(q.y);
 }
 
 // Exact software bilinear at a pixel-centre coordinate with clamp-to-edge:
@@ -255,10 +267,10 @@ __device__ __forceinline__ float sample(const Sampler& v, float sx, float sy) {
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
   const uint32_t qi = static_cast<uint32_t>(fmaf(fy0, v.pitch_f, fx0 + v.origin_f));
-  float t00, t01, t10, t11;
-  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, t01, t10, t11, T{});
-  const float v0 = fmaf(ax, t01 - t00, t00);
-  const float v1 = fmaf(ax, t11 - t10, t10);
+  float t00, d0, t10, d1;
+  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, d0, t10, d1, T{});
+  const float v0 = fmaf(ax, d0, t00);
+  const float v1 = fmaf(ax, d1, t10);
   return fmaf(ay, v1 - v0, v0);
 }
 
@@ -272,11 +284,11 @@ __device__ __forceinline__ float sample_grad(const Sampler& v, float sx, float s
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
   const uint32_t qi = static_cast<uint32_t>(fmaf(fy0, v.pitch_f, fx0 + v.origin_f));
-  float t00, t01, t10, t11;
-  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, t01, t10, t11, T{});
-  const float v0 = fmaf(ax, t01 - t00, t00);
-  const float v1 = fmaf(ax, t11 - t10, t10);
-  gx = fmaf(ay, (t11 - t10) - (t01 - t00), t01 - t00);
+  float t00, d0, t10, d1;
+  ld_quad(v.base + qi * QuadOf<T>::bytes, t00, d0, t10, d1, T{});
+  const float v0 = fmaf(ax, d0, t00);
+  const float v1 = fmaf(ax, d1, t10);
+  gx = fmaf(ay, d1 - d0, d0);
   gy = v1 - v0;
   return fmaf(ay, v1 - v0, v0);
 }

@@ -12,6 +12,18 @@ namespace pmhip {
 namespace {
 
 // Quad copy of the padded views (see Sampler): one lane per padded texel.
+// f32 views store each quad as (t00, t01 - t00, t10, t11 - t10): the two
+// differences are the f32 subtractions the bilinear lerp would otherwise
+// do per sample.  f16 views store the four texels.
+template <typename T>
+__device__ __forceinline__ void put_quad(PM_GLOBAL T* q, T t00, T t01, T t10, T t11) {
+  q[0] = t00; q[1] = t01; q[2] = t10; q[3] = t11;
+}
+template <>
+__device__ __forceinline__ void put_quad(PM_GLOBAL float* q, float t00, float t01, float t10, float t11) {
+  *reinterpret_cast<PM_GLOBAL float4*>(q) = make_float4(t00, t01 - t00, t10, t11 - t10);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void quad_kernel(PmArgs a, int nviews) {
   for (int vi = 0; vi < nviews; ++vi) {
@@ -23,10 +35,8 @@ __global__ __launch_bounds__(256) void quad_kernel(PmArgs a, int nviews) {
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
       const int x = static_cast<int>(i % v.pitch), y = static_cast<int>(i / v.pitch);
       const int x1 = min(x + 1, v.pitch - 1), y1 = min(y + 1, v.h + 1);
-      dst[4 * i + 0] = src[static_cast<int64_t>(y) * v.pitch + x];
-      dst[4 * i + 1] = src[static_cast<int64_t>(y) * v.pitch + x1];
-      dst[4 * i + 2] = src[static_cast<int64_t>(y1) * v.pitch + x];
-      dst[4 * i + 3] = src[static_cast<int64_t>(y1) * v.pitch + x1];
+      put_quad(dst + 4 * i, src[static_cast<int64_t>(y) * v.pitch + x], src[static_cast<int64_t>(y) * v.pitch + x1],
+               src[static_cast<int64_t>(y1) * v.pitch + x], src[static_cast<int64_t>(y1) * v.pitch + x1]);
     }
   }
 }
