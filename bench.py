@@ -46,11 +46,11 @@ FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap 
 # WRITE_SIZE (x 1.0), each factor read off the same counter on a 1 GiB
 # device-to-device copy (profiles/*_fetch.txt, profiles/*_write.txt).
 # candidates_kernel<3, float, 4>, 1600x1200 with 8 sources:
-TRAFFIC_PER_LAUNCH = 2.135e9
+TRAFFIC_PER_LAUNCH = 2.138e9
 TRAFFIC_TILE = 4
-# candidates_batch<3, _Float16, 4> over a group of ten 4000x3000 references
-# with 10 sources each:
-TRAFFIC_PER_LAUNCH_C5 = 1.4345e11
+# candidates_batch<3, _Float16, 4> per 4000x3000 reference with 10 sources
+# in the launch (a launch covers every reference of an admitted group):
+TRAFFIC_PER_REF_C5 = 1.761e10
 
 
 def log(msg: str) -> None:
@@ -533,19 +533,24 @@ def run_c5(args, pydense) -> int:
     # colour of every reference of the group, 21 FLOP per tap and 75 per
     # (hypothesis, view).  Launch times: HIP events on the batch stream
     # around every launch, over the timed steps.
-    # A launch covers one colour of every reference of an admitted group, so
-    # the work per launch is the timed steps' total over the launch count.
-    cand_ms, cand_n = stages.get("candidates", (0.0, 0))
-    total_flop = nrefs * (w * h / 2.0) * 9 * args.neighbours * (21 * PATCH * PATCH + 75) * 2 * args.iters
-    flop_per_launch = total_flop / max(cand_n, 1)
+    # A launch covers one colour of every reference of an admitted group.
+    # Groups differ in size (the last one is smaller), so the roofline takes
+    # the launches of the groups of the most common size: one launch scores
+    # refs x (W*H/2) pixels x 9 hypotheses x N sources.
+    sizes = [len(g.refs) for g in groups]
+    modal = max(sorted(set(sizes)), key=sizes.count)
+    cand_ms = sum(g.stage_ms.get("candidates", (0.0, 0))[0] for g in groups if len(g.refs) == modal)
+    cand_n = sum(g.stage_ms.get("candidates", (0.0, 0))[1] for g in groups if len(g.refs) == modal)
+    flop_per_launch = modal * (w * h / 2.0) * 9 * args.neighbours * (21 * PATCH * PATCH + 75)
     avg_launch_s = cand_ms / max(cand_n, 1) / 1e3
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     sweep_ms = sum(v[0] for k, v in stages.items() if k != "init")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": (TRAFFIC_PER_LAUNCH_C5 if storage == "f16" and group == 10 and (w, h) == (4000, 3000)
+                "traffic": (TRAFFIC_PER_REF_C5 * modal if storage == "f16" and (w, h) == (4000, 3000)
                             and args.neighbours == 10 else None),
                 "kernel": f"candidates_batch<3,{'_Float16' if storage == 'f16' else 'float'},*>",
+                "references_per_launch": modal, "launches": cand_n,
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flop_per_launch": flop_per_launch}
     path_tflops = units * 1e6 * FLOP_PER_UNIT / (pm_ms / 1e3) / 1e12 if pm_ms > 0 else 0.0
 
