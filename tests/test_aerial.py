@@ -318,3 +318,25 @@ def test_run_admitted_errors_reset_the_cache():
         run_admitted(lib, 1, build, budget=0)
     res, groups = run_admitted(lib, 0, build, budget=100)
     assert res == {} and groups == []
+
+
+@pytest.mark.gpu
+def test_aerial_full_size_band_matches_oracle(pydense):
+    """The aerial workload at its own size: a middle reference of a 4000x3000
+    survey (3 strips of 80 %/70 % overlap, the same flight as the 200-view
+    job) with its 10 nearest views as sources, patch 7, 3 iterations, f16
+    storage; a band of 64 reference rows through the middle, on the GPU and
+    on the oracle, bit for bit."""
+    from opensfm_amd import synthetic
+    s = synthetic.aerial_survey(seed=5, num_views=22, width=4000, height=3000, neighbours=10, device="cuda")
+    ref = s.reference(12)
+    assert ref.num_src == 10 and ref.gt_depth.shape == (3000, 4000)
+    band = ref.crop_rows(1468, 1532)
+    p = params_for(band, patch_size=7, max_iterations=3)
+    print(f"aerial 4000x3000 reference 12, rows 1468..1532, 10 sources", flush=True)
+    d = gpu_estimator(pydense, band, p, storage="f16").compute()["depth"]
+    d_o, _, _, _, _ = po.estimate(half_round(band.images), band.Ks, band.Rs, band.ts, p)
+    assert d.shape == (64, 4000)
+    assert np.array_equal(d, d_o)
+    gt = band.gt_depth
+    print(f"aerial 4000x3000 band: GT within 1% {np.mean(np.abs(d - gt) < 0.01 * gt):.4f}", flush=True)
