@@ -679,8 +679,8 @@ def main() -> int:
     ap.add_argument("--storage", choices=["f32", "f16"], default=None,
                     help="image storage (default f32; f16 for the c3 workload)")
     ap.add_argument("--cpu-rows", type=int, default=None,
-                    help="reference rows in the CPU-oracle sample (default 128 for c5, the whole frame for c4, "
-                         "else 384; about 10 s on 16 host threads; 0 = skip)")
+                    help="reference rows in the CPU-oracle sample (default 128 for c5, the whole frame for c2 "
+                         "and c4, else 384; 10-30 s on 16 host threads; 0 = skip)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5",
                     help="c5 (default): the aerial survey, batched f16 PatchMatch + Gauss-Newton in memory-admitted "
                          "groups, with the plain single-view leg; c2: the plain single-view workload alone; c3: "
@@ -704,8 +704,8 @@ def main() -> int:
     args.storage_set = args.storage is not None
     if args.steps is None:
         args.steps = {"c5": 6, "cluster": 1}.get(args.workload, 5)
-    if args.cpu_rows is None:  # c4: the whole 1024x768 frame
-        args.cpu_rows = {"c5": 128, "c4": 768}.get(args.workload, 384)
+    if args.cpu_rows is None:  # c2 and c4: the whole frame (1600x1200, 1024x768)
+        args.cpu_rows = {"c5": 128, "c4": 768, "c2": 1200}.get(args.workload, 384)
     if args.storage is None:
         args.storage = "f32"
     if args.gpus != 1:
