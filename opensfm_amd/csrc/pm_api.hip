@@ -388,9 +388,8 @@ int validate(pm_handle* h) {
   if (p.max_iterations < 0) return fail(PM_E_INVALID, "max_iterations must be >= 0");
   if (p.hierarchy_levels < 1 || p.hierarchy_levels > 8)
     return fail(PM_E_INVALID, "hierarchy_levels must be in 1..8");
-  const int div = 1 << (p.hierarchy_levels - 1);
-  if (h->views[0].w / div < 8 || h->views[0].h / div < 8)
-    return fail(PM_E_INVALID, "hierarchy_levels too large for the reference image (coarsest level below 8 px)");
+  // Any image size runs: pyramid levels shrink to at least 1 x 1, as in
+  // BuildImagePyramid (estimator.cc:728-735).
   if (!(p.sigma_spatial > 0.0f) || !(p.sigma_color > 0.0f)) return fail(PM_E_INVALID, "sigmas must be > 0");
   if (p.speckle_min_size < 0 || p.gap_max_size < 0) return fail(PM_E_INVALID, "speckle/gap sizes must be >= 0");
   return PM_OK;

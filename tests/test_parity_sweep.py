@@ -62,3 +62,30 @@ def test_config_matches_oracle(pydense, cfg):
     assert np.array_equal(out["depth"], d_o)
     assert np.array_equal(out["cost"], c_o)
     assert np.array_equal(out["normal"], n_o)
+
+
+TINY = [(1, 1, 1, 5, "f32", 1), (2, 2, 2, 3, "f16", 1), (3, 5, 2, 7, "f32", 1), (8, 3, 3, 5, "f16", 1),
+        (17, 9, 2, 7, "f32", 1), (4, 70, 2, 5, "f32", 1), (70, 4, 3, 7, "f16", 1),
+        (20, 13, 2, 5, "f32", 4), (5, 3, 2, 5, "f16", 3), (40, 9, 3, 3, "f32", 5)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h,n,patch,storage,levels", TINY, ids=[f"{t[0]}x{t[1]}-l{t[5]}" for t in TINY])
+def test_tiny_images_match_oracle(pydense, w, h, n, patch, storage, levels):
+    """Images smaller than a patch or a colour tile, one-pixel-thin strips,
+    and pyramids whose coarsest levels shrink to one or two pixels (levels
+    stop at 1 x 1, as in BuildImagePyramid, estimator.cc:728-735): partial
+    tiles, empty colours and clamped taps everywhere, bit for bit against the
+    oracle."""
+    from opensfm_amd import synthetic
+    s = synthetic.three_planes(3, width=w, height=h, num_src=n, base_cell=0.3, supersample=1)
+    p = params_for(s, patch_size=patch)
+    print(f"tiny {w}x{h}, {n} sources, patch {patch}, {storage}, {levels} levels", flush=True)
+    est = gpu_estimator(pydense, s, p, storage=storage)
+    est.set_hierarchy_levels(levels)
+    out = est.compute()
+    imgs = half_round(s.images) if storage == "f16" else s.images
+    d_o, _, c_o, _, _ = po.estimate_levels(imgs, s.Ks, s.Rs, s.ts, p, levels)
+    assert out["depth"].shape == (h, w)
+    assert np.array_equal(out["depth"], d_o)
+    assert np.array_equal(out["cost"], c_o)
