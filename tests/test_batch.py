@@ -266,3 +266,23 @@ def test_download_has_the_staged_level_size(pydense, scenes):
     h, w = s.gt_depth.shape
     d = e.download(["depth", "normal"])
     assert d["depth"].shape == (h // 2, w // 2) and d["normal"].shape == (h // 2, w // 2, 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_batch_with_tiny_references_equals_single_runs(pydense, storage):
+    """References of a few pixels batched with ordinary ones: each reference
+    owns whole groups of 8 workgroups, most of them past its last tile; the
+    batch must still equal the references run alone, bit for bit."""
+    from opensfm_amd import synthetic
+    scenes = [synthetic.three_planes(21, width=1, height=1, num_src=1, base_cell=0.3, supersample=1),
+              synthetic.three_planes(22, width=160, height=120, num_src=4, base_cell=0.3),
+              synthetic.three_planes(23, width=5, height=3, num_src=2, base_cell=0.3, supersample=1),
+              synthetic.three_planes(24, width=70, height=4, num_src=3, base_cell=0.3, supersample=1),
+              synthetic.three_planes(25, width=2, height=2, num_src=2, base_cell=0.3, supersample=1)]
+    alone = [_alone(pydense, s, storage) for s in scenes]
+    for config in ((4, 32, True), (16, 16, False)):
+        b, ests, got, ms = _batched(pydense, scenes, storage, config)
+        print(f"tiny batch {storage} {config}: {ms:.2f} ms", flush=True)
+        for a, g in zip(alone, got):
+            _same(a, g)
