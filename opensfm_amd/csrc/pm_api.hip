@@ -542,12 +542,28 @@ int pm_destroy(pm_handle* h) {
   return PM_OK;
 }
 
+// Largest view: the kernels address a view's quad copy with 32-bit byte
+// offsets (16 bytes per padded texel) and its rows with 24-bit multiplies,
+// so a view has at most 2^28 padded texels (16000 x 16000) and 65536 pixels
+// per side.
+constexpr int64_t kMaxViewTexels = int64_t(1) << 28;
+constexpr int kMaxViewSide = 65536;
+
+static int check_view_size(int width, int height, const char* who) {
+  if (width <= 0 || height <= 0) return fail(PM_E_INVALID, std::string(who) + ": empty image");
+  if (width > kMaxViewSide || height > kMaxViewSide ||
+      static_cast<int64_t>(width + 2) * (height + 2) > kMaxViewTexels)
+    return fail(PM_E_INVALID, std::string(who) + ": image too large (at most " + std::to_string(kMaxViewSide) +
+                                  " pixels per side and " + std::to_string(kMaxViewTexels) + " padded pixels)");
+  return PM_OK;
+}
+
 int pm_add_view(pm_handle* h, const double* K, const double* R, const double* t, const void* image,
                 int format, int width, int height, int64_t row_stride, const uint8_t* /*mask*/) {
   if (int r = check_handle(h)) return r;
   ++h->gen;
   if (!K || !R || !t || !image) return fail(PM_E_INVALID, "add_view: null argument");
-  if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "add_view: empty image");
+  if (int r = check_view_size(width, height, "add_view")) return r;
   int esize = format == PM_U8 ? 1 : format == PM_F32 ? 4 : format == PM_F16 ? 2 : 0;
   if (!esize) return fail(PM_E_INVALID, "add_view: unknown pixel format");
   if (row_stride == 0) row_stride = static_cast<int64_t>(width) * esize;
@@ -588,7 +604,7 @@ int pm_add_view_device(pm_handle* h, const double* K, const double* R, const dou
   if (int r = check_handle(h)) return r;
   ++h->gen;
   if (!K || !R || !t || !image) return fail(PM_E_INVALID, "add_view_device: null argument");
-  if (width <= 0 || height <= 0) return fail(PM_E_INVALID, "add_view_device: empty image");
+  if (int r = check_view_size(width, height, "add_view_device")) return r;
   const int esize = format == PM_U8 ? 1 : format == PM_F32 ? 4 : format == PM_F16 ? 2 : 0;
   if (!esize) return fail(PM_E_INVALID, "add_view_device: unknown pixel format");
   if (row_stride == 0) row_stride = static_cast<int64_t>(width) * esize;
