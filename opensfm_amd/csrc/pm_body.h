@@ -939,17 +939,38 @@ __device__ __forceinline__ Plane random_plane(const PmArgs& a, int x, int y, uin
 // column of costs (top-k, selected views).  Every pixel regenerates its
 // plane from its seeded stream in every view's wave; the view-0 wave stores
 // it and the generator state.
+#ifndef PM_INIT_STRIP
+#define PM_INIT_STRIP 16
+#endif
+constexpr int kInitStrip = PM_INIT_STRIP;  // tiles per strip (0: whole rows)
+
+// Tile rows of each XCD's band of the init: the bands are whole tile rows.
+__device__ __host__ __forceinline__ int init_band_rows(int height) {
+  return ((height + kFullTileY - 1) / kFullTileY + kXcds - 1) / kXcds;
+}
+
 template <int HP, typename T>
 __device__ __forceinline__ void init_views_body(const PmArgs& a, int b) {
   const int tiles_x = (a.width + kFullTileX - 1) / kFullTileX;
-  const int ntiles = tiles_x * ((a.height + kFullTileY - 1) / kFullTileY);
-  const int per = (ntiles + kXcds - 1) / kXcds;
-  const int q = b / kXcds;  // position in this XCD's sequence
+  const int tiles_y = (a.height + kFullTileY - 1) / kFullTileY;
+  const int rows = init_band_rows(a.height);
+  const int per = rows * tiles_x;  // slots of one view in this XCD's sequence
+  const int q = b / kXcds;         // position in this XCD's sequence
   const int v = q / per;
-  const int t = (b % kXcds) * per + (q - v * per);
-  if (v >= a.n_src || t >= ntiles) return;
-  const int x = (t % tiles_x) * kFullTileX + static_cast<int>(threadIdx.x % kFullTileX);
-  const int y = (t / tiles_x) * kFullTileY + static_cast<int>(threadIdx.x / kFullTileX);
+  const int tl = q - v * per;
+  // Within a view, the band is walked in vertical strips of kInitStrip tiles,
+  // row by row inside a strip: a random depth moves a patch along its
+  // epipolar segment, and a strip's segments fit one L2 whichever way they
+  // run, while neighbouring tiles still share reference-side cache lines.
+  const int sw = kInitStrip > 0 ? kInitStrip : tiles_x;
+  const int s = tl / (sw * rows);
+  const int rem = tl - s * sw * rows;
+  const int ws = min(sw, tiles_x - s * sw);
+  const int ty = (b % kXcds) * rows + rem / ws;
+  const int tx = s * sw + rem % ws;
+  if (v >= a.n_src || ty >= tiles_y) return;
+  const int x = tx * kFullTileX + static_cast<int>(threadIdx.x % kFullTileX);
+  const int y = ty * kFullTileY + static_cast<int>(threadIdx.x / kFullTileX);
   if (x >= a.width || y >= a.height) return;
   const T* img = images_of<T>(a);
   const int idx = y * a.width + x;
