@@ -29,25 +29,55 @@ constexpr int kSweepSlots = 9;   // 8 propagation candidates + current/refinemen
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
 
 // Tile of this workgroup.  The dispatcher hands workgroups to the XCDs
-// round-robin (linear id mod 8), so consecutive ids land on different L2s;
-// remapping id -> tile gives every XCD one contiguous band of tiles, whose
-// source-image footprint stays resident in that XCD's L2.  The grid is 1-D
-// with kXcds * ceil(tiles / kXcds) workgroups; b is the workgroup's index
-// in the grid of its reference (a multiple of kXcds apart from blockIdx.x,
-// so b mod kXcds is still the XCD).
-__device__ __forceinline__ bool tile_of(int tiles_x, int tiles_y, int b, int& tx, int& ty) {
+// round-robin (linear id mod 8), so consecutive ids land on different L2s.
+// The tiles are ordered in vertical strips of about kBandStripPx pixels,
+// each strip top to bottom and row by row, strip after strip; every XCD
+// takes one contiguous eighth of that order (equal tile counts), so the
+// tiles in flight on one XCD are a few rows of one strip.  Their gathers then
+// stay within that strip's epipolar segments whichever way a view's baseline
+// runs, and neighbouring tiles still share cache lines of the
+// reference-side data.  The grid is 1-D with kXcds * ceil(tiles / kXcds)
+// workgroups; b is the workgroup's index in the grid of its reference (a
+// multiple of kXcds apart from blockIdx.x, so b mod kXcds is still the XCD).
+#ifndef PM_BAND_STRIP_PX
+#define PM_BAND_STRIP_PX 64
+#endif
+#ifndef PM_BAND_SLAB_PX
+#define PM_BAND_SLAB_PX 384
+#endif
+constexpr int kBandStripPx = PM_BAND_STRIP_PX;  // 0: whole rows
+constexpr int kBandSlabPx = PM_BAND_SLAB_PX;    // 0: the whole height
+
+// Tile t of the strip order: horizontal slabs of `slab` tile rows (0: the
+// whole height) one after the other; inside a slab, vertical strips of
+// `strip` tiles (0: whole rows) left to right; inside a strip, row by row.
+__device__ __forceinline__ void strip_tile(int tiles_x, int tiles_y, int t, int strip, int slab, int& tx, int& ty) {
+  const int sh = slab > 0 ? min(slab, tiles_y) : tiles_y;
+  const int k = t / (sh * tiles_x);
+  const int rows = min(sh, tiles_y - k * sh);  // the last slab may be shorter
+  const int r0 = t - k * sh * tiles_x;
+  const int sw = strip > 0 ? min(strip, tiles_x) : tiles_x;
+  const int s = r0 / (sw * rows);
+  const int rem = r0 - s * sw * rows;
+  const int ws = min(sw, tiles_x - s * sw);  // the last strip may be narrower
+  ty = k * sh + rem / ws;
+  tx = s * sw + rem % ws;
+}
+
+__device__ __forceinline__ bool tile_of(int tiles_x, int tiles_y, int b, int strip, int slab, int& tx, int& ty) {
   const int ntiles = tiles_x * tiles_y;
   const int per = (ntiles + kXcds - 1) / kXcds;
   const int t = (b % kXcds) * per + b / kXcds;
   if (t >= ntiles) return false;
-  tx = t % tiles_x;
-  ty = t / tiles_x;
+  strip_tile(tiles_x, tiles_y, t, strip, slab, tx, ty);
   return true;
 }
 
 __device__ __forceinline__ bool full_tile_pixel(int width, int height, int b, int& x, int& y) {
   int tx, ty;
-  if (!tile_of((width + kFullTileX - 1) / kFullTileX, (height + kFullTileY - 1) / kFullTileY, b, tx, ty)) return false;
+  if (!tile_of((width + kFullTileX - 1) / kFullTileX, (height + kFullTileY - 1) / kFullTileY, b,
+               kBandStripPx / kFullTileX, kBandSlabPx / kFullTileY, tx, ty))
+    return false;
   x = tx * kFullTileX + static_cast<int>(threadIdx.x % kFullTileX);
   y = ty * kFullTileY + static_cast<int>(threadIdx.x / kFullTileX);
   return x < width && y < height;
@@ -63,7 +93,8 @@ __device__ __forceinline__ int colour_tiles_y(const PmArgs& a) { return (a.heigh
 // workgroup).
 template <int TX>
 __device__ __forceinline__ bool colour_tile(const PmArgs& a, int b, int& tx, int& ty) {
-  return tile_of(colour_tiles_x<TX>(a), colour_tiles_y<TX>(a), b, tx, ty);
+  return tile_of(colour_tiles_x<TX>(a), colour_tiles_y<TX>(a), b, kBandStripPx / (2 * TX), kBandSlabPx / (kWave / TX),
+                 tx, ty);
 }
 
 __device__ __forceinline__ Plane to_plane(float4 v) { return Plane{v.x, v.y, v.z, v.w}; }
@@ -944,31 +975,20 @@ __device__ __forceinline__ Plane random_plane(const PmArgs& a, int x, int y, uin
 #endif
 constexpr int kInitStrip = PM_INIT_STRIP;  // tiles per strip (0: whole rows)
 
-// Tile rows of each XCD's band of the init: the bands are whole tile rows.
-__device__ __host__ __forceinline__ int init_band_rows(int height) {
-  return ((height + kFullTileY - 1) / kFullTileY + kXcds - 1) / kXcds;
-}
-
 template <int HP, typename T>
 __device__ __forceinline__ void init_views_body(const PmArgs& a, int b) {
   const int tiles_x = (a.width + kFullTileX - 1) / kFullTileX;
-  const int tiles_y = (a.height + kFullTileY - 1) / kFullTileY;
-  const int rows = init_band_rows(a.height);
-  const int per = rows * tiles_x;  // slots of one view in this XCD's sequence
-  const int q = b / kXcds;         // position in this XCD's sequence
+  const int ntiles = tiles_x * ((a.height + kFullTileY - 1) / kFullTileY);
+  const int per = (ntiles + kXcds - 1) / kXcds;  // slots of one view in this XCD's sequence
+  const int q = b / kXcds;                       // position in this XCD's sequence
   const int v = q / per;
-  const int tl = q - v * per;
-  // Within a view, the band is walked in vertical strips of kInitStrip tiles,
-  // row by row inside a strip: a random depth moves a patch along its
-  // epipolar segment, and a strip's segments fit one L2 whichever way they
-  // run, while neighbouring tiles still share reference-side cache lines.
-  const int sw = kInitStrip > 0 ? kInitStrip : tiles_x;
-  const int s = tl / (sw * rows);
-  const int rem = tl - s * sw * rows;
-  const int ws = min(sw, tiles_x - s * sw);
-  const int ty = (b % kXcds) * rows + rem / ws;
-  const int tx = s * sw + rem % ws;
-  if (v >= a.n_src || ty >= tiles_y) return;
+  const int t = (b % kXcds) * per + (q - v * per);
+  if (v >= a.n_src || t >= ntiles) return;
+  // Within a view the tiles go in the strip order of tile_of, strips of
+  // kInitStrip tiles: a random depth moves a patch along its epipolar
+  // segment, and a strip's segments fit one L2 whichever way they run.
+  int tx, ty;
+  strip_tile(tiles_x, (a.height + kFullTileY - 1) / kFullTileY, t, kInitStrip, 0, tx, ty);
   const int x = tx * kFullTileX + static_cast<int>(threadIdx.x % kFullTileX);
   const int y = ty * kFullTileY + static_cast<int>(threadIdx.x / kFullTileX);
   if (x >= a.width || y >= a.height) return;

@@ -292,8 +292,9 @@ int64_t kernel_blocks(int kernel, int width, int height, int n_src, int tile_x) 
     case PM_K_SWEEP:
       return xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
                        ((height + kFullTileY - 1) / kFullTileY));
-    case PM_K_INIT_VIEWS:  // one workgroup per (source view, full tile); each XCD a band of whole tile rows
-      return static_cast<int64_t>(n_src) * kXcds * init_band_rows(height) * ((width + kFullTileX - 1) / kFullTileX);
+    case PM_K_INIT_VIEWS:  // one workgroup per (source view, full tile)
+      return static_cast<int64_t>(n_src) * xcd_round(static_cast<int64_t>((width + kFullTileX - 1) / kFullTileX) *
+                                                     ((height + kFullTileY - 1) / kFullTileY));
     case PM_K_FALLBACK: {
       return xcd_round(colour_tiles(width, height, tile_x));  // one workgroup per colour tile
     }
