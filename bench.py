@@ -46,11 +46,11 @@ FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap 
 # WRITE_SIZE (x 1.0), each factor read off the same counter on a 1 GiB
 # device-to-device copy (profiles/*_fetch.txt, profiles/*_write.txt).
 # candidates_kernel<3, float, 4>, 1600x1200 with 8 sources:
-TRAFFIC_PER_LAUNCH = 2.138e9
+TRAFFIC_PER_LAUNCH = 1.446e9
 TRAFFIC_TILE = 4
-# candidates_batch<3, _Float16, 4> per 4000x3000 reference with 10 sources
+# candidates_batch<3, _Float16, 16> per 4000x3000 reference with 10 sources
 # in the launch (a launch covers every reference of an admitted group):
-TRAFFIC_PER_REF_C5 = 1.761e10
+TRAFFIC_PER_REF_C5 = 1.125e10
 
 
 def log(msg: str) -> None:
@@ -492,6 +492,8 @@ def run_c5(args, pydense) -> int:
             f"{g.refine_ms:.0f} ms, results {g.results_s:.1f} s, release {g.release_s:.1f} s, wall {g.wall_s:.1f} s")
 
     cfg = {"use_graph": False}     # per-launch HIP events on the batch stream (the roofline below)
+    if args.tiles:
+        cfg["cost_tile_x"], cfg["pixel_tile_x"] = args.tiles
     for i in range(args.warmup):  # first use: autotuner, code objects
         run_admitted(pydense, group, build_at, budget=budget, refine_steps=args.refine_steps,
                      on_result=lambda i, e: None, overlap_staging=not args.no_overlap, batch_config=cfg)
@@ -688,6 +690,8 @@ def main() -> int:
                          "cluster: the cluster surface with pyramid levels and geometric links")
     ap.add_argument("--no-plain-leg", action="store_true", help="c5: skip the plain single-view leg")
     ap.add_argument("--refs-per-step", type=int, default=10, help="c5: references per timed step")
+    ap.add_argument("--tiles", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    help="c5: cost and pixel tile widths of the batches, e.g. 4,32 (default: the autotuner's)")
     ap.add_argument("--refine-steps", type=int, default=10, help="Gauss-Newton steps of the c4 and c5 workloads")
     ap.add_argument("--views", type=int, default=None, help="references of the c3 (64) and c5 (200) workloads")
     ap.add_argument("--levels", type=int, default=3, help="pyramid levels of the cluster workload")
