@@ -542,11 +542,11 @@ int pm_destroy(pm_handle* h) {
   return PM_OK;
 }
 
-// Largest view: the kernels address a view's quad copy with 32-bit byte
-// offsets (16 bytes per padded texel) and its rows with 24-bit multiplies,
-// so a view has at most 2^28 padded texels (16000 x 16000) and 65536 pixels
-// per side.
-constexpr int64_t kMaxViewTexels = int64_t(1) << 28;
+// Largest view: the bilinear sampler forms a view's quad index in float,
+// exact below 2^24, and addresses its rows with 24-bit multiplies, so a view
+// has at most 2^24 padded texels (4000 x 4000; the dense pipeline caps
+// images at 3200 px, config.py:417) and 65536 pixels per side.
+constexpr int64_t kMaxViewTexels = int64_t(1) << 24;
 constexpr int kMaxViewSide = 65536;
 
 static int check_view_size(int width, int height, const char* who) {

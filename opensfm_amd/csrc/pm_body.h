@@ -238,6 +238,18 @@ __device__ __forceinline__ bool plane_points(const PmCam& c0, int x, int y, Plan
 #endif
 constexpr int kViewChunk = PM_VIEW_CHUNK;
 
+// True when every tap of a patch of half-size HP (taps spaced `stride`
+// apart) warped by w lies in [-1, w-1] x [-1, h-1] of the view, where
+// sample() needs no clamp.  The corners bound the affine tap positions; the
+// margin covers the rounding of the taps' two FMAs (views are at most 65536
+// pixels wide); NaN fails.
+template <int HP>
+__device__ __forceinline__ bool taps_in_view(const Warp& w, const Sampler& sm, float stride = 1.0f) {
+  const float ex = (fabsf(w.jxx) + fabsf(w.jyx)) * (stride * static_cast<float>(HP)) + 0.05f;
+  const float ey = (fabsf(w.jxy) + fabsf(w.jyy)) * (stride * static_cast<float>(HP)) + 0.05f;
+  return w.scx - ex >= -1.0f && w.scx + ex <= sm.xmax && w.scy - ey >= -1.0f && w.scy + ey <= sm.ymax;
+}
+
 // kLdsTaps: the pixel's stride-1 (w, w*r) are read from the workgroup's LDS
 // copy (lt[tap * kWave], see stage_tile_taps) instead of ref_taps.
 // Only the views [v_lo, v_hi) are scored (all by default).
@@ -283,39 +295,49 @@ __device__ __forceinline__ uint32_t cost_vector(const PmArgs& a, const T* __rest
       scen[k] = sample<T>(sm[k], wp[k].scx, wp[k].scy);
       s_ws[k] = 0.0f; s_wss[k] = 0.0f; s_wrs[k] = 0.0f;
     }
+    // The tap loop, with or without the coordinate clamp: when every lane's
+    // patch lies inside its view the clamp is the identity and is left out.
+    auto taps = [&](auto clamp) {
+      constexpr bool kClamp = decltype(clamp)::value;
 #pragma unroll 1
-    for (int iy = -HP; iy <= HP; ++iy) {
-      const float fiy = static_cast<float>(iy);
-      float bx[kViewChunk], by[kViewChunk];
-#pragma unroll
-      for (int k = 0; k < kViewChunk; ++k) {
-        bx[k] = fmaf(wp[k].jyx, fiy, wp[k].scx);
-        by[k] = fmaf(wp[k].jyy, fiy, wp[k].scy);
-      }
-      const PM_GLOBAL float2* trow = gp(a.ref_taps) + static_cast<int64_t>((iy + HP) * D) * plane_stride;
-#pragma unroll
-      for (int ix = -HP; ix <= HP; ++ix) {
-        const float fix = static_cast<float>(ix);
-        float2 wv;
-        if constexpr (kLdsTaps) {
-          wv = lt[((iy + HP) * D + (ix + HP)) * kWave];
-        } else {
-          const PM_GLOBAL char* tp =
-              reinterpret_cast<const PM_GLOBAL char*>(trow + static_cast<int64_t>(ix + HP) * plane_stride);
-          wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + (ucidx << 3));
-        }
+      for (int iy = -HP; iy <= HP; ++iy) {
+        const float fiy = static_cast<float>(iy);
+        float bx[kViewChunk], by[kViewChunk];
 #pragma unroll
         for (int k = 0; k < kViewChunk; ++k) {
-          const float sx = fmaf(wp[k].jxx, fix, bx[k]);
-          const float sy = fmaf(wp[k].jxy, fix, by[k]);
-          const float sd = sample<T>(sm[k], sx, sy) - scen[k];
-          const float wsv = wv.x * sd;
-          s_ws[k] += wsv;
-          s_wss[k] = fmaf(wsv, sd, s_wss[k]);
-          s_wrs[k] = fmaf(wv.y, sd, s_wrs[k]);
+          bx[k] = fmaf(wp[k].jyx, fiy, wp[k].scx);
+          by[k] = fmaf(wp[k].jyy, fiy, wp[k].scy);
+        }
+        const PM_GLOBAL float2* trow = gp(a.ref_taps) + static_cast<int64_t>((iy + HP) * D) * plane_stride;
+#pragma unroll
+        for (int ix = -HP; ix <= HP; ++ix) {
+          const float fix = static_cast<float>(ix);
+          float2 wv;
+          if constexpr (kLdsTaps) {
+            wv = lt[((iy + HP) * D + (ix + HP)) * kWave];
+          } else {
+            const PM_GLOBAL char* tp =
+                reinterpret_cast<const PM_GLOBAL char*>(trow + static_cast<int64_t>(ix + HP) * plane_stride);
+            wv = *reinterpret_cast<const PM_GLOBAL float2*>(tp + (ucidx << 3));
+          }
+#pragma unroll
+          for (int k = 0; k < kViewChunk; ++k) {
+            const float sx = fmaf(wp[k].jxx, fix, bx[k]);
+            const float sy = fmaf(wp[k].jxy, fix, by[k]);
+            const float sd = sample<T, kClamp>(sm[k], sx, sy) - scen[k];
+            const float wsv = wv.x * sd;
+            s_ws[k] += wsv;
+            s_wss[k] = fmaf(wsv, sd, s_wss[k]);
+            s_wrs[k] = fmaf(wv.y, sd, s_wrs[k]);
+          }
         }
       }
-    }
+    };
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < kViewChunk; ++k) inside = inside && taps_in_view<HP>(wp[k], sm[k]);
+    if (__all(inside)) taps(std::false_type{});
+    else taps(std::true_type{});
 #pragma unroll
     for (int k = 0; k < kViewChunk; ++k) {
       const int s = c0v + k;

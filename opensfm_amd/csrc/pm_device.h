@@ -259,11 +259,15 @@ __device__ __forceinline__ void ld_quad(const PM_GLOBAL char* p, float& t00, flo
 // are the same edge texel, so lerp(a, a, t) == a keeps the clamp-to-edge
 // value; one med3 per axis, coordinates reaching here are finite), floor,
 // lerp along x on both rows, then along y.  The quad index is formed in float, exact
-// because it is an integer below 2^24 (a view is < 16M texels).
-template <typename T>
+// because it is an integer below 2^24 (add_view refuses views of more padded texels).
+// kClamp = false: the caller has checked that (sx, sy) lies in
+// [-1, w-1] x [-1, h-1], where the clamp changes nothing.
+template <typename T, bool kClamp = true>
 __device__ __forceinline__ float sample(const Sampler& v, float sx, float sy) {
-  sx = __builtin_amdgcn_fmed3f(sx, -1.0f, v.xmax);
-  sy = __builtin_amdgcn_fmed3f(sy, -1.0f, v.ymax);
+  if constexpr (kClamp) {
+    sx = __builtin_amdgcn_fmed3f(sx, -1.0f, v.xmax);
+    sy = __builtin_amdgcn_fmed3f(sy, -1.0f, v.ymax);
+  }
   const float fx0 = floorf(sx), fy0 = floorf(sy);
   const float ax = sx - fx0, ay = sy - fy0;
   const uint32_t qi = static_cast<uint32_t>(fmaf(fy0, v.pitch_f, fx0 + v.origin_f));

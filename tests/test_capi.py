@@ -238,16 +238,17 @@ def test_too_many_sources(pydense, scene_tiny):
 
 @pytest.mark.gpu
 def test_oversized_views_are_refused(lib):
-    """A view's quad copy is addressed with 32-bit byte offsets: views beyond
-    65536 pixels per side or 2^28 padded pixels are refused by both add-view
-    entry points before any pixel is read (the buffer here holds 16 bytes)."""
+    """The sampler's quad index is a float, exact below 2^24: views beyond
+    65536 pixels per side or 2^24 padded pixels (4096 x 4096 is one) are
+    refused by both add-view entry points before any pixel is read (the
+    buffer here holds 16 bytes)."""
     h = ctypes.c_void_p()
     assert lib.pm_create(0, ctypes.byref(h)) == 0, lib.pm_last_error()
     eye = (ctypes.c_double * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
     zero = (ctypes.c_double * 3)(0, 0, 0)
     buf = (ctypes.c_uint8 * 16)()
     try:
-        for w, hh in ((70000, 1), (1, 70000), (20000, 15000)):
+        for w, hh in ((70000, 1), (1, 70000), (20000, 15000), (4096, 4096), (5000, 4000)):
             r = lib.pm_add_view(h, eye, eye, zero, buf, 0, w, hh, ctypes.c_int64(0), None)
             assert r == -1 and b"too large" in lib.pm_last_error(), (w, hh)
             r = lib.pm_add_view_device(h, eye, eye, zero, buf, 0, w, hh, ctypes.c_int64(0))
