@@ -902,6 +902,29 @@ __device__ __forceinline__ float mc_cost(const PmArgs& a, const float* cv, int64
   const float gw = a.geom_weight;
   const bool use_geom = gw > 0.0f && a.prev_depth_mask != 0u;
   float total = 0.0f;
+  if (!use_geom) {
+    // The hypothesis's view costs are loaded once, before the draws; each
+    // draw takes its view's cost by the same comparisons that pick the view,
+    // so no draw waits on a load of its own.
+    float c[NV];
+    float last = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      c[i] = i < n_src ? cv[i * cs] : 0.0f;
+      if (i == n_src - 1) last = c[i];
+    }
+    const float geom_term = gw * 0.0f;
+    for (int smp = 0; smp < kMcSamples; ++smp) {
+      const float u = rand_float(s0, s1, key + static_cast<uint32_t>(smp) * 7919u);
+      const float thr = u * cum;
+      float photo = last;
+#pragma unroll
+      for (int i = NV - 1; i >= 0; --i)
+        if (i < n_src && cdf[i] > thr) photo = c[i];
+      total += photo + geom_term;
+    }
+    return total / static_cast<float>(kMcSamples);
+  }
   for (int smp = 0; smp < kMcSamples; ++smp) {
     const float u = rand_float(s0, s1, key + static_cast<uint32_t>(smp) * 7919u);
     const float thr = u * cum;
