@@ -784,8 +784,14 @@ __device__ __forceinline__ void full_cost_vector(const PmArgs& a, const T* __res
 // Smoothness against the 4 neighbours (opencl_kernels.h:649-707); the
 // neighbour planes are preloaded (they have the other colour and cannot
 // change during the pass).
+// The 4-neighbourhood of a pixel (left, right, up, down): whether each
+// neighbour is inside the image, its depth on its own plane at its own
+// pixel, and its unit normal.  These do not depend on the hypothesis being
+// scored, so they are computed once per pixel (0 depth and a zero normal
+// outside the image, as the zero plane gives).
 struct Nbrs {
-  Plane p[4];     // left, right, up, down
+  float nd[4];
+  V3 nn[4];
   bool ok[4];
 };
 
@@ -802,13 +808,11 @@ __device__ __forceinline__ float smoothness_cost(const PmArgs& a, const Nbrs& nb
   for (int k = 0; k < 4; ++k) {
     if (!nb.ok[k]) continue;
     const int nx = x + dxs[k], ny = y + dys[k];
-    const Plane q = nb.p[k];
-    const float nd = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), q, c0);
+    const float nd = nb.nd[k];
     if (nd <= 0.0f) continue;
     const float rel = fabsf(cur_depth - nd) / cur_depth;
     const float w = pm_expf(-rel * rel / (2.0f * sigma_rel * sigma_rel));
-    const V3 nn = normalize3(V3{q.x, q.y, q.z});
-    const float normal_cost = 1.0f - dot3(cur_n, nn);
+    const float normal_cost = 1.0f - dot3(cur_n, nb.nn[k]);
     const float predicted = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), cur, c0);
     const float dev = (predicted > 0.0f) ? fabsf(nd - predicted) / cur_depth : 0.0f;
     const float cost = 0.5f * clampf(normal_cost, 0.0f, 1.0f) + 0.5f * clampf(dev * 10.0f, 0.0f, 1.0f);
@@ -821,13 +825,16 @@ __device__ __forceinline__ float smoothness_cost(const PmArgs& a, const Nbrs& nb
 
 __device__ __forceinline__ void load_nbrs(const PmArgs& a, const float4* __restrict__ planes, int x, int y, Nbrs& nb) {
   const int W = a.width, H = a.height;
+  const PmCam c0 = kcam(a, 0);
   const int dxs[4] = {-1, 1, 0, 0};
   const int dys[4] = {0, 0, -1, 1};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int nx = x + dxs[k], ny = y + dys[k];
     nb.ok[k] = !(nx < 0 || nx >= W || ny < 0 || ny >= H);
-    nb.p[k] = nb.ok[k] ? to_plane(gp(planes)[static_cast<int64_t>(ny) * W + nx]) : Plane{0, 0, 0, 0};
+    const Plane q = nb.ok[k] ? to_plane(gp(planes)[static_cast<int64_t>(ny) * W + nx]) : Plane{0, 0, 0, 0};
+    nb.nd[k] = depth_from_plane(static_cast<float>(nx), static_cast<float>(ny), q, c0);
+    nb.nn[k] = normalize3(V3{q.x, q.y, q.z});
   }
 }
 
@@ -838,10 +845,7 @@ __device__ __forceinline__ V3 surface_normal(const PmArgs& a, const Nbrs& nb, in
   if (x < 1 || x >= W - 1 || y < 1 || y >= H - 1) return V3{0, 0, 0};
   const PmCam c0 = kcam(a, 0);
   const float fx = c0.K[0], fy = c0.K[4], cx = c0.K[2], cy = c0.K[5];
-  const float dl = depth_from_plane(static_cast<float>(x - 1), static_cast<float>(y), nb.p[0], c0);
-  const float dr = depth_from_plane(static_cast<float>(x + 1), static_cast<float>(y), nb.p[1], c0);
-  const float du = depth_from_plane(static_cast<float>(x), static_cast<float>(y - 1), nb.p[2], c0);
-  const float dd = depth_from_plane(static_cast<float>(x), static_cast<float>(y + 1), nb.p[3], c0);
+  const float dl = nb.nd[0], dr = nb.nd[1], du = nb.nd[2], dd = nb.nd[3];
   if (dl <= 0.0f || dr <= 0.0f || du <= 0.0f || dd <= 0.0f) return V3{0, 0, 0};
   const float dx = dr - dl;
   const float dy = dd - du;
