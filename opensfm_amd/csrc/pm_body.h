@@ -1741,6 +1741,9 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 #ifndef PM_COST_WAVES_PER_EU
 #define PM_COST_WAVES_PER_EU 0
 #endif
+#ifndef PM_REFINE_COST_WAVES_PER_EU
+#define PM_REFINE_COST_WAVES_PER_EU 0
+#endif
 #ifndef PM_FALLBACK_WAVES_PER_EU
 #define PM_FALLBACK_WAVES_PER_EU 0
 #endif
@@ -1751,6 +1754,11 @@ __device__ __forceinline__ void batch_item(const PmBatch& bt, F&& body) {
 #define PM_COST_ATTR __attribute__((amdgpu_waves_per_eu(PM_COST_WAVES_PER_EU)))
 #else
 #define PM_COST_ATTR
+#endif
+#if PM_REFINE_COST_WAVES_PER_EU
+#define PM_REFINE_COST_ATTR __attribute__((amdgpu_waves_per_eu(PM_REFINE_COST_WAVES_PER_EU)))
+#else
+#define PM_REFINE_COST_ATTR
 #endif
 #if PM_FALLBACK_WAVES_PER_EU
 #define PM_FALLBACK_ATTR __attribute__((amdgpu_waves_per_eu(PM_FALLBACK_WAVES_PER_EU)))
@@ -1798,11 +1806,11 @@ __global__ __launch_bounds__(kWave * 9) PM_COST_ATTR void candidates_batch(PmBat
 }
 
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 6) PM_COST_ATTR void refine_cost_kernel(PmArgs a) {
+__global__ __launch_bounds__(kWave * 6) PM_REFINE_COST_ATTR void refine_cost_kernel(PmArgs a) {
   refine_cost_body<HP, T, TX>(a, PM_BLOCK);
 }
 template <int HP, typename T, int TX>
-__global__ __launch_bounds__(kWave * 6) PM_COST_ATTR void refine_cost_batch(PmBatch bt) {
+__global__ __launch_bounds__(kWave * 6) PM_REFINE_COST_ATTR void refine_cost_batch(PmBatch bt) {
   batch_item(bt, [](const PmArgs& a, int b) { refine_cost_body<HP, T, TX>(a, b); });
 }
 
