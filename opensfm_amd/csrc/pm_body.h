@@ -1823,18 +1823,17 @@ __global__ __launch_bounds__(kWave * kFbWaves) PM_FALLBACK_ATTR void fallback_ba
   batch_item(bt, [](const PmArgs& a, int b) { fallback_tile_body<HP, T, TX>(a, b); });
 }
 
-// Occupancy target of the select and refine kernels (waves per SIMD; 0 =
-// the compiler's choice).  3 takes select<16> from 173 to 168 VGPRs, so 3
-// waves fit per SIMD instead of 2 (aerial survey, 10 sources: 562 -> 446 ms
-// per step of ten 4000x3000 references); 4 would spill.
+// Occupancy target of the select and refine kernels (waves per SIMD) for
+// up to NV views.  3 takes select<16> from 173 to 168 VGPRs, so 3 waves fit
+// per SIMD instead of 2 (aerial survey, 10 sources: 562 -> 446 ms per step
+// of ten 4000x3000 references); at 4 it would spill 136 bytes per lane.
+// Up to 12 views 4 waves fit with a spill of at most 44 bytes outside the
+// MC draws: select 232 -> 221 ms per aerial step (10 sources), 3.37 ->
+// 3.17 ms per 1600x1200 step with 8 sources.
 #ifndef PM_SELECT_WAVES_PER_EU
-#define PM_SELECT_WAVES_PER_EU 3
+#define PM_SELECT_WAVES_PER_EU(NV) ((NV) <= 12 ? 4 : 3)
 #endif
-#if PM_SELECT_WAVES_PER_EU > 0
-#define PM_SELECT_ATTR __attribute__((amdgpu_waves_per_eu(PM_SELECT_WAVES_PER_EU)))
-#else
-#define PM_SELECT_ATTR
-#endif
+#define PM_SELECT_ATTR __attribute__((amdgpu_waves_per_eu(PM_SELECT_WAVES_PER_EU(NV))))
 
 template <int NV, int TX>
 __global__ __launch_bounds__(kWave) PM_SELECT_ATTR void select_kernel(PmArgs a) { select_body<NV, TX>(a, PM_BLOCK); }
