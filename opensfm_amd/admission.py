@@ -19,7 +19,10 @@ dropped.  While a group runs on the device, a worker thread builds the next
 group's estimators (host work: views and parameters) and, with
 ``overlap_staging``, stages it (host conversion, uploads and graph capture
 on the batch's own stream): each group then gets half the budget, so that
-two groups fit at once.  The host holds at most two groups' images.  During
+two groups fit at once.  With ``overlap_staging`` the worker also takes a
+finished group's results and release, while the next group runs; it stages
+the group after that only once they are done, so at most two groups hold
+device memory at any time.  The host holds at most two groups' images.  During
 the run the library keeps released device buffers for reuse
 (``DepthmapEstimator.set_buffer_cache``, up to the budget): groups of
 same-shaped references then reuse the blocks of an earlier group instead of
@@ -42,8 +45,8 @@ class GroupStats:
     prepare_s: float = 0.0     # staging, first-use tuning and graph capture (on the worker when overlapped)
     run_ms: float = 0.0        # device time of the batched PatchMatch run
     refine_ms: float = 0.0     # device time of the Gauss-Newton steps (sum over the group)
-    results_s: float = 0.0     # on_result calls (downloads)
-    release_s: float = 0.0     # destroying the batch, releasing the device memory
+    results_s: float = 0.0     # on_result calls (downloads; on the worker when overlapped)
+    release_s: float = 0.0     # releasing the device memory (on the worker when overlapped)
     wall_s: float = 0.0        # the whole group, end to end
     stage_ms: Dict[str, List[float]] = dataclasses.field(default_factory=dict)  # {stage: [ms, launches]}, eager runs
 
@@ -93,7 +96,8 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
     finished group's timing.  ``reuse_buffers``: keep released device
     buffers for the next group (the cache is emptied at the end).
     ``overlap_staging``: stage the next group while the current one runs
-    (groups of half the budget).  Returns
+    (groups of half the budget), and collect a group's results and release
+    it on the worker while the next one runs.  Returns
     ``(results: {i: value}, groups: [GroupStats])``.
     """
     if count < 0:
@@ -147,16 +151,36 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
             prep_s = time.perf_counter() - t0
         return ests, refs, used, pending, i, build_s, batch, prep_s
 
+    def finish(ests, refs, st, t_group):
+        """Results, then release, of a group whose device work is done."""
+        t0 = time.perf_counter()
+        for idx, e in zip(refs, ests):
+            results[idx] = on_result(idx, e)
+        st.results_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for e in ests:
+            e.release_device()
+        ests.clear()
+        st.release_s = time.perf_counter() - t0
+        st.wall_s = time.perf_counter() - t_group
+        groups.append(st)
+        if on_group is not None:
+            on_group(st)
+
     est_cls = pydense.DepthmapEstimator
     if reuse_buffers:
         est_cls.set_buffer_cache(device, budget)
     try:
         with ThreadPoolExecutor(max_workers=1) as pool:
+            finishing = []
             fut = pool.submit(next_group, None, 0)
             while fut is not None:
                 t_group = time.perf_counter()
                 ests, refs, used, pending, i, build_s, batch, prep_s = fut.result()
                 wait_s = time.perf_counter() - t_group
+                # with overlap_staging the worker runs this after the previous
+                # group's finish(): that group's memory is free before the
+                # next one is staged
                 fut = pool.submit(next_group, pending, i) if (pending is not None or i < count) else None
                 if not ests:
                     break
@@ -185,20 +209,17 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
                     if refine_steps > 0:
                         e.gauss_newton(refine_steps, damping)
                         st.refine_ms += e.last_refine_ms()
-                t0 = time.perf_counter()
-                for idx, e in zip(refs, ests):
-                    results[idx] = on_result(idx, e)
-                st.results_s = time.perf_counter() - t0
-                t0 = time.perf_counter()
                 del batch
-                for e in ests:
-                    e.release_device()
-                ests.clear()
-                st.release_s = time.perf_counter() - t0
-                st.wall_s = time.perf_counter() - t_group
-                groups.append(st)
-                if on_group is not None:
-                    on_group(st)
+                if overlap_staging:
+                    finishing.append(pool.submit(finish, list(ests), refs, st, t_group))
+                    ests.clear()
+                    # a failed finish ends the run before the next group starts
+                    while finishing and finishing[0].done():
+                        finishing.pop(0).result()
+                else:
+                    finish(ests, refs, st, t_group)
+            for f in finishing:
+                f.result()
     finally:
         if reuse_buffers:
             est_cls.set_buffer_cache(device, 0)

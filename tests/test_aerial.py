@@ -285,6 +285,43 @@ def test_run_admitted_control_flow(overlap):
     assert all(g.run_ms == 5.0 and g.refine_ms == len(g.refs) for g in groups)
 
 
+def test_overlapped_groups_hold_at_most_two_groups_of_memory():
+    """With overlap_staging the worker collects and releases a finished group
+    while the next one runs, and stages the group after that only then: the
+    bytes of groups staged and not yet released never exceed the budget."""
+    from opensfm_amd.admission import run_admitted
+    lib = _StubLibrary()
+    sizes = [40, 40, 30, 60, 50, 50, 20, 80, 10]
+    res, groups = run_admitted(lib, len(sizes), lambda i: _StubEstimator(lib.log, i, sizes[i]), budget=200,
+                               refine_steps=1, overlap_staging=True)
+    assert [g.refs for g in groups] == [[0, 1], [2, 3], [4, 5], [6, 7], [8]]
+    held = peak = 0
+    for e in lib.log:
+        if e[0] == "prepare":
+            held += sum(sizes[i] for i in e[1])
+            peak = max(peak, held)
+        elif e[0] == "release":
+            held -= sizes[e[1]]
+    assert held == 0 and 100 < peak <= 200
+    assert sorted(res) == list(range(len(sizes))) and all(res[i]["depth"][0, 0] == i for i in res)
+    assert lib.log[-1] == ("cache", 0)
+
+
+def test_overlapped_result_errors_end_the_run():
+    from opensfm_amd.admission import run_admitted
+    lib = _StubLibrary()
+
+    def on_result(i, e):
+        if i == 3:
+            raise RuntimeError("disk full")
+        return i
+
+    with pytest.raises(RuntimeError, match="disk full"):
+        run_admitted(lib, 8, lambda i: _StubEstimator(lib.log, i, 30), budget=200, on_result=on_result,
+                     overlap_staging=True)
+    assert lib.log[-1] == ("cache", 0)
+
+
 def test_run_admitted_runs_every_pyramid_level_in_order():
     # a batch starts at level 0 of each reference's pyramid; the finer levels
     # run after it, in order, before the refinement and the results
