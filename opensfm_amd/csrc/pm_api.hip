@@ -1489,8 +1489,9 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   for (int i = 0; i < steps; ++i)
     PM_HIP(launch_gauss_newton(a, hp, h->f16, h->vgeom.p, damping, state, i == 0, h->stream),
            "Gauss-Newton kernel launch");
-  // the guard's list of moved pixels takes the state buffer, free after the
-  // last step (it holds gauss_newton_state_floats() >= 2 floats per pixel)
+  // the guard's list of moved pixels, its count and its block offsets take
+  // the state buffer, free after the last step (2n + 1 ints at most; it holds
+  // gauss_newton_state_floats() = 16 floats per pixel)
   if (steps > 0)
     PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, reinterpret_cast<int*>(state), h->stream),
            "refinement guard launch");

@@ -132,30 +132,71 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
 // ---------------------------------------------------------------------------
 // The guard runs on the moved pixels only, compacted into a list first:
 // moves are sparse, and one moved pixel in a wave would otherwise hold the
-// whole wave for two full cost vectors.  list[] and *count live in the
-// refinement's state buffer, free once its steps are done; the order of the
-// list is arbitrary and changes no result (pixels are independent).
+// whole wave for two full cost vectors.  The list is in pixel order (a count
+// per block of 256 pixels, an exclusive prefix of the counts, then each
+// block writes its moved pixels from its offset), so the 64 entries a guard
+// wave takes are neighbouring pixels whose patches share cache lines.
+// list[], the block offsets and *count live in the refinement's state
+// buffer, free once its steps are done.
 // One-wave workgroups looping over the list, 32 per CU: 8 waves per SIMD hide
 // the gathers of the full cost vectors (Gauss-Newton with its guard, ten
 // 4000x3000 references: 535 ms at 2048 workgroups, 524 ms at 8192).
 constexpr unsigned kGuardWaves = 8192;
-__global__ __launch_bounds__(kWave) void guard_clear_kernel(int* count) { *count = 0; }
+constexpr int kGuardBlock = 256;
 
-__global__ __launch_bounds__(256) void guard_list_kernel(const float4* __restrict__ planes,
-                                                         const float4* __restrict__ orig, int64_t n, int* list,
-                                                         int* count) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool moved = false;
-  if (i < n) {
-    const float4 pn = planes[i], po = orig[i];
-    moved = !(pn.x == po.x && pn.y == po.y && pn.z == po.z && pn.w == po.w);
+__device__ __forceinline__ bool plane_moved(const float4* __restrict__ planes, const float4* __restrict__ orig,
+                                            int64_t i) {
+  const float4 pn = planes[i], po = orig[i];
+  return !(pn.x == po.x && pn.y == po.y && pn.z == po.z && pn.w == po.w);
+}
+
+__global__ __launch_bounds__(kGuardBlock) void guard_count_kernel(const float4* __restrict__ planes,
+                                                                  const float4* __restrict__ orig, int64_t n,
+                                                                  int* offs) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kGuardBlock + threadIdx.x;
+  const int c = __syncthreads_count(i < n && plane_moved(planes, orig, i));
+  if (threadIdx.x == 0) offs[blockIdx.x] = c;
+}
+
+// Exclusive prefix of offs[0..nb) in place (one workgroup: a serial run per
+// thread over its share, a scan of the 1024 shares in LDS); the total goes
+// to *count.
+__global__ __launch_bounds__(1024) void guard_scan_kernel(int* offs, int nb, int* count) {
+  __shared__ int part[1024];
+  const int t = static_cast<int>(threadIdx.x);
+  const int per = (nb + 1023) / 1024;
+  const int lo = min(t * per, nb), hi = min(lo + per, nb);
+  int own = 0;
+  for (int k = lo; k < hi; ++k) own += offs[k];
+  part[t] = own;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
+  int run = part[t] - own;
+  for (int k = lo; k < hi; ++k) {
+    const int c = offs[k];
+    offs[k] = run;
+    run += c;
+  }
+  if (t == 1023) *count = part[1023];
+}
+
+__global__ __launch_bounds__(kGuardBlock) void guard_list_kernel(const float4* __restrict__ planes,
+                                                                 const float4* __restrict__ orig, int64_t n,
+                                                                 const int* __restrict__ offs, int* list) {
+  __shared__ int wave_count[kGuardBlock / kWave];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kGuardBlock + threadIdx.x;
+  const bool moved = i < n && plane_moved(planes, orig, i);
   const uint64_t m = __ballot(moved);
-  if (m == 0ull) return;
-  const int lane = static_cast<int>(threadIdx.x % kWave);
-  int base = 0;
-  if (lane == __ffsll(static_cast<long long>(m)) - 1) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, __ffsll(static_cast<long long>(m)) - 1);
+  const int lane = static_cast<int>(threadIdx.x % kWave), w = static_cast<int>(threadIdx.x / kWave);
+  if (lane == 0) wave_count[w] = __popcll(m);
+  __syncthreads();
+  int base = offs[blockIdx.x];
+  for (int k = 0; k < w; ++k) base += wave_count[k];
   if (moved) list[base + __popcll(m & ((1ull << lane) - 1ull))] = static_cast<int>(i);
 }
 
@@ -581,11 +622,17 @@ hipError_t launch_score(const PmArgs& a, int hp, bool f16, int n, const int32_t*
 
 hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, int* scratch,
                                hipStream_t stream) {
+  // scratch: list[n], *count, then the ceil(n / 256) block offsets: at most
+  // 2n + 1 ints, within the state buffer's 16 floats per pixel for any n >= 1
   const int64_t n = static_cast<int64_t>(a.width) * a.height;
+  const int nb = static_cast<int>((n + kGuardBlock - 1) / kGuardBlock);
   int* count = scratch + n;
-  hipLaunchKernelGGL(guard_clear_kernel, dim3(1), dim3(1), 0, stream, count);
-  hipLaunchKernelGGL(guard_list_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, a.planes,
-                     orig, n, scratch, count);
+  int* offs = scratch + n + 1;
+  hipLaunchKernelGGL(guard_count_kernel, dim3(static_cast<unsigned>(nb)), dim3(kGuardBlock), 0, stream, a.planes, orig,
+                     n, offs);
+  hipLaunchKernelGGL(guard_scan_kernel, dim3(1), dim3(1024), 0, stream, offs, nb, count);
+  hipLaunchKernelGGL(guard_list_kernel, dim3(static_cast<unsigned>(nb)), dim3(kGuardBlock), 0, stream, a.planes, orig,
+                     n, offs, scratch);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const dim3 grid(kGuardWaves);
