@@ -210,3 +210,26 @@ def test_gauss_newton_state_buffers(pydense, num_src):
     assert np.all(np.isfinite(after["depth"]))
     assert np.mean(c1[m] <= c0[m] + 1e-4) >= 0.999 and c1[m].mean() < c0[m].mean()
     assert not np.array_equal(mid["plane"], before["plane"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h", [(1, 1), (5, 3), (300, 1), (97, 61)])
+def test_gauss_newton_on_small_and_thin_images(pydense, w, h):
+    """The refinement and its guard on images of one pixel, a few pixels, one
+    row (the guard's pixel-order list then spans two blocks of 256 pixels)
+    and an odd size: the run completes, its depths are finite, and two runs
+    give the same planes."""
+    from opensfm_amd import synthetic
+    s = synthetic.three_planes(9, width=w, height=h, num_src=2, base_cell=0.3, supersample=1)
+    p = params_for(s, patch_size=5)
+    outs = []
+    for _ in range(2):
+        est = gpu_estimator(pydense, s, p)
+        est.compute()
+        est.gauss_newton(4, 1e-3)
+        outs.append(est.download())
+    print(f"Gauss-Newton on {w}x{h}: {np.count_nonzero(outs[0]['depth'])} pixels with a depth", flush=True)
+    assert outs[0]["depth"].shape == (h, w)
+    assert np.all(np.isfinite(outs[0]["depth"]))
+    for k in ("depth", "plane"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
