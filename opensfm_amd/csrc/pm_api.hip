@@ -269,6 +269,12 @@ struct pm_handle {
   // Bumped by every change of views, parameters or device buffers, so a
   // batch knows when its argument tables are stale.
   uint64_t gen = 0;
+  // Slot 8 of ca holds, for the pixels of colour c, the per-view costs of
+  // their current planes since the last colour-c pass (select and refine
+  // copy the winner's costs there); cleared by anything else that changes
+  // planes or ca, and by a new generation.
+  bool cur_valid[2] = {false, false};
+  uint64_t cur_gen = 0;
 
   // Device buffers into the block cache (or freed); the stream has drained.
   // The coarse level's result stays unless keep_coarse is false: the next
@@ -1196,6 +1202,18 @@ int pm_upload(pm_handle* h) {
 
 namespace {
 
+// Whether the candidates pass of colour c may take the current planes'
+// costs from slot 8 of ca instead of scoring them again.
+bool current_cached(const pm_handle* h, int c) { return h->cur_valid[c] && h->cur_gen == h->gen; }
+void current_stale(pm_handle* h) { h->cur_valid[0] = h->cur_valid[1] = false; }
+void current_filled(pm_handle* h, int c) {
+  if (h->cur_gen != h->gen) {
+    current_stale(h);
+    h->cur_gen = h->gen;
+  }
+  h->cur_valid[c] = true;
+}
+
 // Prior re-seeding (PriorReinit, estimator.cc:574-614), when a prior is set.
 int run_prior(pm_handle* h, const PmArgs& a) {
   if (!h->masks.p) return PM_OK;
@@ -1238,6 +1256,7 @@ int pm_init_state(pm_handle* h) {
   PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
   if (int r = ensure_refpatch(h)) return r;
+  current_stale(h);
   if (int r = timed(h, PM_STAGE_INIT, "init kernel launch", [&] { return launch_init(a, hp, h->f16, h->stream); }))
     return r;
   if (int r = run_prior(h, a)) return r;  // estimator.cc:574-614
@@ -1259,6 +1278,7 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
   for (int c = 0; c < 2; ++c) {
     if (color >= 0 && c != color) continue;
     a.color = c;
+    a.cur_cached = current_cached(h, c) ? 1 : 0;
     if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
                       [&] { return launch_candidates(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
@@ -1269,6 +1289,7 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
                       [&] { return launch_fallback(a, hp, f16, 1, cost_tile(h), fallback_cap(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_REFINE, "refine kernel launch", [&] { return launch_refine(a, pixel_tile(h), st); })) return r;
+    current_filled(h, c);
   }
   return PM_OK;
 }
@@ -1276,6 +1297,7 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
 int pm_filter(pm_handle* h) {
   if (int r = check_handle(h)) return r;
   if (!h->state_ready) return fail(PM_E_STATE, "pm_filter: state is not initialised");
+  current_stale(h);
   PmArgs a = make_args(h);
   for (int c = 0; c < 2; ++c) {
     a.color = c;
@@ -1300,6 +1322,7 @@ int pm_plane_sweep(pm_handle* h, int num_planes) {
   if (int r = validate(h)) return r;
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   if (int r = ensure_refpatch(h)) return r;
+  current_stale(h);
   PmArgs a = make_args(h);
   PM_HIP(launch_plane_sweep(a, h->params.patch_size / 2, h->f16, num_planes, h->stream), "plane sweep kernel launch");
   h->state_ready = true;
@@ -1323,6 +1346,7 @@ int run_level(pm_handle* h, int level, int nlevels) {
     if (int r = pm_init_state(h)) return r;
   } else {
     if (h->coarse_w <= 0 || !h->coarse_depth.p) return fail(PM_E_STATE, "no coarser level to start from");
+    current_stale(h);
     if (int r = ensure_refpatch(h)) return r;
     PmArgs a = make_args(h);
     if (int r = timed(h, PM_STAGE_INIT, "upsample kernel launch", [&] {
@@ -1468,6 +1492,7 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   if (steps < 0 || !(damping >= 0.0f)) return fail(PM_E_INVALID, "pm_gauss_newton: bad steps or damping");
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   if (int r = ensure_refpatch(h)) return r;
+  current_stale(h);  // its state takes ca, and it moves planes
   const PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
   PM_HIP(hipEventRecord(h->ev[0], h->stream), "event");
@@ -1583,6 +1608,7 @@ int pm_write_state(pm_handle* h, const float* planes, const float* costs, const 
   PM_HIP(hipMemcpyAsync(h->sel.p, sel, npix * 4, hipMemcpyHostToDevice, h->stream), "write");
   PM_HIP(hipStreamSynchronize(h->stream), "write sync");
   h->state_ready = true;
+  current_stale(h);
   return PM_OK;
 }
 
@@ -2111,6 +2137,9 @@ int batch_prepare(pm_batch* b) {
           s.smooth_w = smooth_weight_at(h->params, it);
           s.color = c;
           s.phase = ph;
+          // a batch run starts from new planes (init or upsample, prior):
+          // iteration 0 scores the current planes and fills slot 8
+          s.cur_cached = it > 0 ? 1 : 0;
           tab[static_cast<size_t>(batch_slice(it, c, ph)) * n + r] = s;
         }
   }
@@ -2311,6 +2340,7 @@ int pm_batch_run(pm_batch* b) {
   b->kev_marks.clear();
   for (pm_handle* h : b->refs) {
     h->state_ready = true;
+    current_stale(h);
     if (int r = postprocess(h)) return r;
     h->next_level = b->level + 1;
     if (b->level < level_count(h->params) - 1) {  // keep the decoded level for the next one

@@ -1338,7 +1338,8 @@ __device__ __forceinline__ void candidates_body(const PmArgs& a, int b) {
   const int h = threadIdx.y;  // 0..7 candidates, 8 = current plane (one wave each)
   ColourPixel cp;
   uint32_t pending = 0u;
-  if (colour_lane_pixel<TX>(a, tx, ty, cp)) {
+  // the current plane's costs are already in slot 8 when cur_cached
+  if (!(h == 8 && a.cur_cached) && colour_lane_pixel<TX>(a, tx, ty, cp)) {
     const int pos = h == 8 ? cp.idx : candidate_position(a, h, cp.x, cp.y);
     if (h < 8) gpw(a.cpos)[static_cast<int64_t>(h) * a.ref_plane + cp.cidx] = pos;
     if (pos >= 0) {
@@ -1349,6 +1350,17 @@ __device__ __forceinline__ void candidates_body(const PmArgs& a, int b) {
     }
   }
   write_fb_masks(a, ty * colour_tiles_x<TX>(a) + tx, h, pending);
+}
+
+// The per-view costs of hypothesis `slot` become the costs of the pixel's
+// current plane (slot 8), which the next candidates pass of this colour
+// then need not score again.
+__device__ __forceinline__ void copy_costs_to_current(const PmArgs& a, int cidx, int slot) {
+  const int64_t pl = a.ref_plane;
+  const float* src = a.ca + static_cast<int64_t>(slot) * a.n_src * pl + cidx;
+  float* dst = a.ca + static_cast<int64_t>(8) * a.n_src * pl + cidx;
+#pragma unroll 1
+  for (int s = 0; s < a.n_src; ++s) dst[s * pl] = src[s * pl];
 }
 
 template <int NV, int TX>
@@ -1485,6 +1497,7 @@ __device__ __forceinline__ void select_body(const PmArgs& a, int b) {
       plane_now = bp;
       cost_now = best_cost;
       sel_out = new_sel;
+      copy_costs_to_current(a, cp.cidx, best);
     }
   }
 
@@ -1585,6 +1598,7 @@ __device__ __forceinline__ void refine_body(const PmArgs& a, int b) {
   const float smooth_w = a.smooth_w;
   const float* ca = a.ca + cp.cidx;
   // Step 4: score the refinement hypotheses in order (:1744-1771).
+  int win = -1;
   for (int r = 0; r < n_ref; ++r) {
     const Plane hyp = to_plane(gp(a.refp)[r * pl + cp.cidx]);
     float tc = mc_cost<NV>(a, ca + static_cast<int64_t>(r) * a.n_src * pl, pl, vw, wnorm, hyp, x, y, s0, s1,
@@ -1594,8 +1608,10 @@ __device__ __forceinline__ void refine_body(const PmArgs& a, int b) {
     if (!(db < a.depth_min || db > a.depth_max) && tc < cost_now) {
       plane_now = hyp;
       cost_now = tc;
+      win = r;
     }
   }
+  if (win >= 0) copy_costs_to_current(a, cp.cidx, win);
   gpw(a.planes)[idx] = to_f4(plane_now);
   gpw(a.costs)[idx] = cost_now;
   gpw(a.rng)[idx] = make_uint2(s0, s1);

@@ -49,6 +49,10 @@ struct PmArgs {
   int* nref;                   // number of refinement hypotheses (5 or 6)
   uint64_t* fb_masks;          // pending textureless fallbacks, [(tile * 9 + slot) * n_src + view]
   int phase;                   // fallback jobs of the candidate (0) or refinement (1) slots
+  // 1: slot 8 of ca already holds, for this colour's pixels, the per-view
+  // costs of their current planes (select and refine copy the winning
+  // hypothesis's costs there), so the candidates pass does not score it again
+  int cur_cached;
   uint32_t prev_depth_mask;
   int width, height, n_src;
   float depth_min, depth_max;
