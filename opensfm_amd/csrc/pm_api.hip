@@ -1259,6 +1259,10 @@ int pm_init_state(pm_handle* h) {
   current_stale(h);
   if (int r = timed(h, PM_STAGE_INIT, "init kernel launch", [&] { return launch_init(a, hp, h->f16, h->stream); }))
     return r;
+  if (!h->masks.p) {  // the init leaves the random planes' costs in slot 8; a prior re-seeds planes
+    current_filled(h, 0);
+    current_filled(h, 1);
+  }
   if (int r = run_prior(h, a)) return r;  // estimator.cc:574-614
   h->state_ready = true;
   return PM_OK;
@@ -2137,9 +2141,10 @@ int batch_prepare(pm_batch* b) {
           s.smooth_w = smooth_weight_at(h->params, it);
           s.color = c;
           s.phase = ph;
-          // a batch run starts from new planes (init or upsample, prior):
-          // iteration 0 scores the current planes and fills slot 8
-          s.cur_cached = it > 0 ? 1 : 0;
+          // a batch run starts from new planes: at level 0 the init leaves
+          // the random planes' costs in slot 8, unless a prior re-seeds
+          // planes; after an upsample iteration 0 scores the current planes
+          s.cur_cached = (it > 0 || (b->level == 0 && !h->masks.p)) ? 1 : 0;
           tab[static_cast<size_t>(batch_slice(it, c, ph)) * n + r] = s;
         }
   }

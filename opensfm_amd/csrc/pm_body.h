@@ -1065,7 +1065,14 @@ __device__ __forceinline__ void init_select_body(const PmArgs& a, int b) {
   const int cidx = colour_index(a, x, y);
   float* cv = smem + lane;
   float* tmp = smem + a.n_src * kWave + lane;
-  for (int s = 0; s < a.n_src; ++s) cv[s * kWave] = a.ca[s * a.ref_plane + cidx];
+  // The random plane is the pixel's current plane: its costs also go to
+  // slot 8, where the first candidates pass finds them.
+  float* cur = a.ca + static_cast<int64_t>(8) * a.n_src * a.ref_plane + cidx;
+  for (int s = 0; s < a.n_src; ++s) {
+    const float c = a.ca[s * a.ref_plane + cidx];
+    cv[s * kWave] = c;
+    cur[s * a.ref_plane] = c;
+  }
   uint32_t sel = 0u;
   a.costs[idx] = initial_cost_and_views(a, cv, tmp, &sel);
   a.sel[idx] = sel;
