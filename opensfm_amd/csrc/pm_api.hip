@@ -1496,6 +1496,10 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   if (steps < 0 || !(damping >= 0.0f)) return fail(PM_E_INVALID, "pm_gauss_newton: bad steps or damping");
   PM_HIP(hipSetDevice(h->device), "hipSetDevice");
   if (int r = ensure_refpatch(h)) return r;
+  // The PatchMatch planes' per-view costs, for the guard: slot 8 of ca when
+  // it is current for both colours (no filter or state change since the
+  // last passes)
+  const bool pm_costs = current_cached(h, 0) && current_cached(h, 1);
   current_stale(h);  // its state takes ca, and it moves planes
   const PmArgs a = make_args(h);
   const int hp = h->params.patch_size / 2;
@@ -1521,8 +1525,12 @@ int pm_gauss_newton(pm_handle* h, int steps, float damping) {
   // the guard's list of moved pixels, its count and its block offsets take
   // the state buffer, free after the last step (2n + 1 ints at most; it holds
   // gauss_newton_state_floats() = 16 floats per pixel)
+  // slot 8 lies past the state (16 floats per pixel) whenever the state
+  // lives in ca, which takes at least two sources
+  const size_t slot8 = static_cast<size_t>(8) * h->n_src * static_cast<size_t>(h->ref_plane);
+  const float* orig_costs = pm_costs && (state != h->ca.p || need <= slot8) ? h->ca.p + slot8 : nullptr;
   if (steps > 0)
-    PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, reinterpret_cast<int*>(state), h->stream),
+    PM_HIP(launch_refine_guard(a, hp, h->f16, h->gn_orig.p, orig_costs, reinterpret_cast<int*>(state), h->stream),
            "refinement guard launch");
   PM_HIP(hipEventRecord(h->ev[1], h->stream), "event");
   if (int r = pm_decode(h)) return r;
@@ -2345,7 +2353,13 @@ int pm_batch_run(pm_batch* b) {
   b->kev_marks.clear();
   for (pm_handle* h : b->refs) {
     h->state_ready = true;
+    // the last passes left slot 8 current for both colours, unless the
+    // filter changed planes after them
     current_stale(h);
+    if (b->iters > 0 && !h->params.checkerboard_filter) {
+      current_filled(h, 0);
+      current_filled(h, 1);
+    }
     if (int r = postprocess(h)) return r;
     h->next_level = b->level + 1;
     if (b->level < level_count(h->params) - 1) {  // keep the decoded level for the next one

@@ -170,8 +170,8 @@ int gauss_newton_state_floats();
 // After the Gauss-Newton steps: pixels whose full PatchMatch cost (fallbacks
 // included) rose go back to their plane in `orig`.  scratch: W*H + 1 ints
 // of device memory (the moved pixels' list and its length).
-hipError_t launch_refine_guard(const PmArgs& a, int half_patch, bool f16, const float4* orig, int* scratch,
-                               hipStream_t stream);
+hipError_t launch_refine_guard(const PmArgs& a, int half_patch, bool f16, const float4* orig,
+                               const float* orig_costs, int* scratch, hipStream_t stream);
 hipError_t launch_gauss_newton(const PmArgs& a, int half_patch, bool f16, const float* vgeom, float lambda,
                                float* state, bool first, hipStream_t stream);
 // conf = logistic(cost) where depth > 0, else 0 (n pixels).

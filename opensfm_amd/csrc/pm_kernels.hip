@@ -201,7 +201,10 @@ __global__ __launch_bounds__(kGuardBlock) void guard_list_kernel(const float4* _
 }
 
 template <int HP, typename T>
+// orig_costs, when not null: the per-view full costs of the PatchMatch
+// planes (slot 8 of ca, [view][cidx]), which then need not be scored again.
 __global__ __launch_bounds__(kWave) void refine_guard_kernel(PmArgs a, const float4* __restrict__ orig,
+                                                             const float* __restrict__ orig_costs,
                                                              const int* __restrict__ list,
                                                              const int* __restrict__ count) {
   extern __shared__ float smem[];
@@ -220,10 +223,15 @@ __global__ __launch_bounds__(kWave) void refine_guard_kernel(PmArgs a, const flo
     load_nbrs(a, orig, x, y, nb);
     float obj[2];
     const float4 pl[2] = {po, pn};
+    const int cidx = colour_index(a, x, y);
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
       const Plane p = to_plane(pl[k]);
-      full_cost_vector<HP>(a, img, x, y, colour_index(a, x, y), p, cv);
+      if (k == 0 && orig_costs) {
+        for (int s = 0; s < a.n_src; ++s) cv[s * kWave] = orig_costs[s * a.ref_plane + cidx];
+      } else {
+        full_cost_vector<HP>(a, img, x, y, cidx, p, cv);
+      }
       float sum = 0.0f;
       int n = 0;
       for (int s = 0; s < a.n_src; ++s) {
@@ -620,8 +628,8 @@ hipError_t launch_score(const PmArgs& a, int hp, bool f16, int n, const int32_t*
   PM_DISPATCH(score_kernel, hp, f16, grid, lds, stream, a, n, xy, planes, costs, init_cost, sel);
 }
 
-hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, int* scratch,
-                               hipStream_t stream) {
+hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* orig, const float* orig_costs,
+                               int* scratch, hipStream_t stream) {
   // scratch: list[n], *count, then the ceil(n / 256) block offsets: at most
   // 2n + 1 ints, within the state buffer's 16 floats per pixel for any n >= 1
   const int64_t n = static_cast<int64_t>(a.width) * a.height;
@@ -637,7 +645,7 @@ hipError_t launch_refine_guard(const PmArgs& a, int hp, bool f16, const float4* 
   if (e != hipSuccess) return e;
   const dim3 grid(kGuardWaves);
   const size_t lds = cost_column_lds(a.n_src);
-  PM_DISPATCH(refine_guard_kernel, hp, f16, grid, lds, stream, a, orig, scratch, count);
+  PM_DISPATCH(refine_guard_kernel, hp, f16, grid, lds, stream, a, orig, orig_costs, scratch, count);
 }
 
 }  // namespace pmhip

@@ -233,3 +233,31 @@ def test_gauss_newton_on_small_and_thin_images(pydense, w, h):
     assert np.all(np.isfinite(outs[0]["depth"]))
     for k in ("depth", "plane"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("num_src", [1, 4])
+def test_guard_reads_the_patchmatch_costs_it_would_score(pydense, bumps, num_src):
+    """After a run without the checkerboard filter, the guard takes the
+    PatchMatch planes' per-view costs from the colour passes' buffer instead
+    of scoring them again.  Writing the state back (the same values) makes
+    that buffer stale, so the guard scores the planes itself: both ways must
+    keep exactly the same pixels, bit for bit."""
+    from opensfm_amd import synthetic
+    # one source: the Gauss-Newton state has a buffer of its own; four: it
+    # shares the colour passes' buffer, below the PatchMatch costs
+    s = bumps if num_src == 4 else synthetic.three_planes(5, width=160, height=120, num_src=1, base_cell=0.3)
+    assert s.num_src == num_src
+    p = params_for(s, patch_size=5, checkerboard_filter=False)
+    outs = []
+    for rewrite in (False, True):
+        est = gpu_estimator(pydense, s, p)
+        est.compute()
+        if rewrite:
+            est.write_state(*est.read_state())
+        est.gauss_newton(10, 1e-3)
+        outs.append(est.download(["plane", "depth"]))
+    moved = np.count_nonzero(np.any(outs[0]["plane"] != outs[1]["plane"], axis=-1))
+    print(f"{num_src} source(s): planes that differ between the two guards: {moved}", flush=True)
+    assert np.array_equal(outs[0]["plane"], outs[1]["plane"])
+    assert np.array_equal(outs[0]["depth"], outs[1]["depth"])
