@@ -41,7 +41,11 @@ METRIC = "Mpix·views·iters/s on 1 GPU vs host CPU; % pixels |Δd|/d<1% vs refe
 UNIT = "Mpix·views·iters/s"
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PATCH = 7
-FLOP_PER_UNIT = 15 * (21 * PATCH * PATCH + 75)   # 15 hypotheses x (21 FLOP/tap x P taps + 75)
+# The reference scores 15 hypotheses per pixel and iteration (8 candidates,
+# the current plane, 6 refinements); the current plane's per-view costs are
+# carried over from its last scoring here, so 14 are scored.
+CAND_HYPS = 8                     # hypotheses one candidates launch scores
+FLOP_PER_UNIT = (CAND_HYPS + 6) * (21 * PATCH * PATCH + 75)   # x (21 FLOP/tap x P taps + 75)
 # HBM bytes per candidates launch: rocprofv3 --pmc FETCH_SIZE (x 1.9999) +
 # WRITE_SIZE (x 1.0), each factor read off the same counter on a 1 GiB
 # device-to-device copy (profiles/*_fetch.txt, profiles/*_write.txt).
@@ -531,19 +535,19 @@ def run_c5(args, pydense) -> int:
     del depths
 
     # Roofline of the candidate-scoring kernel (fixed algorithmic work per
-    # launch): one batched launch scores 9 hypotheses of every pixel of one
-    # colour of every reference of the group, 21 FLOP per tap and 75 per
-    # (hypothesis, view).  Launch times: HIP events on the batch stream
+    # launch): one batched launch scores the 8 candidates of every pixel of
+    # one colour of every reference of the group (the current plane's costs
+    # are carried over), 21 FLOP per tap and 75 per (hypothesis, view).  Launch times: HIP events on the batch stream
     # around every launch, over the timed steps.
     # A launch covers one colour of every reference of an admitted group.
     # Groups differ in size (the last one is smaller), so the roofline takes
     # the launches of the groups of the most common size: one launch scores
-    # refs x (W*H/2) pixels x 9 hypotheses x N sources.
+    # refs x (W*H/2) pixels x 8 hypotheses x N sources.
     sizes = [len(g.refs) for g in groups]
     modal = max(sorted(set(sizes)), key=sizes.count)
     cand_ms = sum(g.stage_ms.get("candidates", (0.0, 0))[0] for g in groups if len(g.refs) == modal)
     cand_n = sum(g.stage_ms.get("candidates", (0.0, 0))[1] for g in groups if len(g.refs) == modal)
-    flop_per_launch = modal * (w * h / 2.0) * 9 * args.neighbours * (21 * PATCH * PATCH + 75)
+    flop_per_launch = modal * (w * h / 2.0) * CAND_HYPS * args.neighbours * (21 * PATCH * PATCH + 75)
     avg_launch_s = cand_ms / max(cand_n, 1) / 1e3
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     sweep_ms = sum(v[0] for k, v in stages.items() if k != "init")
@@ -785,11 +789,12 @@ def main() -> int:
     f16_effect = storage_effect(pydense, scene, args.iters, d) if args.storage == "f16" else None
 
     # Roofline of the candidates kernel (fixed work per launch): one launch scores the
-    # 8 propagated candidates + the current plane of every pixel of one
-    # colour, i.e. (W*H/2) x 9 hypotheses x N_src x (21 P + 75) FLOP.
+    # 8 propagated candidates of every pixel of one colour (the current
+    # plane's costs are carried over), i.e. (W*H/2) x 8 hypotheses x N_src x
+    # (21 P + 75) FLOP.
     cand_ms, cand_n = stages["candidates"]
     avg_launch_s = cand_ms / max(cand_n, 1) / 1e3
-    flop_per_launch = (w * h / 2.0) * 9 * n_src * (21 * PATCH * PATCH + 75)
+    flop_per_launch = (w * h / 2.0) * CAND_HYPS * n_src * (21 * PATCH * PATCH + 75)
     achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
