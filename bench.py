@@ -50,11 +50,11 @@ FLOP_PER_UNIT = (CAND_HYPS + 6) * (21 * PATCH * PATCH + 75)   # x (21 FLOP/tap x
 # WRITE_SIZE (x 1.0), each factor read off the same counter on a 1 GiB
 # device-to-device copy (profiles/*_fetch.txt, profiles/*_write.txt).
 # candidates_kernel<3, float, 4>, 1600x1200 with 8 sources:
-TRAFFIC_PER_LAUNCH = 1.446e9
+TRAFFIC_PER_LAUNCH = 1.245e9
 TRAFFIC_TILE = 4
 # candidates_batch<3, _Float16, 16> per 4000x3000 reference with 10 sources
 # in the launch (a launch covers every reference of an admitted group):
-TRAFFIC_PER_REF_C5 = 1.125e10
+TRAFFIC_PER_REF_C5 = 1.012e10
 
 
 def log(msg: str) -> None:
