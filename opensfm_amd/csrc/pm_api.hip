@@ -1283,6 +1283,7 @@ int pm_iterate(pm_handle* h, int iteration, int color) {
     if (color >= 0 && c != color) continue;
     a.color = c;
     a.cur_cached = current_cached(h, c) ? 1 : 0;
+    h->cur_valid[c] = false;  // until every launch of this pass went out
     if (int r = timed(h, PM_STAGE_CANDIDATES, "candidates kernel launch",
                       [&] { return launch_candidates(a, hp, f16, cost_tile(h), st); })) return r;
     if (int r = timed(h, PM_STAGE_FALLBACK, "fallback kernel launch",
