@@ -93,7 +93,8 @@ def run_admitted(pydense, count: int, build: Callable[[int], Any], budget: Optio
     reference after its group's run and refinement; its return value is
     collected (default: ``est.download(["depth"])``).  ``batch_config`` goes
     to ``DepthmapBatch.set_config``.  ``on_group`` is called with each
-    finished group's timing.  ``reuse_buffers``: keep released device
+    finished group's timing.  With ``overlap_staging`` both callbacks run on
+    the worker thread, one group at a time and in group order.  ``reuse_buffers``: keep released device
     buffers for the next group (the cache is emptied at the end).
     ``overlap_staging``: stage the next group while the current one runs
     (groups of half the budget), and collect a group's results and release
