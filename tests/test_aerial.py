@@ -8,6 +8,8 @@ its grouping rule; on the GPU, references run in admitted groups must equal
 the same references run alone (and refined alone) bit for bit, and one
 aerial reference must equal the oracle.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -190,8 +192,8 @@ def test_buffer_cache_reuses_released_blocks(pydense, survey):
 
 
 class _StubEstimator:
-    def __init__(self, log, i, nbytes, levels=1):
-        self.log, self.i, self.nbytes, self.levels = log, i, nbytes, levels
+    def __init__(self, log, i, nbytes, levels=1, delay=0.0):
+        self.log, self.i, self.nbytes, self.levels, self.delay = log, i, nbytes, levels, delay
 
     def device_bytes(self):
         return self.nbytes
@@ -206,6 +208,8 @@ class _StubEstimator:
         return 1.0
 
     def download(self, maps=None):
+        if self.delay:   # a download that takes a while (a copy behind running kernels)
+            time.sleep(self.delay)
         return {"depth": np.full((2, 2), float(self.i), np.float32)}
 
     def release_device(self):
@@ -304,6 +308,42 @@ def test_overlapped_groups_hold_at_most_two_groups_of_memory():
             held -= sizes[e[1]]
     assert held == 0 and 100 < peak <= 200
     assert sorted(res) == list(range(len(sizes))) and all(res[i]["depth"][0, 0] == i for i in res)
+    assert lib.log[-1] == ("cache", 0)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_admission_bounds_and_order_on_random_sizes(seed):
+    """Random reference sizes and budgets, staging overlapped or not: groups
+    take references in order, no group exceeds its share of the budget
+    unless it is a single reference larger than that share, the bytes of
+    groups staged and not yet released never exceed the budget (two
+    references too large for a share aside), and every result is collected
+    exactly once."""
+    from opensfm_amd.admission import run_admitted
+    rng = np.random.default_rng(seed)
+    count = int(rng.integers(1, 30))
+    sizes = [int(v) for v in rng.integers(1, 120, count)]
+    budget = int(rng.integers(60, 400))
+    overlap = bool(seed % 2)
+    lib = _StubLibrary()
+    res, groups = run_admitted(lib, count, lambda i: _StubEstimator(lib.log, i, sizes[i], delay=0.002),
+                               budget=budget, refine_steps=int(seed % 3), overlap_staging=overlap)
+    share = budget // 2 if overlap else budget
+    assert [i for g in groups for i in g.refs] == list(range(count))
+    for g in groups:
+        assert g.device_bytes <= share or len(g.refs) == 1
+        assert g.device_bytes == sum(sizes[i] for i in g.refs)
+    held = peak = 0
+    for e in lib.log:
+        if e[0] == "prepare":
+            held += sum(sizes[i] for i in e[1])
+            peak = max(peak, held)
+        elif e[0] == "release":
+            held -= sizes[e[1]]
+    assert held == 0
+    if all(sz <= share for sz in sizes):
+        assert peak <= (budget if overlap else share)
+    assert sorted(res) == list(range(count)) and all(res[i]["depth"][0, 0] == i for i in res)
     assert lib.log[-1] == ("cache", 0)
 
 
