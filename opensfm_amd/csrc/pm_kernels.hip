@@ -130,9 +130,9 @@ __global__ __launch_bounds__(kWave) void score_kernel(PmArgs a, int n, const int
 // the PatchMatch plane `orig`.  (Without the smoothness term, a step that
 // fits the images' noise moves a pixel away from its neighbours' surface.)
 // ---------------------------------------------------------------------------
-// The guard runs on the moved pixels only, compacted into a list first:
-// moves are sparse, and one moved pixel in a wave would otherwise hold the
-// whole wave for two full cost vectors.  The list is in pixel order (a count
+// The guard runs on the moved pixels only, compacted into a list first: they
+// are a minority of the pixels, and one moved pixel in a wave would
+// otherwise hold the whole wave for its full cost vectors.  The list is in pixel order (a count
 // per block of 256 pixels, an exclusive prefix of the counts, then each
 // block writes its moved pixels from its offset), so the 64 entries a guard
 // wave takes are neighbouring pixels whose patches share cache lines.
