@@ -1851,8 +1851,8 @@ __global__ __launch_bounds__(kWave * kFbWaves) PM_FALLBACK_ATTR void fallback_ba
 // per SIMD instead of 2 (aerial survey, 10 sources: 562 -> 446 ms per step
 // of ten 4000x3000 references); at 4 it would spill 136 bytes per lane.
 // Up to 12 views 4 waves fit with a spill of at most 44 bytes outside the
-// MC draws: select 232 -> 221 ms per aerial step (10 sources), 3.37 ->
-// 3.17 ms per 1600x1200 step with 8 sources.
+// MC draws: select 232 -> 221 ms per aerial step (10 sources, 12-view
+// instance), 3.37 -> 3.17 ms per 1600x1200 step with 8 sources.
 #ifndef PM_SELECT_WAVES_PER_EU
 #define PM_SELECT_WAVES_PER_EU(NV) ((NV) <= 12 ? 4 : 3)
 #endif

@@ -285,6 +285,8 @@ void launch_pixel_tx(int kernel, const PmArgs* a, const PmBatch* bt, dim3 g, int
     else hipLaunchKernelGGL((filter_batch<TX>), g, dim3(kWave), 0, s, *bt);
   } else if (max_src <= 8) {
     launch_pixel_nv<8, TX>(kernel, a, bt, g, s);
+  } else if (max_src <= 10) {
+    launch_pixel_nv<10, TX>(kernel, a, bt, g, s);
   } else if (max_src <= 12) {
     launch_pixel_nv<12, TX>(kernel, a, bt, g, s);
   } else {
